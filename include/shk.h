@@ -1,0 +1,244 @@
+/*
+ * shk.h -- C ABI of the MI355X (gfx950) sparse Krylov library "shk"
+ * (Seldon HIP Krylov).
+ *
+ * This is the drop-in boundary for Seldon's sparse Krylov hot path.  Seldon
+ * has no runtime plugin layer: its solvers are templates that reach the
+ * kernels through unqualified calls (Mlt, MltAdd, DotProd, Norm2, Add, Copy,
+ * Mlt(alpha, v), M.Solve) resolved at compile time
+ * (computation/solver/iterative/Cg.cxx:45-116, BiCgStab.cxx:44-145,
+ * Gmres.cxx:45-182; doc/guide/iterative.dox:62,223).  Each entry point below
+ * replaces one of those calls (or a whole solver template) and is cited to it.
+ * The Seldon-compatible C++ overloads in include/seldon_amd/ call these.
+ *
+ * Conventions
+ *   - Every function returns an int status: SHK_OK (0) or a negative
+ *     SHK_ERR_*; shk_last_error() gives the message (thread-local).  No C++
+ *     exception crosses this boundary.
+ *   - Vectors are raw device pointers of the matrix's value type; every call
+ *     is ordered on the context's stream.  Scalars that Seldon returns by
+ *     value (DotProd, Norm2) come back on the host (the call synchronises);
+ *     the *_device variants leave them in device memory instead.
+ *   - A context is single-threaded (Seldon is not thread-safe either,
+ *     vector/Vector.hxx:50); separate contexts may be used from separate
+ *     threads.  Device memory is owned by the handles.
+ *   - Arithmetic order: SpMV row sums are taken sequentially in stored order,
+ *     exactly as Functions_MatVect.cxx:424-430 does, and FP64/FP32 products
+ *     and sums are not contracted into FMAs, so SpMV results are bitwise
+ *     equal to Seldon's generic templates.  Reductions (dot, nrm2) are
+ *     deterministic fixed-order trees.
+ */
+#ifndef SHK_H
+#define SHK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHK_OK 0
+#define SHK_ERR_ARG (-1)     /* bad argument (null handle, negative size, ...) */
+#define SHK_ERR_HIP (-2)     /* HIP runtime error; message has the HIP text */
+#define SHK_ERR_NOMEM (-3)   /* device allocation failed (Seldon NoMemory) */
+#define SHK_ERR_DIM (-4)     /* dimension mismatch (Seldon WrongDim) */
+#define SHK_ERR_NODEV (-5)   /* no usable gfx950 device */
+#define SHK_ERR_STATE (-6)   /* call not valid in the current state */
+
+typedef enum { SHK_F64 = 0, SHK_F32 = 1 } shk_dtype;
+
+/* The three Seldon storages of the path (share/Storage.hxx:82,115,138). */
+typedef enum {
+    SHK_ROWSPARSE = 0,       /* Matrix<T,General,RowSparse>: full CSR */
+    SHK_ROWSYMSPARSE = 1,    /* Matrix<T,Symmetric,RowSymSparse>: upper CSR */
+    SHK_ARRAYROWSPARSE = 2   /* Matrix<T,General,ArrayRowSparse>: rows laid
+                                end to end as by ConvertToCSR
+                                (matrix_sparse/Matrix_Conversions.cxx:3676-3705) */
+} shk_storage;
+
+typedef enum { SHK_PREC_IDENTITY = 0, SHK_PREC_JACOBI = 1 } shk_precond;
+
+typedef struct shk_ctx_s *shk_ctx;
+typedef struct shk_csr_s *shk_csr;
+
+const char *shk_last_error(void);
+const char *shk_version(void);
+
+/* ---- context (device + stream + workspace) ----------------------------- */
+int shk_ctx_create(int device, shk_ctx *out);
+int shk_ctx_destroy(shk_ctx ctx);
+/* Run on an external hipStream_t (e.g. torch.cuda.current_stream()), taken
+ * verbatim: NULL is the default (null) stream.  shk_ctx_create starts on a
+ * stream of the context's own (default priority). */
+int shk_ctx_set_stream(shk_ctx ctx, void *hip_stream);
+int shk_ctx_synchronize(shk_ctx ctx);
+int shk_ctx_device_info(shk_ctx ctx, int *num_cus, int *num_xcds, char *arch, int arch_len);
+
+/* ---- device memory (Allocator role: share/Allocator.cxx:34-65) --------- */
+int shk_malloc(shk_ctx ctx, size_t bytes, void **dptr);
+int shk_free(shk_ctx ctx, void *dptr);
+int shk_memcpy_h2d(shk_ctx ctx, void *dst, const void *src, size_t bytes);
+int shk_memcpy_d2h(shk_ctx ctx, void *dst, const void *src, size_t bytes);
+int shk_memcpy_d2d(shk_ctx ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- sparse matrices --------------------------------------------------- */
+/* Upload a matrix.  ptr/ind/val describe the storage as Seldon holds it
+ * (0-based; RowSymSparse = upper triangle with diagonal).  src_on_device
+ * says where the three arrays live.  The caller keeps ownership of them.
+ * Replaces Matrix_Sparse::SetData / the Matrix(m,n,values,ptr,ind)
+ * constructor (matrix_sparse/Matrix_Sparse.cxx:105-181, 300-412) for a
+ * device-resident copy. */
+int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n,
+                   int64_t nnz, const int *ptr, const int *ind, const void *val,
+                   int src_on_device, shk_csr *out);
+int shk_csr_destroy(shk_csr A);
+/* New values on the same pattern, in the caller's stored order (RowSymSparse:
+ * upper part).  Used when A depends on parameters (the differentiable solve). */
+int shk_csr_set_values(shk_ctx ctx, shk_csr A, const void *val, int src_on_device);
+/* m, n, stored nnz (as the caller gave it), nnz held on device after the
+ * symmetric expansion, value type, storage. */
+int shk_csr_info(shk_csr A, int *m, int *n, int64_t *nnz_stored, int64_t *nnz_device,
+                 int *dtype, int *storage);
+/* Algorithmic bytes of one MltAdd on this matrix (what a Seldon-style CSR
+ * traversal must move: stored ind+val, ptr, x, y write, + y read if beta!=0). */
+double shk_csr_spmv_bytes(shk_csr A, int beta_nonzero);
+
+/* ---- SpMV -------------------------------------------------------------- */
+/* MltAdd(alpha, A, X, beta, Y): Y <- beta*Y + alpha*A*X
+ *   RowSparse      Functions_MatVect.cxx:398-431 (beta multiplies, NaN kept)
+ *   RowSymSparse   Functions_MatVect.cxx:526-564
+ *   ArrayRowSparse matrix_sparse/Functions_MatrixArray.cxx:534-575
+ *                  (beta == 0 zero-fills; alpha != 1 applied per term)
+ * Trans/SeldonNoTrans for the symmetric storage forward here
+ * (Functions_MatVect.cxx:905-927). */
+int shk_spmv(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double beta, void *dy);
+/* Out-of-place MltAdd: Y_out <- beta*Y_in + alpha*A*X (Y_in may equal Y_out). */
+int shk_spmv_out(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double beta,
+                 const void *dy_in, void *dy_out);
+/* Mlt(M, X, Y): Y.Fill(0); MltAdd(1, M, X, 0, Y) (Functions_MatVect.cxx:72-81). */
+int shk_mlt(shk_ctx ctx, shk_csr A, const void *dx, void *dy);
+/* Fused: Y = A*X and returns (X, Y) -- the CG step q = A p, (p, q)
+ * (computation/solver/iterative/Cg.cxx:97-98). */
+int shk_mlt_dot(shk_ctx ctx, shk_csr A, const void *dx, void *dy, double *result);
+
+/* ---- BLAS-1 (computation/basic_functions/Functions_Vector.cxx) --------- */
+/* DotProd / DotProdConj (:330-346, :391-397): accumulated in the vector type. */
+int shk_dot(shk_ctx ctx, shk_dtype dt, int64_t n, const void *dx, const void *dy, double *result);
+int shk_dot_device(shk_ctx ctx, shk_dtype dt, int64_t n, const void *dx, const void *dy,
+                   void *dresult);
+/* Norm2 (:540-549): sqrt of the sum of squares, in the vector type. */
+int shk_nrm2(shk_ctx ctx, shk_dtype dt, int64_t n, const void *dx, double *result);
+/* Add(alpha, X, Y) (:85-105): Y += alpha X; alpha == 0 is a no-op. */
+int shk_axpy(shk_ctx ctx, shk_dtype dt, int64_t n, double alpha, const void *dx, void *dy);
+/* Mlt(alpha, X) (:69-74 -> vector/Vector.cxx:607-615): X *= alpha. */
+int shk_scal(shk_ctx ctx, shk_dtype dt, int64_t n, double alpha, void *dx);
+/* Copy(X, Y) (:240-246). */
+int shk_copy(shk_ctx ctx, shk_dtype dt, int64_t n, const void *dx, void *dy);
+/* Vector::Fill(x) / Zero() (vector/Vector.cxx:686-713). */
+int shk_fill(shk_ctx ctx, shk_dtype dt, int64_t n, double value, void *dx);
+/* Fused Add + DotProd: Y += alpha X, then returns (Y, Z). */
+int shk_axpy_dot(shk_ctx ctx, shk_dtype dt, int64_t n, double alpha, const void *dx,
+                 void *dy, const void *dz, double *result);
+/* Cast between the two value types (FP64 <-> FP32 copies for refinement). */
+int shk_convert(shk_ctx ctx, int64_t n, shk_dtype src_dt, const void *src, shk_dtype dst_dt,
+                void *dst);
+
+/* ---- preconditioners (Preconditioner_Base, computation/solver/iterative/
+ *      Iterative.hxx:25-39 / Iterative.cxx:56-77) -------------------------- */
+/* Jacobi: z_i = r_i / a_ii, with a_ii from A (new: Seldon has no Jacobi; the
+ * division form follows SOR's / a_jj, matrix_sparse/Relaxation_MatVect.cxx:79).
+ * Identity (Preconditioner_Base::Solve) is shk_copy. */
+int shk_jacobi(shk_ctx ctx, shk_csr A, const void *dr, void *dz);
+
+/* ---- Iteration + solvers ----------------------------------------------- */
+/* Mirrors Iteration<double> (computation/solver/iterative/Iterative.hxx:47-108).
+ * In: tolerance, max_iter, nb_iter (kept when Init fails, as Seldon does),
+ * init_guess_null, parameter_restart, hist/hist_it/hist_cap (optional host
+ * arrays for the residual history).  Out: facteur_reste, nb_iter,
+ * error_code/fail_convergence, hist_len (number of Finished() calls, which
+ * may exceed hist_cap; only hist_cap are stored). */
+typedef struct {
+    double tolerance;
+    double facteur_reste;
+    int max_iter;
+    int nb_iter;
+    int error_code;
+    int fail_convergence;
+    int print_level;
+    int init_guess_null;
+    int parameter_restart;
+    int hist_cap;
+    int hist_len;
+    double *hist;
+    int *hist_it;
+} shk_iteration;
+
+/* Return value of the three solvers = Iteration::ErrorCode()
+ * (Iterative.cxx:344-354): 0, -2 at max_iter, or the breakdown code.
+ * A negative SHK_ERR_* (<= -1, != -2 ...) cannot be confused: solver status
+ * is returned through *code, the function result is the call status. */
+/* Cg (computation/solver/iterative/Cg.cxx:45-116). */
+int shk_cg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+           shk_iteration *iter, int *code);
+/* BiCgStab (computation/solver/iterative/BiCgStab.cxx:44-145). */
+int shk_bicgstab(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+                 shk_iteration *iter, int *code);
+/* Gmres(m) (computation/solver/iterative/Gmres.cxx:45-182), m =
+ * iter->parameter_restart; Hessenberg, Givens rotations and the triangular
+ * solve stay in FP64 on the device for both value types. */
+int shk_gmres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+              shk_iteration *iter, int *code);
+
+/* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
+ *      Seldon; its Gmres is the inner solver) ------------------------------ */
+typedef struct {
+    double tolerance;       /* on ||b - A x||_2 / ||b||_2, true FP64 residual */
+    int max_refinements;
+    int inner_restart;      /* GMRES(m) restart of the FP32 inner solve */
+    int inner_max_iter;     /* per refinement step */
+    double inner_tolerance; /* relative, on the preconditioned FP32 residual */
+    int inner_method;       /* 0 = GMRES, 1 = CG */
+    /* out */
+    int refinements;
+    int total_inner_iter;
+    double final_relres;
+    double *relres_hist;    /* optional, max_refinements + 1 entries */
+} shk_ir_params;
+/* A64 and A32 hold the same pattern (FP64 and FP32 values).  x, b: FP64. */
+int shk_solve_ir(shk_ctx ctx, shk_csr A64, shk_csr A32, void *dx, const void *db,
+                 shk_precond prec, shk_ir_params *p, int *code);
+
+/* ---- adjoint gradient on the pattern (no counterpart in Seldon) -------- */
+/* g_j = -(lam_i * x_col(j)) for every stored entry j of row i: dL/dA_ij for
+ * x = A^{-1} b and adjoint lam = A^{-T} dL/dx.  For RowSymSparse the value of
+ * an upper entry (i,j), i != j, also stands for (j,i):
+ * g = -(lam_i x_j + lam_j x_i). */
+int shk_sddmm(shk_ctx ctx, shk_csr A, const void *dlam, const void *dx, void *dg);
+
+/* ---- batched CG over many independent systems (no counterpart) --------- */
+/* `count` systems stacked block-diagonally in one CSR: system s owns rows
+ * [row_off[s], row_off[s+1]).  One persistent kernel runs every CG (identity
+ * preconditioner, Seldon's Cg recurrences per system) to tol relative to
+ * ||b_s||; per-system iterations and codes come back in its[], codes[].
+ * group = threads per system work-group (64..1024, power of two; 0 = the
+ * tuned default). */
+int shk_cg_batched(shk_ctx ctx, shk_csr A, int count, const int *row_off_host,
+                   void *dx, const void *db, double tol, int max_iter, int group,
+                   int use_graph, int *its, int *codes);
+/* Mixed variant: FP32 inner CG per system with FP64 residual refinement
+ * in the same persistent kernel; A64/A32 share the pattern. */
+int shk_cg_batched_ir(shk_ctx ctx, shk_csr A64, shk_csr A32, int count,
+                      const int *row_off_host, void *dx, const void *db, double tol,
+                      int max_iter, double inner_tol, int group, int use_graph,
+                      int *its, int *codes);
+
+/* ---- timing of the last launch of a kernel family (HIP events on the
+ *      stream the kernel ran on) ---------------------------------------- */
+int shk_time_spmv(shk_ctx ctx, shk_csr A, int mode, const void *dx, void *dy, int reps,
+                  double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+"""TEST INFRASTRUCTURE ONLY -- the CPU oracle for the sparse Krylov path.
+
+ctypes front-end over ``oracle/liboracle.so`` (built from ``seldon_oracle.c``
+by ``oracle/Makefile``), a plain-C restatement of Seldon's generic templates:
+SpMV on RowSparse / RowSymSparse / ArrayRowSparse, DotProd / Norm2 / Add /
+Mlt / Copy, Iteration, Cg, BiCgStab and Gmres (file:line citations are in
+``seldon_oracle_body.h``).  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may use it, and only as the checker.
+
+Parity pins (see ``tests/test_oracle.py``):
+  * bitwise equality with drivers compiled from the real Seldon headers
+    (``oracle/_ref``) on SpMV, Cg and BiCgStab, and on committed fixtures
+    (``tests/golden``) made by them;
+  * Gmres: iteration counts and residual histories against the MKL-linked
+    real Seldon build (tolerance, MKL's reductions differ in order);
+  * closed forms (5-point Laplacian eigenpairs), dense numpy solves,
+    scipy/numpy triangular solves and the textbook Givens rotation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ROWSPARSE, ROWSYMSPARSE, ARRAYROWSPARSE = 0, 1, 2
+PREC = {"id": 0, "identity": 0, "jacobi": 1}
+
+
+class OrcMat(ctypes.Structure):
+    _fields_ = [("storage", ctypes.c_int), ("m", ctypes.c_int), ("n", ctypes.c_int),
+                ("nnz", ctypes.c_int64), ("ptr", ctypes.c_void_p),
+                ("ind", ctypes.c_void_p), ("val", ctypes.c_void_p)]
+
+
+class OrcIter(ctypes.Structure):
+    _fields_ = [("tolerance", ctypes.c_double), ("facteur_reste", ctypes.c_double),
+                ("max_iter", ctypes.c_int), ("nb_iter", ctypes.c_int),
+                ("error_code", ctypes.c_int), ("fail_convergence", ctypes.c_int),
+                ("print_level", ctypes.c_int), ("init_guess_null", ctypes.c_int),
+                ("parameter_restart", ctypes.c_int),
+                ("hist", ctypes.c_void_p), ("hist_it", ctypes.c_void_p),
+                ("hist_len", ctypes.c_int), ("hist_cap", ctypes.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        for sfx, T in (("d", ctypes.c_double), ("s", ctypes.c_float)):
+            getattr(L, f"orc_{sfx}_mltadd").argtypes = [T, ctypes.POINTER(OrcMat), vp, T, vp]
+            getattr(L, f"orc_{sfx}_mlt").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
+            getattr(L, f"orc_{sfx}_dot").argtypes = [ctypes.c_int, vp, vp]
+            getattr(L, f"orc_{sfx}_dot").restype = T
+            getattr(L, f"orc_{sfx}_norm2").argtypes = [ctypes.c_int, vp]
+            getattr(L, f"orc_{sfx}_norm2").restype = T
+            for solver in ("cg", "bicgstab", "gmres"):
+                f = getattr(L, f"orc_{sfx}_{solver}")
+                f.argtypes = [ctypes.POINTER(OrcMat), vp, vp, ctypes.c_int, vp,
+                              ctypes.POINTER(OrcIter)]
+                f.restype = ctypes.c_int
+            getattr(L, f"orc_{sfx}_jacobi_diag").argtypes = [ctypes.POINTER(OrcMat), vp]
+            getattr(L, f"orc_{sfx}_sddmm").argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_iter_init.argtypes = [ctypes.POINTER(OrcIter), ctypes.c_int, ctypes.c_double]
+        L.orc_trsv_upper.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp]
+        L.orc_genrot.argtypes = [ctypes.POINTER(ctypes.c_double)] * 4
+        L.orc_applyrot.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.c_double, ctypes.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sfx(dtype):
+    return "s" if np.dtype(dtype) == np.float32 else "d"
+
+
+class _Mat:
+    """Keeps the numpy arrays alive alongside the C struct."""
+
+    def __init__(self, A, storage, dtype):
+        self.ptr = np.ascontiguousarray(A.ptr, np.int32)
+        self.ind = np.ascontiguousarray(A.ind, np.int32)
+        self.val = np.ascontiguousarray(A.val, dtype)
+        self.s = OrcMat(storage, A.m, A.n, int(self.ptr[-1]), _p(self.ptr), _p(self.ind), _p(self.val))
+
+
+def storage_of(A, storage=None):
+    if storage is not None:
+        return {"row": ROWSPARSE, "sym": ROWSYMSPARSE, "array": ARRAYROWSPARSE}.get(storage, storage)
+    return ROWSYMSPARSE if getattr(A, "sym", False) else ROWSPARSE
+
+
+def mltadd(alpha, A, x, beta, y, storage=None, dtype=np.float64):
+    M = _Mat(A, storage_of(A, storage), dtype)
+    x = np.ascontiguousarray(x, dtype)
+    y = np.array(y, dtype=dtype, copy=True)
+    getattr(lib(), f"orc_{_sfx(dtype)}_mltadd")(alpha, ctypes.byref(M.s), _p(x), beta, _p(y))
+    return y
+
+
+def mlt(A, x, storage=None, dtype=np.float64):
+    M = _Mat(A, storage_of(A, storage), dtype)
+    x = np.ascontiguousarray(x, dtype)
+    y = np.empty(A.m, dtype)
+    getattr(lib(), f"orc_{_sfx(dtype)}_mlt")(ctypes.byref(M.s), _p(x), _p(y))
+    return y
+
+
+def dot(x, y):
+    x = np.ascontiguousarray(x); y = np.ascontiguousarray(y, x.dtype)
+    return getattr(lib(), f"orc_{_sfx(x.dtype)}_dot")(x.size, _p(x), _p(y))
+
+
+def norm2(x):
+    x = np.ascontiguousarray(x)
+    return getattr(lib(), f"orc_{_sfx(x.dtype)}_norm2")(x.size, _p(x))
+
+
+def jacobi_diag(A, storage=None, dtype=np.float64):
+    M = _Mat(A, storage_of(A, storage), dtype)
+    d = np.empty(A.m, dtype)
+    getattr(lib(), f"orc_{_sfx(dtype)}_jacobi_diag")(ctypes.byref(M.s), _p(d))
+    return d
+
+
+@dataclass
+class SolveResult:
+    x: np.ndarray
+    code: int
+    nb_iter: int
+    hist: np.ndarray      # every relative residual Finished() compared
+    hist_it: np.ndarray   # iteration number at each of those calls
+
+
+def solve(method, A, b, precond="id", tol=1e-10, max_iter=10000, restart=10,
+          x0=None, storage=None, dtype=np.float64, hist_cap=None) -> SolveResult:
+    st = storage_of(A, storage)
+    M = _Mat(A, st, dtype)
+    b = np.ascontiguousarray(b, dtype)
+    x = np.zeros(A.m, dtype) if x0 is None else np.array(x0, dtype=dtype, copy=True)
+    diag = jacobi_diag(A, st, dtype)
+    cap = hist_cap or (2 * max_iter + 4 * (max_iter // max(restart, 1)) + 16)
+    hist = np.zeros(cap, np.float64)
+    hist_it = np.zeros(cap, np.int32)
+    it = OrcIter()
+    lib().orc_iter_init(ctypes.byref(it), max_iter, tol)
+    it.parameter_restart = restart
+    it.init_guess_null = 1 if x0 is None else 0
+    it.hist, it.hist_it, it.hist_cap = _p(hist), _p(hist_it), cap
+    code = getattr(lib(), f"orc_{_sfx(dtype)}_{method}")(
+        ctypes.byref(M.s), _p(x), _p(b), PREC[precond], _p(diag), ctypes.byref(it))
+    n = min(it.hist_len, cap)
+    return SolveResult(x, code, it.nb_iter, hist[:n].copy(), hist_it[:n].copy())
+
+
+def sddmm(A, lam, x):
+    dtype = np.asarray(lam).dtype
+    lam = np.ascontiguousarray(lam); x = np.ascontiguousarray(x, dtype)
+    g = np.empty(A.nnz, dtype)
+    ptr = np.ascontiguousarray(A.ptr, np.int32); ind = np.ascontiguousarray(A.ind, np.int32)
+    getattr(lib(), f"orc_{_sfx(dtype)}_sddmm")(A.m, _p(ptr), _p(ind), _p(lam), _p(x), _p(g))
+    return g
+
+
+def trsv_upper(H, s):
+    H = np.asfortranarray(H, np.float64)
+    s = np.array(s, np.float64, copy=True)
+    n = s.size
+    lib().orc_trsv_upper(n, H.ctypes.data_as(ctypes.c_void_p), H.shape[0], _p(s))
+    return s
+
+
+def genrot(a, b):
+    a_, b_, c_, s_ = (ctypes.c_double(a), ctypes.c_double(b), ctypes.c_double(), ctypes.c_double())
+    lib().orc_genrot(ctypes.byref(a_), ctypes.byref(b_), ctypes.byref(c_), ctypes.byref(s_))
+    return a_.value, b_.value, c_.value, s_.value
+
+
+# ------------------------------------------------------------ reference run --
+
+REF_DIR = os.path.join(HERE, "_ref")
+
+
+def ref_available(kind="generic"):
+    return os.path.exists(os.path.join(REF_DIR, f"seldon_ref_{kind}"))
+
+
+def write_seldon_csr(path, A, dtype=np.float64):
+    """Seldon binary CSR: int m, n, nz; ptr[m+1]; ind[nz]; data[nz]
+    (matrix_sparse/Matrix_Sparse.cxx:1532-1556)."""
+    with open(path, "wb") as f:
+        np.array([A.m, A.n, A.nnz], np.int32).tofile(f)
+        np.ascontiguousarray(A.ptr, np.int32).tofile(f)
+        np.ascontiguousarray(A.ind, np.int32).tofile(f)
+        np.ascontiguousarray(A.val, dtype).tofile(f)
+
+
+def write_seldon_vec(path, x):
+    """Seldon binary vector: int m; T[m] (vector/Vector.cxx:846-874)."""
+    with open(path, "wb") as f:
+        np.array([x.size], np.int32).tofile(f)
+        np.ascontiguousarray(x).tofile(f)
+
+
+def read_seldon_vec(path, dtype=np.float64):
+    with open(path, "rb") as f:
+        m = int(np.fromfile(f, np.int32, 1)[0])
+        return np.fromfile(f, dtype, m)
+
+
+def run_ref(args, kind="generic", cwd=None):
+    exe = os.path.join(REF_DIR, f"seldon_ref_{kind}")
+    env = dict(os.environ, MKL_NUM_THREADS="1", MKL_CBWR="COMPATIBLE")
+    out = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True,
+                         cwd=cwd, env=env, check=True)
+    return out.stdout
+
+
+def parse_ref_history(stdout):
+    hist, its, code, nb = [], [], None, None
+    for line in stdout.splitlines():
+        if line.startswith("Residu at iteration number"):
+            parts = line.split()
+            its.append(int(parts[4])); hist.append(float(parts[5]))
+        elif line.startswith("ERRCODE"):
+            parts = line.split()
+            code, nb = int(parts[1]), int(parts[3])
+    return np.array(hist), np.array(its, np.int32), code, nb

@@ -1,0 +1,98 @@
+/*
+ * seldon_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C CPU restatement of Seldon's sparse Krylov path (SpMV on the
+ * RowSparse / RowSymSparse / ArrayRowSparse storages, the BLAS-1 steps, the
+ * Iteration stopping logic, Cg, BiCgStab and Gmres), used as the checker for
+ * the HIP library.  Only the tests, the smoke check and the benchmark's CPU
+ * baseline leg may load it; the product never does.
+ *
+ * Every function follows the reference file:line named in its comment, in
+ * the same arithmetic order, and is built with -ffp-contract=off so that
+ * each a*b+c rounds twice, as the reference's C++ does under ISO rules.
+ *
+ * Two families are generated from one body: orc_d_* (double vectors) and
+ * orc_s_* (float vectors).  Iteration bookkeeping (tolerance, 1/|r0|,
+ * residual history) is always double, matching Iteration<double>.
+ */
+#ifndef SELDON_ORACLE_H
+#define SELDON_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_ROWSPARSE = 0, ORC_ROWSYMSPARSE = 1, ORC_ARRAYROWSPARSE = 2 };
+enum { ORC_PREC_IDENTITY = 0, ORC_PREC_JACOBI = 1 };
+
+/* A sparse matrix in one of the three storages, as flat CSR arrays:
+ * ROWSPARSE: full CSR; ROWSYMSPARSE: upper triangle incl. diagonal;
+ * ARRAYROWSPARSE: the per-row sparse vectors laid end to end (the
+ * ConvertToCSR flattening, matrix_sparse/Matrix_Conversions.cxx:3676-3705). */
+typedef struct {
+    int storage;
+    int m, n;
+    int64_t nnz;
+    const int *ptr;
+    const int *ind;
+    const void *val; /* double* or float* by family */
+} orc_mat;
+
+/* Mirrors Iteration<double> (computation/solver/iterative/Iterative.hxx:47-108). */
+typedef struct {
+    double tolerance;
+    double facteur_reste;
+    int max_iter;
+    int nb_iter;
+    int error_code;
+    int fail_convergence;
+    int print_level; /* kept for completeness; the oracle never prints */
+    int init_guess_null;
+    int parameter_restart;
+    /* residual history: every value Finished() compared with the
+     * tolerance, in call order, with the iteration number at the call */
+    double *hist;
+    int *hist_it;
+    int hist_len;
+    int hist_cap;
+} orc_iter;
+
+void orc_iter_init(orc_iter *it, int max_iter, double tol);
+
+/* ---- family-generated entry points (SFX = d | s, T = double | float) ---- */
+#define ORC_DECLARE(SFX, T)                                                    \
+    void orc_##SFX##_mltadd(T alpha, const orc_mat *A, const T *x, T beta,     \
+                            T *y);                                             \
+    void orc_##SFX##_mlt(const orc_mat *A, const T *x, T *y);                  \
+    T orc_##SFX##_dot(int n, const T *x, const T *y);                          \
+    T orc_##SFX##_norm2(int n, const T *x);                                    \
+    void orc_##SFX##_add(int n, T alpha, const T *x, T *y);                    \
+    void orc_##SFX##_scal(int n, T alpha, T *x);                               \
+    void orc_##SFX##_copy(int n, const T *x, T *y);                            \
+    void orc_##SFX##_jacobi_diag(const orc_mat *A, T *diag);                   \
+    int orc_##SFX##_cg(const orc_mat *A, T *x, const T *b, int precond,        \
+                       const T *diag, orc_iter *it);                           \
+    int orc_##SFX##_bicgstab(const orc_mat *A, T *x, const T *b, int precond,  \
+                             const T *diag, orc_iter *it);                     \
+    int orc_##SFX##_gmres(const orc_mat *A, T *x, const T *b, int precond,     \
+                          const T *diag, orc_iter *it);                        \
+    void orc_##SFX##_sddmm(int m, const int *ptr, const int *ind,              \
+                           const T *lam, const T *x, T *g);
+
+ORC_DECLARE(d, double)
+ORC_DECLARE(s, float)
+
+/* Back substitution on the leading n x n upper triangle of a column-major
+ * array with leading dimension ld (the role of cblas_dtrsv in
+ * computation/interfaces/Blas_2.cxx:3226-3238). */
+void orc_trsv_upper(int n, const double *H, int ld, double *s);
+/* Old-BLAS Givens generation (computation/basic_functions/
+ * Functions_Vector.cxx:612-649) and application (:681-687). */
+void orc_genrot(double *a, double *b, double *c, double *s);
+void orc_applyrot(double *x, double *y, double c, double s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

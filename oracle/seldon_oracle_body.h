@@ -1,0 +1,333 @@
+/*
+ * seldon_oracle_body.h -- TEST INFRASTRUCTURE ONLY (see seldon_oracle.h).
+ *
+ * Type-generic body, included by seldon_oracle.c once per family with
+ *   T   = double | float      (vector / matrix value type)
+ *   SFX = d      | s
+ * Scalars of the Krylov recurrences are T, as Seldon's `Complexe`
+ * (= Vector1::value_type) is; the Iteration bookkeeping is double.
+ * The GMRES Hessenberg matrix, its rotations and right-hand side are double
+ * in both families (for T = double this is exactly Seldon's arithmetic; for
+ * T = float it is the FP32-inner definition used by the mixed-precision
+ * solver, whose vector arithmetic stays in float).
+ */
+#define CAT_(a, b) a##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(CAT(orc_, SFX), CAT(_, name))
+
+/* ---- BLAS-1, generic templates --------------------------------------- */
+
+/* DotProd: computation/basic_functions/Functions_Vector.cxx:330-346
+ * (DotProdConj forwards to it for real types, :391-397). */
+T FN(dot)(int n, const T *x, const T *y)
+{
+    T value = 0;
+    for (int i = 0; i < n; i++) value += x[i] * y[i];
+    return value;
+}
+
+/* Norm2: Functions_Vector.cxx:540-549 (unscaled sqrt of the sum). */
+T FN(norm2)(int n, const T *x)
+{
+    T value = 0;
+    for (int i = 0; i < n; i++) value += x[i] * x[i];
+    return (T)sqrt(value);
+}
+
+/* Add(alpha, X, Y): Functions_Vector.cxx:85-105; alpha == 0 is a no-op. */
+void FN(add)(int n, T alpha, const T *x, T *y)
+{
+    if (alpha != (T)0) {
+        T alpha_ = alpha;
+        for (int i = 0; i < n; i++) y[i] += alpha_ * x[i];
+    }
+}
+
+/* Mlt(alpha, X) -> X *= alpha: Functions_Vector.cxx:69-74,
+ * vector/Vector.cxx:607-615. */
+void FN(scal)(int n, T alpha, T *x)
+{
+    for (int i = 0; i < n; i++) x[i] *= alpha;
+}
+
+/* Copy(X, Y): Functions_Vector.cxx:240-246 -> Vector.cxx:579-586. */
+void FN(copy)(int n, const T *x, T *y)
+{
+    memcpy(y, x, sizeof(T) * (size_t)n);
+}
+
+static void FN(fill)(int n, T v, T *x)
+{
+    for (int i = 0; i < n; i++) x[i] = v; /* Vector.cxx:707-713 */
+}
+
+static T *FN(dup)(int n, const T *x)
+{
+    T *y = (T *)malloc(sizeof(T) * (size_t)(n > 0 ? n : 1));
+    if (n > 0) memcpy(y, x, sizeof(T) * (size_t)n);
+    return y;
+}
+
+/* ---- SpMV ---------------------------------------------------------- */
+
+void FN(mltadd)(T alpha, const orc_mat *A, const T *x, T beta, T *y)
+{
+    const int m = A->m;
+    const int *ptr = A->ptr, *ind = A->ind;
+    const T *data = (const T *)A->val;
+    if (A->storage == ORC_ROWSPARSE) {
+        /* computation/basic_functions/Functions_MatVect.cxx:398-431 */
+        FN(scal)(m, beta, y);
+        for (int i = 0; i < m; i++) {
+            T temp = 0;
+            for (int j = ptr[i]; j < ptr[i + 1]; j++) temp += data[j] * x[ind[j]];
+            y[i] += alpha * temp;
+        }
+    } else if (A->storage == ORC_ROWSYMSPARSE) {
+        /* Functions_MatVect.cxx:526-564: gather over the stored upper part,
+         * then the transposed scatter of the strict upper part. */
+        FN(scal)(m, beta, y);
+        for (int i = 0; i < m; i++) {
+            T temp = 0;
+            for (int j = ptr[i]; j < ptr[i + 1]; j++) temp += data[j] * x[ind[j]];
+            y[i] += alpha * temp;
+        }
+        for (int i = 0; i < m - 1; i++)
+            for (int j = ptr[i]; j < ptr[i + 1]; j++)
+                if (ind[j] != i) y[ind[j]] += alpha * data[j] * x[i];
+    } else {
+        /* matrix_sparse/Functions_MatrixArray.cxx:534-575 */
+        if (beta == (T)0) FN(fill)(m, (T)0, y);
+        else FN(scal)(m, beta, y);
+        if (alpha == (T)1) {
+            for (int i = 0; i < m; i++)
+                for (int j = ptr[i]; j < ptr[i + 1]; j++) y[i] += data[j] * x[ind[j]];
+        } else {
+            for (int i = 0; i < m; i++)
+                for (int j = ptr[i]; j < ptr[i + 1]; j++)
+                    y[i] += alpha * data[j] * x[ind[j]];
+        }
+    }
+}
+
+/* Mlt(M, X, Y): Functions_MatVect.cxx:72-81 = Fill(0) then MltAdd(1,..,0,..). */
+void FN(mlt)(const orc_mat *A, const T *x, T *y)
+{
+    FN(fill)(A->m, (T)0, y);
+    FN(mltadd)((T)1, A, x, (T)0, y);
+}
+
+/* Diagonal for the Jacobi preconditioner (an entry with ind == row; 0 when
+ * the row stores none). */
+void FN(jacobi_diag)(const orc_mat *A, T *diag)
+{
+    const T *data = (const T *)A->val;
+    for (int i = 0; i < A->m; i++) {
+        diag[i] = 0;
+        for (int j = A->ptr[i]; j < A->ptr[i + 1]; j++)
+            if (A->ind[j] == i) diag[i] = data[j];
+    }
+}
+
+/* Preconditioner application M.Solve(A, r, z):
+ * identity = Copy(r, z) (computation/solver/iterative/Iterative.cxx:60-65);
+ * Jacobi  = z_i = r_i / a_ii (the division form of SOR's /a_jj,
+ * matrix_sparse/Relaxation_MatVect.cxx:79). */
+static void FN(precond)(int precond, int n, const T *diag, const T *r, T *z)
+{
+    if (precond == ORC_PREC_JACOBI) {
+        for (int i = 0; i < n; i++) z[i] = r[i] / diag[i];
+    } else {
+        FN(copy)(n, r, z);
+    }
+}
+
+/* Iteration::Init (Iterative.cxx:235-249): 1/|r| or -1 for a null rhs. */
+static int FN(iter_init)(orc_iter *it, int n, const T *r)
+{
+    double norme_rhs = (double)FN(norm2)(n, r);
+    if (norme_rhs == 0.0) return -1;
+    it->facteur_reste = 1.0 / norme_rhs;
+    it->nb_iter = 0;
+    return 0;
+}
+
+/* ---- Cg: computation/solver/iterative/Cg.cxx:45-116 ---------------- */
+int FN(cg)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+           orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho = 1, rho_1 = 1, alpha, beta, delta;
+    T *p = FN(dup)(N, b), *q = FN(dup)(N, b), *r = FN(dup)(N, b), *z = FN(dup)(N, b);
+    (void)rho;
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        FN(precond)(precond, N, diag, r, z);
+        rho = FN(dot)(N, r, z);
+        if (rho == (T)0) { orc_fail(it, 1); break; }
+        if (it->nb_iter == 0) {
+            FN(copy)(N, z, p);
+        } else {
+            beta = rho / rho_1;
+            FN(scal)(N, beta, p);
+            FN(add)(N, (T)1, z, p);
+        }
+        FN(mlt)(A, p, q);
+        delta = FN(dot)(N, p, q);
+        if (delta == (T)0) { orc_fail(it, 2); break; }
+        alpha = rho / delta;
+        FN(add)(N, alpha, p, x);
+        FN(add)(N, -alpha, q, r);
+        rho_1 = rho;
+        it->nb_iter++;
+    }
+done:
+    free(p); free(q); free(r); free(z);
+    return orc_error_code(it);
+}
+
+/* ---- BiCgStab: computation/solver/iterative/BiCgStab.cxx:44-145 ---- */
+int FN(bicgstab)(const orc_mat *A, T *x, const T *b, int precond,
+                 const T *diag, orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho_1, rho_2 = 0, alpha = 0, beta, omega = 0, sigma;
+    T *p = FN(dup)(N, b), *phat = FN(dup)(N, b), *s = FN(dup)(N, b),
+      *shat = FN(dup)(N, b), *t = FN(dup)(N, b), *v = FN(dup)(N, b),
+      *r = FN(dup)(N, b), *rtilde = FN(dup)(N, b);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(copy)(N, r, rtilde);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        rho_1 = FN(dot)(N, rtilde, r);
+        if (rho_1 == (T)0) { orc_fail(it, 1); break; }
+        if (it->nb_iter == 0) {
+            FN(copy)(N, r, p);
+        } else {
+            if (omega == (T)0) { orc_fail(it, 2); break; }
+            beta = (rho_1 / rho_2) * (alpha / omega);
+            FN(add)(N, -omega, v, p);
+            FN(scal)(N, beta, p);
+            FN(add)(N, (T)1, r, p);
+        }
+        FN(precond)(precond, N, diag, p, phat);
+        FN(mlt)(A, phat, v);
+        sigma = FN(dot)(N, rtilde, v);
+        if (sigma == (T)0) { orc_fail(it, 3); break; }
+        alpha = rho_1 / sigma;
+        FN(copy)(N, r, s);
+        FN(add)(N, -alpha, v, s);
+        it->nb_iter++;
+        if (orc_finished(it, (double)FN(norm2)(N, s), it->nb_iter, it->max_iter)) {
+            FN(add)(N, alpha, phat, x);
+            break;
+        }
+        FN(precond)(precond, N, diag, s, shat);
+        FN(mlt)(A, shat, t);
+        omega = FN(dot)(N, t, s) / FN(dot)(N, t, t);
+        FN(add)(N, alpha, phat, x);
+        FN(add)(N, omega, shat, x);
+        FN(copy)(N, s, r);
+        FN(add)(N, -omega, t, r);
+        rho_2 = rho_1;
+        it->nb_iter++;
+    }
+done:
+    free(p); free(phat); free(s); free(shat); free(t); free(v); free(r); free(rtilde);
+    return orc_error_code(it);
+}
+
+/* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
+int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+              orc_iter *outer)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    const int m = outer->parameter_restart;
+    const int ld = m + 1;
+    T **V = (T **)malloc(sizeof(T *) * (size_t)(m + 1));
+    for (int i = 0; i < m + 1; i++) { V[i] = FN(dup)(N, b); FN(fill)(N, (T)0, V[i]); }
+    /* H: (m+1) x (m+1), column-major, H(k,i) = H[i*ld + k] (Gmres.cxx:63) */
+    double *H = (double *)calloc((size_t)ld * ld, sizeof(double));
+    double *s = (double *)calloc((size_t)(m + 1), sizeof(double));
+    double *rc = (double *)calloc((size_t)(m + 1), sizeof(double));
+    double *rs = (double *)calloc((size_t)(m + 1), sizeof(double));
+    T *w = FN(dup)(N, b), *r = FN(dup)(N, b), *u = FN(dup)(N, b);
+    FN(fill)(N, (T)0, w); FN(fill)(N, (T)0, r); FN(fill)(N, (T)0, u);
+
+    /* residual w = b - A x, then r = M^{-1} w  (Gmres.cxx:82-90) */
+    FN(copy)(N, b, w);
+    if (!outer->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, w);
+    else FN(fill)(N, (T)0, x);
+    FN(precond)(precond, N, diag, w, r);
+    double beta = (double)FN(norm2)(N, r);
+    if (FN(iter_init)(outer, N, r) != 0) goto done;
+
+    outer->nb_iter = 0;
+    while (!orc_finished(outer, beta, outer->nb_iter, outer->max_iter)) {
+        /* V0 = r / beta, s = beta e1   (Gmres.cxx:105-108) */
+        FN(copy)(N, r, V[0]);
+        FN(scal)(N, (T)(1.0 / beta), V[0]);
+        for (int k = 0; k < m + 1; k++) s[k] = 0.0;
+        s[0] = beta;
+        int i = 0, k;
+        /* inner iteration copies outer (Iterative.cxx:113-124) with
+         * nb = outer nb and max = outer nb + m (Gmres.cxx:114-116) */
+        int inner_nb = outer->nb_iter, inner_max = outer->nb_iter + m;
+        do {
+            FN(mlt)(A, V[i], u);
+            FN(precond)(precond, N, diag, u, w);
+            /* modified Gram-Schmidt (Gmres.cxx:127-132) */
+            for (k = 0; k <= i; k++) {
+                H[i * ld + k] = (double)FN(dot)(N, V[k], w);
+                FN(add)(N, (T)(-H[i * ld + k]), V[k], w);
+            }
+            double hi_ip1 = (double)FN(norm2)(N, w);
+            FN(copy)(N, w, V[i + 1]);
+            if (hi_ip1 != 0.0) FN(scal)(N, (T)(1.0 / hi_ip1), V[i + 1]);
+            for (k = 0; k < i; k++)
+                orc_applyrot(&H[i * ld + k], &H[i * ld + k + 1], rc[k], rs[k]);
+            if (hi_ip1 != 0.0) {
+                orc_genrot(&H[i * ld + i], &hi_ip1, &rc[i], &rs[i]);
+                orc_applyrot(&s[i], &s[i + 1], rc[i], rs[i]);
+            }
+            inner_nb++; outer->nb_iter++; i++;
+        } while (!orc_finished(outer, fabs(s[i]), inner_nb, inner_max));
+
+        /* H.Resize(i,i); s.Resize(i); Solve(H, s)  (Gmres.cxx:165-166) */
+        orc_trsv_upper(i, H, ld, s);
+        /* x += sum_k s(k) V(k)  (Gmres.cxx:169-170) */
+        for (k = 0; k < i; k++)
+            if (s[k] != 0.0) FN(add)(N, (T)s[k], V[k], x);
+        /* true preconditioned residual (Gmres.cxx:173-178) */
+        FN(copy)(N, b, w);
+        FN(mltadd)((T)-1, A, x, (T)1, w);
+        FN(precond)(precond, N, diag, w, r);
+        beta = (double)FN(norm2)(N, r);
+    }
+done:
+    for (int i = 0; i < m + 1; i++) free(V[i]);
+    free(V); free(H); free(s); free(rc); free(rs); free(w); free(r); free(u);
+    return orc_error_code(outer);
+}
+
+/* ---- SDDMM on the pattern: g_j = -(lam_i * x_col(j)) -------------- */
+void FN(sddmm)(int m, const int *ptr, const int *ind, const T *lam, const T *x,
+               T *g)
+{
+    for (int i = 0; i < m; i++)
+        for (int j = ptr[i]; j < ptr[i + 1]; j++) g[j] = -(lam[i] * x[ind[j]]);
+}
+
+#undef FN
+#undef CAT
+#undef CAT_

@@ -1,0 +1,232 @@
+// seldon_ref_harness.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A driver around the *real* Seldon headers (read in place from the
+// reference tree by oracle/Makefile, never copied).  It loads inputs in
+// Seldon's own binary layouts (CSR: matrix_sparse/Matrix_Sparse.cxx:
+// 1532-1556 / 1649-1682; vectors: vector/Vector.cxx:846-874 / 1060-1091),
+// calls Seldon's MltAdd / Mlt / Cg / BiCgStab / Gmres, and prints what the
+// golden fixtures and the oracle cross-check need:
+//   - every "Residu at iteration number k  r" line that Iteration::Finished
+//     prints at print level 6, at 17 significant digits;
+//   - "ERRCODE e NBITER n" at the end of a solve;
+// and writes output vectors in Seldon's binary layout.
+//
+// Usage:
+//   seldon_ref spmv  <row|sym|array> <d|f> A x y alpha beta out
+//   seldon_ref mlt   <row|sym|array> <d|f> A x out
+//   seldon_ref solve <cg|bicgstab|gmres> <row|sym|array> <id|jacobi>
+//                    A b x0|- tol maxit restart out
+// The Jacobi preconditioner does not exist in the reference; it is defined
+// here as the Preconditioner_Base-shaped class z_i = r_i / a_ii (the
+// division form of SOR's /a_jj, matrix_sparse/Relaxation_MatVect.cxx:79).
+#include "Seldon.hxx"
+#include "SeldonSolver.hxx"
+
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+
+using namespace Seldon;
+
+template <class T>
+class JacobiPreconditioner : public Preconditioner_Base
+{
+public:
+  Vector<T> diag;
+  template <class Matrix1, class Vector1>
+  void Solve(const Matrix1&, const Vector1& r, Vector1& z)
+  {
+    for (int i = 0; i < r.GetM(); i++)
+      z(i) = r(i) / diag(i);
+  }
+  template <class Matrix1, class Vector1>
+  void TransSolve(const Matrix1& A, const Vector1& r, Vector1& z)
+  {
+    Solve(A, r, z);
+  }
+};
+
+template <class T>
+static void csr_diag(const Matrix<T, General, RowSparse>& A, Vector<T>& d)
+{
+  d.Reallocate(A.GetM());
+  d.Fill(T(0));
+  for (int i = 0; i < A.GetM(); i++)
+    for (int j = A.GetPtr()[i]; j < A.GetPtr()[i + 1]; j++)
+      if (A.GetInd()[j] == i)
+        d(i) = A.GetData()[j];
+}
+
+template <class T>
+static void sym_diag(const Matrix<T, Symmetric, RowSymSparse>& A, Vector<T>& d)
+{
+  d.Reallocate(A.GetM());
+  d.Fill(T(0));
+  for (int i = 0; i < A.GetM(); i++)
+    for (int j = A.GetPtr()[i]; j < A.GetPtr()[i + 1]; j++)
+      if (A.GetInd()[j] == i)
+        d(i) = A.GetData()[j];
+}
+
+template <class T>
+static void to_array(const Matrix<T, General, RowSparse>& A,
+                     Matrix<T, General, ArrayRowSparse>& B)
+{
+  B.Reallocate(A.GetM(), A.GetN());
+  for (int i = 0; i < A.GetM(); i++)
+    {
+      int b = A.GetPtr()[i], e = A.GetPtr()[i + 1];
+      B.ReallocateRow(i, e - b);
+      for (int k = 0; k < e - b; k++)
+        {
+          B.Index(i, k) = A.GetInd()[b + k];
+          B.Value(i, k) = A.GetData()[b + k];
+        }
+    }
+}
+
+template <class T>
+static int run_spmv(bool add, const string& st, char** a)
+{
+  Vector<T> x, y;
+  x.Read(a[1]);
+  T alpha = 1, beta = 0;
+  string out;
+  if (add)
+    {
+      y.Read(a[2]);
+      alpha = T(atof(a[3]));
+      beta = T(atof(a[4]));
+      out = a[5];
+    }
+  else
+    {
+      y.Reallocate(0);
+      out = a[2];
+    }
+  if (st == "sym")
+    {
+      Matrix<T, Symmetric, RowSymSparse> A;
+      A.Read(a[0]);
+      if (!add) { y.Reallocate(A.GetM()); y.Fill(T(0)); Mlt(A, x, y); }
+      else MltAdd(alpha, A, x, beta, y);
+    }
+  else
+    {
+      Matrix<T, General, RowSparse> A;
+      A.Read(a[0]);
+      if (st == "array")
+        {
+          Matrix<T, General, ArrayRowSparse> B;
+          to_array(A, B);
+          if (!add) { y.Reallocate(B.GetM()); y.Fill(T(0)); Mlt(B, x, y); }
+          else MltAdd(alpha, B, x, beta, y);
+        }
+      else
+        {
+          if (!add) { y.Reallocate(A.GetM()); y.Fill(T(0)); Mlt(A, x, y); }
+          else MltAdd(alpha, A, x, beta, y);
+        }
+    }
+  y.Write(out);
+  return 0;
+}
+
+template <class Mat, class Prec>
+static int solve_with(const string& method, Mat& A, Vector<double>& x,
+                      const Vector<double>& b, Prec& M, Iteration<double>& it)
+{
+  if (method == "cg")
+    return Cg(A, x, b, M, it);
+  if (method == "bicgstab")
+    return BiCgStab(A, x, b, M, it);
+#ifdef SELDON_WITH_BLAS
+  if (method == "gmres")
+    return Gmres(A, x, b, M, it);
+#endif
+  cerr << "method not available in this build: " << method << endl;
+  exit(3);
+}
+
+template <class Mat>
+static int solve_prec(const string& method, const string& prec, Mat& A,
+                      const Vector<double>& d, Vector<double>& x,
+                      const Vector<double>& b, Iteration<double>& it)
+{
+  if (prec == "jacobi")
+    {
+      JacobiPreconditioner<double> M;
+      M.diag = d;
+      return solve_with(method, A, x, b, M, it);
+    }
+  Preconditioner_Base M;
+  return solve_with(method, A, x, b, M, it);
+}
+
+static int run_solve(char** a)
+{
+  string method = a[0], st = a[1], prec = a[2];
+  Vector<double> b, x, d;
+  b.Read(a[4]);
+  Iteration<double> it(atoi(a[7]), atof(a[6]));
+  it.SetRestart(atoi(a[8]));
+  it.ShowFullHistory();
+  if (string(a[5]) != "-")
+    {
+      x.Read(a[5]);
+      it.SetInitGuess(false);
+    }
+  else
+    {
+      x.Reallocate(b.GetM());
+      x.Fill(0.0);
+    }
+  cout.precision(17);
+  int code;
+  if (st == "sym")
+    {
+      Matrix<double, Symmetric, RowSymSparse> A;
+      A.Read(a[3]);
+      sym_diag(A, d);
+      code = solve_prec(method, prec, A, d, x, b, it);
+    }
+  else
+    {
+      Matrix<double, General, RowSparse> A;
+      A.Read(a[3]);
+      csr_diag(A, d);
+      if (st == "array")
+        {
+          Matrix<double, General, ArrayRowSparse> B;
+          to_array(A, B);
+          code = solve_prec(method, prec, B, d, x, b, it);
+        }
+      else
+        code = solve_prec(method, prec, A, d, x, b, it);
+    }
+  cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+  x.Write(a[9]);
+  return 0;
+}
+
+int main(int argc, char** argv)
+{
+  try
+    {
+      if (argc >= 10 && string(argv[1]) == "spmv")
+        return string(argv[3]) == "f" ? run_spmv<float>(true, argv[2], argv + 4)
+                                      : run_spmv<double>(true, argv[2], argv + 4);
+      if (argc >= 7 && string(argv[1]) == "mlt")
+        return string(argv[3]) == "f" ? run_spmv<float>(false, argv[2], argv + 4)
+                                      : run_spmv<double>(false, argv[2], argv + 4);
+      if (argc >= 12 && string(argv[1]) == "solve")
+        return run_solve(argv + 2);
+    }
+  catch (Error& e)
+    {
+      cerr << e.What() << endl;
+      return 2;
+    }
+  cerr << "usage: see the header of seldon_ref_harness.cpp" << endl;
+  return 1;
+}

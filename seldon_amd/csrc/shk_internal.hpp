@@ -1,0 +1,200 @@
+// shk_internal.hpp -- shared internals of the shk HIP library (gfx950).
+//
+// Contents: error plumbing, the context / matrix handle layouts, the
+// deterministic reduction helpers used by every fused kernel, and the
+// XCD-aware block mapping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "shk.h"
+
+namespace shk {
+
+// ------------------------------------------------------------- errors ----
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define SHK_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return ::shk::fail(SHK_ERR_HIP, std::string(#call) + ": " +        \
+                                                hipGetErrorString(e_));        \
+    } while (0)
+
+#define SHK_CHECK(cond, code, msg)                                             \
+    do {                                                                       \
+        if (!(cond)) return ::shk::fail((code), (msg));                        \
+    } while (0)
+
+#define SHK_TRY(expr)                                                          \
+    do {                                                                       \
+        int s_ = (expr);                                                       \
+        if (s_ != SHK_OK) return s_;                                           \
+    } while (0)
+
+// ------------------------------------------------------------- layout ----
+constexpr int kBlock = 256;         // threads per work-group for most kernels
+constexpr int kWave = 64;           // CDNA wavefront
+constexpr int kNumXcd = 8;          // MI355X: 8 XCDs, work-groups dealt round-robin
+constexpr int kMaxRedBlocks = 2048; // BLAS-1 reduction grid cap (fixed per n)
+constexpr uint32_t kLowerFlag = 0x80000000u; // ind high bit: "direct" term
+
+inline size_t dsize(int dt) { return dt == SHK_F32 ? 4 : 8; }
+
+// Device-side scalar workspace for one in-flight reduction family.
+struct RedSlot {
+    void *partials = nullptr; // [nv][nblocks] of T
+    unsigned *ticket = nullptr;
+    size_t cap_bytes = 0;
+};
+
+}  // namespace shk
+
+struct shk_ctx_s {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    char arch[64] = {0};
+    // reduction workspace: one partial buffer + ticket per concurrent user
+    shk::RedSlot red;
+    // small device scratch for scalar results and solver state
+    void *scalars = nullptr;      // 4 KiB
+    void *state = nullptr;        // solver state (device)
+    void *state_host = nullptr;   // pinned mirror
+    size_t state_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // grow-only work-vector arena for the solvers (Seldon allocates its
+    // work vectors per call, Cg.cxx:55; here they are reused across calls)
+    void *ws = nullptr;
+    size_t ws_cap = 0;
+};
+
+struct shk_csr_s {
+    shk_ctx ctx = nullptr;
+    int storage = SHK_ROWSPARSE;
+    int dtype = SHK_F64;
+    int m = 0, n = 0;
+    int64_t nnz_stored = 0;   // as given by the caller
+    int64_t nnz_dev = 0;      // after the symmetric expansion
+    int *ptr = nullptr;       // [m+1]
+    int *mid = nullptr;       // [m] RowSymSparse: start of the transposed part
+    uint32_t *ind = nullptr;  // [nnz_dev + pad], kLowerFlag on direct terms
+    void *val = nullptr;      // [nnz_dev + pad]
+    void *diag = nullptr;     // [m] a_ii for Jacobi (0 when absent)
+    int *blk_rows = nullptr;  // [nblk+1] row-block boundaries
+    int nblk = 0;             // logical row blocks
+    int nblk_launch = 0;      // launched blocks (padded to a multiple of 8)
+    int xcd_q = 0;            // logical blocks per XCD
+    int budget = 0;           // products staged per block (LDS)
+    int max_row = 0;
+    // RowSymSparse values: device position of each stored entry and of its
+    // transposed copy (-1 for the diagonal) so new values can be scattered.
+    int64_t *map_up = nullptr;
+    int64_t *map_lo = nullptr;
+    int64_t *map_diag = nullptr; // [m] position of a_ii on device (-1: none)
+    // ArrayRowSparse / RowSparse values are stored in caller order (map = id)
+};
+
+namespace shk {
+
+// ----------------------------------------------------- device helpers ----
+
+// Logical block index for an XCD-aware launch of `nblk_launch` = 8*q blocks:
+// hardware block b runs on XCD (b % 8); logical blocks [k*q, (k+1)*q) all
+// land on XCD k, so neighbouring row blocks share one L2.
+__device__ __forceinline__ int xcd_logical_block(int q)
+{
+    int b = blockIdx.x;
+    return (b % kNumXcd) * q + b / kNumXcd;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_lane0(T v)
+{
+    // xor butterfly; every lane ends with a full sum, lane 0's is the one used
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block reduction of NV values, fixed order: lanes by xor butterfly (lane 0),
+// waves summed sequentially by thread 0.  Result valid in thread 0.
+template <typename T, int NV>
+__device__ __forceinline__ void block_sum(T (&v)[NV], T *lds /*[NV*16]*/)
+{
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        T s = wave_sum_lane0(v[k]);
+        if (lane == 0) lds[k * 16 + wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            T s = lds[k * 16];
+            for (int w = 1; w < nw; w++) s += lds[k * 16 + w];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Grid reduction epilogue: every block of the launch calls this once with its
+// per-thread contributions.  Block sums go to partials[k*nblocks + b]; the
+// last block to arrive (ticket) sums them in a fixed order and returns true in
+// all its threads with the totals in out[] (valid in thread 0).  The ticket is
+// reset by that block, so the workspace needs no clearing between launches.
+template <typename T, int NV>
+__device__ __forceinline__ bool grid_sum(T (&v)[NV], T *partials, unsigned *ticket,
+                                         int nblocks, int bidx, T (&out)[NV])
+{
+    __shared__ T lds[NV * 16];
+    __shared__ int is_last;
+    block_sum<T, NV>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[(size_t)k * nblocks + bidx] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        is_last = (t == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        T s = 0;
+        const volatile T *pk = partials + (size_t)k * nblocks;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x) s += pk[i];
+        out[k] = s;
+    }
+    block_sum<T, NV>(out, lds);
+    if (threadIdx.x == 0) *ticket = 0u;
+    return true;
+}
+
+// number of blocks used by the grid-stride BLAS-1 reductions (fixed per n)
+inline int red_blocks(int64_t n, int items_per_thread = 8)
+{
+    int64_t b = (n + (int64_t)kBlock * items_per_thread - 1) / ((int64_t)kBlock * items_per_thread);
+    if (b < 1) b = 1;
+    if (b > kMaxRedBlocks) b = kMaxRedBlocks;
+    return (int)b;
+}
+
+// ----------------------------------------------------- host helpers ----
+int ensure_red(shk_ctx ctx, size_t bytes);
+int ensure_ws(shk_ctx ctx, size_t bytes);
+int check_device_ptr(const void *p, const char *what);
+
+}  // namespace shk

@@ -1,0 +1,323 @@
+// shk_solver_state.hpp -- device-resident Krylov state and the scalar
+// "finalize" steps that the last work-group of each fused reduction runs.
+//
+// The host never reads a scalar inside an iteration: every dot / norm ends in
+// a finalize below that applies Seldon's scalar recurrence and its
+// Iteration::Finished / Fail logic on the device, then sets `skip` when the
+// solve (or a GMRES cycle) is over so the kernels already queued return at
+// once.  The host checks the state once per chunk of iterations.
+#pragma once
+
+#include "shk_kernels.hpp"
+
+namespace shk {
+
+template <typename T>
+struct KState {
+    // Iteration<double> (computation/solver/iterative/Iterative.hxx:47-108)
+    double tol, facteur, last_res;
+    int max_iter, nb_iter, error_code, fail;
+    int done, init_failed, hist_len, hist_cap;
+    int skip;          // done, or (GMRES) the current cycle is over
+    int stop_slot;     // BiCgStab: loop slot whose Finished(s) ended the solve
+    double *hist;
+    int *hist_it;
+    // Cg / BiCgStab scalars, typed like Seldon's `Complexe` (= value type)
+    T rho, rho_1, rho_2, alpha, beta, delta, omega, sigma;
+    // Gmres(m)
+    int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
+    int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
+    double gm_beta, gm_hi;
+    T gm_scale;        // (T)(1 / hi) for V_{i+1}, (T)(1 / beta) for V_0
+    double *H;         // (m+1) x (m+1), column-major, H(k,i) = H[i*(m+1)+k]
+    double *gs, *gc, *gsn;  // s, rotation cosines, sines
+};
+
+__device__ __forceinline__ double sqrt_t(double v) { return sqrt(v); }
+__device__ __forceinline__ float sqrt_t(float v) { return sqrtf(v); }
+
+// Iteration::Finished (Iterative.cxx:272-319) with the history record.
+template <typename T>
+__device__ inline bool it_finished(KState<T> *st, double norm, int nb, int maxit)
+{
+    double reste = st->facteur * norm;
+    st->last_res = reste;
+    int k = st->hist_len;
+    if (k < st->hist_cap) {
+        st->hist[k] = reste;
+        st->hist_it[k] = nb;
+    }
+    st->hist_len = k + 1;
+    return (reste < st->tol) || (nb >= maxit);
+}
+
+// Iteration::Fail (Iterative.cxx:323-331); the solver then breaks out.
+template <typename T>
+__device__ inline void it_fail(KState<T> *st, int code)
+{
+    st->fail = 1;
+    st->error_code = code;
+    st->done = 1;
+    st->skip = 1;
+}
+
+template <typename T>
+__device__ inline void it_stop(KState<T> *st)
+{
+    st->done = 1;
+    st->skip = 1;
+}
+
+// Iteration::Init on |v| (Iterative.cxx:235-249); false when |v| == 0.
+template <typename T>
+__device__ inline bool it_init(KState<T> *st, T sumsq)
+{
+    double nr = (double)sqrt_t(sumsq);
+    if (nr == 0.0) {
+        st->init_failed = 1;
+        it_stop(st);
+        return false;
+    }
+    st->facteur = 1.0 / nr;
+    st->nb_iter = 0;
+    return true;
+}
+
+// ---------------------------------------------------------------- Cg ----
+// After rho = (r, z) (Cg.cxx:78-94): breakdown test and beta.
+template <typename T>
+__device__ inline void cg_after_rho(KState<T> *st, T rho)
+{
+    st->rho = rho;
+    if (rho == (T)0) {
+        it_fail(st, 1);
+        return;
+    }
+    if (st->nb_iter != 0) st->beta = rho / st->rho_1;
+}
+
+template <typename T>
+struct FinInitNorm {  // Init(b): 1/|b|
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const { it_init(st, s[0]); }
+};
+
+// Loop head of Cg: [rho_1 = rho; ++iter;] Finished(r); identity: rho = |r|^2.
+template <typename T>
+struct FinCgTop {
+    KState<T> *st;
+    int inc, identity;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (inc) {
+            st->rho_1 = st->rho;
+            st->nb_iter++;
+        }
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        if (identity) cg_after_rho(st, s[0]);
+    }
+};
+
+template <typename T>
+struct FinCgRho {  // Jacobi: rho = (r, z)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const { cg_after_rho(st, s[0]); }
+};
+
+template <typename T>
+struct FinCgDelta {  // delta = (p, q) (Cg.cxx:98-104)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->delta = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 2);
+            return;
+        }
+        st->alpha = st->rho / s[0];
+    }
+};
+
+// ---------------------------------------------------------- BiCgStab ----
+// Loop head (BiCgStab.cxx:71-96): [rho_2 = rho_1; ++iter;] Finished(r);
+// rho_1 = (rtilde, r); breakdowns #1, #2; beta.
+template <typename T>
+struct FinBiTop {
+    KState<T> *st;
+    int inc;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (inc) {
+            st->rho_2 = st->rho_1;
+            st->nb_iter++;
+        }
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        st->rho_1 = s[1];
+        if (s[1] == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        if (st->nb_iter != 0) {
+            if (st->omega == (T)0) {
+                it_fail(st, 2);
+                return;
+            }
+            st->beta = (st->rho_1 / st->rho_2) * (st->alpha / st->omega);
+        }
+    }
+};
+
+template <typename T>
+struct FinBiSigma {  // sigma = (rtilde, v); breakdown #3; alpha (:104-110)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->sigma = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 3);
+            return;
+        }
+        st->alpha = st->rho_1 / s[0];
+    }
+};
+
+template <typename T>
+struct FinBiS {  // ++iter; Finished(s) (:115-121)
+    KState<T> *st;
+    int slot;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->nb_iter++;
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) {
+            st->stop_slot = slot;
+            it_stop(st);
+        }
+    }
+};
+
+template <typename T>
+struct FinBiOmega {  // omega = (t, s) / (t, t) (:129)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const { st->omega = s[0] / s[1]; }
+};
+
+// ------------------------------------------------------------- Gmres ----
+// GenRot "old BLAS version" (Functions_Vector.cxx:612-649) and ApplyRot
+// (:681-687), in double.
+__device__ inline void dev_genrot(double &a_in, double &b_in, double &c_, double &s_)
+{
+    double roe = fabs(a_in) > fabs(b_in) ? a_in : b_in;
+    double scal = fabs(a_in) + fabs(b_in);
+    double r, z;
+    if (scal != 0.0) {
+        double a_scl = a_in / scal;
+        double b_scl = b_in / scal;
+        r = scal * sqrt(a_scl * a_scl + b_scl * b_scl);
+        if (roe < 0.0) r *= -1.0;
+        c_ = a_in / r;
+        s_ = b_in / r;
+        z = 1.0;
+        if (fabs(a_in) > fabs(b_in)) z = s_;
+        else if (fabs(b_in) >= fabs(a_in) && c_ != 0.0) z = 1.0 / c_;
+    } else {
+        c_ = 1.0;
+        s_ = 0.0;
+        r = 0.0;
+        z = 0.0;
+    }
+    a_in = r;
+    b_in = z;
+}
+
+__device__ inline void dev_applyrot(double &x, double &y, double c_, double s_)
+{
+    double temp = c_ * x + s_ * y;
+    y = c_ * y - s_ * x;
+    x = temp;
+}
+
+// Start of an outer cycle (Gmres.cxx:102-116) after outer.Finished(beta).
+template <typename T>
+__device__ inline void gm_outer_check(KState<T> *st, double beta)
+{
+    st->gm_beta = beta;
+    if (it_finished(st, beta, st->nb_iter, st->max_iter)) {
+        st->gm_active = 0;
+        st->cyc_skip = 1;
+        it_stop(st);
+        return;
+    }
+    const int m = st->gm_m;
+    for (int k = 0; k < m + 1; k++) st->gs[k] = 0.0;
+    st->gs[0] = beta;
+    st->gm_i = 0;
+    st->gm_cycle_done = 0;
+    st->gm_active = 1;
+    st->cyc_skip = 0;
+    st->gm_inner_nb = st->nb_iter;
+    st->gm_inner_max = st->nb_iter + m;
+    st->gm_scale = (T)(1.0 / beta);
+    st->skip = 0;
+}
+
+template <typename T>
+struct FinGmInit {  // beta = |r|; outer.Init(r); outer.Finished(beta)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (!it_init(st, s[0])) {
+            st->gm_active = 0;
+            st->cyc_skip = 1;
+            return;
+        }
+        gm_outer_check(st, (double)sqrt_t(s[0]));
+    }
+};
+
+template <typename T>
+struct FinGmOuter {  // true residual after the update (Gmres.cxx:173-178)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const { gm_outer_check(st, (double)sqrt_t(s[0])); }
+};
+
+template <typename T>
+struct FinGmH {  // H(k, i) = (V_k, w)
+    KState<T> *st;
+    int k, i;
+    __device__ void operator()(const T (&s)[2]) const { st->H[i * (st->gm_m + 1) + k] = (double)s[0]; }
+};
+
+// End of Arnoldi step i (Gmres.cxx:135-162).
+template <typename T>
+struct FinGmStep {
+    KState<T> *st;
+    int i;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const int ld = st->gm_m + 1;
+        double *Hc = st->H + (size_t)i * ld;
+        double hi = (double)sqrt_t(s[0]);
+        st->gm_hi = hi;
+        st->gm_scale = hi != 0.0 ? (T)(1.0 / hi) : (T)1;
+        for (int k = 0; k < i; k++) dev_applyrot(Hc[k], Hc[k + 1], st->gc[k], st->gsn[k]);
+        if (hi != 0.0) {
+            dev_genrot(Hc[i], hi, st->gc[i], st->gsn[i]);
+            dev_applyrot(st->gs[i], st->gs[i + 1], st->gc[i], st->gsn[i]);
+        }
+        st->gm_inner_nb++;
+        st->nb_iter++;
+        st->gm_i = i + 1;
+        if (it_finished(st, fabs(st->gs[i + 1]), st->gm_inner_nb, st->gm_inner_max)) {
+            st->gm_cycle_done = 1;
+            st->skip = 1;
+        }
+    }
+};
+
+}  // namespace shk

@@ -1,0 +1,650 @@
+// shk_solvers.hip -- Cg, BiCgStab and Gmres(m) driven from the device.
+//
+// Each solver is the reference template's loop body unrolled into fused
+// kernels; the scalar recurrences and Iteration semantics run in the
+// finalize steps of shk_solver_state.hpp.  The host enqueues iterations in
+// chunks and reads the state back once per chunk; kernels queued past the end
+// of the solve see `skip` and return at once.
+#include <algorithm>
+#include <cstring>
+
+#include "shk_solver_state.hpp"
+
+namespace shk {
+
+#define GRID_STRIDE(i, n) \
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (n); i += (int64_t)gridDim.x * kBlock)
+
+// ---------------------------------------------------------------- Cg ----
+// Init: x = 0 (x.Zero(), Cg.cxx:67) and r = b (Copy, :63); sum r^2.
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_cg_init0(int64_t n, const T *b, T *x, T *r,
+                                                     KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T v = b[i];
+        x[i] = (T)0;
+        r[i] = v;
+        c[0] += v * v;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// z = M^{-1} r (Jacobi), sum r*z  (Cg.cxx:75-78)
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_cg_z(int64_t n, const T *r, const T *d, T *z,
+                                                 KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T rv = r[i];
+        T zv = rv / d[i];
+        z[i] = zv;
+        c[0] += rv * zv;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// First ? p = z : { p *= beta; p += 1 * z }   (Cg.cxx:86-94)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *z, T *p, const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T beta = st->beta;
+    GRID_STRIDE(i, n) {
+        if (first) {
+            p[i] = z[i];
+        } else {
+            T v = p[i] * beta;
+            v += (T)1 * z[i];
+            p[i] = v;
+        }
+    }
+}
+
+// x += alpha p; r += (-alpha) q; sum r^2   (Cg.cxx:107-108, then Finished)
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *p, const T *q, T *x, T *r,
+                                                  KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    const T alpha = st->alpha, malpha = -st->alpha;
+    const bool ax = alpha != (T)0, ar = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (ax) x[i] += alpha * p[i];
+        T rv = r[i];
+        if (ar) {
+            rv += malpha * q[i];
+            r[i] = rv;
+        }
+        c[0] += rv * rv;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// ---------------------------------------------------------- BiCgStab ----
+// [x = 0, r = b]; rtilde = r; sums r^2 and rtilde*r  (BiCgStab.cxx:61-71)
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_bi_init(int64_t n, const T *b, T *x, T *r, T *rt, int zero_x,
+                                                    KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T v;
+        if (zero_x) {
+            x[i] = (T)0;
+            v = b[i];
+            r[i] = v;
+        } else {
+            v = r[i];
+        }
+        rt[i] = v;
+        c[0] += v * v;
+        c[1] += v * v;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// p update (:81-96) and phat = M^{-1} p (:98)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const T *r, const T *v, T *p, const T *d,
+                                                 T *phat, const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T momega = -st->omega, beta = st->beta;
+    const bool aw = momega != (T)0;
+    GRID_STRIDE(i, n) {
+        T pv;
+        if (first) {
+            pv = r[i];
+        } else {
+            pv = p[i];
+            if (aw) pv += momega * v[i];
+            pv *= beta;
+            pv += (T)1 * r[i];
+        }
+        p[i] = pv;
+        if (d) phat[i] = pv / d[i];
+    }
+}
+
+// s = r; s += (-alpha) v (:111-112); shat = M^{-1} s (:124); sum s^2
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *r, const T *v, T *s, const T *d,
+                                                 T *shat, KState<T> *st, T *partials, unsigned *ticket,
+                                                 Fin fin)
+{
+    if (st->skip) return;
+    const T malpha = -st->alpha;
+    const bool aa = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T sv = r[i];
+        if (aa) sv += malpha * v[i];
+        s[i] = sv;
+        if (d) shat[i] = sv / d[i];
+        c[0] += sv * sv;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// Finished(s) ended the solve in this slot: x += alpha phat (:116-120)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_fix(int64_t n, const T *phat, T *x, const KState<T> *st,
+                                                   int slot)
+{
+    if (st->stop_slot != slot) return;
+    const T alpha = st->alpha;
+    if (alpha == (T)0) return;
+    GRID_STRIDE(i, n) x[i] += alpha * phat[i];
+}
+
+// x += alpha phat; x += omega shat; r = s; r += (-omega) t (:132-137);
+// sums r^2, rtilde*r for the next loop head
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *phat, const T *shat, const T *s,
+                                                  const T *t, const T *rt, T *x, T *r, KState<T> *st,
+                                                  T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    const T alpha = st->alpha, omega = st->omega, momega = -st->omega;
+    const bool aa = alpha != (T)0, ao = omega != (T)0, am = momega != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T xv = x[i];
+        if (aa) xv += alpha * phat[i];
+        if (ao) xv += omega * shat[i];
+        x[i] = xv;
+        T rv = s[i];
+        if (am) rv += momega * t[i];
+        r[i] = rv;
+        c[0] += rv * rv;
+        c[1] += rt[i] * rv;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// ------------------------------------------------------------- Gmres ----
+// x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_gm_init0(int64_t n, const T *b, const T *d, T *x, T *r,
+                                                     KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        x[i] = (T)0;
+        T v = d ? b[i] / d[i] : b[i];
+        r[i] = v;
+        c[0] += v * v;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// V_0 = r * (1/beta)  (:105-106)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_v0(int64_t n, const T *r, T *v0, const KState<T> *st)
+{
+    if (st->cyc_skip) return;
+    const T sc = st->gm_scale;
+    GRID_STRIDE(i, n) v0[i] = r[i] * sc;
+}
+
+// MGS step k >= 1 of Arnoldi step i: w += (-H(k-1,i)) V_{k-1}; H(k,i) = (V_k, w)
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *vprev, const T *vk, T *w,
+                                                   int hpos, KState<T> *st, T *partials, unsigned *ticket,
+                                                   Fin fin)
+{
+    if (st->skip) return;
+    const T a = (T)(-st->H[hpos]);
+    const bool aa = a != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(j, n) {
+        T wv = w[j];
+        if (aa) {
+            wv += a * vprev[j];
+            w[j] = wv;
+        }
+        c[0] += vk[j] * wv;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// last MGS update w += (-H(i,i)) V_i, then hi_ip1 = |w| (:131-135)
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_gm_last(int64_t n, const T *vi, T *w, int hpos, KState<T> *st,
+                                                    T *partials, unsigned *ticket, Fin fin)
+{
+    if (st->skip) return;
+    const T a = (T)(-st->H[hpos]);
+    const bool aa = a != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(j, n) {
+        T wv = w[j];
+        if (aa) {
+            wv += a * vi[j];
+            w[j] = wv;
+        }
+        c[0] += wv * wv;
+    }
+    T out[2];
+    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+}
+
+// V_{i+1} = w, scaled by 1/hi when hi != 0 (:136-140); runs before the
+// cycle-done test of the step has any effect (V_{i+1} is unused after it)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_vnext(int64_t n, const T *w, T *vnext, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T sc = st->gm_scale;
+    GRID_STRIDE(j, n) vnext[j] = w[j] * sc;
+}
+
+// H.Resize(i,i); Solve(H, s): back substitution (:165-166, the dtrsv loop)
+template <typename T>
+__global__ void k_gm_solve(KState<T> *st)
+{
+    if (st->cyc_skip || threadIdx.x != 0) return;
+    const int i = st->gm_i, ld = st->gm_m + 1;
+    double *H = st->H, *s = st->gs;
+    for (int j = i - 1; j >= 0; j--) {
+        if (s[j] != 0.0) {
+            s[j] = s[j] / H[j * ld + j];
+            double temp = s[j];
+            for (int q = j - 1; q >= 0; q--) s[q] -= temp * H[j * ld + q];
+        }
+    }
+}
+
+// x += sum_k s(k) V_k, Adds in k order per element (:169-170)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_xupd(int64_t n, const T *V, T *x, const KState<T> *st)
+{
+    if (st->cyc_skip) return;
+    __shared__ T coef[1024];
+    const int i = st->gm_i;
+    for (int k = threadIdx.x; k < i; k += kBlock) coef[k] = (T)st->gs[k];
+    __syncthreads();
+    GRID_STRIDE(j, n) {
+        T xv = x[j];
+        for (int k = 0; k < i; k++) {
+            const T a = coef[k];
+            if (a != (T)0) xv += a * V[(size_t)k * n + j];
+        }
+        x[j] = xv;
+    }
+}
+
+#undef GRID_STRIDE
+
+// -------------------------------------------------------------- host ----
+
+template <typename T>
+struct Solve {
+    shk_ctx ctx;
+    shk_csr A;
+    int64_t n;
+    int nb;                 // reduction grid
+    KState<T> *dst;         // device state
+    KState<T> *hst;         // pinned mirror
+    double *dhist = nullptr;
+    int *dhist_it = nullptr;
+    T *partials;
+    unsigned *ticket;
+    char *ws;
+    size_t off = 0;
+
+    T *take(int64_t count)
+    {
+        T *p = (T *)(ws + off);
+        off += ((size_t)count * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+
+    int setup(shk_ctx c, shk_csr a, size_t vec_count, size_t extra_bytes, const shk_iteration *it)
+    {
+        ctx = c;
+        A = a;
+        n = a->m;
+        nb = red_blocks(n);
+        size_t vb = (((size_t)n * sizeof(T) + 255) & ~(size_t)255);
+        size_t hist_cap = it->hist && it->hist_cap > 0 ? (size_t)it->hist_cap : 0;
+        size_t need = vec_count * vb + extra_bytes + 256 + hist_cap * (sizeof(double) + sizeof(int)) + 512;
+        SHK_TRY(ensure_ws(ctx, need));
+        SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * (size_t)std::max(nb, a->nblk_launch)));
+        ws = (char *)ctx->ws;
+        off = 0;
+        partials = (T *)ctx->red.partials;
+        ticket = ctx->red.ticket;
+        dst = (KState<T> *)ctx->state;
+        hst = (KState<T> *)ctx->state_host;
+        if (hist_cap) {
+            dhist = (double *)(ws + off);
+            off += (hist_cap * sizeof(double) + 255) & ~(size_t)255;
+            dhist_it = (int *)(ws + off);
+            off += (hist_cap * sizeof(int) + 255) & ~(size_t)255;
+        }
+        KState<T> s;
+        std::memset(&s, 0, sizeof(s));
+        s.tol = it->tolerance;
+        s.facteur = it->facteur_reste;
+        s.max_iter = it->max_iter;
+        s.nb_iter = it->nb_iter;
+        s.hist = dhist;
+        s.hist_it = dhist_it;
+        s.hist_cap = (int)hist_cap;
+        s.stop_slot = -1;
+        s.cyc_skip = 1;
+        std::memcpy(hst, &s, sizeof(s));
+        SHK_HIP(hipMemcpyAsync(dst, hst, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
+        return SHK_OK;
+    }
+
+    int read_state()
+    {
+        SHK_HIP(hipMemcpyAsync(hst, dst, sizeof(KState<T>), hipMemcpyDeviceToHost, ctx->stream));
+        SHK_HIP(hipStreamSynchronize(ctx->stream));
+        return SHK_OK;
+    }
+
+    // Copy the outcome into the caller's Iteration mirror; returns ErrorCode().
+    int finish(shk_iteration *it, int *code)
+    {
+        SHK_TRY(read_state());
+        const KState<T> &s = *hst;
+        if (!s.init_failed) {
+            it->facteur_reste = s.facteur;
+            it->nb_iter = s.nb_iter;
+        }
+        it->fail_convergence = s.fail;
+        if (s.fail) it->error_code = s.error_code;
+        it->hist_len = s.hist_len;
+        int ncopy = std::min(s.hist_len, s.hist_cap);
+        if (ncopy > 0) {
+            SHK_HIP(hipMemcpyAsync(it->hist, dhist, sizeof(double) * ncopy, hipMemcpyDeviceToHost, ctx->stream));
+            if (it->hist_it)
+                SHK_HIP(hipMemcpyAsync(it->hist_it, dhist_it, sizeof(int) * ncopy, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            SHK_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        // Iteration::ErrorCode (Iterative.cxx:344-354)
+        if (it->nb_iter >= it->max_iter) *code = -2;
+        else if (s.fail) *code = s.error_code;
+        else *code = 0;
+        return SHK_OK;
+    }
+
+    template <typename Fin>
+    int spmv(SpmvArgs<T> &a, Fin fin)
+    {
+        return launch_spmv<T, Fin>(A, a, fin);
+    }
+};
+
+constexpr int kChunk = 16;
+
+template <typename T>
+int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, jac ? 4 : 3, 0, it));
+    const int64_t n = S.n;
+    T *p = S.take(n), *q = S.take(n), *r = S.take(n), *z = jac ? S.take(n) : r;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (Cg.cxx:58)
+    hipLaunchKernelGGL((k_axpy_dot<T, FinInitNorm<T>>), g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, S.ticket, (const int *)&S.dst->skip,
+                       FinInitNorm<T>{S.dst});
+    // r = b - A x or x = 0, r = b; then the first Finished(r)
+    FinCgTop<T> top0{S.dst, 0, jac ? 0 : 1};
+    if (it->init_guess_null) {
+        hipLaunchKernelGGL((k_cg_init0<T, FinCgTop<T>>), g, blk, 0, st, n, b, x, r, S.dst, S.partials, S.ticket,
+                           top0);
+    } else {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1;
+        a.dot0 = kDotSelf; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, top0));
+    }
+    SHK_HIP(hipGetLastError());
+    const T *dg = (const T *)A->diag;
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
+    for (int64_t done_its = 0; done_its <= limit; done_its += kChunk) {
+        for (int k = 0; k < kChunk; k++) {
+            if (jac)
+                hipLaunchKernelGGL((k_cg_z<T, FinCgRho<T>>), g, blk, 0, st, n, r, dg, z, S.dst, S.partials,
+                                   S.ticket, FinCgRho<T>{S.dst});
+            hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, st, n, z, p, S.dst);
+            SpmvArgs<T> a{};
+            a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = p; a.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a, FinCgDelta<T>{S.dst}));
+            hipLaunchKernelGGL((k_cg_xr<T, FinCgTop<T>>), g, blk, 0, st, n, p, q, x, r, S.dst, S.partials,
+                               S.ticket, FinCgTop<T>{S.dst, 1, jac ? 0 : 1});
+        }
+        SHK_HIP(hipGetLastError());
+        SHK_TRY(S.read_state());
+        if (S.hst->done) break;
+    }
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "cg: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
+template <typename T>
+int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, jac ? 8 : 6, 0, it));
+    const int64_t n = S.n;
+    T *p = S.take(n), *s = S.take(n), *t = S.take(n), *v = S.take(n), *r = S.take(n), *rt = S.take(n);
+    T *phat = jac ? S.take(n) : p, *shat = jac ? S.take(n) : s;
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    hipLaunchKernelGGL((k_axpy_dot<T, FinInitNorm<T>>), g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, S.ticket, (const int *)&S.dst->skip,
+                       FinInitNorm<T>{S.dst});
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+    }
+    hipLaunchKernelGGL((k_bi_init<T, FinBiTop<T>>), g, blk, 0, st, n, b, x, r, rt, it->init_guess_null ? 1 : 0,
+                       S.dst, S.partials, S.ticket, FinBiTop<T>{S.dst, 0});
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    int slot = 0;
+    for (int64_t loops = 0; loops <= limit; loops += kChunk) {
+        for (int k = 0; k < kChunk; k++, slot++) {
+            hipLaunchKernelGGL(k_bi_p<T>, g, blk, 0, st, n, r, v, p, dg, phat, S.dst);
+            SpmvArgs<T> a{};
+            a.x = phat; a.y_out = v; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = rt; a.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a, FinBiSigma<T>{S.dst}));
+            hipLaunchKernelGGL((k_bi_s<T, FinBiS<T>>), g, blk, 0, st, n, r, v, s, dg, shat, S.dst, S.partials,
+                               S.ticket, FinBiS<T>{S.dst, slot});
+            hipLaunchKernelGGL(k_bi_fix<T>, g, blk, 0, st, n, phat, x, S.dst, slot);
+            SpmvArgs<T> b2{};
+            b2.x = shat; b2.y_out = t; b2.alpha = (T)1; b2.mlt = 1;
+            b2.dot0 = kDotVec; b2.u0 = s; b2.dot1 = kDotSelf; b2.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(b2, FinBiOmega<T>{S.dst}));
+            hipLaunchKernelGGL((k_bi_xr<T, FinBiTop<T>>), g, blk, 0, st, n, phat, shat, s, t, rt, x, r, S.dst,
+                               S.partials, S.ticket, FinBiTop<T>{S.dst, 1});
+        }
+        SHK_HIP(hipGetLastError());
+        SHK_TRY(S.read_state());
+        if (S.hst->done) break;
+    }
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "bicgstab: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
+template <typename T>
+int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    const int m = it->parameter_restart;
+    SHK_CHECK(m >= 1 && m <= 1000, SHK_ERR_ARG, "gmres: restart must be in [1, 1000]");
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    const size_t small = sizeof(double) * ((size_t)(m + 1) * (m + 1) + 3 * (size_t)(m + 1)) + 1024;
+    SHK_TRY(S.setup(ctx, A, (size_t)m + 3, small, it));
+    const int64_t n = S.n;
+    T *V = S.take(n * (int64_t)(m + 1));
+    T *w = S.take(n), *r = S.take(n);
+    double *H = (double *)S.take((int64_t)((small + sizeof(T) - 1) / sizeof(T)));
+    double *gs = H + (size_t)(m + 1) * (m + 1), *gc = gs + (m + 1), *gsn = gc + (m + 1);
+    SHK_HIP(hipMemsetAsync(H, 0, small, ctx->stream));
+    {
+        KState<T> *h = S.hst;
+        h->gm_m = m;
+        h->H = H;
+        h->gs = gs;
+        h->gc = gc;
+        h->gsn = gsn;
+        SHK_HIP(hipMemcpyAsync(S.dst, h, sizeof(KState<T>), hipMemcpyHostToDevice, ctx->stream));
+    }
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    auto Vk = [&](int k) { return V + (size_t)k * n; };
+    // residual, preconditioning, beta, outer.Init (Gmres.cxx:82-95)
+    if (it->init_guess_null) {
+        hipLaunchKernelGGL((k_gm_init0<T, FinGmInit<T>>), g, blk, 0, st, n, b, dg, x, r, S.dst, S.partials,
+                           S.ticket, FinGmInit<T>{S.dst});
+    } else {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.diag = dg;
+        a.dot0 = kDotSelf; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinGmInit<T>{S.dst}));
+    }
+    hipLaunchKernelGGL(k_gm_v0<T>, g, blk, 0, st, n, r, Vk(0), S.dst);
+    SHK_HIP(hipGetLastError());
+    SHK_TRY(S.read_state());
+    const int step_chunk = 4;
+    int64_t guard = 0;
+    while (!S.hst->done) {
+        SHK_CHECK(++guard <= (int64_t)it->max_iter + 2, SHK_ERR_STATE, "gmres: no progress");
+        // Arnoldi steps of one cycle, enqueued in chunks
+        for (int i0 = 0; i0 < m; i0 += step_chunk) {
+            for (int i = i0; i < std::min(m, i0 + step_chunk); i++) {
+                SpmvArgs<T> a{};
+                a.x = Vk(i); a.y_out = w; a.alpha = (T)1; a.mlt = 1; a.diag = dg;
+                a.dot0 = kDotVec; a.u0 = Vk(0); a.skip = &S.dst->skip;
+                SHK_TRY(S.spmv(a, FinGmH<T>{S.dst, 0, i}));
+                for (int k = 1; k <= i; k++)
+                    hipLaunchKernelGGL((k_gm_mgs<T, FinGmH<T>>), g, blk, 0, st, n, (const T *)Vk(k - 1),
+                                       (const T *)Vk(k), w, i * (m + 1) + (k - 1), S.dst, S.partials, S.ticket,
+                                       FinGmH<T>{S.dst, k, i});
+                hipLaunchKernelGGL((k_gm_last<T, FinGmStep<T>>), g, blk, 0, st, n, (const T *)Vk(i), w,
+                                   i * (m + 1) + i, S.dst, S.partials, S.ticket, FinGmStep<T>{S.dst, i});
+                hipLaunchKernelGGL(k_gm_vnext<T>, g, blk, 0, st, n, (const T *)w, Vk(i + 1), S.dst);
+            }
+            SHK_HIP(hipGetLastError());
+            SHK_TRY(S.read_state());
+            if (S.hst->gm_cycle_done || S.hst->done) break;
+        }
+        // cycle end: least squares, update, true residual (Gmres.cxx:165-178)
+        hipLaunchKernelGGL(k_gm_solve<T>, dim3(1), dim3(64), 0, st, S.dst);
+        hipLaunchKernelGGL(k_gm_xupd<T>, g, blk, 0, st, n, (const T *)V, x, S.dst);
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.diag = dg;
+        a.dot0 = kDotSelf; a.skip = &S.dst->cyc_skip;
+        SHK_TRY(S.spmv(a, FinGmOuter<T>{S.dst}));
+        hipLaunchKernelGGL(k_gm_v0<T>, g, blk, 0, st, n, r, Vk(0), S.dst);
+        SHK_HIP(hipGetLastError());
+        SHK_TRY(S.read_state());
+    }
+    return S.finish(it, code);
+}
+
+template <typename T>
+int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
+                   shk_iteration *it, int *code)
+{
+    if (which == 0) return cg_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 1) return bicgstab_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+}
+
+static int solve_entry(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+                       shk_iteration *it, int *code)
+{
+    SHK_CHECK(ctx && A && it && code, SHK_ERR_ARG, "solver: null argument");
+    SHK_CHECK(A->ctx == ctx, SHK_ERR_ARG, "solver: matrix belongs to another context");
+    SHK_CHECK(prec == SHK_PREC_IDENTITY || prec == SHK_PREC_JACOBI, SHK_ERR_ARG, "solver: bad preconditioner");
+    SHK_CHECK(A->m == A->n, SHK_ERR_DIM, "solver: matrix must be square");
+    *code = 0;
+    // const int N = A.GetM(); if (N <= 0) return 0;  (Cg.cxx:49-51)
+    if (A->m <= 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(db, "b"));
+    SHK_CHECK(!(it->hist_cap > 0 && !it->hist), SHK_ERR_ARG, "solver: hist_cap > 0 needs hist");
+    it->hist_len = 0;
+    if (A->dtype == SHK_F64) return solve_dispatch<double>(which, ctx, A, dx, db, prec, it, code);
+    return solve_dispatch<float>(which, ctx, A, dx, db, prec, it, code);
+}
+
+}  // namespace shk
+
+using namespace shk;
+
+extern "C" {
+
+int shk_cg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(0, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_bicgstab(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it,
+                 int *code)
+{
+    return solve_entry(1, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_gmres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it,
+              int *code)
+{
+    return solve_entry(2, ctx, A, dx, db, prec, it, code);
+}
+
+}  // extern "C"

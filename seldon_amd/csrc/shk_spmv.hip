@@ -1,0 +1,417 @@
+// shk_spmv.hip -- sparse matrix upload, row-block planning and the SpMV
+// entry points of the C ABI.
+#include <algorithm>
+#include <cstring>
+
+#include "shk_kernels.hpp"
+
+namespace shk {
+
+// ------------------------------------------------------------ planning ----
+// Greedy row blocks: at most kBlock rows and `budget` entries per block; a row
+// longer than the budget is a block by itself.
+static void plan_blocks(int m, const std::vector<int> &ptr, int budget, std::vector<int> &blk)
+{
+    blk.clear();
+    blk.push_back(0);
+    int r = 0;
+    while (r < m) {
+        int start = r;
+        int64_t base = ptr[r];
+        if (ptr[r + 1] - base > budget) {
+            r++;
+        } else {
+            while (r < m && r - start < kBlock && ptr[r + 1] - base <= budget) r++;
+        }
+        blk.push_back(r);
+    }
+}
+
+template <typename T>
+static __global__ void k_scatter_vals(int64_t n, const T *src, const int64_t *map_up,
+                                      const int64_t *map_lo, T *dst)
+{
+    for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+        T v = src[j];
+        dst[map_up[j]] = v;
+        if (map_lo[j] >= 0) dst[map_lo[j]] = v;
+    }
+}
+
+template <typename T>
+static __global__ void k_gather_diag(int m, const int64_t *pos, const T *val, T *diag)
+{
+    for (int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x; r < m; r += (int64_t)gridDim.x * kBlock)
+        diag[r] = pos[r] >= 0 ? val[pos[r]] : (T)0;
+}
+
+struct HostPlan {
+    std::vector<int> ptr, mid, blk;
+    std::vector<uint32_t> ind;
+    std::vector<int64_t> src_of;   // expanded position -> stored entry index
+    std::vector<int64_t> map_up, map_lo, diag_pos;
+    int max_row = 0;
+};
+
+// Build the device layout (expanded for RowSymSparse) from the caller's CSR.
+static int build_plan(int storage, int m, int n, int64_t nnz, const int *ptr, const int *ind,
+                      int budget, HostPlan &P)
+{
+    SHK_CHECK(ptr[0] == 0 && ptr[m] == nnz, SHK_ERR_ARG, "csr: ptr[0] must be 0 and ptr[m] == nnz");
+    for (int i = 0; i < m; i++)
+        SHK_CHECK(ptr[i + 1] >= ptr[i], SHK_ERR_ARG, "csr: ptr must be non-decreasing");
+    for (int64_t j = 0; j < nnz; j++)
+        SHK_CHECK(ind[j] >= 0 && ind[j] < n, SHK_ERR_ARG, "csr: column index out of range");
+    SHK_CHECK(nnz < (int64_t)INT32_MAX, SHK_ERR_ARG, "csr: nnz must fit in int (Seldon's int indices)");
+    P.diag_pos.assign(m, -1);
+    if (storage == SHK_ROWSPARSE || storage == SHK_ARRAYROWSPARSE) {
+        P.ptr.assign(ptr, ptr + m + 1);
+        P.ind.resize(nnz);
+        const uint32_t flag = storage == SHK_ARRAYROWSPARSE ? kLowerFlag : 0u;
+        for (int64_t j = 0; j < nnz; j++) P.ind[j] = (uint32_t)ind[j] | flag;
+        for (int i = 0; i < m; i++)
+            for (int j = ptr[i]; j < ptr[i + 1]; j++)
+                if (ind[j] == i) P.diag_pos[i] = j;  // last match, as the oracle
+        P.src_of.clear();
+    } else {
+        SHK_CHECK(m == n, SHK_ERR_DIM, "RowSymSparse needs a square matrix");
+        // transposed part: the scatter of Functions_MatVect.cxx:560-563 runs
+        // i = 0..m-2, entries j of row i with ind[j] != i, adding to row ind[j]
+        std::vector<int64_t> cnt(m + 1, 0);
+        for (int i = 0; i < m - 1; i++)
+            for (int j = ptr[i]; j < ptr[i + 1]; j++)
+                if (ind[j] != i) cnt[ind[j]]++;
+        int64_t total = nnz;
+        for (int i = 0; i < m; i++) total += cnt[i];
+        SHK_CHECK(total < (int64_t)INT32_MAX, SHK_ERR_ARG, "csr: expanded nnz must fit in int");
+        P.ptr.assign(m + 1, 0);
+        P.mid.assign(m, 0);
+        for (int i = 0; i < m; i++) {
+            int64_t up = ptr[i + 1] - ptr[i];
+            P.ptr[i + 1] = (int)(P.ptr[i] + up + cnt[i]);
+            P.mid[i] = (int)(P.ptr[i] + up);
+        }
+        P.ind.resize(total);
+        P.src_of.resize(total);
+        P.map_up.resize(nnz);
+        P.map_lo.assign(nnz, -1);
+        std::vector<int64_t> fillpos(m);
+        for (int i = 0; i < m; i++) {
+            int64_t pos = P.ptr[i];
+            for (int j = ptr[i]; j < ptr[i + 1]; j++, pos++) {
+                P.ind[pos] = (uint32_t)ind[j];
+                P.src_of[pos] = j;
+                P.map_up[j] = pos;
+                if (ind[j] == i) P.diag_pos[i] = pos;
+            }
+            fillpos[i] = P.mid[i];
+        }
+        for (int i = 0; i < m - 1; i++)
+            for (int j = ptr[i]; j < ptr[i + 1]; j++)
+                if (ind[j] != i) {
+                    int c = ind[j];
+                    int64_t pos = fillpos[c]++;
+                    P.ind[pos] = (uint32_t)i | kLowerFlag;
+                    P.src_of[pos] = j;
+                    P.map_lo[j] = pos;
+                }
+    }
+    for (int i = 0; i < m; i++) P.max_row = std::max(P.max_row, P.ptr[i + 1] - P.ptr[i]);
+    plan_blocks(m, P.ptr, budget, P.blk);
+    return SHK_OK;
+}
+
+template <typename T>
+static int upload(shk_csr A, const HostPlan &P, const T *val_host)
+{
+    const int64_t nd = (int64_t)P.ind.size();
+    const int64_t pad = 16;
+    SHK_HIP(hipMalloc(&A->ptr, sizeof(int) * (A->m + 1)));
+    SHK_HIP(hipMalloc(&A->ind, sizeof(uint32_t) * (nd + pad)));
+    SHK_HIP(hipMalloc(&A->val, sizeof(T) * (nd + pad)));
+    SHK_HIP(hipMalloc(&A->diag, sizeof(T) * std::max(A->m, 1)));
+    SHK_HIP(hipMalloc(&A->blk_rows, sizeof(int) * P.blk.size()));
+    hipStream_t s = A->ctx->stream;
+    SHK_HIP(hipMemcpyAsync(A->ptr, P.ptr.data(), sizeof(int) * (A->m + 1), hipMemcpyHostToDevice, s));
+    SHK_HIP(hipMemsetAsync(A->ind, 0, sizeof(uint32_t) * (nd + pad), s));
+    SHK_HIP(hipMemsetAsync(A->val, 0, sizeof(T) * (nd + pad), s));
+    if (nd) SHK_HIP(hipMemcpyAsync(A->ind, P.ind.data(), sizeof(uint32_t) * nd, hipMemcpyHostToDevice, s));
+    SHK_HIP(hipMemcpyAsync(A->blk_rows, P.blk.data(), sizeof(int) * P.blk.size(), hipMemcpyHostToDevice, s));
+    if (!P.mid.empty()) {
+        SHK_HIP(hipMalloc(&A->mid, sizeof(int) * A->m));
+        SHK_HIP(hipMemcpyAsync(A->mid, P.mid.data(), sizeof(int) * A->m, hipMemcpyHostToDevice, s));
+        SHK_HIP(hipMalloc(&A->map_up, sizeof(int64_t) * std::max<int64_t>(A->nnz_stored, 1)));
+        SHK_HIP(hipMalloc(&A->map_lo, sizeof(int64_t) * std::max<int64_t>(A->nnz_stored, 1)));
+        if (A->nnz_stored) {
+            SHK_HIP(hipMemcpyAsync(A->map_up, P.map_up.data(), sizeof(int64_t) * A->nnz_stored,
+                                   hipMemcpyHostToDevice, s));
+            SHK_HIP(hipMemcpyAsync(A->map_lo, P.map_lo.data(), sizeof(int64_t) * A->nnz_stored,
+                                   hipMemcpyHostToDevice, s));
+        }
+    }
+    // diag positions stay on device for value updates
+    int64_t *dpos = nullptr;
+    SHK_HIP(hipMalloc(&dpos, sizeof(int64_t) * std::max(A->m, 1)));
+    SHK_HIP(hipMemcpyAsync(dpos, P.diag_pos.data(), sizeof(int64_t) * A->m, hipMemcpyHostToDevice, s));
+    SHK_HIP(hipStreamSynchronize(s));
+    A->map_diag = dpos;
+    if (val_host) {
+        std::vector<T> v(nd);
+        if (P.src_of.empty()) std::memcpy(v.data(), val_host, sizeof(T) * nd);
+        else
+            for (int64_t q = 0; q < nd; q++) v[q] = val_host[P.src_of[q]];
+        if (nd) SHK_HIP(hipMemcpyAsync(A->val, v.data(), sizeof(T) * nd, hipMemcpyHostToDevice, s));
+        if (A->m)
+            hipLaunchKernelGGL(k_gather_diag<T>, dim3(red_blocks(A->m)), dim3(kBlock), 0, s, A->m,
+                               (const int64_t *)A->map_diag, (const T *)A->val, (T *)A->diag);
+        SHK_HIP(hipGetLastError());
+        SHK_HIP(hipStreamSynchronize(s));
+    }
+    return SHK_OK;
+}
+
+template <typename T>
+static int set_values_dev(shk_csr A, const T *dval)
+{
+    hipStream_t s = A->ctx->stream;
+    if (A->storage == SHK_ROWSYMSPARSE) {
+        if (A->nnz_stored)
+            hipLaunchKernelGGL(k_scatter_vals<T>, dim3(red_blocks(A->nnz_stored)), dim3(kBlock), 0, s,
+                               A->nnz_stored, dval, (const int64_t *)A->map_up,
+                               (const int64_t *)A->map_lo, (T *)A->val);
+    } else if (A->nnz_stored) {
+        SHK_HIP(hipMemcpyAsync(A->val, dval, sizeof(T) * A->nnz_stored, hipMemcpyDeviceToDevice, s));
+    }
+    if (A->m)
+        hipLaunchKernelGGL(k_gather_diag<T>, dim3(red_blocks(A->m)), dim3(kBlock), 0, s, A->m,
+                           (const int64_t *)A->map_diag, (const T *)A->val, (T *)A->diag);
+    SHK_HIP(hipGetLastError());
+    return SHK_OK;
+}
+
+template <typename T>
+static int spmv_typed(shk_csr A, int mlt, T alpha, const void *dx, T beta, const void *dy_in,
+                      void *dy_out)
+{
+    SpmvArgs<T> a{};
+    a.x = (const T *)dx;
+    a.y_in = (const T *)dy_in;
+    a.y_out = (T *)dy_out;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.mlt = mlt;
+    return launch_spmv<T>(A, a, FinNone{});
+}
+
+}  // namespace shk
+
+using namespace shk;
+
+static int check_csr(shk_ctx ctx, shk_csr A)
+{
+    SHK_CHECK(ctx && A, SHK_ERR_ARG, "null ctx or matrix");
+    SHK_CHECK(A->ctx == ctx, SHK_ERR_ARG, "matrix belongs to another context");
+    return SHK_OK;
+}
+
+extern "C" {
+
+int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n, int64_t nnz,
+                   const int *ptr, const int *ind, const void *val, int src_on_device, shk_csr *out)
+{
+    SHK_CHECK(ctx && out, SHK_ERR_ARG, "shk_csr_create: null argument");
+    *out = nullptr;
+    SHK_CHECK(m >= 0 && n >= 0 && nnz >= 0, SHK_ERR_ARG, "shk_csr_create: negative size");
+    SHK_CHECK(storage >= 0 && storage <= 2, SHK_ERR_ARG, "shk_csr_create: bad storage");
+    SHK_CHECK(dtype == SHK_F64 || dtype == SHK_F32, SHK_ERR_ARG, "shk_csr_create: bad dtype");
+    SHK_CHECK(ptr && (nnz == 0 || (ind && val)), SHK_ERR_ARG, "shk_csr_create: null array");
+    SHK_HIP(hipSetDevice(ctx->device));
+    const size_t vs = dsize(dtype);
+    std::vector<int> hptr(m + 1), hind(nnz);
+    std::vector<char> hval(vs * (size_t)nnz);
+    if (src_on_device) {
+        SHK_HIP(hipMemcpy(hptr.data(), ptr, sizeof(int) * (m + 1), hipMemcpyDeviceToHost));
+        if (nnz) {
+            SHK_HIP(hipMemcpy(hind.data(), ind, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+            SHK_HIP(hipMemcpy(hval.data(), val, vs * nnz, hipMemcpyDeviceToHost));
+        }
+    } else {
+        std::memcpy(hptr.data(), ptr, sizeof(int) * (m + 1));
+        if (nnz) {
+            std::memcpy(hind.data(), ind, sizeof(int) * nnz);
+            std::memcpy(hval.data(), val, vs * nnz);
+        }
+    }
+    HostPlan P;
+    const int budget = dtype == SHK_F64 ? budget_of<double>() : budget_of<float>();
+    SHK_TRY(build_plan(storage, m, n, nnz, hptr.data(), hind.data(), budget, P));
+    shk_csr A = new shk_csr_s();
+    A->ctx = ctx;
+    A->storage = storage;
+    A->dtype = dtype;
+    A->m = m;
+    A->n = n;
+    A->nnz_stored = nnz;
+    A->nnz_dev = (int64_t)P.ind.size();
+    A->budget = budget;
+    A->max_row = P.max_row;
+    A->nblk = (int)P.blk.size() - 1;
+    A->xcd_q = (A->nblk + kNumXcd - 1) / kNumXcd;
+    A->nblk_launch = A->xcd_q * kNumXcd;
+    int st = dtype == SHK_F64 ? upload<double>(A, P, (const double *)hval.data())
+                              : upload<float>(A, P, (const float *)hval.data());
+    if (st != SHK_OK) {
+        shk_csr_destroy(A);
+        return st;
+    }
+    *out = A;
+    return SHK_OK;
+}
+
+int shk_csr_destroy(shk_csr A)
+{
+    if (!A) return SHK_OK;
+    if (A->ctx) hipStreamSynchronize(A->ctx->stream);
+    hipFree(A->ptr);
+    hipFree(A->mid);
+    hipFree(A->ind);
+    hipFree(A->val);
+    hipFree(A->diag);
+    hipFree(A->blk_rows);
+    hipFree(A->map_up);
+    hipFree(A->map_lo);
+    hipFree(A->map_diag);
+    delete A;
+    return SHK_OK;
+}
+
+int shk_csr_set_values(shk_ctx ctx, shk_csr A, const void *val, int src_on_device)
+{
+    SHK_TRY(check_csr(ctx, A));
+    const size_t vs = dsize(A->dtype);
+    if (A->nnz_stored == 0) return SHK_OK;
+    const void *dsrc = val;
+    void *tmp = nullptr;
+    if (!src_on_device) {
+        SHK_HIP(hipMalloc(&tmp, vs * A->nnz_stored));
+        SHK_HIP(hipMemcpyAsync(tmp, val, vs * A->nnz_stored, hipMemcpyHostToDevice, ctx->stream));
+        dsrc = tmp;
+    }
+    int st = A->dtype == SHK_F64 ? set_values_dev<double>(A, (const double *)dsrc)
+                                 : set_values_dev<float>(A, (const float *)dsrc);
+    if (tmp) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(tmp);
+    }
+    return st;
+}
+
+int shk_csr_info(shk_csr A, int *m, int *n, int64_t *nnz_stored, int64_t *nnz_dev, int *dtype,
+                 int *storage)
+{
+    SHK_CHECK(A, SHK_ERR_ARG, "null matrix");
+    if (m) *m = A->m;
+    if (n) *n = A->n;
+    if (nnz_stored) *nnz_stored = A->nnz_stored;
+    if (nnz_dev) *nnz_dev = A->nnz_dev;
+    if (dtype) *dtype = A->dtype;
+    if (storage) *storage = A->storage;
+    return SHK_OK;
+}
+
+double shk_csr_spmv_bytes(shk_csr A, int beta_nonzero)
+{
+    if (!A) return 0.0;
+    const double V = (double)dsize(A->dtype);
+    return (4.0 + V) * (double)A->nnz_stored + 4.0 * (A->m + 1) + V * A->n + V * A->m +
+           (beta_nonzero ? V * A->m : 0.0);
+}
+
+int shk_spmv(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double beta, void *dy)
+{
+    return shk_spmv_out(ctx, A, alpha, dx, beta, dy, dy);
+}
+
+int shk_spmv_out(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double beta,
+                 const void *dy_in, void *dy_out)
+{
+    SHK_TRY(check_csr(ctx, A));
+    if (A->m == 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(dy_in, "y_in"));
+    SHK_TRY(check_device_ptr(dy_out, "y_out"));
+    if (A->dtype == SHK_F64) return spmv_typed<double>(A, 0, alpha, dx, beta, dy_in, dy_out);
+    return spmv_typed<float>(A, 0, (float)alpha, dx, (float)beta, dy_in, dy_out);
+}
+
+int shk_mlt(shk_ctx ctx, shk_csr A, const void *dx, void *dy)
+{
+    SHK_TRY(check_csr(ctx, A));
+    if (A->m == 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(dy, "y"));
+    if (A->dtype == SHK_F64) return spmv_typed<double>(A, 1, 1.0, dx, 0.0, nullptr, dy);
+    return spmv_typed<float>(A, 1, 1.0f, dx, 0.0f, nullptr, dy);
+}
+
+int shk_mlt_dot(shk_ctx ctx, shk_csr A, const void *dx, void *dy, double *result)
+{
+    SHK_TRY(check_csr(ctx, A));
+    SHK_CHECK(result, SHK_ERR_ARG, "null result");
+    if (A->m == 0) { *result = 0.0; return SHK_OK; }
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(dy, "y"));
+    if (A->dtype == SHK_F64) {
+        SpmvArgs<double> a{};
+        a.x = (const double *)dx; a.y_out = (double *)dy; a.alpha = 1.0; a.mlt = 1;
+        a.dot0 = kDotVec; a.u0 = (const double *)dx;
+        SHK_TRY(launch_spmv<double>(A, a, FinStore<double>{(double *)ctx->scalars}));
+        double h[2];
+        SHK_HIP(hipMemcpyAsync(h, ctx->scalars, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        SHK_HIP(hipStreamSynchronize(ctx->stream));
+        *result = h[0];
+    } else {
+        SpmvArgs<float> a{};
+        a.x = (const float *)dx; a.y_out = (float *)dy; a.alpha = 1.0f; a.mlt = 1;
+        a.dot0 = kDotVec; a.u0 = (const float *)dx;
+        SHK_TRY(launch_spmv<float>(A, a, FinStore<float>{(float *)ctx->scalars}));
+        float h[2];
+        SHK_HIP(hipMemcpyAsync(h, ctx->scalars, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        SHK_HIP(hipStreamSynchronize(ctx->stream));
+        *result = h[0];
+    }
+    return SHK_OK;
+}
+
+int shk_jacobi(shk_ctx ctx, shk_csr A, const void *dr, void *dz)
+{
+    SHK_TRY(check_csr(ctx, A));
+    if (A->m == 0) return SHK_OK;
+    if (A->dtype == SHK_F64)
+        hipLaunchKernelGGL(k_jacobi<double>, dim3(red_blocks(A->m)), dim3(kBlock), 0, ctx->stream,
+                           (int64_t)A->m, (const double *)dr, (const double *)A->diag, (double *)dz);
+    else
+        hipLaunchKernelGGL(k_jacobi<float>, dim3(red_blocks(A->m)), dim3(kBlock), 0, ctx->stream,
+                           (int64_t)A->m, (const float *)dr, (const float *)A->diag, (float *)dz);
+    SHK_HIP(hipGetLastError());
+    return SHK_OK;
+}
+
+int shk_time_spmv(shk_ctx ctx, shk_csr A, int mode, const void *dx, void *dy, int reps, double *avg_ms)
+{
+    SHK_TRY(check_csr(ctx, A));
+    SHK_CHECK(reps > 0 && avg_ms, SHK_ERR_ARG, "shk_time_spmv: bad reps/out");
+    SHK_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < reps; i++) {
+        int st = mode == 0 ? shk_mlt(ctx, A, dx, dy) : shk_spmv(ctx, A, 1.0, dx, 1.0, dy);
+        if (st != SHK_OK) return st;
+    }
+    SHK_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    SHK_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    SHK_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *avg_ms = ms / reps;
+    return SHK_OK;
+}
+
+}  // extern "C"

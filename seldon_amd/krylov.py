@@ -1,0 +1,360 @@
+"""Seldon-named Python front-end over the HIP library.
+
+Mirrors the names and argument meaning of Seldon's sparse Krylov API
+(computation/solver/iterative/Iterative.hxx, Cg.cxx, BiCgStab.cxx,
+Gmres.cxx, computation/basic_functions/Functions_Vector.cxx,
+Functions_MatVect.cxx) so the parity tests read like Seldon code:
+
+    A = Matrix(csr, storage="row")          # Matrix<double,General,RowSparse>
+    it = Iteration(max_iter, tol)            # Iteration<double>
+    code = Cg(A, x, b, Preconditioner_Base(), it)
+
+Vectors are torch tensors on the GPU (torch is used for device memory and
+streams only); every operation runs in libshk.so.  The calls are ordered on
+torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+_DT = {torch.float64: L.SHK_F64, torch.float32: L.SHK_F32}
+_NP = {torch.float64: np.float64, torch.float32: np.float32}
+_STORAGE = {"row": L.SHK_ROWSPARSE, "sym": L.SHK_ROWSYMSPARSE, "array": L.SHK_ARRAYROWSPARSE}
+
+
+class Context:
+    """One device, one stream (the current torch stream), its workspace."""
+
+    _default = None
+
+    def __init__(self, device: int = 0):
+        lib = L.load()
+        h = ctypes.c_void_p()
+        L.check(lib.shk_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        self.bind_stream()
+
+    def bind_stream(self, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        L.check(L.load().shk_ctx_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)))
+
+    def synchronize(self):
+        L.check(L.load().shk_ctx_synchronize(self.h))
+
+    def info(self):
+        cus, xcds = ctypes.c_int(), ctypes.c_int()
+        arch = ctypes.create_string_buffer(64)
+        L.check(L.load().shk_ctx_device_info(self.h, ctypes.byref(cus), ctypes.byref(xcds), arch, 64))
+        return dict(cus=cus.value, xcds=xcds.value, arch=arch.value.decode())
+
+    def close(self):
+        if self.h:
+            L.load().shk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context(torch.cuda.current_device())
+        return cls._default
+
+
+def _ptr(t: torch.Tensor):
+    assert t.is_cuda and t.is_contiguous(), "device vectors must be contiguous CUDA tensors"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Matrix:
+    """A sparse matrix resident on the GPU.
+
+    storage: "row" (RowSparse), "sym" (RowSymSparse: upper triangle given),
+    "array" (ArrayRowSparse, rows laid end to end)."""
+
+    def __init__(self, csr, storage: str | None = None, dtype=torch.float64, ctx: Context | None = None):
+        self.ctx = ctx or Context.default()
+        if storage is None:
+            storage = "sym" if getattr(csr, "sym", False) else "row"
+        self.storage = storage
+        self.dtype = dtype
+        self.m, self.n = int(csr.m), int(csr.n)
+        ptr = np.ascontiguousarray(csr.ptr, np.int32)
+        ind = np.ascontiguousarray(csr.ind, np.int32)
+        val = np.ascontiguousarray(np.asarray(csr.val), _NP[dtype])
+        self.nnz = int(ptr[-1])
+        h = ctypes.c_void_p()
+        L.check(L.load().shk_csr_create(self.ctx.h, _STORAGE[storage], _DT[dtype], self.m, self.n, self.nnz,
+                                        ptr.ctypes.data_as(ctypes.c_void_p), ind.ctypes.data_as(ctypes.c_void_p),
+                                        val.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(h)))
+        self.h = h
+
+    def GetM(self):
+        return self.m
+
+    def GetN(self):
+        return self.n
+
+    def set_values(self, val):
+        if isinstance(val, torch.Tensor) and val.is_cuda:
+            assert val.dtype == self.dtype and val.numel() == self.nnz
+            L.check(L.load().shk_csr_set_values(self.ctx.h, self.h, _ptr(val.contiguous()), 1))
+        else:
+            v = np.ascontiguousarray(np.asarray(val), _NP[self.dtype])
+            L.check(L.load().shk_csr_set_values(self.ctx.h, self.h, v.ctypes.data_as(ctypes.c_void_p), 0))
+
+    def spmv_bytes(self, beta_nonzero=False):
+        return L.load().shk_csr_spmv_bytes(self.h, int(beta_nonzero))
+
+    def device_nnz(self):
+        m, n, a, b, d, s = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(),
+                            ctypes.c_int(), ctypes.c_int())
+        L.check(L.load().shk_csr_info(self.h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(a),
+                                      ctypes.byref(b), ctypes.byref(d), ctypes.byref(s)))
+        return b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().shk_csr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ctx_of(*objs):
+    for o in objs:
+        if isinstance(o, Matrix):
+            return o.ctx
+    return Context.default()
+
+
+# ----------------------------------------------------------- MltAdd / Mlt --
+
+def MltAdd(alpha, A: Matrix, X: torch.Tensor, beta, Y: torch.Tensor):
+    """Y <- beta Y + alpha A X (Functions_MatVect.cxx:398-431, :526-564;
+    Functions_MatrixArray.cxx:534-575)."""
+    L.check(L.load().shk_spmv(A.ctx.h, A.h, float(alpha), _ptr(X), float(beta), _ptr(Y)))
+
+
+def Mlt(*args):
+    """Mlt(A, X, Y): Y = A X (Functions_MatVect.cxx:72-81);
+    Mlt(alpha, X): X *= alpha (Functions_Vector.cxx:69-74)."""
+    if len(args) == 3:
+        A, X, Y = args
+        L.check(L.load().shk_mlt(A.ctx.h, A.h, _ptr(X), _ptr(Y)))
+    else:
+        alpha, X = args
+        L.check(L.load().shk_scal(Context.default().h, _DT[X.dtype], X.numel(), float(alpha), _ptr(X)))
+
+
+def MltDot(A: Matrix, X, Y) -> float:
+    r = ctypes.c_double()
+    L.check(L.load().shk_mlt_dot(A.ctx.h, A.h, _ptr(X), _ptr(Y), ctypes.byref(r)))
+    return r.value
+
+
+def DotProd(X, Y) -> float:
+    r = ctypes.c_double()
+    L.check(L.load().shk_dot(Context.default().h, _DT[X.dtype], X.numel(), _ptr(X), _ptr(Y), ctypes.byref(r)))
+    return r.value
+
+
+DotProdConj = DotProd
+
+
+def Norm2(X) -> float:
+    r = ctypes.c_double()
+    L.check(L.load().shk_nrm2(Context.default().h, _DT[X.dtype], X.numel(), _ptr(X), ctypes.byref(r)))
+    return r.value
+
+
+def Add(alpha, X, Y):
+    L.check(L.load().shk_axpy(Context.default().h, _DT[X.dtype], X.numel(), float(alpha), _ptr(X), _ptr(Y)))
+
+
+def Copy(X, Y):
+    L.check(L.load().shk_copy(Context.default().h, _DT[X.dtype], X.numel(), _ptr(X), _ptr(Y)))
+
+
+def Fill(X, value):
+    L.check(L.load().shk_fill(Context.default().h, _DT[X.dtype], X.numel(), float(value), _ptr(X)))
+
+
+def AddDot(alpha, X, Y, Z) -> float:
+    r = ctypes.c_double()
+    L.check(L.load().shk_axpy_dot(Context.default().h, _DT[X.dtype], X.numel(), float(alpha), _ptr(X), _ptr(Y),
+                                  _ptr(Z), ctypes.byref(r)))
+    return r.value
+
+
+# ------------------------------------------------------- preconditioners --
+
+class Preconditioner_Base:
+    """Identity preconditioner (Iterative.cxx:56-77)."""
+    kind = L.SHK_PREC_IDENTITY
+
+    def Solve(self, A: Matrix, r, z):
+        Copy(r, z)
+
+    TransSolve = Solve
+
+
+class JacobiPreconditioner(Preconditioner_Base):
+    """z_i = r_i / a_ii (new; division form of SOR's /a_jj,
+    matrix_sparse/Relaxation_MatVect.cxx:79)."""
+    kind = L.SHK_PREC_JACOBI
+
+    def Solve(self, A: Matrix, r, z):
+        L.check(L.load().shk_jacobi(A.ctx.h, A.h, _ptr(r), _ptr(z)))
+
+    TransSolve = Solve
+
+
+# --------------------------------------------------------------- Iteration --
+
+@dataclass
+class Iteration:
+    """Iteration<double> (Iterative.hxx:47-108, Iterative.cxx:81-354)."""
+    max_iter: int = 100
+    tolerance: float = 1e-6
+    nb_iter: int = 0
+    facteur_reste: float = 0.0
+    error_code: int = 0
+    fail_convergence: bool = False
+    print_level: int = 1
+    init_guess_null: bool = True
+    parameter_restart: int = 10
+    history: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    history_it: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    hist_len: int = 0
+
+    def __init__(self, max_iter: int = 100, tol: float = 1e-6):
+        self.max_iter, self.tolerance = int(max_iter), float(tol)
+        self.nb_iter, self.facteur_reste, self.error_code = 0, 0.0, 0
+        self.fail_convergence, self.print_level = False, 1
+        self.init_guess_null, self.parameter_restart = True, 10
+        self.history = np.zeros(0)
+        self.history_it = np.zeros(0, np.int32)
+        self.hist_len = 0
+
+    def SetRestart(self, m):
+        self.parameter_restart = int(m)
+
+    def GetRestart(self):
+        return self.parameter_restart
+
+    def SetTolerance(self, t):
+        self.tolerance = float(t)
+
+    def SetMaxNumberIteration(self, m):
+        self.max_iter = int(m)
+
+    def SetNumberIteration(self, nb):
+        self.nb_iter = int(nb)
+
+    def GetNumberIteration(self):
+        return self.nb_iter
+
+    def SetInitGuess(self, null: bool):
+        self.init_guess_null = bool(null)
+
+    def IsInitGuess_Null(self):
+        return self.init_guess_null
+
+    def ShowMessages(self):
+        self.print_level = 1
+
+    def ShowFullHistory(self):
+        self.print_level = 6
+
+    def HideMessages(self):
+        self.print_level = 0
+
+    def ErrorCode(self):
+        if self.nb_iter >= self.max_iter:
+            return -2
+        if self.fail_convergence:
+            return self.error_code
+        return 0
+
+    def _to_c(self, cap):
+        self._hist = np.zeros(max(cap, 1), np.float64)
+        self._hist_it = np.zeros(max(cap, 1), np.int32)
+        c = L.ShkIteration()
+        c.tolerance, c.facteur_reste = self.tolerance, self.facteur_reste
+        c.max_iter, c.nb_iter = self.max_iter, self.nb_iter
+        c.error_code, c.fail_convergence = self.error_code, int(self.fail_convergence)
+        c.print_level, c.init_guess_null = self.print_level, int(self.init_guess_null)
+        c.parameter_restart = self.parameter_restart
+        c.hist_cap = cap
+        c.hist = self._hist.ctypes.data_as(ctypes.c_void_p)
+        c.hist_it = self._hist_it.ctypes.data_as(ctypes.c_void_p)
+        return c
+
+    def _from_c(self, c):
+        self.facteur_reste, self.nb_iter = c.facteur_reste, c.nb_iter
+        self.error_code, self.fail_convergence = c.error_code, bool(c.fail_convergence)
+        self.hist_len = c.hist_len
+        n = min(c.hist_len, c.hist_cap)
+        self.history = self._hist[:n].copy()
+        self.history_it = self._hist_it[:n].copy()
+        if self.print_level >= 1:
+            for k, (it, res) in enumerate(zip(self.history_it, self.history)):
+                if (self.print_level >= 6) or it % 100 == 0:
+                    print(f"Residu at iteration number {it}  {res:.17g}")
+
+
+def _solve(fn, A: Matrix, x, b, M, it: Iteration, hist_cap=None):
+    assert x.dtype == A.dtype and b.dtype == A.dtype
+    cap = hist_cap if hist_cap is not None else 2 * it.max_iter + 4 * (it.max_iter // max(it.parameter_restart, 1)) + 16
+    c = it._to_c(cap)
+    code = ctypes.c_int()
+    L.check(fn(A.ctx.h, A.h, _ptr(x), _ptr(b), M.kind, ctypes.byref(c), ctypes.byref(code)))
+    it._from_c(c)
+    return code.value
+
+
+def Cg(A, x, b, M, it):
+    """Cg (computation/solver/iterative/Cg.cxx:45-116) on the GPU."""
+    return _solve(L.load().shk_cg, A, x, b, M, it)
+
+
+def BiCgStab(A, x, b, M, it):
+    """BiCgStab (computation/solver/iterative/BiCgStab.cxx:44-145) on the GPU."""
+    return _solve(L.load().shk_bicgstab, A, x, b, M, it)
+
+
+def Gmres(A, x, b, M, it):
+    """Gmres(m) (computation/solver/iterative/Gmres.cxx:45-182) on the GPU."""
+    return _solve(L.load().shk_gmres, A, x, b, M, it)
+
+
+def solve_ir(A64: Matrix, A32: Matrix, x, b, M, tol=1e-10, inner_restart=50, inner_tol=1e-6,
+             inner_max_iter=2000, max_refinements=50, inner_method="gmres"):
+    """FP32 inner Krylov + FP64 iterative refinement (no Seldon counterpart)."""
+    p = L.ShkIrParams()
+    p.tolerance, p.max_refinements = tol, max_refinements
+    p.inner_restart, p.inner_max_iter, p.inner_tolerance = inner_restart, inner_max_iter, inner_tol
+    p.inner_method = 0 if inner_method == "gmres" else 1
+    hist = np.zeros(max_refinements + 1)
+    p.relres_hist = hist.ctypes.data_as(ctypes.c_void_p)
+    code = ctypes.c_int()
+    L.check(L.load().shk_solve_ir(A64.ctx.h, A64.h, A32.h, _ptr(x), _ptr(b), M.kind, ctypes.byref(p),
+                                  ctypes.byref(code)))
+    return dict(code=code.value, refinements=p.refinements, inner_iters=p.total_inner_iter,
+                relres=p.final_relres, relres_hist=hist[:p.refinements + 1].copy())

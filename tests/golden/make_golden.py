@@ -1,0 +1,194 @@
+"""Write the golden fixtures under tests/golden/ from the real Seldon build.
+
+Runs the drivers that oracle/Makefile compiles from the reference headers
+(oracle/_ref/seldon_ref_generic: Seldon's generic templates;
+oracle/_ref/seldon_ref_blas: the same with SELDON_WITH_BLAS linked to the
+image's MKL, needed by Gmres) on seeded inputs from seldon_amd.workloads and
+stores their outputs as small .npz files.  The inputs themselves are not
+stored: each fixture names its generator call, and a checksum of the
+generated arrays guards against generator drift.
+
+Usage:  python tests/golden/make_golden.py      (needs /root/reference)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as orc  # noqa: E402
+from seldon_amd import workloads as wl  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def csr_digest(A) -> str:
+    h = hashlib.sha256()
+    for a in (A.ptr, A.ind, A.val):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()[:16]
+
+
+def build_case(spec):
+    """Regenerate (A, b) from a fixture spec (also used by the tests)."""
+    kind = spec["kind"]
+    if kind == "laplace2d":
+        A = wl.laplace2d(spec["grid"])
+    elif kind == "convdiff3d":
+        A = wl.convdiff3d(spec["grid"])
+    elif kind == "powerlaw":
+        A = wl.powerlaw(spec["n"], spec["seed"], half_width=spec["half_width"])
+    elif kind == "diffusion3d":
+        A = wl.diffusion3d(spec["grid"], wl.normal(spec["grid"] ** 3, spec["seed"], 0.0, 0.5))
+    elif kind == "aniso2d":
+        A = wl.aniso2d(spec["grid"], spec["grid"], spec["eps"])
+    elif kind == "diag":
+        m = spec["n"]
+        vals = np.array(spec["values"], np.float64)
+        A = wl.CSR(m, m, np.arange(m + 1, dtype=np.int32), np.arange(m, dtype=np.int32),
+                   vals[np.arange(m) % vals.size].copy())
+    else:
+        raise ValueError(kind)
+    xt = wl.uniform(A.m, spec.get("xseed", wl.SEED_XTRUE))
+    b = wl.apply(A, xt)
+    if spec.get("zero_rhs"):
+        b = np.zeros_like(b)
+    if spec.get("storage") == "sym":
+        A = wl.upper(A)
+    return A, b
+
+
+def run_solve(tmp, spec, kind):
+    A, b = build_case(spec)
+    orc.write_seldon_csr(f"{tmp}/A.bin", A)
+    orc.write_seldon_vec(f"{tmp}/b.bin", b)
+    x0 = "-"
+    if spec.get("x0seed") is not None:
+        x0v = wl.uniform(A.m, spec["x0seed"])
+        orc.write_seldon_vec(f"{tmp}/x0.bin", x0v)
+        x0 = f"{tmp}/x0.bin"
+    out = orc.run_ref(["solve", spec["method"], spec.get("storage", "row"), spec["precond"],
+                       f"{tmp}/A.bin", f"{tmp}/b.bin", x0, spec["tol"], spec["max_iter"],
+                       spec.get("restart", 10), f"{tmp}/x.bin"], kind=kind)
+    hist, its, code, nb = orc.parse_ref_history(out)
+    x = orc.read_seldon_vec(f"{tmp}/x.bin")
+    return A, hist, its, code, nb, x
+
+
+SOLVES = {
+    "cg_laplace64": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-10, max_iter=2000),
+    "cg_laplace256": dict(kind="laplace2d", grid=256, method="cg", precond="id", tol=1e-10, max_iter=5000),
+    "cg_laplace64_sym": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-10,
+                             max_iter=2000, storage="sym"),
+    "cg_laplace64_array": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-10,
+                               max_iter=2000, storage="array"),
+    "cg_laplace64_x0": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-10,
+                            max_iter=2000, x0seed=99),
+    "cg_laplace64_maxit": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-10, max_iter=7),
+    "cg_laplace64_zero_rhs": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-10,
+                                  max_iter=100, zero_rhs=True),
+    "cg_diffusion16_jacobi": dict(kind="diffusion3d", grid=16, seed=31, method="cg", precond="jacobi",
+                                  tol=1e-12, max_iter=2000),
+    "cg_aniso64_eps30": dict(kind="aniso2d", grid=64, eps=30.0, method="cg", precond="id", tol=1e-8,
+                             max_iter=5000),
+    "cg_diag5": dict(kind="diag", n=500, values=[1.0, 2.0, 3.5, 7.0, 11.0], method="cg", precond="id",
+                     tol=1e-12, max_iter=100),
+    "bicgstab_convdiff16_jacobi": dict(kind="convdiff3d", grid=16, method="bicgstab", precond="jacobi",
+                                       tol=1e-10, max_iter=2000, xseed=wl.SEED_XTRUE + 1),
+    "bicgstab_convdiff24_jacobi": dict(kind="convdiff3d", grid=24, method="bicgstab", precond="jacobi",
+                                       tol=1e-10, max_iter=2000, xseed=wl.SEED_XTRUE + 1),
+    "bicgstab_laplace64_id": dict(kind="laplace2d", grid=64, method="bicgstab", precond="id", tol=1e-10,
+                                  max_iter=2000),
+    "gmres30_convdiff16_jacobi": dict(kind="convdiff3d", grid=16, method="gmres", precond="jacobi",
+                                      tol=1e-10, max_iter=2000, restart=30, xseed=wl.SEED_XTRUE + 1),
+    "gmres30_convdiff24_jacobi": dict(kind="convdiff3d", grid=24, method="gmres", precond="jacobi",
+                                      tol=1e-10, max_iter=2000, restart=30, xseed=wl.SEED_XTRUE + 1),
+    "gmres50_powerlaw20k_jacobi": dict(kind="powerlaw", n=20000, seed=2024, half_width=2000,
+                                       method="gmres", precond="jacobi", tol=1e-10, max_iter=2000,
+                                       restart=50, xseed=wl.SEED_XTRUE + 2),
+    "gmres10_laplace32_id": dict(kind="laplace2d", grid=32, method="gmres", precond="id", tol=1e-10,
+                                 max_iter=3000, restart=10),
+    "gmres5_diag_identity": dict(kind="diag", n=300, values=[1.0], method="gmres", precond="id",
+                                 tol=1e-10, max_iter=50, restart=5),
+}
+
+
+def make_solves():
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, spec in SOLVES.items():
+            res = {}
+            kinds = ["blas"] if spec["method"] == "gmres" else ["generic", "blas"]
+            for kind in kinds:
+                A, hist, its, code, nb, x = run_solve(tmp, spec, kind)
+                res[kind] = (hist, its, code, nb, x)
+            primary = res["blas"] if spec["method"] == "gmres" else res["generic"]
+            hist, its, code, nb, x = primary
+            other = res["blas"]
+            if spec["method"] == "gmres":
+                # envelope for Gmres: the generic-template restatement against
+                # the MKL build (the only build of Seldon's Gmres there is)
+                Ar, br = build_case(spec)
+                x0 = None if spec.get("x0seed") is None else wl.uniform(Ar.m, spec["x0seed"])
+                g = orc.solve("gmres", Ar, br, precond=spec["precond"], tol=spec["tol"],
+                              max_iter=spec["max_iter"], restart=spec.get("restart", 10), x0=x0,
+                              storage=spec.get("storage"))
+                other = (g.hist, g.hist_it, g.code, g.nb_iter, g.x)
+            n = min(len(other[0]), len(hist))
+            spread = np.zeros(len(hist))
+            if n:
+                spread[:n] = np.abs(other[0][:n] - hist[:n]) / np.maximum(np.abs(hist[:n]), 1e-300)
+            np.savez_compressed(
+                os.path.join(OUT, f"solve_{name}.npz"),
+                spec=json.dumps(spec), digest=csr_digest(A),
+                hist=hist, hist_it=its, code=code, nb_iter=nb,
+                x_sum=np.sum(x), x_norm=np.linalg.norm(x), x_head=x[:16],
+                build="blas" if spec["method"] == "gmres" else "generic",
+                other_hist=other[0], other_code=other[2], other_nb_iter=other[3],
+                spread=spread)
+            print(f"{name}: code {code} its {nb} hist {len(hist)} "
+                  f"(MKL build: code {other[2]} its {other[3]}, max spread {spread.max() if spread.size else 0:.2e})")
+
+
+SPMV_ALPHA_BETA = [(1.0, 0.0), (0.3, -1.0), (-1.0, 1.0), (0.0, 0.3), (1.0, 1.0), (2.5, 0.0)]
+
+
+def make_spmv():
+    A = wl.powerlaw(300, 5, kmin=2, kmax=40, half_width=40)
+    U = wl.upper(A)
+    x = wl.uniform(A.m, 11)
+    y = wl.uniform(A.m, 12)
+    y[3] = np.nan  # beta*NaN: RowSparse/RowSym propagate it, ArrayRowSparse with beta=0 does not
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for st, M in (("row", A), ("sym", U), ("array", A)):
+            for dt, c in ((np.float64, "d"), (np.float32, "f")):
+                orc.write_seldon_csr(f"{tmp}/M.bin", M, dt)
+                orc.write_seldon_vec(f"{tmp}/x.bin", x.astype(dt))
+                orc.write_seldon_vec(f"{tmp}/y.bin", y.astype(dt))
+                for k, (al, be) in enumerate(SPMV_ALPHA_BETA):
+                    orc.run_ref(["spmv", st, c, f"{tmp}/M.bin", f"{tmp}/x.bin", f"{tmp}/y.bin",
+                                 repr(al), repr(be), f"{tmp}/o.bin"])
+                    out[f"{st}_{c}_{k}"] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
+                orc.run_ref(["mlt", st, c, f"{tmp}/M.bin", f"{tmp}/x.bin", f"{tmp}/o.bin"])
+                out[f"{st}_{c}_mlt"] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
+    np.savez_compressed(os.path.join(OUT, "spmv_powerlaw300.npz"),
+                        spec=json.dumps(dict(n=300, seed=5, kmin=2, kmax=40, half_width=40,
+                                             xseed=11, yseed=12, nan_at=3)),
+                        alpha_beta=np.array(SPMV_ALPHA_BETA), digest=csr_digest(A), **out)
+    print("spmv fixtures:", len(out))
+
+
+if __name__ == "__main__":
+    if not orc.ref_available("generic") or not orc.ref_available("blas"):
+        orc.build()
+        import subprocess
+        subprocess.run(["make", "-s", "-C", orc.HERE, "ref"], check=True)
+    make_spmv()
+    make_solves()

@@ -1,0 +1,121 @@
+"""GPU parity of the SpMV path (MltAdd / Mlt on RowSparse, RowSymSparse,
+ArrayRowSparse) through the C ABI: bitwise against the real-Seldon fixtures
+and against the oracle at larger sizes (row sums are sequential in stored
+order on the GPU too, with no FMA contraction)."""
+import numpy as np
+import pytest
+import torch
+
+from seldon_amd import workloads as wl
+from tests.conftest import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+DT = {"d": (torch.float64, np.float64), "f": (torch.float32, np.float32)}
+
+
+def _dev(a, npdt):
+    return torch.tensor(np.ascontiguousarray(a, npdt), device="cuda")
+
+
+def _same(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_spmv_golden_fixture(gpu, st, c):
+    from seldon_amd import krylov as K
+    fx = load_fixture("spmv_powerlaw300.npz")
+    sp = fx["spec"]
+    A = wl.powerlaw(sp["n"], sp["seed"], kmin=sp["kmin"], kmax=sp["kmax"], half_width=sp["half_width"])
+    M = wl.upper(A) if st == "sym" else A
+    tdt, ndt = DT[c]
+    x = wl.uniform(A.m, sp["xseed"]); y = wl.uniform(A.m, sp["yseed"]); y[sp["nan_at"]] = np.nan
+    Ag = K.Matrix(M, storage=st, dtype=tdt)
+    xg = _dev(x, ndt)
+    for k, (al, be) in enumerate(fx["alpha_beta"]):
+        yg = _dev(y, ndt)
+        K.MltAdd(al, Ag, xg, be, yg)
+        got = yg.cpu().numpy()
+        assert _same(got, fx[f"{st}_{c}_{k}"]), (st, c, al, be)
+    yg = torch.full((A.m,), 7.0, dtype=tdt, device="cuda")
+    K.Mlt(Ag, xg, yg)
+    assert _same(yg.cpu().numpy(), fx[f"{st}_{c}_mlt"])
+
+
+def _cases():
+    yield "powerlaw20k", wl.powerlaw(20000, 3, half_width=3000)
+    yield "convdiff24", wl.convdiff3d(24)
+    yield "laplace64", wl.laplace2d(64)
+    # rows longer than the LDS budget (2048 / 4096 entries) and empty rows
+    A = wl.powerlaw(12000, 9, kmin=1, kmax=6000, expo=0.9, half_width=5000)
+    yield "longrows", A
+    ptr = A.ptr.copy()
+    lens = np.diff(ptr)
+    lens[::7] = 0
+    ptr2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    keep = np.concatenate([np.arange(A.ptr[i], A.ptr[i] + lens[i]) for i in range(A.m)])
+    yield "emptyrows", wl.CSR(A.m, A.n, ptr2, A.ind[keep].copy(), A.val[keep].copy())
+
+
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_spmv_bitwise_vs_oracle(gpu, st, c):
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    tdt, ndt = DT[c]
+    for name, A in _cases():
+        M = wl.upper(A) if st == "sym" else A
+        x = wl.uniform(A.n, 21)
+        y = wl.uniform(A.m, 22)
+        Ag = K.Matrix(M, storage=st, dtype=tdt)
+        xg = _dev(x, ndt)
+        for al, be in ((1.0, 0.0), (-1.0, 1.0), (0.7, -0.3)):
+            ref = orc.mltadd(ndt(al), M, x.astype(ndt), ndt(be), y.astype(ndt), storage=st, dtype=ndt)
+            yg = _dev(y, ndt)
+            K.MltAdd(al, Ag, xg, be, yg)
+            assert _same(yg.cpu().numpy(), ref), (name, st, c, al, be)
+        ref = orc.mlt(M, x.astype(ndt), storage=st, dtype=ndt)
+        yg = torch.empty(A.m, dtype=tdt, device="cuda")
+        K.Mlt(Ag, xg, yg)
+        assert _same(yg.cpu().numpy(), ref), (name, st, c, "mlt")
+
+
+def test_spmv_closed_form_laplacian(gpu):
+    """5-point Laplacian eigenpairs: A u_jk = lambda_jk u_jk with
+    lambda = 4 - 2cos(j pi/(g+1)) - 2cos(k pi/(g+1)), u = sin x sin."""
+    from seldon_amd import krylov as K
+    g = 64
+    A = wl.laplace2d(g)
+    i = np.arange(1, g + 1)
+    for (j, k) in ((1, 1), (3, 7), (g, g - 1)):
+        u = np.outer(np.sin(k * i * np.pi / (g + 1)), np.sin(j * i * np.pi / (g + 1))).ravel()
+        lam = 4 - 2 * np.cos(j * np.pi / (g + 1)) - 2 * np.cos(k * np.pi / (g + 1))
+        Ag = K.Matrix(A)
+        ug = _dev(u, np.float64)
+        yg = torch.empty_like(ug)
+        K.Mlt(Ag, ug, yg)
+        assert np.allclose(yg.cpu().numpy(), lam * u, rtol=0, atol=1e-12)
+
+
+def test_spmv_empty_matrix(gpu):
+    from seldon_amd import krylov as K
+    A = wl.CSR(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    Ag = K.Matrix(A)
+    x = torch.zeros(1, dtype=torch.float64, device="cuda")
+    K.Mlt(Ag, x, x)
+
+
+def test_mlt_dot_fused(gpu):
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    A = wl.convdiff3d(20)
+    x = wl.uniform(A.m, 5)
+    Ag = K.Matrix(A)
+    xg = _dev(x, np.float64)
+    yg = torch.empty_like(xg)
+    d = K.MltDot(Ag, xg, yg)
+    y = orc.mlt(A, x)
+    assert np.array_equal(yg.cpu().numpy(), y)
+    assert abs(d - orc.dot(x, y)) <= 1e-13 * abs(orc.dot(np.abs(x), np.abs(y)))
