@@ -90,6 +90,7 @@ struct shk_csr_s {
     void *val = nullptr;      // [nnz_dev + pad]
     void *diag = nullptr;     // [m] a_ii for Jacobi (0 when absent)
     int *blk_rows = nullptr;  // [nblk+1] row-block boundaries
+    int *blk_info = nullptr;  // [4*nblk] {rs, re, ptr[rs], ptr[re]} per block
     int nblk = 0;             // logical row blocks
     int nblk_launch = 0;      // launched blocks (padded to a multiple of 8)
     int xcd_q = 0;            // logical blocks per XCD

@@ -32,6 +32,7 @@ struct SpmvArgs {
     int m;
     int nblk, nblk_launch, xcd_q;
     const int *blk_rows;
+    const int4 *blk_info;  // per block {first row, end row, first entry, end entry}
     const int *ptr;
     const int *mid;
     const uint32_t *ind;
@@ -50,7 +51,7 @@ struct SpmvArgs {
 };
 
 template <typename T, int BUDGET>
-struct SpmvSmem {
+struct alignas(16) SpmvSmem {
     T prod[BUDGET];
 };
 
@@ -99,6 +100,91 @@ __device__ __forceinline__ T spmv_product(const SpmvArgs<T> &a, int j)
     return (c & kLowerFlag) ? (a.alpha * v) * xv : v * xv;
 }
 
+// Vector widths of the coalesced streaming loads: lane t of load k reads
+// elements [k*256*V + t*V, +V) of the block's range -- 512 B (ind) and 1 KiB
+// (values) contiguous per wave instruction.
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<double> {
+    static constexpr int V = 2;
+    typedef unsigned int IV __attribute__((ext_vector_type(2)));
+    typedef double FV __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<float> {
+    static constexpr int V = 4;
+    typedef unsigned int IV __attribute__((ext_vector_type(4)));
+    typedef float FV __attribute__((ext_vector_type(4)));
+};
+
+// Phase 1, branch-free so that every load of the block is in flight at once:
+// the streaming loads of ind/val (non-temporal, so they do not evict x from
+// L2) read the whole budget window [js0, js0 + BUDGET) -- the arrays are padded
+// by BUDGET entries -- then every x gather is issued with its index clamped to
+// 0 outside [js, je), then the products go to LDS.  prod[e - js0] holds the
+// product of entry e (0 outside the block).
+template <typename T, int BUDGET>
+__device__ __forceinline__ void spmv_stage_products(const SpmvArgs<T> &a, int js0, int js, int je, T *prod)
+{
+    typedef VecOf<T> VT;
+    constexpr int V = VT::V;
+    constexpr int K = BUDGET / (kBlock * V);
+    typename VT::IV iv[K];
+    typename VT::FV fv[K];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = js0 + (k * kBlock + t) * V;
+        iv[k] = __builtin_nontemporal_load((const typename VT::IV *)(a.ind + e));
+        fv[k] = __builtin_nontemporal_load((const typename VT::FV *)(a.val + e));
+    }
+    T xv[K][V];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = js0 + (k * kBlock + t) * V;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const bool in = (e + v < je) & (e + v >= js);
+            const unsigned col = in ? (iv[k][v] & ~kLowerFlag) : 0u;
+            xv[k][v] = a.x[col];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = js0 + (k * kBlock + t) * V;
+        typename VT::FV p;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const T val = fv[k][v];
+            // direct terms (transposed symmetric part, ArrayRowSparse)
+            // multiply alpha into the value first: `alpha * data[j] * X(i)`
+            // (Functions_MatVect.cxx:563, Functions_MatrixArray.cxx:571)
+            const T pr = (iv[k][v] & kLowerFlag) ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
+            const bool in = (e + v < je) & (e + v >= js);
+            p[v] = in ? pr : (T)0;
+        }
+        *(typename VT::FV *)(prod + (e - js0)) = p;
+    }
+}
+
+// Sequential sum of prod[q0, q1) onto acc, four products per step so the LDS
+// reads overlap; the additions stay strictly left to right.
+template <typename T>
+__device__ __forceinline__ T seq_sum(T acc, const T *prod, int q0, int q1)
+{
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {
+        const T b0 = prod[q], b1 = prod[q + 1], b2 = prod[q + 2], b3 = prod[q + 3];
+        acc += b0;
+        acc += b1;
+        acc += b2;
+        acc += b3;
+    }
+    for (; q < q1; q++) acc += prod[q];
+    return acc;
+}
+
 struct FinNone {
     template <typename T>
     __device__ void operator()(const T (&)[2]) const {}
@@ -123,15 +209,33 @@ __global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<T> a, Fin fin)
     const int lb = xcd_logical_block(a.xcd_q);
     T c[2] = {(T)0, (T)0};
     if (lb < a.nblk) {
-        const int rs = a.blk_rows[lb], re = a.blk_rows[lb + 1];
-        const int js = a.ptr[rs], je = a.ptr[re];
-        if (je - js <= BUDGET) {
-            for (int j = js + (int)threadIdx.x; j < je; j += kBlock) sm.prod[j - js] = spmv_product(a, j);
+        const int4 bi = a.blk_info[lb];
+        const int rs = bi.x, re = bi.y, js = bi.z, je = bi.w;
+        constexpr int V = VecOf<T>::V;
+        const int js0 = js & ~(V - 1);  // the planner keeps je - js0 <= BUDGET
+        if (je - js0 <= BUDGET) {
+            // the planner puts at most kBlock rows in a block: thread t owns
+            // row rs + t; its metadata and y_in are fetched before phase 1
+            const int r = rs + (int)threadIdx.x;
+            const bool own = r < re;
+            const int rr = own ? r : rs;
+            int p0 = a.ptr[rr], p1 = a.ptr[rr + 1];
+            int pm = MODE == kSym ? a.mid[rr] : (MODE == kCsr ? p1 : p0);
+            T yin = (T)0;
+            if (!a.mlt) yin = a.y_in[rr];
+            spmv_stage_products<T, BUDGET>(a, js0, js, je, sm.prod);
             __syncthreads();
-            for (int r = rs + (int)threadIdx.x; r < re; r += kBlock) {
-                const int p0 = a.ptr[r] - js, p1 = a.ptr[r + 1] - js;
-                const int pm = MODE == kSym ? a.mid[r] - js : (MODE == kCsr ? p1 : p0);
-                T y = spmv_row_result<T, MODE>(a, r, sm.prod, p0, pm, p1);
+            if (own) {
+                p0 -= js0; p1 -= js0; pm -= js0;
+                T y;
+                if (a.mlt) y = (T)0;
+                else if (MODE == kArray) y = (a.beta == (T)0) ? (T)0 : yin * a.beta;
+                else y = yin * a.beta;
+                if (MODE != kArray) {
+                    const T temp = seq_sum((T)0, sm.prod, p0, pm);
+                    y += a.alpha * temp;
+                }
+                y = seq_sum(y, sm.prod, pm, p1);
                 spmv_epilogue<T, MODE>(a, r, y, c);
             }
         } else {
@@ -190,6 +294,7 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     a.nblk_launch = A->nblk_launch;
     a.xcd_q = A->xcd_q;
     a.blk_rows = A->blk_rows;
+    a.blk_info = (const int4 *)A->blk_info;
     a.ptr = A->ptr;
     a.mid = A->mid;
     a.ind = A->ind;

@@ -8,16 +8,18 @@
 namespace shk {
 
 // ------------------------------------------------------------ planning ----
-// Greedy row blocks: at most kBlock rows and `budget` entries per block; a row
-// longer than the budget is a block by itself.
-static void plan_blocks(int m, const std::vector<int> &ptr, int budget, std::vector<int> &blk)
+// Greedy row blocks: at most kBlock rows and `budget` entries (counted from
+// the block's first entry rounded down to the load vector width) per block;
+// a row longer than that is a block by itself.
+static void plan_blocks(int m, const std::vector<int> &ptr, int budget, int align, std::vector<int> &blk)
 {
     blk.clear();
     blk.push_back(0);
     int r = 0;
     while (r < m) {
         int start = r;
-        int64_t base = ptr[r];
+        // the kernel stages from the vector-aligned start of the block
+        int64_t base = ptr[r] & ~(align - 1);
         if (ptr[r + 1] - base > budget) {
             r++;
         } else {
@@ -55,7 +57,7 @@ struct HostPlan {
 
 // Build the device layout (expanded for RowSymSparse) from the caller's CSR.
 static int build_plan(int storage, int m, int n, int64_t nnz, const int *ptr, const int *ind,
-                      int budget, HostPlan &P)
+                      int budget, int align, HostPlan &P)
 {
     SHK_CHECK(ptr[0] == 0 && ptr[m] == nnz, SHK_ERR_ARG, "csr: ptr[0] must be 0 and ptr[m] == nnz");
     for (int i = 0; i < m; i++)
@@ -117,7 +119,7 @@ static int build_plan(int storage, int m, int n, int64_t nnz, const int *ptr, co
                 }
     }
     for (int i = 0; i < m; i++) P.max_row = std::max(P.max_row, P.ptr[i + 1] - P.ptr[i]);
-    plan_blocks(m, P.ptr, budget, P.blk);
+    plan_blocks(m, P.ptr, budget, align, P.blk);
     return SHK_OK;
 }
 
@@ -125,7 +127,8 @@ template <typename T>
 static int upload(shk_csr A, const HostPlan &P, const T *val_host)
 {
     const int64_t nd = (int64_t)P.ind.size();
-    const int64_t pad = 16;
+    // the SpMV streams whole budget windows: pad so the last one stays in bounds
+    const int64_t pad = A->budget + 16;
     SHK_HIP(hipMalloc(&A->ptr, sizeof(int) * (A->m + 1)));
     SHK_HIP(hipMalloc(&A->ind, sizeof(uint32_t) * (nd + pad)));
     SHK_HIP(hipMalloc(&A->val, sizeof(T) * (nd + pad)));
@@ -137,6 +140,19 @@ static int upload(shk_csr A, const HostPlan &P, const T *val_host)
     SHK_HIP(hipMemsetAsync(A->val, 0, sizeof(T) * (nd + pad), s));
     if (nd) SHK_HIP(hipMemcpyAsync(A->ind, P.ind.data(), sizeof(uint32_t) * nd, hipMemcpyHostToDevice, s));
     SHK_HIP(hipMemcpyAsync(A->blk_rows, P.blk.data(), sizeof(int) * P.blk.size(), hipMemcpyHostToDevice, s));
+    {
+        const size_t nb = P.blk.size() - 1;
+        std::vector<int> info(4 * std::max<size_t>(nb, 1), 0);
+        for (size_t b = 0; b < nb; b++) {
+            info[4 * b + 0] = P.blk[b];
+            info[4 * b + 1] = P.blk[b + 1];
+            info[4 * b + 2] = P.ptr[P.blk[b]];
+            info[4 * b + 3] = P.ptr[P.blk[b + 1]];
+        }
+        SHK_HIP(hipMalloc(&A->blk_info, sizeof(int) * info.size()));
+        SHK_HIP(hipMemcpyAsync(A->blk_info, info.data(), sizeof(int) * info.size(), hipMemcpyHostToDevice, s));
+        SHK_HIP(hipStreamSynchronize(s));
+    }
     if (!P.mid.empty()) {
         SHK_HIP(hipMalloc(&A->mid, sizeof(int) * A->m));
         SHK_HIP(hipMemcpyAsync(A->mid, P.mid.data(), sizeof(int) * A->m, hipMemcpyHostToDevice, s));
@@ -244,7 +260,8 @@ int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int
     }
     HostPlan P;
     const int budget = dtype == SHK_F64 ? budget_of<double>() : budget_of<float>();
-    SHK_TRY(build_plan(storage, m, n, nnz, hptr.data(), hind.data(), budget, P));
+    const int align = dtype == SHK_F64 ? VecOf<double>::V : VecOf<float>::V;
+    SHK_TRY(build_plan(storage, m, n, nnz, hptr.data(), hind.data(), budget, align, P));
     shk_csr A = new shk_csr_s();
     A->ctx = ctx;
     A->storage = storage;
@@ -278,6 +295,7 @@ int shk_csr_destroy(shk_csr A)
     hipFree(A->val);
     hipFree(A->diag);
     hipFree(A->blk_rows);
+    hipFree(A->blk_info);
     hipFree(A->map_up);
     hipFree(A->map_lo);
     hipFree(A->map_diag);

@@ -1,0 +1,222 @@
+"""CPU tests that pin the oracle (oracle/seldon_oracle.c).
+
+Pins, each chosen so that a plausible slip (a dropped term, a wrong sign or
+index, a transposed operand, a wrong stopping count) fails one of them:
+  * the golden fixtures written by the real Seldon build (tests/golden):
+    SpMV outputs on three storages x two precisions x six (alpha, beta),
+    bitwise; Cg / BiCgStab residual histories, iteration counts, error codes
+    and solutions, bitwise against the generic-template build; Gmres against
+    the MKL-linked build (the only one that compiles) within its recorded
+    spread, and bitwise against its own recorded history;
+  * live runs of oracle/_ref (real Seldon) on fresh random inputs when that
+    build is present;
+  * closed forms and textbook special cases: Laplacian eigenpairs, A = I,
+    diagonal A with k distinct eigenvalues, the Givens rotation, upper
+    triangular solves against numpy, small dense solves.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from seldon_amd import workloads as wl
+from tests.conftest import GOLDEN, load_fixture
+from tests.golden.make_golden import SOLVES, build_case, csr_digest
+
+SOLVE_FIXTURES = sorted(f for f in os.listdir(GOLDEN) if f.startswith("solve_"))
+
+
+def _solve(spec):
+    A, b = build_case(spec)
+    x0 = None if spec.get("x0seed") is None else wl.uniform(A.m, spec["x0seed"])
+    return A, orc.solve(spec["method"], A, b, precond=spec["precond"], tol=spec["tol"],
+                        max_iter=spec["max_iter"], restart=spec.get("restart", 10), x0=x0,
+                        storage=spec.get("storage"))
+
+
+def test_fixture_specs_match_generator():
+    for f in SOLVE_FIXTURES:
+        fx = load_fixture(f)
+        A, _ = build_case(fx["spec"])
+        assert csr_digest(A) == str(fx["digest"]), f
+
+
+@pytest.mark.parametrize("fname", SOLVE_FIXTURES)
+def test_oracle_solver_matches_seldon_fixture(fname):
+    fx = load_fixture(fname)
+    spec = fx["spec"]
+    A, r = _solve(spec)
+    assert r.code == int(fx["code"])
+    assert r.nb_iter == int(fx["nb_iter"])
+    if str(fx["build"]) == "generic":
+        # Seldon's generic templates: the restatement is bitwise equal
+        assert np.array_equal(r.hist, fx["hist"])
+        assert np.array_equal(r.hist_it, fx["hist_it"])
+        if not spec.get("zero_rhs"):
+            assert np.sum(r.x) == float(fx["x_sum"])
+            assert np.array_equal(r.x[:16], fx["x_head"])
+    else:
+        # Gmres exists only in the MKL-linked build: same iteration count and
+        # Finished() sequence, history within the recorded spread, and the
+        # oracle's own history is pinned bitwise
+        assert np.array_equal(r.hist_it, fx["hist_it"])
+        assert np.array_equal(r.hist, fx["other_hist"])
+        rel = np.abs(r.hist - fx["hist"]) / np.abs(fx["hist"])
+        assert np.all(rel <= np.maximum(fx["spread"], 0) + 1e-300)
+        assert abs(np.sum(r.x) - float(fx["x_sum"])) <= 1e-9 * max(1.0, abs(float(fx["x_sum"])))
+
+
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_oracle_spmv_matches_seldon_fixture(st, c):
+    fx = load_fixture("spmv_powerlaw300.npz")
+    sp = fx["spec"]
+    dt = np.float64 if c == "d" else np.float32
+    A = wl.powerlaw(sp["n"], sp["seed"], kmin=sp["kmin"], kmax=sp["kmax"], half_width=sp["half_width"])
+    assert csr_digest(A) == str(fx["digest"])
+    M = wl.upper(A) if st == "sym" else A
+    x = wl.uniform(A.m, sp["xseed"]).astype(dt)
+    y = wl.uniform(A.m, sp["yseed"])
+    y[sp["nan_at"]] = np.nan
+    y = y.astype(dt)
+    for k, (al, be) in enumerate(fx["alpha_beta"]):
+        got = orc.mltadd(dt(al), M, x, dt(be), y, storage=st, dtype=dt)
+        assert np.array_equal(got, fx[f"{st}_{c}_{k}"], equal_nan=True), (st, c, al, be)
+    assert np.array_equal(orc.mlt(M, x, storage=st, dtype=dt), fx[f"{st}_{c}_mlt"])
+
+
+def test_fixture_check_has_teeth():
+    """A transposed operand or the wrong storage semantics must not pass."""
+    fx = load_fixture("spmv_powerlaw300.npz")
+    sp = fx["spec"]
+    A = wl.powerlaw(sp["n"], sp["seed"], kmin=sp["kmin"], kmax=sp["kmax"], half_width=sp["half_width"])
+    At = A.to_scipy().T.tocsr()
+    T = wl.CSR(A.m, A.n, At.indptr.astype(np.int32), At.indices.astype(np.int32), At.data.copy())
+    x = wl.uniform(A.m, sp["xseed"])
+    assert not np.array_equal(orc.mlt(T, x), fx["row_d_mlt"])
+    y = wl.uniform(A.m, sp["yseed"])
+    y[sp["nan_at"]] = np.nan
+    # RowSparse keeps beta*NaN, ArrayRowSparse zero-fills on beta == 0
+    row = orc.mltadd(1.0, A, x, 0.0, y, storage="row")
+    arr = orc.mltadd(1.0, A, x, 0.0, y, storage="array")
+    assert np.isnan(row[sp["nan_at"]]) and not np.isnan(arr[sp["nan_at"]])
+
+
+@pytest.mark.skipif(not orc.ref_available("generic"), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_bitwise_vs_live_seldon():
+    rng = np.random.default_rng(7)
+    with tempfile.TemporaryDirectory() as tmp:
+        for seed in range(3):
+            A = wl.powerlaw(2000 + 500 * seed, 100 + seed, kmin=1, kmax=60, half_width=300)
+            x = rng.standard_normal(A.m)
+            y = rng.standard_normal(A.m)
+            for st, M in (("row", A), ("sym", wl.upper(A)), ("array", A)):
+                orc.write_seldon_csr(f"{tmp}/A.bin", M)
+                orc.write_seldon_vec(f"{tmp}/x.bin", x)
+                orc.write_seldon_vec(f"{tmp}/y.bin", y)
+                orc.run_ref(["spmv", st, "d", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/y.bin", "0.7", "-0.2",
+                             f"{tmp}/o.bin"])
+                ref = orc.read_seldon_vec(f"{tmp}/o.bin")
+                assert np.array_equal(ref, orc.mltadd(0.7, M, x, -0.2, y, storage=st))
+        A, b, _ = wl.cg_laplace(48)
+        orc.write_seldon_csr(f"{tmp}/A.bin", A)
+        orc.write_seldon_vec(f"{tmp}/b.bin", b)
+        for meth, prec in (("cg", "jacobi"), ("bicgstab", "id")):
+            out = orc.run_ref(["solve", meth, "row", prec, f"{tmp}/A.bin", f"{tmp}/b.bin", "-", 1e-9, 1000, 10,
+                               f"{tmp}/x.bin"])
+            h, its, code, nb = orc.parse_ref_history(out)
+            r = orc.solve(meth, A, b, precond=prec, tol=1e-9, max_iter=1000)
+            assert np.array_equal(h, r.hist) and code == r.code and nb == r.nb_iter
+            assert np.array_equal(orc.read_seldon_vec(f"{tmp}/x.bin"), r.x)
+
+
+def test_laplacian_eigenpairs():
+    g = 40
+    A = wl.laplace2d(g)
+    i = np.arange(1, g + 1)
+    for j, k in ((1, 1), (2, 5), (g, 1)):
+        u = np.outer(np.sin(k * i * np.pi / (g + 1)), np.sin(j * i * np.pi / (g + 1))).ravel()
+        lam = 4 - 2 * np.cos(j * np.pi / (g + 1)) - 2 * np.cos(k * np.pi / (g + 1))
+        assert np.allclose(orc.mlt(A, u), lam * u, atol=1e-13)
+
+
+def test_identity_and_diagonal_cases():
+    n = 200
+    I = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n))
+    b = wl.uniform(n, 3)
+    r = orc.solve("cg", I, b, tol=1e-12, max_iter=10)
+    assert r.code == 0 and r.nb_iter == 1 and np.array_equal(r.x, b)
+    r = orc.solve("gmres", I, b, tol=1e-12, max_iter=10, restart=5)  # happy breakdown
+    assert r.code == 0 and r.nb_iter == 1 and np.allclose(r.x, b, rtol=1e-15)
+    d = np.array([1.0, 2.0, 3.5, 7.0, 11.0])[np.arange(n) % 5]
+    D = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    r = orc.solve("cg", D, b, tol=1e-12, max_iter=100)
+    assert r.code == 0 and r.nb_iter <= 5
+    assert np.allclose(r.x, b / d, rtol=1e-12)
+
+
+def test_iteration_edge_cases():
+    A = wl.laplace2d(8)
+    b = np.zeros(A.m)
+    x0 = wl.uniform(A.m, 5)
+    r = orc.solve("cg", A, b, x0=x0)          # Init fails: x untouched, code 0
+    assert r.code == 0 and r.nb_iter == 0 and np.array_equal(r.x, x0)
+    b = wl.uniform(A.m, 6)
+    r = orc.solve("cg", A, b, tol=1e-14, max_iter=3)
+    assert r.code == -2 and r.nb_iter == 3
+    r = orc.solve("bicgstab", A, b, tol=1e-14, max_iter=4)
+    assert r.code == -2 and r.nb_iter == 4
+    E = wl.CSR(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    assert orc.solve("cg", E, np.zeros(0)).code == 0
+
+
+def test_givens_textbook():
+    for a, b in ((3.0, 4.0), (-2.0, 0.5), (0.0, 1.0), (1e-300, -1e-300), (0.0, 0.0)):
+        r, z, c, s = orc.genrot(a, b)
+        if a == 0 and b == 0:
+            assert (c, s, r) == (1.0, 0.0, 0.0)
+            continue
+        assert abs(abs(r) - np.hypot(a, b)) <= 1e-15 * np.hypot(a, b)
+        assert abs(c * c + s * s - 1) < 1e-15
+        assert abs(c * a + s * b - r) <= 1e-15 * abs(r)       # rotation maps (a,b) to (r,0)
+        assert abs(-s * a + c * b) <= 1e-15 * abs(r)
+
+
+def test_trsv_against_numpy():
+    rng = np.random.default_rng(2)
+    for n in (1, 5, 30):
+        H = np.triu(rng.standard_normal((n + 3, n + 3))) + 5 * np.eye(n + 3)
+        s = rng.standard_normal(n)
+        got = orc.trsv_upper(H, s)
+        assert np.allclose(H[:n, :n] @ got, s, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab", "gmres"])
+def test_small_dense_solution(method):
+    A = wl.convdiff3d(6) if method != "cg" else wl.laplace2d(12)
+    xt = wl.uniform(A.m, 8)
+    b = wl.apply(A, xt)
+    r = orc.solve(method, A, b, precond="jacobi", tol=1e-12, max_iter=2000, restart=20)
+    assert r.code == 0
+    ref = np.linalg.solve(A.to_scipy().toarray(), b)
+    assert np.linalg.norm(r.x - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+def test_gmres_matches_scipy_solution():
+    import scipy.sparse.linalg as sla
+    A, b, xt = wl.convdiff(10)
+    r = orc.solve("gmres", A, b, precond="id", tol=1e-11, max_iter=3000, restart=30)
+    xs = sla.spsolve(A.to_scipy().tocsc(), b)
+    assert np.linalg.norm(r.x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_sddmm_matches_dense_outer_product():
+    A = wl.powerlaw(300, 4, kmin=2, kmax=20, half_width=30)
+    lam = wl.uniform(A.m, 1)
+    x = wl.uniform(A.m, 2)
+    g = orc.sddmm(A, lam, x)
+    dense = -np.outer(lam, x)
+    rows = np.repeat(np.arange(A.m), np.diff(A.ptr))
+    assert np.array_equal(g, dense[rows, A.ind])
