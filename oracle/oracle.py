@@ -245,3 +245,59 @@ def parse_ref_history(stdout):
             parts = line.split()
             code, nb = int(parts[1]), int(parts[3])
     return np.array(hist), np.array(its, np.int32), code, nb
+
+
+# ------------------------------------------------ adjoint reference (FP64) --
+
+def sddmm_sym_fold(U, lam, x):
+    """RowSymSparse gradient: stored upper (i, c) -> -(lam_i x_c) + -(lam_c x_i)
+    (diagonal: first term only), from the plain per-entry products."""
+    g_up = sddmm(U, lam, x)
+    rows = np.repeat(np.arange(U.m), np.diff(U.ptr))
+    g_lo = -(lam[U.ind] * x[rows])
+    off = U.ind != rows
+    g = g_up.copy()
+    g[off] = g_up[off] + g_lo[off]
+    return g
+
+
+def diffusion_grad_reference(n, theta, values, pattern, b, x_obs, tol=1e-13, max_iter=20000):
+    """dL/dtheta and dL/db for L = |x - x_obs|^2, x = A(theta)^{-1} b, with the
+    oracle's FP64 Cg (Jacobi) for the forward and adjoint solves, the oracle's
+    SDDMM for dL/dA, and the chain rule through the harmonic-mean faces written
+    out in numpy (independent of the torch assembly)."""
+    import copy
+    A = copy.copy(pattern)
+    A.val = np.ascontiguousarray(values, np.float64)
+    fx = solve("cg", A, b, precond="jacobi", tol=tol, max_iter=max_iter)
+    x = fx.x
+    gx = 2.0 * (x - x_obs)
+    ad = solve("cg", A, gx, precond="jacobi", tol=tol, max_iter=max_iter)
+    lam = ad.x
+    G = sddmm(A, lam, x)
+    N = n ** 3
+    ptr = A.ptr.astype(np.int64)
+    rows = np.repeat(np.arange(N), np.diff(ptr))
+    key = rows * N + A.ind.astype(np.int64)
+    order = np.argsort(key)
+    ks = key[order]
+
+    def pos(i, j):
+        return order[np.searchsorted(ks, i * N + j)]
+
+    k = np.exp(theta)
+    r = np.arange(N)
+    ix, iy, iz = r % n, (r // n) % n, r // (n * n)
+    dk = np.zeros(N)
+    gd = G[pos(r, r)]
+    for f, (cond, off) in enumerate(((iz > 0, -n * n), (iy > 0, -n), (ix > 0, -1),
+                                     (ix < n - 1, 1), (iy < n - 1, n), (iz < n - 1, n * n))):
+        c = r[cond]
+        nb = c + off
+        kc, kn = k[c], k[nb]
+        dLdkf = gd[c] + gd[nb] - G[pos(c, nb)] - G[pos(nb, c)]
+        dk[c] += dLdkf * 2.0 * kn * kn / ((kc + kn) ** 2)
+        bnd = r[~cond]
+        dk[bnd] += gd[bnd]  # boundary face: k_f = k_c
+    return dict(x=x, lam=lam, grad_theta=k * dk, grad_b=lam, loss=float(np.sum((x - x_obs) ** 2)),
+                its=(fx.nb_iter, ad.nb_iter))

@@ -84,7 +84,7 @@ def build_cpp_programs():
         exe = os.path.join(bindir, name)
         _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I", INCLUDE,
               "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-o", exe, path,
-              "-L", PKG, "-lshk", "-Wl,-rpath," + PKG])
+              "-L", PKG, "-lshk", "-Wl,-rpath,$ORIGIN/../.."])
         outs.append(exe)
     return outs
 

@@ -41,6 +41,22 @@ int ensure_ws(shk_ctx ctx, size_t bytes)
     return SHK_OK;
 }
 
+int ensure_ws2(shk_ctx ctx, size_t bytes)
+{
+    if (ctx->ws2_cap >= bytes) return SHK_OK;
+    if (ctx->ws2) {
+        SHK_HIP(hipStreamSynchronize(ctx->stream));
+        SHK_HIP(hipFree(ctx->ws2));
+        ctx->ws2 = nullptr;
+        ctx->ws2_cap = 0;
+    }
+    hipError_t e = hipMalloc(&ctx->ws2, bytes);
+    if (e != hipSuccess)
+        return fail(SHK_ERR_NOMEM, std::string("refinement workspace: ") + hipGetErrorString(e));
+    ctx->ws2_cap = bytes;
+    return SHK_OK;
+}
+
 int check_device_ptr(const void *p, const char *what)
 {
     if (!p) return fail(SHK_ERR_ARG, std::string("null device pointer: ") + what);
@@ -107,6 +123,7 @@ int shk_ctx_destroy(shk_ctx c)
     if (c->red.ticket) hipFree(c->red.ticket);
     if (c->scalars) hipFree(c->scalars);
     if (c->ws) hipFree(c->ws);
+    if (c->ws2) hipFree(c->ws2);
     if (c->state) hipFree(c->state);
     if (c->state_host) hipHostFree(c->state_host);
     if (c->ev0) hipEventDestroy(c->ev0);

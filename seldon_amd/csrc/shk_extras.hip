@@ -1,21 +1,86 @@
-// shk_extras.hip -- entry points beyond Seldon's API: FP32+FP64 iterative
-// refinement, the adjoint SDDMM gradient and the batched CG.  (Filled in by
-// the dedicated sources; these definitions report "not available" until the
-// corresponding kernels are linked.)
-#include "shk_internal.hpp"
+// shk_extras.hip -- the adjoint SDDMM gradient kernel (no Seldon
+// counterpart) and the batched-CG entry points' argument checking.
+#include "shk_kernels.hpp"
+
+namespace shk {
+
+// g_pos = -(lam_row * x_col) for every device entry of the row blocks.  The
+// block's row pointers and lam values are staged in LDS; entries are taken
+// by consecutive threads (coalesced), each finding its row by binary search
+// over the staged pointers.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sddmm(int nblk, int xcd_q, const int4 *blk_info, const int *ptr,
+                                                  const uint32_t *ind, const T *lam, const T *x, T *g)
+{
+    __shared__ int sptr[kBlock + 1];
+    __shared__ T slam[kBlock];
+    const int lb = xcd_logical_block(xcd_q);
+    if (lb >= nblk) return;
+    const int4 bi = blk_info[lb];
+    const int rs = bi.x, re = bi.y, js = bi.z, je = bi.w;
+    const int nr = re - rs;  // <= kBlock (the planner's row cap)
+    for (int i = threadIdx.x; i <= nr; i += kBlock) sptr[i] = ptr[rs + i];
+    for (int i = threadIdx.x; i < nr; i += kBlock) slam[i] = lam[rs + i];
+    __syncthreads();
+    for (int j = js + (int)threadIdx.x; j < je; j += kBlock) {
+        int lo = 0, hi = nr - 1;  // last row with sptr[row] <= j
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sptr[mid] <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        g[j] = -(slam[lo] * x[ind[j] & ~kLowerFlag]);
+    }
+}
+
+// RowSymSparse: an upper entry (i, c) stands for a_ic and a_ci:
+// g = -(lam_i x_c) + -(lam_c x_i); the diagonal has only the first term.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sddmm_fold(int64_t nnz, const T *G, const int64_t *map_up,
+                                                       const int64_t *map_lo, T *g)
+{
+    for (int64_t j = blockIdx.x * (int64_t)kBlock + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * kBlock) {
+        T v = G[map_up[j]];
+        if (map_lo[j] >= 0) v += G[map_lo[j]];
+        g[j] = v;
+    }
+}
+
+template <typename T>
+static int sddmm_typed(shk_ctx ctx, shk_csr A, const T *lam, const T *x, T *g)
+{
+    T *G = g;
+    if (A->storage == SHK_ROWSYMSPARSE) {
+        SHK_TRY(ensure_ws2(ctx, sizeof(T) * (size_t)std::max<int64_t>(A->nnz_dev, 1)));
+        G = (T *)ctx->ws2;
+    }
+    if (A->nblk > 0)
+        hipLaunchKernelGGL(k_sddmm<T>, dim3(A->nblk_launch), dim3(kBlock), 0, ctx->stream, A->nblk, A->xcd_q,
+                           (const int4 *)A->blk_info, (const int *)A->ptr, (const uint32_t *)A->ind, lam, x, G);
+    if (A->storage == SHK_ROWSYMSPARSE && A->nnz_stored > 0)
+        hipLaunchKernelGGL(k_sddmm_fold<T>, dim3(red_blocks(A->nnz_stored)), dim3(kBlock), 0, ctx->stream,
+                           A->nnz_stored, (const T *)G, (const int64_t *)A->map_up, (const int64_t *)A->map_lo, g);
+    SHK_HIP(hipGetLastError());
+    return SHK_OK;
+}
+
+}  // namespace shk
 
 using namespace shk;
 
 extern "C" {
 
-int shk_solve_ir(shk_ctx, shk_csr, shk_csr, void *, const void *, shk_precond, shk_ir_params *, int *)
+int shk_sddmm(shk_ctx ctx, shk_csr A, const void *dlam, const void *dx, void *dg)
 {
-    return fail(SHK_ERR_STATE, "shk_solve_ir: not available in this build");
-}
-
-int shk_sddmm(shk_ctx, shk_csr, const void *, const void *, void *)
-{
-    return fail(SHK_ERR_STATE, "shk_sddmm: not available in this build");
+    SHK_CHECK(ctx && A, SHK_ERR_ARG, "shk_sddmm: null argument");
+    SHK_CHECK(A->ctx == ctx, SHK_ERR_ARG, "shk_sddmm: matrix of another context");
+    if (A->nnz_stored == 0 || A->m == 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dlam, "lam"));
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(dg, "g"));
+    if (A->dtype == SHK_F64)
+        return sddmm_typed<double>(ctx, A, (const double *)dlam, (const double *)dx, (double *)dg);
+    return sddmm_typed<float>(ctx, A, (const float *)dlam, (const float *)dx, (float *)dg);
 }
 
 int shk_cg_batched(shk_ctx, shk_csr, int, const int *, void *, const void *, double, int, int, int, int *,

@@ -75,6 +75,9 @@ struct shk_ctx_s {
     // work vectors per call, Cg.cxx:55; here they are reused across calls)
     void *ws = nullptr;
     size_t ws_cap = 0;
+    // second arena for drivers that call a solver (iterative refinement)
+    void *ws2 = nullptr;
+    size_t ws2_cap = 0;
 };
 
 struct shk_csr_s {
@@ -196,6 +199,7 @@ inline int red_blocks(int64_t n, int items_per_thread = 8)
 // ----------------------------------------------------- host helpers ----
 int ensure_red(shk_ctx ctx, size_t bytes);
 int ensure_ws(shk_ctx ctx, size_t bytes);
+int ensure_ws2(shk_ctx ctx, size_t bytes);
 int check_device_ptr(const void *p, const char *what);
 
 }  // namespace shk

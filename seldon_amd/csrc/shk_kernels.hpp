@@ -20,6 +20,8 @@
 //     fixed-order grid reduction, no float atomics).
 #pragma once
 
+#include <algorithm>
+
 #include "shk_internal.hpp"
 
 namespace shk {
@@ -118,54 +120,105 @@ struct VecOf<float> {
     typedef float FV __attribute__((ext_vector_type(4)));
 };
 
-// Phase 1, branch-free so that every load of the block is in flight at once:
-// the streaming loads of ind/val (non-temporal, so they do not evict x from
-// L2) read the whole budget window [js0, js0 + BUDGET) -- the arrays are padded
-// by BUDGET entries -- then every x gather is issued with its index clamped to
-// 0 outside [js, je), then the products go to LDS.  prod[e - js0] holds the
-// product of entry e (0 outside the block).
+// Phase 1 is split in two so that the streaming loads of the *next* row block
+// are in flight while the current one is processed:
+//   spmv_load   -- non-temporal, coalesced loads of the whole budget window
+//                  [js0, js0 + BUDGET) of ind/val (arrays padded by BUDGET);
+//   spmv_gather -- every x gather issued at once (index clamped to 0 outside
+//                  [js, je));
+//   spmv_products -- the products written to LDS: prod[e - js0] holds the
+//                  product of entry e (0 outside the block).
+// The kernel issues block i's gathers, then block i+W's streaming loads, and
+// only then consumes the gathers: vector loads retire in order, so this way
+// waiting on the gathers never waits on the next block's loads.
 template <typename T, int BUDGET>
-__device__ __forceinline__ void spmv_stage_products(const SpmvArgs<T> &a, int js0, int js, int je, T *prod)
+struct SpmvRegs {
+    static constexpr int V = VecOf<T>::V;
+    static constexpr int K = BUDGET / (kBlock * V);
+    typename VecOf<T>::IV iv[K];
+    typename VecOf<T>::FV fv[K];
+};
+
+template <typename T, int BUDGET>
+__device__ __forceinline__ void spmv_load(const SpmvArgs<T> &a, int js0, SpmvRegs<T, BUDGET> &R)
 {
     typedef VecOf<T> VT;
-    constexpr int V = VT::V;
-    constexpr int K = BUDGET / (kBlock * V);
-    typename VT::IV iv[K];
-    typename VT::FV fv[K];
+    constexpr int V = VT::V, K = SpmvRegs<T, BUDGET>::K;
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int e = js0 + (k * kBlock + t) * V;
-        iv[k] = __builtin_nontemporal_load((const typename VT::IV *)(a.ind + e));
-        fv[k] = __builtin_nontemporal_load((const typename VT::FV *)(a.val + e));
+        R.iv[k] = __builtin_nontemporal_load((const typename VT::IV *)(a.ind + e));
+        R.fv[k] = __builtin_nontemporal_load((const typename VT::FV *)(a.val + e));
     }
-    T xv[K][V];
+}
+
+template <typename T, int BUDGET>
+struct SpmvGather {
+    T xv[SpmvRegs<T, BUDGET>::K][VecOf<T>::V];
+};
+
+template <typename T, int BUDGET>
+__device__ __forceinline__ void spmv_gather(const SpmvArgs<T> &a, int js0, int js, int je,
+                                            const SpmvRegs<T, BUDGET> &R, SpmvGather<T, BUDGET> &G)
+{
+    constexpr int V = VecOf<T>::V, K = SpmvRegs<T, BUDGET>::K;
+    const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int e = js0 + (k * kBlock + t) * V;
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const bool in = (e + v < je) & (e + v >= js);
-            const unsigned col = in ? (iv[k][v] & ~kLowerFlag) : 0u;
-            xv[k][v] = a.x[col];
+            const unsigned col = in ? (R.iv[k][v] & ~kLowerFlag) : 0u;
+            G.xv[k][v] = a.x[col];
         }
     }
+}
+
+template <typename T, int BUDGET>
+__device__ __forceinline__ void spmv_products(const SpmvArgs<T> &a, int js0, int js, int je,
+                                              const SpmvRegs<T, BUDGET> &R, const SpmvGather<T, BUDGET> &G,
+                                              T *prod)
+{
+    typedef VecOf<T> VT;
+    constexpr int V = VT::V, K = SpmvRegs<T, BUDGET>::K;
+    const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int e = js0 + (k * kBlock + t) * V;
         typename VT::FV p;
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            const T val = fv[k][v];
+            const T val = R.fv[k][v];
             // direct terms (transposed symmetric part, ArrayRowSparse)
             // multiply alpha into the value first: `alpha * data[j] * X(i)`
             // (Functions_MatVect.cxx:563, Functions_MatrixArray.cxx:571)
-            const T pr = (iv[k][v] & kLowerFlag) ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
+            const T pr = (R.iv[k][v] & kLowerFlag) ? (a.alpha * val) * G.xv[k][v] : val * G.xv[k][v];
             const bool in = (e + v < je) & (e + v >= js);
             p[v] = in ? pr : (T)0;
         }
         *(typename VT::FV *)(prod + (e - js0)) = p;
     }
+}
+
+// Per-thread row metadata of a block (thread t owns row rs + t: the planner
+// puts at most kBlock rows in a block).
+template <typename T>
+struct RowMeta {
+    int p0, p1, pm;
+    T yin;
+};
+
+template <typename T, int MODE>
+__device__ __forceinline__ void row_meta_load(const SpmvArgs<T> &a, int4 bi, RowMeta<T> &M)
+{
+    const int r = bi.x + (int)threadIdx.x;
+    const int rr = r < bi.y ? r : bi.x;
+    M.p0 = a.ptr[rr];
+    M.p1 = a.ptr[rr + 1];
+    M.pm = MODE == kSym ? a.mid[rr] : (MODE == kCsr ? M.p1 : M.p0);
+    M.yin = a.mlt ? (T)0 : a.y_in[rr];
 }
 
 // Sequential sum of prod[q0, q1) onto acc, four products per step so the LDS
@@ -201,88 +254,138 @@ struct FinStore {
     }
 };
 
+// One row block whose products are staged: sequential row sums and epilogue.
+template <typename T, int MODE, int BUDGET>
+__device__ __forceinline__ void spmv_rows(const SpmvArgs<T> &a, int4 bi, int js0, const RowMeta<T> &M,
+                                          const T *prod, T (&c)[2])
+{
+    const int r = bi.x + (int)threadIdx.x;
+    if (r >= bi.y) return;
+    const int p0 = M.p0 - js0, p1 = M.p1 - js0, pm = M.pm - js0;
+    T y;
+    if (a.mlt) y = (T)0;
+    else if (MODE == kArray) y = (a.beta == (T)0) ? (T)0 : M.yin * a.beta;
+    else y = M.yin * a.beta;
+    if (MODE != kArray) {
+        const T temp = seq_sum((T)0, prod, p0, pm);
+        y += a.alpha * temp;
+    }
+    y = seq_sum(y, prod, pm, p1);
+    spmv_epilogue<T, MODE>(a, r, y, c);
+}
+
+// A row longer than the budget: staged chunk by chunk, thread 0 keeps the
+// sequential accumulators across chunks.
+template <typename T, int MODE, int BUDGET>
+__device__ __forceinline__ void spmv_long_row(const SpmvArgs<T> &a, int4 bi, T *prod, T (&c)[2])
+{
+    const int r = bi.x, js = bi.z, je = bi.w;
+    const int jm = MODE == kSym ? a.mid[r] : (MODE == kCsr ? je : js);
+    T y = (T)0, temp = (T)0;
+    if (threadIdx.x == 0 && !a.mlt) {
+        const T yin = a.y_in[r];
+        if (MODE == kArray) y = (a.beta == (T)0) ? (T)0 : yin * a.beta;
+        else y = yin * a.beta;
+    }
+    for (int cs = js; cs < je; cs += BUDGET) {
+        const int ce = min(je, cs + BUDGET);
+        for (int j = cs + (int)threadIdx.x; j < ce; j += kBlock) prod[j - cs] = spmv_product(a, j);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int j = cs; j < ce; j++) {
+                if (j < jm) {
+                    temp += prod[j - cs];
+                } else {
+                    if (MODE != kArray && j == jm) y += a.alpha * temp;
+                    y += prod[j - cs];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (MODE != kArray && jm >= je) y += a.alpha * temp;
+        spmv_epilogue<T, MODE>(a, r, y, c);
+    }
+}
+
+// Persistent, software-pipelined SpMV.  The grid is 8 * W work-groups; the
+// ones with blockIdx % 8 == k run on XCD k and sweep that XCD's contiguous
+// range of row blocks [k*q, (k+1)*q) together, W apart, so the x entries
+// they gather are shared through one L2.  While a work-group stages and sums
+// block i, the streaming loads and row metadata of block i+W are already in
+// flight, and the block table entry of block i+2W is being fetched.
 template <typename T, int MODE, int BUDGET, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<T> a, Fin fin)
+__global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<T> a, const int4 *__restrict__ blk_info, Fin fin)
 {
     if (a.skip && *a.skip) return;
     __shared__ SpmvSmem<T, BUDGET> sm;
-    const int lb = xcd_logical_block(a.xcd_q);
+    constexpr int V = VecOf<T>::V;
+    const int xcd = blockIdx.x % kNumXcd;
+    const int stride = gridDim.x / kNumXcd;
+    const int lb_end = min(a.nblk, (xcd + 1) * a.xcd_q);
+    int lb = xcd * a.xcd_q + (int)(blockIdx.x / kNumXcd);
     T c[2] = {(T)0, (T)0};
-    if (lb < a.nblk) {
-        const int4 bi = a.blk_info[lb];
-        const int rs = bi.x, re = bi.y, js = bi.z, je = bi.w;
-        constexpr int V = VecOf<T>::V;
-        const int js0 = js & ~(V - 1);  // the planner keeps je - js0 <= BUDGET
-        if (je - js0 <= BUDGET) {
-            // the planner puts at most kBlock rows in a block: thread t owns
-            // row rs + t; its metadata and y_in are fetched before phase 1
-            const int r = rs + (int)threadIdx.x;
-            const bool own = r < re;
-            const int rr = own ? r : rs;
-            int p0 = a.ptr[rr], p1 = a.ptr[rr + 1];
-            int pm = MODE == kSym ? a.mid[rr] : (MODE == kCsr ? p1 : p0);
-            T yin = (T)0;
-            if (!a.mlt) yin = a.y_in[rr];
-            spmv_stage_products<T, BUDGET>(a, js0, js, je, sm.prod);
-            __syncthreads();
-            if (own) {
-                p0 -= js0; p1 -= js0; pm -= js0;
-                T y;
-                if (a.mlt) y = (T)0;
-                else if (MODE == kArray) y = (a.beta == (T)0) ? (T)0 : yin * a.beta;
-                else y = yin * a.beta;
-                if (MODE != kArray) {
-                    const T temp = seq_sum((T)0, sm.prod, p0, pm);
-                    y += a.alpha * temp;
-                }
-                y = seq_sum(y, sm.prod, pm, p1);
-                spmv_epilogue<T, MODE>(a, r, y, c);
-            }
-        } else {
-            // one long row, staged chunk by chunk; thread 0 keeps the
-            // sequential accumulators across chunks
-            const int r = rs;
-            const int jm = MODE == kSym ? a.mid[r] : (MODE == kCsr ? je : js);
-            T y = (T)0, temp = (T)0;
-            if (threadIdx.x == 0) {
-                if (!a.mlt) {
-                    T yin = a.y_in[r];
-                    if (MODE == kArray) y = (a.beta == (T)0) ? (T)0 : yin * a.beta;
-                    else y = yin * a.beta;
-                }
-            }
-            for (int cs = js; cs < je; cs += BUDGET) {
-                const int ce = min(je, cs + BUDGET);
-                for (int j = cs + (int)threadIdx.x; j < ce; j += kBlock) sm.prod[j - cs] = spmv_product(a, j);
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    for (int j = cs; j < ce; j++) {
-                        if (j < jm) {
-                            temp += sm.prod[j - cs];
-                        } else {
-                            if (MODE != kArray && j == jm) y += a.alpha * temp;
-                            y += sm.prod[j - cs];
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) {
-                if (MODE != kArray && jm >= je) y += a.alpha * temp;
-                spmv_epilogue<T, MODE>(a, r, y, c);
-            }
+
+    int4 bi = make_int4(0, 0, 0, 0), bi_next = make_int4(0, 0, 0, 0);
+    SpmvRegs<T, BUDGET> R;
+    RowMeta<T> M;
+    if (lb < lb_end) {
+        bi = blk_info[lb];
+        if (lb + stride < lb_end) bi_next = blk_info[lb + stride];
+        if (bi.w - (bi.z & ~(V - 1)) <= BUDGET) spmv_load<T, BUDGET>(a, bi.z & ~(V - 1), R);
+        row_meta_load<T, MODE>(a, bi, M);
+    }
+    while (lb < lb_end) {
+        const int lbn = lb + stride;
+        const bool haven = lbn < lb_end;
+        int4 bi_nn = make_int4(0, 0, 0, 0);
+        if (lbn + stride < lb_end) bi_nn = blk_info[lbn + stride];
+        const int js0 = bi.z & ~(V - 1);
+        const bool short_blk = bi.w - js0 <= BUDGET;
+        SpmvGather<T, BUDGET> G;
+        if (short_blk) spmv_gather<T, BUDGET>(a, js0, bi.z, bi.w, R, G);
+        SpmvRegs<T, BUDGET> Rn;
+        RowMeta<T> Mn;
+        if (haven) {
+            if (bi_next.w - (bi_next.z & ~(V - 1)) <= BUDGET) spmv_load<T, BUDGET>(a, bi_next.z & ~(V - 1), Rn);
+            row_meta_load<T, MODE>(a, bi_next, Mn);
         }
+        if (short_blk) {
+            spmv_products<T, BUDGET>(a, js0, bi.z, bi.w, R, G, sm.prod);
+            __syncthreads();
+            spmv_rows<T, MODE, BUDGET>(a, bi, js0, M, sm.prod, c);
+            __syncthreads();
+        } else {
+            spmv_long_row<T, MODE, BUDGET>(a, bi, sm.prod, c);
+        }
+        bi = bi_next;
+        bi_next = bi_nn;
+        R = Rn;
+        M = Mn;
+        lb = lbn;
     }
     if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
         T out[2];
-        if (grid_sum<T, 2>(c, a.partials, a.ticket, a.nblk_launch, blockIdx.x, out)) {
+        if (grid_sum<T, 2>(c, a.partials, a.ticket, gridDim.x, blockIdx.x, out)) {
             if (threadIdx.x == 0) fin(out);
         }
     }
 }
 
+// Entries staged per row block (LDS products): 16 KiB per work-group.
+#ifndef SHK_BUDGET_F64
+#define SHK_BUDGET_F64 2048
+#endif
+#ifndef SHK_BUDGET_F32
+#define SHK_BUDGET_F32 4096
+#endif
 template <typename T>
-constexpr int budget_of() { return sizeof(T) == 8 ? 2048 : 4096; }
+constexpr int budget_of() { return sizeof(T) == 8 ? SHK_BUDGET_F64 : SHK_BUDGET_F32; }
+#ifndef SHK_SPMV_WG_PER_CU
+#define SHK_SPMV_WG_PER_CU 4
+#endif
+constexpr int kSpmvWgPerCu = SHK_SPMV_WG_PER_CU;
 
 // ------------------------------------------------------------ launch ----
 template <typename T, typename Fin>
@@ -306,16 +409,18 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     }
     if (A->m == 0) return SHK_OK;
     constexpr int B = budget_of<T>();
-    dim3 grid(A->nblk_launch), block(kBlock);
+    // persistent grid: at most kSpmvWgPerCu work-groups per CU, a multiple of 8
+    const int per_xcd = std::min(A->xcd_q, std::max(1, ctx->num_cus * kSpmvWgPerCu / kNumXcd));
+    dim3 grid(per_xcd * kNumXcd), block(kBlock);
     switch (A->storage) {
     case SHK_ROWSPARSE:
-        hipLaunchKernelGGL((k_spmv<T, kCsr, B, Fin>), grid, block, 0, ctx->stream, a, fin);
+        hipLaunchKernelGGL((k_spmv<T, kCsr, B, Fin>), grid, block, 0, ctx->stream, a, a.blk_info, fin);
         break;
     case SHK_ROWSYMSPARSE:
-        hipLaunchKernelGGL((k_spmv<T, kSym, B, Fin>), grid, block, 0, ctx->stream, a, fin);
+        hipLaunchKernelGGL((k_spmv<T, kSym, B, Fin>), grid, block, 0, ctx->stream, a, a.blk_info, fin);
         break;
     default:
-        hipLaunchKernelGGL((k_spmv<T, kArray, B, Fin>), grid, block, 0, ctx->stream, a, fin);
+        hipLaunchKernelGGL((k_spmv<T, kArray, B, Fin>), grid, block, 0, ctx->stream, a, a.blk_info, fin);
         break;
     }
     SHK_HIP(hipGetLastError());

@@ -6,6 +6,7 @@
 // chunks and reads the state back once per chunk; kernels queued past the end
 // of the solve see `skip` and return at once.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "shk_solver_state.hpp"
@@ -624,6 +625,110 @@ static int solve_entry(int which, shk_ctx ctx, shk_csr A, void *dx, const void *
     return solve_dispatch<float>(which, ctx, A, dx, db, prec, it, code);
 }
 
+
+// ------------------------------------------------- iterative refinement ----
+// FP32 inner Krylov solve + FP64 refinement (no Seldon counterpart).  FP64
+// is used only for the true residual r = b - A x, its norm and the update
+// x += |r| d; the inner GMRES keeps its Hessenberg / Givens least squares in
+// FP64 (shk_solver_state.hpp) while every vector, the SpMV and the dot
+// products run in FP32.
+
+// r32 = (float)(r * (1/|r|))  -- the scaled correction equation A d = r/|r|
+__global__ __launch_bounds__(kBlock) void k_ir_down(int64_t n, const double *r, double inv, float *r32, float *d32)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        r32[i] = (float)(r[i] * inv);
+        d32[i] = 0.0f;
+    }
+}
+
+// x += |r| * (double)d
+__global__ __launch_bounds__(kBlock) void k_ir_up(int64_t n, const float *d32, double scale, double *x)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        x[i] += scale * (double)d32[i];
+}
+
+// true residual r = b - A x (FP64) and its squared norm to ctx scalars
+static int ir_residual(shk_ctx ctx, shk_csr A64, const double *x, const double *b, double *r, double *nrm)
+{
+    SpmvArgs<double> a{};
+    a.x = x; a.y_in = b; a.y_out = r; a.alpha = -1.0; a.beta = 1.0; a.dot0 = kDotSelf;
+    SHK_TRY((launch_spmv<double, FinStore<double>>(A64, a, FinStore<double>{(double *)ctx->scalars})));
+    double h[2];
+    SHK_HIP(hipMemcpyAsync(h, ctx->scalars, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    SHK_HIP(hipStreamSynchronize(ctx->stream));
+    *nrm = std::sqrt(h[0]);
+    return SHK_OK;
+}
+
+static int ir_impl(shk_ctx ctx, shk_csr A64, shk_csr A32, double *x, const double *b, int prec,
+                   shk_ir_params *p, int *code)
+{
+    const int64_t n = A64->m;
+    const size_t vb8 = ((size_t)n * 8 + 255) & ~(size_t)255, vb4 = ((size_t)n * 4 + 255) & ~(size_t)255;
+    SHK_TRY(ensure_ws2(ctx, vb8 + 2 * vb4));
+    SHK_TRY(ensure_red(ctx, sizeof(double) * 2 * (size_t)std::max(red_blocks(n), A64->nblk_launch)));
+    char *w = (char *)ctx->ws2;
+    double *r = (double *)w;
+    float *r32 = (float *)(w + vb8), *d32 = (float *)(w + vb8 + vb4);
+    // |b|
+    double bn2 = 0;
+    {
+        const int nb = red_blocks(n);
+        hipLaunchKernelGGL((k_axpy_dot<double, FinStore<double>>), dim3(nb), dim3(kBlock), 0, ctx->stream, n, 0.0,
+                           (const double *)nullptr, (double *)b, b, (const double *)nullptr,
+                           (double *)ctx->red.partials, ctx->red.ticket, (const int *)nullptr,
+                           FinStore<double>{(double *)ctx->scalars});
+        SHK_HIP(hipGetLastError());
+        SHK_HIP(hipMemcpyAsync(&bn2, ctx->scalars, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        SHK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    const double bn = std::sqrt(bn2);
+    p->refinements = 0;
+    p->total_inner_iter = 0;
+    *code = 0;
+    if (bn == 0.0) {
+        SHK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, ctx->stream));
+        p->final_relres = 0.0;
+        if (p->relres_hist) p->relres_hist[0] = 0.0;
+        return SHK_OK;
+    }
+    double rn = 0;
+    SHK_TRY(ir_residual(ctx, A64, x, b, r, &rn));
+    double rel = rn / bn;
+    if (p->relres_hist) p->relres_hist[0] = rel;
+    const int nb = red_blocks(n);
+    while (rel >= p->tolerance && p->refinements < p->max_refinements) {
+        hipLaunchKernelGGL(k_ir_down, dim3(nb), dim3(kBlock), 0, ctx->stream, n, (const double *)r, 1.0 / rn, r32,
+                           d32);
+        SHK_HIP(hipGetLastError());
+        shk_iteration it{};
+        it.tolerance = p->inner_tolerance;
+        it.max_iter = p->inner_max_iter;
+        it.init_guess_null = 1;
+        it.parameter_restart = p->inner_restart;
+        int icode = 0;
+        if (p->inner_method == 1) SHK_TRY(cg_impl<float>(ctx, A32, d32, r32, prec, &it, &icode));
+        else SHK_TRY(gmres_impl<float>(ctx, A32, d32, r32, prec, &it, &icode));
+        p->total_inner_iter += it.nb_iter;
+        hipLaunchKernelGGL(k_ir_up, dim3(nb), dim3(kBlock), 0, ctx->stream, n, (const float *)d32, rn, x);
+        SHK_HIP(hipGetLastError());
+        SHK_TRY(ir_residual(ctx, A64, x, b, r, &rn));
+        const double prev = rel;
+        rel = rn / bn;
+        p->refinements++;
+        if (p->relres_hist) p->relres_hist[p->refinements] = rel;
+        if (!(rel < prev)) {  // refinement stopped contracting (or NaN)
+            *code = 1;
+            break;
+        }
+    }
+    p->final_relres = rel;
+    if (*code == 0 && rel >= p->tolerance) *code = -2;
+    return SHK_OK;
+}
+
 }  // namespace shk
 
 using namespace shk;
@@ -648,3 +753,23 @@ int shk_gmres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec
 }
 
 }  // extern "C"
+
+extern "C" int shk_solve_ir(shk_ctx ctx, shk_csr A64, shk_csr A32, void *dx, const void *db, shk_precond prec,
+                            shk_ir_params *p, int *code)
+{
+    using namespace shk;
+    SHK_CHECK(ctx && A64 && A32 && p && code, SHK_ERR_ARG, "shk_solve_ir: null argument");
+    SHK_CHECK(A64->ctx == ctx && A32->ctx == ctx, SHK_ERR_ARG, "shk_solve_ir: matrices of another context");
+    SHK_CHECK(A64->dtype == SHK_F64 && A32->dtype == SHK_F32, SHK_ERR_ARG,
+              "shk_solve_ir: A64 must be FP64 and A32 FP32");
+    SHK_CHECK(A64->m == A32->m && A64->n == A32->n && A64->nnz_stored == A32->nnz_stored &&
+                  A64->storage == A32->storage,
+              SHK_ERR_DIM, "shk_solve_ir: A64 and A32 must share the pattern");
+    SHK_CHECK(A64->m == A64->n, SHK_ERR_DIM, "shk_solve_ir: square matrix needed");
+    SHK_CHECK(p->max_refinements >= 0 && p->max_refinements < 100000, SHK_ERR_ARG, "shk_solve_ir: bad max_refinements");
+    *code = 0;
+    if (A64->m == 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(db, "b"));
+    return ir_impl(ctx, A64, A32, (double *)dx, (const double *)db, prec, p, code);
+}

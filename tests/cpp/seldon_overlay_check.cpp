@@ -1,0 +1,93 @@
+// seldon_overlay_check.cpp -- the GPU storages dropped into the real Seldon.
+//
+// Compiled against the reference headers (oracle/Makefile, into oracle/_ref)
+// and linked to libshk.so.  On one 2-D Laplacian system it runs
+//   1. Seldon's own Cg on its CPU RowSparse matrix,
+//   2. Cg on RowSparseHip / VectHip with Preconditioner_Base: the fused GPU
+//      solver selected by overload resolution,
+//   3. Cg on the same GPU storages with a user preconditioner type: Seldon's
+//      generic Cg template, each primitive (Mlt, DotProdConj, Add, Norm2,
+//      Copy, ...) reaching the GPU through the overload set,
+// and prints each residual history (Iteration::Finished lines) after a
+// "== <label>" marker, then "ERRCODE e NBITER n".
+#include "Seldon.hxx"
+#include "SeldonSolver.hxx"
+#include "seldon_amd/SeldonHip.hxx"
+
+#include <cstdlib>
+
+using namespace Seldon;
+
+// A preconditioner Seldon does not know: identity through Copy
+class UserIdentity : public Preconditioner_Base
+{
+public:
+  template <class Matrix1, class Vector1>
+  void Solve(const Matrix1&, const Vector1& r, Vector1& z) { Copy(r, z); }
+  template <class Matrix1, class Vector1>
+  void TransSolve(const Matrix1& A, const Vector1& r, Vector1& z) { Solve(A, r, z); }
+};
+
+int main(int argc, char** argv)
+{
+  const int g = argc > 1 ? atoi(argv[1]) : 64, n = g * g;
+  // 5-point Laplacian, natural order, sorted columns
+  Vector<double> values, xt(n), bh(n);
+  Vector<int> ptr(n + 1), ind;
+  std::vector<double> vv;
+  std::vector<int> vi;
+  ptr(0) = 0;
+  for (int iy = 0; iy < g; iy++)
+    for (int ix = 0; ix < g; ix++)
+      {
+        int r = iy * g + ix;
+        if (iy > 0) { vi.push_back(r - g); vv.push_back(-1); }
+        if (ix > 0) { vi.push_back(r - 1); vv.push_back(-1); }
+        vi.push_back(r); vv.push_back(4);
+        if (ix < g - 1) { vi.push_back(r + 1); vv.push_back(-1); }
+        if (iy < g - 1) { vi.push_back(r + g); vv.push_back(-1); }
+        ptr(r + 1) = (int)vi.size();
+      }
+  values.Reallocate(vv.size());
+  ind.Reallocate(vi.size());
+  for (size_t k = 0; k < vv.size(); k++) { values(k) = vv[k]; ind(k) = vi[k]; }
+  Matrix<double, General, RowSparse> A;
+  A.SetData(n, n, values, ptr, ind);  // takes ownership (Matrix_Sparse.cxx:300-377)
+  for (int i = 0; i < n; i++) xt(i) = std::sin(0.37 * i) + 0.1 * std::cos(1.3 * i);
+  Mlt(A, xt, bh);
+  cout.precision(17);
+
+  {
+    cout << "== cpu" << endl;
+    Vector<double> x(n);
+    x.Fill(0.0);
+    Iteration<double> it(2000, 1e-10);
+    it.ShowFullHistory();
+    Preconditioner_Base M;
+    int code = Cg(A, x, bh, M, it);
+    cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+  }
+  Matrix<double, General, RowSparseHip, HipAlloc<double> > Ag(A);
+  Vector<double, VectHip, HipAlloc<double> > bg(bh);
+  {
+    cout << "== gpu_fused" << endl;
+    Vector<double, VectHip, HipAlloc<double> > x(n);
+    x.Fill(0.0);
+    Iteration<double> it(2000, 1e-10);
+    it.ShowFullHistory();
+    Preconditioner_Base M;
+    int code = Cg(Ag, x, bg, M, it);
+    cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+  }
+  {
+    cout << "== gpu_generic" << endl;
+    Vector<double, VectHip, HipAlloc<double> > x(n);
+    x.Fill(0.0);
+    Iteration<double> it(2000, 1e-10);
+    it.ShowFullHistory();
+    UserIdentity M;
+    int code = Cg(Ag, x, bg, M, it);
+    cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+  }
+  return 0;
+}
