@@ -1,5 +1,4 @@
-// shk_extras.hip -- the adjoint SDDMM gradient kernel (no Seldon
-// counterpart) and the batched-CG entry points' argument checking.
+// shk_extras.hip -- the adjoint SDDMM gradient kernel (no Seldon counterpart).
 #include "shk_kernels.hpp"
 
 namespace shk {
@@ -81,18 +80,6 @@ int shk_sddmm(shk_ctx ctx, shk_csr A, const void *dlam, const void *dx, void *dg
     if (A->dtype == SHK_F64)
         return sddmm_typed<double>(ctx, A, (const double *)dlam, (const double *)dx, (double *)dg);
     return sddmm_typed<float>(ctx, A, (const float *)dlam, (const float *)dx, (float *)dg);
-}
-
-int shk_cg_batched(shk_ctx, shk_csr, int, const int *, void *, const void *, double, int, int, int, int *,
-                   int *)
-{
-    return fail(SHK_ERR_STATE, "shk_cg_batched: not available in this build");
-}
-
-int shk_cg_batched_ir(shk_ctx, shk_csr, shk_csr, int, const int *, void *, const void *, double, int, double,
-                      int, int, int *, int *)
-{
-    return fail(SHK_ERR_STATE, "shk_cg_batched_ir: not available in this build");
 }
 
 }  // extern "C"
