@@ -14,7 +14,11 @@ time to a 1e-10 relative true residual with plain FP64 GMRES(50) and with the
 FP32 GMRES(50) + FP64 iterative refinement solver, and the oracle's CPU SpMV
 on the same matrix as `cpu_baseline`.
 
+Also: the 128^3 convection-diffusion BiCgStab / Gmres(30) solves and the
+512-system batched Cg (FP64 and FP32+IR), each timed to its tolerance.
+
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--quick] [--no-cpu]
+                       [--no-solve] [--no-extra] [--probe]
 """
 import argparse
 import json
@@ -52,6 +56,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="smaller matrix (1M rows)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--layout", default="auto", choices=["auto", "rows", "panels"],
+                    help="device layout of the benchmark matrix (auto: panels where they apply)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 128^3 convection-diffusion solves and the batched Cg")
     ap.add_argument("--probe", action="store_true",
                     help="profiling run: one 1 GiB copy kernel (counter calibration) and 3 FP64 + 3 FP32 SpMVs")
     args = ap.parse_args()
@@ -72,8 +80,9 @@ def main():
     log(f"matrix {A.m} rows, {A.nnz} nnz, built in {time.time() - t0:.1f}s")
     torch.cuda.set_device(0)
     ctx = K.Context.default()
-    A64 = K.Matrix(A, dtype=torch.float64)
-    A32 = K.Matrix(A, dtype=torch.float32)
+    A64 = K.Matrix(A, dtype=torch.float64, layout=args.layout)
+    A32 = K.Matrix(A, dtype=torch.float32, layout=args.layout)
+    log(f"device layout: fp64 {A64.layout}, fp32 {A32.layout}")
     x64 = torch.tensor(xt, device="cuda")
     y64 = torch.empty_like(x64)
     x32 = x64.float()
@@ -133,7 +142,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded power-law FEM-like CSR)",
-        "config": {"workload": "FEM-like power-law CSR SpMV, FP64, Mlt(A,X,Y)",
+        "config": {"workload": "FEM-like power-law CSR SpMV, FP64, Mlt(A,X,Y)", "layout": A64.layout,
                    "rows": A.m, "nnz": A.nnz, "row_len": "P(k)~k^-1.7452 on [4,512], mean 27",
                    "band_half_width": 20000, "algorithmic_bytes_per_step": bytes64},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -145,6 +154,13 @@ def main():
 
     if not args.no_solve:
         result["solve"] = solve_compare(K, A, A64, A32, b, xt, np, torch)
+
+    if not args.no_extra:
+        del A64, A32, x64, y64, x32, y32
+        torch.cuda.empty_cache()
+        result["convdiff128"] = convdiff_solves(K, wl, np, torch)
+        torch.cuda.empty_cache()
+        result["batched"] = batched_cg(K, np, torch)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(A, xt, bytes64)
@@ -192,6 +208,92 @@ def solve_compare(K, A, A64, A32, b, xt, np, torch):
             f"relres {relir:.2e}, err vs fp64 {err:.2e}")
     except Exception as e:  # reported, not hidden
         out["fp32_ir_gmres50"] = {"error": str(e)}
+    return out
+
+
+def _true_relres(K, A, x, b, bn):
+    import torch
+    r = torch.empty_like(b)
+    K.Copy(b, r)
+    K.MltAdd(-1.0, A, x, 1.0, r)
+    return K.Norm2(r) / bn
+
+
+def convdiff_solves(K, wl, np, torch):
+    """FP64 BiCgStab and Gmres(30) with the Jacobi preconditioner on the
+    128^3 7-point upwind convection-diffusion matrix (2,097,152 rows), to a
+    1e-10 relative residual: time to solution and iterations/s, plus the
+    SpMV bandwidth on this matrix."""
+    import ctypes
+
+    from seldon_amd import _lib as L
+    A, b, xt = wl.convdiff(128)
+    Ad = K.Matrix(A, dtype=torch.float64)
+    bg = torch.tensor(b, device="cuda")
+    bn = float(np.linalg.norm(b))
+    out = {"rows": A.m, "nnz": A.nnz}
+    y = torch.empty_like(bg)
+    avg = ctypes.c_double()
+    L.check(L.load().shk_time_spmv(Ad.ctx.h, Ad.h, 0, K._ptr(bg), K._ptr(y), 20, ctypes.byref(avg)))
+    sb = Ad.spmv_bytes(False)
+    out["spmv"] = {"kernel_ms": round(avg.value, 5), "GBs": round(sb / (avg.value * 1e-3) / 1e9, 2),
+                   "frac": round(sb / (avg.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes": sb}
+    for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30)):
+        x = torch.zeros_like(bg)
+        it = K.Iteration(20000, 1e-10)
+        if restart:
+            it.SetRestart(restart)
+        it.HideMessages()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        code = fn(Ad, x, bg, K.JacobiPreconditioner(), it)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rel = _true_relres(K, Ad, x, bg, bn)
+        out[name] = {"seconds": round(dt, 4), "iterations": it.nb_iter, "code": code,
+                     "iterations_per_s": round(it.nb_iter / dt, 1), "true_relres": rel}
+        log(f"convdiff128 {name}: {dt:.3f}s, {it.nb_iter} its, code {code}, true relres {rel:.2e}")
+    return out
+
+
+def batched_cg(K, np, torch):
+    """512 anisotropic 2-D Laplacians (n x n, n in [32, 256], anisotropy
+    log-uniform in [1, 100]) solved to 1e-8 by the persistent batched Cg,
+    FP64 and FP32 inner + FP64 refinement, each under hipGraph with the
+    autotuned group size.  Bytes per Cg iteration of system s (FP64):
+    (4+8) nnz_s + 4 (n_s+1) for the matrix and 11 n_s x 8 for the vectors
+    (p = r + beta p: 3; q = A p: 2; x += alpha p, r -= alpha q, |r|^2: 6)."""
+    from seldon_amd import batched as B
+    A, row_off, b, xt, sides, eps = B.aniso_batch()
+    A64 = K.Matrix(A, dtype=torch.float64, layout="rows")
+    A32 = K.Matrix(A, dtype=torch.float32, layout="rows")
+    bg = torch.tensor(b, device="cuda")
+    n_s = np.diff(row_off).astype(np.float64)
+    nnz_s = (A.ptr[row_off[1:]] - A.ptr[row_off[:-1]]).astype(np.float64)
+    out = {"systems": len(n_s), "rows": A.m, "nnz": A.nnz}
+    for variant in ("fp64", "ir"):
+        group, entry = B.autotune(A64, row_off, bg, variant=variant, A32=A32, use_cache=False)
+        x = torch.zeros_like(bg)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if variant == "fp64":
+            its, codes = B.cg_batched(A64, row_off, x, bg, tol=1e-8, group=group, use_graph=True)
+        else:
+            its, codes = B.cg_batched_ir(A64, A32, row_off, x, bg, tol=1e-8, group=group, use_graph=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        V = 8.0 if variant == "fp64" else 4.0
+        per_it = (4 + V) * nnz_s + 4 * (n_s + 1) + 11 * n_s * V
+        gbs = float((per_it * its).sum()) / dt / 1e9
+        err = float(torch.linalg.norm(x - torch.tensor(xt, device="cuda")) / np.linalg.norm(xt))
+        out[variant] = {"seconds": round(dt, 4), "group": group, "tuned_seconds": entry["seconds"],
+                        "iterations_total": int(its.sum()), "iterations_max": int(its.max()),
+                        "iterations_per_s": round(float(its.sum()) / dt, 1),
+                        "codes_nonzero": int((codes != 0).sum()),
+                        ("GBs" if variant == "fp64" else "inner_GBs"): round(gbs, 2),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "rel_error_vs_xtrue": err}
+        log(f"batched {variant}: {dt:.3f}s group {group}, {int(its.sum())} its, {gbs:.0f} GB/s, "
+            f"err {err:.2e}, nonzero codes {int((codes != 0).sum())}")
     return out
 
 

@@ -59,6 +59,15 @@ typedef enum {
 
 typedef enum { SHK_PREC_IDENTITY = 0, SHK_PREC_JACOBI = 1 } shk_precond;
 
+/* Device layout of a matrix (an internal choice; results are identical).
+ *   ROWS   -- the caller's CSR order, cut into row blocks;
+ *   PANELS -- panels of 1024 rows whose entries are regrouped by column
+ *             windows so the x entries they read are staged on chip; for
+ *             RowSparse / ArrayRowSparse with sorted columns near the
+ *             diagonal (banded, stencil and FEM matrices in natural order);
+ *   AUTO   -- PANELS where it applies, else ROWS. */
+typedef enum { SHK_LAYOUT_AUTO = 0, SHK_LAYOUT_ROWS = 1, SHK_LAYOUT_PANELS = 2 } shk_layout;
+
 typedef struct shk_ctx_s *shk_ctx;
 typedef struct shk_csr_s *shk_csr;
 
@@ -92,6 +101,12 @@ int shk_memcpy_d2d(shk_ctx ctx, void *dst, const void *src, size_t bytes);
 int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n,
                    int64_t nnz, const int *ptr, const int *ind, const void *val,
                    int src_on_device, shk_csr *out);
+/* Same, with an explicit device layout (SHK_LAYOUT_PANELS fails with
+ * SHK_ERR_ARG where it does not apply). */
+int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n,
+                          int64_t nnz, const int *ptr, const int *ind, const void *val,
+                          int src_on_device, int layout, shk_csr *out);
+int shk_csr_layout(shk_csr A, int *layout);
 int shk_csr_destroy(shk_csr A);
 /* New values on the same pattern, in the caller's stored order (RowSymSparse:
  * upper part).  Used when A depends on parameters (the differentiable solve). */

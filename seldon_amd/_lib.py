@@ -56,6 +56,8 @@ SIGNATURES = [
     ("shk_memcpy_d2h", _I, [_P, _P, _P, _U]),
     ("shk_memcpy_d2d", _I, [_P, _P, _P, _U]),
     ("shk_csr_create", _I, [_P, _I, _I, _I, _I, _I64, _P, _P, _P, _I, _PP]),
+    ("shk_csr_create_layout", _I, [_P, _I, _I, _I, _I, _I64, _P, _P, _P, _I, _I, _PP]),
+    ("shk_csr_layout", _I, [_P, _PI]),
     ("shk_csr_destroy", _I, [_P]),
     ("shk_csr_set_values", _I, [_P, _P, _P, _I]),
     ("shk_csr_info", _I, [_P, _PI, _PI, _PI64, _PI64, _PI, _PI]),

@@ -6,6 +6,8 @@ in one launch of the library's persistent kernel (work-group per system,
 device work queue, largest systems first), optionally captured in a hipGraph;
 ``cg_batched_ir`` is the FP32-inner / FP64-refinement variant.
 
+The stacked matrices must use the row layout: ``K.Matrix(A, layout="rows")``.
+
 ``autotune`` picks the work-group size ("group": threads per system) by timing
 the candidates on the batch itself and caches the choice per batch signature
 (variant, number of systems, rows, stored entries) in ``autotune.json`` next

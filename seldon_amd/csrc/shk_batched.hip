@@ -48,12 +48,37 @@ __device__ __forceinline__ T sqrt_b(T v) { return sqrt(v); }
 template <>
 __device__ __forceinline__ float sqrt_b<float>(float v) { return sqrtf(v); }
 
+// Row access to a matrix in the row-block layout (the caller's CSR order).
+struct RowsView {
+    const int *ptr;
+    const uint32_t *ind;
+    const void *val;
+};
+
+inline RowsView rows_view(shk_csr A)
+{
+    RowsView v;
+    v.ptr = A->ptr;
+    v.ind = A->ind;
+    v.val = A->val;
+    return v;
+}
+
+// sum_j a_ij p_j of row i in stored order
+template <typename T, typename P>
+__device__ __forceinline__ T row_sum(const RowsView &M, const P *p, int i)
+{
+    const T *val = (const T *)M.val;
+    T temp = (T)0;
+    for (int j = M.ptr[i]; j < M.ptr[i + 1]; j++) temp += val[j] * (T)p[M.ind[j]];
+    return temp;
+}
+
 // y_i = (A p)_i for the rows of one system (Mlt: 0 + 1 * temp)
 template <typename T>
-__device__ __forceinline__ T row_apply(const int *ptr, const uint32_t *ind, const T *val, const T *p, int i)
+__device__ __forceinline__ T row_apply(const RowsView &M, const T *p, int i)
 {
-    T temp = (T)0;
-    for (int j = ptr[i]; j < ptr[i + 1]; j++) temp += val[j] * p[ind[j]];
+    const T temp = row_sum<T, T>(M, p, i);
     T y = (T)0;
     y += (T)1 * temp;
     return y;
@@ -63,7 +88,7 @@ __device__ __forceinline__ T row_apply(const int *ptr, const uint32_t *ind, cons
 // initial guess when !zero_x.  Returns the iteration count; *code gets
 // ErrorCode().  facteur is 1/|b| (Init), already checked nonzero.
 template <typename T>
-__device__ int group_cg(int r0, int r1, const int *ptr, const uint32_t *ind, const T *val, const T *b, T *x,
+__device__ int group_cg(int r0, int r1, const RowsView &M, const T *b, T *x,
                         T *r, T *p, T *q, double facteur, double tol, int max_iter, T *buf, int &phase,
                         int *code)
 {
@@ -99,7 +124,7 @@ __device__ int group_cg(int r0, int r1, const int *ptr, const uint32_t *ind, con
         __syncthreads();
         part = (T)0;
         for (int i = r0 + tid; i < r1; i += G) {
-            const T qi = row_apply(ptr, ind, val, p, i);
+            const T qi = row_apply<T>(M, p, i);
             q[i] = qi;
             part += p[i] * qi;
         }
@@ -129,10 +154,7 @@ struct BatchArgs {
     int count;
     const int *order;    // queue position -> system
     const int *row_off;  // [count+1]
-    const int *ptr;
-    const uint32_t *ind;
-    const void *val64;
-    const void *val32;
+    RowsView A64, A32;
     double *x;
     const double *b;
     void *w64;           // FP64 work: r, p, q (3 * n) or r (n) for the mixed variant
@@ -167,7 +189,7 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched(BatchArgs a)
         const double bn = sqrt(group_allreduce(part, buf, phase));
         int code = 0, its = 0;
         if (bn != 0.0)
-            its = group_cg<double>(r0, r1, a.ptr, a.ind, (const double *)a.val64, a.b, a.x, r, p, q, 1.0 / bn,
+            its = group_cg<double>(r0, r1, a.A64, a.b, a.x, r, p, q, 1.0 / bn,
                                    a.tol, a.max_iter, buf, phase, &code);
         if (threadIdx.x == 0) {
             a.its[s] = its;
@@ -185,8 +207,6 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched_ir(BatchArgs a)
     int phase = 0, phase32 = 0;
     double *r64 = (double *)a.w64;
     float *r32 = a.w32, *p32 = r32 + a.n, *q32 = p32 + a.n, *d32 = q32 + a.n;
-    const double *v64 = (const double *)a.val64;
-    const float *v32 = (const float *)a.val32;
     for (;;) {
         if (threadIdx.x == 0) s_q = (int)atomicAdd(a.queue, 1u);
         __syncthreads();
@@ -207,8 +227,7 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched_ir(BatchArgs a)
                 // true residual r = b - A x in FP64
                 part = 0;
                 for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-                    double temp = 0.0;
-                    for (int j = a.ptr[i]; j < a.ptr[i + 1]; j++) temp += v64[j] * a.x[a.ind[j]];
+                    const double temp = row_sum<double, double>(a.A64, a.x, i);
                     double w = a.b[i];
                     w += -1.0 * temp;
                     r64[i] = w;
@@ -228,7 +247,7 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched_ir(BatchArgs a)
                 int icode = 0;
                 // right-hand side and residual share r32: Cg copies b into r
                 // first (r = b) and reads b nowhere else
-                total += group_cg<float>(r0, r1, a.ptr, a.ind, v32, r32, d32, r32, p32, q32,
+                total += group_cg<float>(r0, r1, a.A32, r32, d32, r32, p32, q32,
                                          1.0 / (double)rh, a.inner_tol, a.max_iter, buf32, phase32, &icode);
                 for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) a.x[i] += rn * (double)d32[i];
                 __syncthreads();
@@ -254,6 +273,8 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
     SHK_CHECK(A64->ctx == ctx, SHK_ERR_ARG, "cg_batched: matrix of another context");
     SHK_CHECK(A64->storage == SHK_ROWSPARSE && A64->dtype == SHK_F64, SHK_ERR_ARG,
               "cg_batched: the stacked matrix must be FP64 RowSparse");
+    SHK_CHECK(A64->layout == SHK_LAYOUT_ROWS && (!mixed || !A32 || A32->layout == SHK_LAYOUT_ROWS), SHK_ERR_ARG,
+              "cg_batched: create the stacked matrices with SHK_LAYOUT_ROWS");
     SHK_CHECK(count >= 0, SHK_ERR_ARG, "cg_batched: negative count");
     if (mixed)
         SHK_CHECK(A32 && A32->ctx == ctx && A32->dtype == SHK_F32 && A32->storage == SHK_ROWSPARSE &&
@@ -298,10 +319,8 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
     a.count = count;
     a.order = d_order;
     a.row_off = d_off;
-    a.ptr = A64->ptr;
-    a.ind = A64->ind;
-    a.val64 = A64->val;
-    a.val32 = mixed ? A32->val : nullptr;
+    a.A64 = rows_view(A64);
+    if (mixed) a.A32 = rows_view(A32);
     a.x = (double *)dx;
     a.b = (const double *)db;
     a.w64 = d_w64;
@@ -324,12 +343,19 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
         return SHK_OK;
     };
     if (use_graph) {
-        // the queue reset is a kernel node, not a memset node
+        // Recorded on the context's own non-blocking stream (the legacy
+        // default stream cannot be captured), launched on the caller's stream
+        // so it stays ordered after the copies above.  The queue reset is a
+        // kernel node, not a memset node.
         hipGraph_t graph;
         hipGraphExec_t exec;
-        SHK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipStream_t cs = ctx->own;
+        hipStream_t keep = st;
+        st = cs;
+        SHK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         int e = enqueue();
-        hipError_t ce = hipStreamEndCapture(st, &graph);
+        hipError_t ce = hipStreamEndCapture(cs, &graph);
+        st = keep;
         SHK_TRY(e);
         SHK_HIP(ce);
         SHK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));

@@ -32,6 +32,42 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(int nblk, int xcd_q, const int
     }
 }
 
+// Panel layout: one work-group per panel walks its chunks in order with a
+// per-row count of the entries already seen, so each entry's gradient lands
+// at the caller's stored position ptr[row] + k.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sddmm_panel(const int *pchunk, const int4 *chunk,
+                                                        const uint32_t *word, const int *ptr, const T *lam,
+                                                        const T *x, T *g)
+{
+    __shared__ int cnt[kPanelRows], segs[kPanelRows];
+    const int p = blockIdx.x, r0 = p * kPanelRows, t = threadIdx.x;
+    for (int i = t; i < kPanelRows; i += kBlock) cnt[i] = 0;
+    for (int cc = pchunk[p]; cc < pchunk[p + 1]; cc++) {
+        const int4 ci = chunk[cc];
+        __syncthreads();
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            if (w & kPanFirst) segs[(w >> kPanRowShift) & kPanRowMask] = e;
+        }
+        __syncthreads();
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            const int lr = (w >> kPanRowShift) & kPanRowMask;
+            const int col = ci.x + (int)(w & kPanColMask);
+            g[ptr[r0 + lr] + cnt[lr] + (e - segs[lr])] = -(lam[r0 + lr] * x[col]);
+        }
+        __syncthreads();
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            if (w & kPanLast) {
+                const int lr = (w >> kPanRowShift) & kPanRowMask;
+                cnt[lr] += e - segs[lr] + 1;
+            }
+        }
+    }
+}
+
 // RowSymSparse: an upper entry (i, c) stands for a_ic and a_ci:
 // g = -(lam_i x_c) + -(lam_c x_i); the diagonal has only the first term.
 template <typename T>
@@ -50,10 +86,15 @@ static int sddmm_typed(shk_ctx ctx, shk_csr A, const T *lam, const T *x, T *g)
 {
     T *G = g;
     if (A->storage == SHK_ROWSYMSPARSE) {
-        SHK_TRY(ensure_ws2(ctx, sizeof(T) * (size_t)std::max<int64_t>(A->nnz_dev, 1)));
+        SHK_TRY(ensure_ws2(ctx, sizeof(T) * (size_t)std::max<int64_t>(A->nslots, 1)));
         G = (T *)ctx->ws2;
     }
-    if (A->nblk > 0)
+    if (A->layout == SHK_LAYOUT_PANELS) {
+        if (A->npanel)
+            hipLaunchKernelGGL(k_sddmm_panel<T>, dim3(A->npanel), dim3(kBlock), 0, ctx->stream,
+                               (const int *)A->pchunk, (const int4 *)A->chunk, (const uint32_t *)A->ind,
+                               (const int *)A->ptr, lam, x, G);
+    } else if (A->nblk > 0)
         hipLaunchKernelGGL(k_sddmm<T>, dim3(A->nblk_launch), dim3(kBlock), 0, ctx->stream, A->nblk, A->xcd_q,
                            (const int4 *)A->blk_info, (const int *)A->ptr, (const uint32_t *)A->ind, lam, x, G);
     if (A->storage == SHK_ROWSYMSPARSE && A->nnz_stored > 0)

@@ -104,7 +104,14 @@ struct shk_csr_s {
     int64_t *map_up = nullptr;
     int64_t *map_lo = nullptr;
     int64_t *map_diag = nullptr; // [m] position of a_ii on device (-1: none)
-    // ArrayRowSparse / RowSparse values are stored in caller order (map = id)
+    // Row-block layout: RowSparse / ArrayRowSparse values are stored in the
+    // caller's order.  Panel layout: see k_panel (the maps above point into
+    // the panel arrays; ptr keeps the caller's row pointer).
+    int layout = 0;           // SHK_LAYOUT_ROWS or SHK_LAYOUT_PANELS
+    int64_t nslots = 0;       // length of ind/val without the load padding
+    int npanel = 0, nchunk = 0;
+    int *pchunk = nullptr;    // [npanel+1] first chunk of each panel
+    int *chunk = nullptr;     // [4*nchunk] {c0, c1, e0, e1} per chunk
 };
 
 namespace shk {

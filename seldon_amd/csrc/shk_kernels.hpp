@@ -26,6 +26,22 @@
 
 namespace shk {
 
+// Entries staged per row block (LDS products): 16 KiB per work-group.
+#ifndef SHK_BUDGET_F64
+#define SHK_BUDGET_F64 2048
+#endif
+#ifndef SHK_BUDGET_F32
+#define SHK_BUDGET_F32 4096
+#endif
+template <typename T>
+constexpr int budget_of() { return sizeof(T) == 8 ? SHK_BUDGET_F64 : SHK_BUDGET_F32; }
+#ifndef SHK_SPMV_WG_PER_CU
+#define SHK_SPMV_WG_PER_CU 4
+#endif
+// Work-groups of the persistent SpMV per CU; the kernel is compiled for
+// that occupancy (4 waves per SIMD: at most 128 VGPRs).
+constexpr int kSpmvWgPerCu = SHK_SPMV_WG_PER_CU;
+
 enum SpmvMode { kCsr = 0, kSym = 1, kArray = 2 };
 enum DotKind { kDotNone = 0, kDotVec = 1, kDotSelf = 2 };
 
@@ -50,34 +66,19 @@ struct SpmvArgs {
     T *partials;
     unsigned *ticket;
     const int *skip;   // device flag: nonzero = return at once (solver done)
+    // panel layout
+    int npanel;
+    const int *pchunk;   // [npanel+1] first chunk of each panel
 };
+
+// Entries seq_sum may read past the end of a row (its look-ahead window);
+// their values are never used.
+constexpr int kSeqPad = 8;
 
 template <typename T, int BUDGET>
 struct alignas(16) SpmvSmem {
-    T prod[BUDGET];
+    T prod[BUDGET + kSeqPad];
 };
-
-// Row result from the staged products [p0, p1) with the temp/direct split pm.
-template <typename T, int MODE>
-__device__ __forceinline__ T spmv_row_result(const SpmvArgs<T> &a, int r, const T *prod, int p0,
-                                             int pm, int p1)
-{
-    T y;
-    if (a.mlt) {
-        y = (T)0;
-    } else {
-        T yin = a.y_in[r];
-        if (MODE == kArray) y = (a.beta == (T)0) ? (T)0 : yin * a.beta;
-        else y = yin * a.beta;
-    }
-    if (MODE != kArray) {
-        T temp = (T)0;
-        for (int q = p0; q < pm; q++) temp += prod[q];
-        y += a.alpha * temp;
-    }
-    for (int q = pm; q < p1; q++) y += prod[q];
-    return y;
-}
 
 template <typename T, int MODE>
 __device__ __forceinline__ void spmv_epilogue(const SpmvArgs<T> &a, int r, T y, T (&c)[2])
@@ -221,20 +222,37 @@ __device__ __forceinline__ void row_meta_load(const SpmvArgs<T> &a, int4 bi, Row
     M.yin = a.mlt ? (T)0 : a.y_in[rr];
 }
 
-// Sequential sum of prod[q0, q1) onto acc, four products per step so the LDS
-// reads overlap; the additions stay strictly left to right.
+// Sequential sum of prod[q0, q1) onto acc, strictly left to right.  Two
+// register groups of eight alternate: while one group is added, the LDS reads
+// of the other are in flight, so a long row costs about one dependent add per
+// product instead of one LDS round trip per group.  Reads reach at most
+// kSeqPad entries past q1.
 template <typename T>
 __device__ __forceinline__ T seq_sum(T acc, const T *prod, int q0, int q1)
 {
+    constexpr int W = kSeqPad / 2;
+    T A[W], B[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) A[k] = prod[q0 + k];
+#pragma unroll
+    for (int k = 0; k < W; k++) B[k] = prod[q0 + W + k];
     int q = q0;
-    for (; q + 4 <= q1; q += 4) {
-        const T b0 = prod[q], b1 = prod[q + 1], b2 = prod[q + 2], b3 = prod[q + 3];
-        acc += b0;
-        acc += b1;
-        acc += b2;
-        acc += b3;
+    for (; q + 2 * W <= q1; q += 2 * W) {
+#pragma unroll
+        for (int k = 0; k < W; k++) acc += A[k];
+#pragma unroll
+        for (int k = 0; k < W; k++) A[k] = prod[q + 2 * W + k];
+#pragma unroll
+        for (int k = 0; k < W; k++) acc += B[k];
+#pragma unroll
+        for (int k = 0; k < W; k++) B[k] = prod[q + 3 * W + k];
     }
-    for (; q < q1; q++) acc += prod[q];
+#pragma unroll
+    for (int k = 0; k < W; k++)
+        if (q + k < q1) acc += A[k];
+#pragma unroll
+    for (int k = 0; k < W; k++)
+        if (q + W + k < q1) acc += B[k];
     return acc;
 }
 
@@ -316,7 +334,7 @@ __device__ __forceinline__ void spmv_long_row(const SpmvArgs<T> &a, int4 bi, T *
 // block i, the streaming loads and row metadata of block i+W are already in
 // flight, and the block table entry of block i+2W is being fetched.
 template <typename T, int MODE, int BUDGET, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<T> a, const int4 *__restrict__ blk_info, Fin fin)
+__global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T> a, const int4 *__restrict__ blk_info, Fin fin)
 {
     if (a.skip && *a.skip) return;
     __shared__ SpmvSmem<T, BUDGET> sm;
@@ -373,19 +391,203 @@ __global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<T> a, const int4 *__re
     }
 }
 
-// Entries staged per row block (LDS products): 16 KiB per work-group.
-#ifndef SHK_BUDGET_F64
-#define SHK_BUDGET_F64 2048
-#endif
-#ifndef SHK_BUDGET_F32
-#define SHK_BUDGET_F32 4096
-#endif
+// ------------------------------------------------------- panel layout ----
+// Rows are grouped in panels of kPanelRows consecutive rows, one work-group
+// per panel, each thread owning kPanelRpt rows (t, t+256, ...) and their
+// running sums in registers.  A panel's entries are stored as a sequence of
+// chunks, each chunk holding, row after row, the entries of the panel whose
+// columns fall in one window [c0, c1) of at most panel_win<T>() columns, at
+// most `budget` entries in all.  Seldon's rows have sorted columns, so
+// walking the chunks in order visits every row's entries in stored order:
+// the row sums stay sequential (Functions_MatVect.cxx:424-430) and bitwise
+// equal to the CPU's.  Per chunk the x window is staged in LDS with coalesced
+// loads, so the products read x from LDS instead of gathering one cache line
+// per entry from L2.  Each entry's 32-bit word packs its window-local column
+// (13 bits), its panel-local row (10 bits), "first/last entry of the row in
+// this chunk" flags, and kLowerFlag (ArrayRowSparse: a direct term).  The
+// next chunk's entries and x window are loaded into registers while the
+// current one is summed.
+constexpr int kPanelRows = 1024;
+constexpr int kPanelRpt = kPanelRows / kBlock;
+constexpr uint32_t kPanColMask = 0x1FFFu;
+constexpr int kPanRowShift = 13;
+constexpr uint32_t kPanRowMask = 0x3FFu;
+constexpr uint32_t kPanFirst = 1u << 23;
+constexpr uint32_t kPanLast = 1u << 24;
 template <typename T>
-constexpr int budget_of() { return sizeof(T) == 8 ? SHK_BUDGET_F64 : SHK_BUDGET_F32; }
-#ifndef SHK_SPMV_WG_PER_CU
-#define SHK_SPMV_WG_PER_CU 4
-#endif
-constexpr int kSpmvWgPerCu = SHK_SPMV_WG_PER_CU;
+constexpr int panel_win() { return sizeof(T) == 8 ? 3072 : 6144; }  // 24 KiB of x
+
+template <typename T, int BUDGET>
+struct alignas(16) PanelSmem {
+    T xs[panel_win<T>()];
+    T prod[BUDGET];
+    unsigned short seg_s[kPanelRows], seg_e[kPanelRows];
+};
+
+template <typename T>
+struct PanelX {
+    static constexpr int XK = panel_win<T>() / kBlock;
+    T v[XK];
+};
+
+// x[c0 + i] for i < w (an index past w re-reads x[c0]: in bounds, unused)
+template <typename T>
+__device__ __forceinline__ void panel_xload(const T *x, int4 ci, PanelX<T> &X)
+{
+    const int t = threadIdx.x, w = ci.y - ci.x;
+#pragma unroll
+    for (int j = 0; j < PanelX<T>::XK; j++) {
+        const int i = j * kBlock + t;
+        X.v[j] = x[ci.x + (i < w ? i : 0)];
+    }
+}
+
+template <typename T, int MODE, int BUDGET, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_panel(SpmvArgs<T> a, const int4 *__restrict__ chunk, Fin fin)
+{
+    if (a.skip && *a.skip) return;
+    __shared__ PanelSmem<T, BUDGET> sm;
+    constexpr int V = VecOf<T>::V, K = BUDGET / (kBlock * V);
+    const int p = xcd_logical_block(a.xcd_q);
+    const int t = threadIdx.x;
+    T c[2] = {(T)0, (T)0};
+    if (p < a.npanel) {
+        const int r0 = p * kPanelRows;
+        const int nr = min(kPanelRows, a.m - r0);
+        T acc[kPanelRpt], yin[kPanelRpt];
+#pragma unroll
+        for (int i = 0; i < kPanelRpt; i++) {
+            const int rr = t + i * kBlock;
+            yin[i] = (!a.mlt && rr < nr) ? a.y_in[r0 + rr] : (T)0;
+            sm.seg_s[rr] = 0;
+            sm.seg_e[rr] = 0;
+        }
+#pragma unroll
+        for (int i = 0; i < kPanelRpt; i++) {
+            // ArrayRowSparse adds the direct terms onto beta * y (zero-filled
+            // when beta is 0); RowSparse sums a temp first
+            if (MODE == kArray) acc[i] = a.mlt ? (T)0 : ((a.beta == (T)0) ? (T)0 : yin[i] * a.beta);
+            else acc[i] = (T)0;
+        }
+        const int cb = a.pchunk[p], ce = a.pchunk[p + 1];
+        int4 ci = make_int4(0, 0, 0, 0);
+        SpmvRegs<T, BUDGET> R;
+        PanelX<T> X;
+        if (cb < ce) {
+            ci = chunk[cb];
+            spmv_load<T, BUDGET>(a, ci.z, R);
+            panel_xload<T>(a.x, ci, X);
+        }
+        for (int cc = cb; cc < ce; cc++) {
+            // this chunk's x window into LDS (its loads were issued one chunk
+            // ago), then the next chunk's loads, which have the whole of this
+            // chunk's work to arrive (the barriers pin the order)
+#pragma unroll
+            for (int j = 0; j < PanelX<T>::XK; j++) sm.xs[j * kBlock + t] = X.v[j];
+            __builtin_amdgcn_sched_barrier(0);
+            const bool more = cc + 1 < ce;
+            const int4 cn = more ? chunk[cc + 1] : ci;
+            SpmvRegs<T, BUDGET> Rn;
+            PanelX<T> Xn;
+            if (more) {
+                spmv_load<T, BUDGET>(a, cn.z, Rn);
+                panel_xload<T>(a.x, cn, Xn);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();  // x window in; the previous chunk's sums are done
+            // products a_j * x_col(j) into LDS, row segments marked; the x
+            // reads are all issued before any LDS write
+            T xv[K][V];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    const bool in = ci.z + (k * kBlock + t) * V + v < ci.w;
+                    xv[k][v] = sm.xs[in ? (R.iv[k][v] & kPanColMask) : 0u];
+                }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                typename VecOf<T>::FV pv;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    const bool in = ci.z + (k * kBlock + t) * V + v < ci.w;
+                    const T val = R.fv[k][v];
+                    // direct terms multiply alpha into the value first:
+                    // `alpha * data[j] * X(i)` (Functions_MatrixArray.cxx:571)
+                    const T pr = (MODE == kArray) ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
+                    pv[v] = in ? pr : (T)0;
+                }
+                *(typename VecOf<T>::FV *)(sm.prod + (k * kBlock + t) * V) = pv;
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    const int el = (k * kBlock + t) * V + v;
+                    const bool in = ci.z + el < ci.w;
+                    const uint32_t w = R.iv[k][v];
+                    const int lr = (w >> kPanRowShift) & kPanRowMask;
+                    if (in && (w & kPanFirst)) sm.seg_s[lr] = (unsigned short)el;
+                    if (in && (w & kPanLast)) sm.seg_e[lr] = (unsigned short)(el + 1);
+                }
+            __syncthreads();
+            // each thread adds its rows' segments, in order, its rows side by
+            // side (reads first, then the adds)
+            int s0[kPanelRpt], n0[kPanelRpt], nmax = 0;
+#pragma unroll
+            for (int i = 0; i < kPanelRpt; i++) {
+                const int rr = t + i * kBlock;
+                s0[i] = sm.seg_s[rr];
+                n0[i] = (int)sm.seg_e[rr] - s0[i];
+                nmax = max(nmax, n0[i]);
+            }
+            for (int j = 0; j < nmax; j += 2) {
+                T pv[2][kPanelRpt];
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int i = 0; i < kPanelRpt; i++)
+                        pv[h][i] = sm.prod[(j + h < n0[i]) ? s0[i] + j + h : 0];
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int i = 0; i < kPanelRpt; i++) {
+                        const T sum = acc[i] + pv[h][i];
+                        acc[i] = (j + h < n0[i]) ? sum : acc[i];
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < kPanelRpt; i++) {
+                const int rr = t + i * kBlock;
+                sm.seg_s[rr] = 0;
+                sm.seg_e[rr] = 0;
+            }
+            ci = cn;
+            R = Rn;
+            X = Xn;
+        }
+#pragma unroll
+        for (int i = 0; i < kPanelRpt; i++) {
+            const int rr = t + i * kBlock;
+            if (rr < nr) {
+                T y;
+                if (MODE == kArray) {
+                    y = acc[i];
+                } else {
+                    y = a.mlt ? (T)0 : yin[i] * a.beta;
+                    y += a.alpha * acc[i];
+                }
+                spmv_epilogue<T, MODE>(a, r0 + rr, y, c);
+            }
+        }
+    }
+    if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
+        T out[2];
+        if (grid_sum<T, 2>(c, a.partials, a.ticket, gridDim.x, blockIdx.x, out)) {
+            if (threadIdx.x == 0) fin(out);
+        }
+    }
+}
 
 // ------------------------------------------------------------ launch ----
 template <typename T, typename Fin>
@@ -402,12 +604,25 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     a.mid = A->mid;
     a.ind = A->ind;
     a.val = (const T *)A->val;
+    a.npanel = A->npanel;
+    a.pchunk = A->pchunk;
     if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
         SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * (size_t)A->nblk_launch));
         a.partials = (T *)ctx->red.partials;
         a.ticket = ctx->red.ticket;
     }
     if (A->m == 0) return SHK_OK;
+    if (A->layout == SHK_LAYOUT_PANELS) {
+        constexpr int PB = budget_of<T>();
+        dim3 grid(A->nblk_launch), block(kBlock);
+        const int4 *chunk = (const int4 *)A->chunk;
+        if (A->storage == SHK_ARRAYROWSPARSE)
+            hipLaunchKernelGGL((k_panel<T, kArray, PB, Fin>), grid, block, 0, ctx->stream, a, chunk, fin);
+        else
+            hipLaunchKernelGGL((k_panel<T, kCsr, PB, Fin>), grid, block, 0, ctx->stream, a, chunk, fin);
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }
     constexpr int B = budget_of<T>();
     // persistent grid: at most kSpmvWgPerCu work-groups per CU, a multiple of 8
     const int per_xcd = std::min(A->xcd_q, std::max(1, ctx->num_cus * kSpmvWgPerCu / kNumXcd));

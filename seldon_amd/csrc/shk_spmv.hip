@@ -2,6 +2,7 @@
 // entry points of the C ABI.
 #include <algorithm>
 #include <cstring>
+#include <numeric>
 
 #include "shk_kernels.hpp"
 
@@ -123,6 +124,177 @@ static int build_plan(int storage, int m, int n, int64_t nnz, const int *ptr, co
     return SHK_OK;
 }
 
+// ------------------------------------------------------- panel layout ----
+// Host side of the layout read by k_panel: per panel of kPanelRows rows, the
+// entries are cut into chunks by column windows.  The cut walks the panel's
+// columns in increasing order and closes a chunk before it would exceed
+// `budget` entries or panel_win columns; every chunk then lists, row after
+// row, that row's entries inside the window (a contiguous run, the row's
+// columns being sorted).  Chunk starts are aligned to the load width.
+struct PanelPlan {
+    int npanel = 0;
+    std::vector<int> pchunk;        // [npanel+1]
+    std::vector<int> chunk;         // 4 per chunk: c0, c1, e0, e1
+    std::vector<uint32_t> word;     // packed entry words
+    std::vector<int64_t> src;       // panel position -> expanded CSR index (-1: padding)
+    std::vector<int64_t> diag_pos;  // [m] panel position of a_ii
+    int64_t nslots = 0;
+    double x_bytes = 0;             // x window bytes per SpMV
+};
+
+// Build the panel layout, or return false when it does not apply: a row
+// with unsorted columns, or x windows that would cost more traffic than the
+// matrix itself (columns spread far from the diagonal).
+static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, int align, size_t vsize,
+                        PanelPlan &Q)
+{
+    if (m == 0 || !P.mid.empty()) return false;
+    for (int r = 0; r < m; r++)
+        for (int j = P.ptr[r] + 1; j < P.ptr[r + 1]; j++)
+            if ((P.ind[j] & ~kLowerFlag) < (P.ind[j - 1] & ~kLowerFlag)) return false;
+    const int64_t nnz = P.ptr[m];
+    Q.npanel = (m + kPanelRows - 1) / kPanelRows;
+    Q.pchunk.assign(Q.npanel + 1, 0);
+    Q.chunk.clear();
+    Q.word.clear();
+    Q.src.clear();
+    Q.word.reserve(nnz + nnz / 8 + 64);
+    Q.src.reserve(nnz + nnz / 8 + 64);
+    Q.diag_pos.assign(m, -1);
+    std::vector<int> cols, cur(kPanelRows);
+    for (int p = 0; p < Q.npanel; p++) {
+        const int r0 = p * kPanelRows, r1 = std::min(m, r0 + kPanelRows);
+        cols.assign(P.ind.begin() + P.ptr[r0], P.ind.begin() + P.ptr[r1]);
+        for (int &c : cols) c &= (int)~kLowerFlag;
+        std::sort(cols.begin(), cols.end());
+        for (int r = r0; r < r1; r++) cur[r - r0] = P.ptr[r];
+        size_t i = 0;
+        while (i < cols.size()) {
+            const int c0 = cols[i];
+            size_t k = i;
+            // whole columns only, at most `budget` entries and `win` columns
+            while (k < cols.size() && cols[k] - c0 < win) {
+                size_t k2 = k;
+                while (k2 < cols.size() && cols[k2] == cols[k]) k2++;
+                if ((int64_t)(k2 - i) > budget) break;
+                k = k2;
+            }
+            if (k == i) return false;  // one column with more entries than a chunk holds
+            const int c1 = cols[k - 1] + 1;
+            while ((int64_t)Q.word.size() % align) {
+                Q.word.push_back(0u);
+                Q.src.push_back(-1);
+            }
+            const int64_t e0 = (int64_t)Q.word.size();
+            for (int r = r0; r < r1; r++) {
+                int j = cur[r - r0];
+                const int jb = j;
+                while (j < P.ptr[r + 1] && (int)(P.ind[j] & ~kLowerFlag) < c1) j++;
+                for (int q = jb; q < j; q++) {
+                    const uint32_t col = P.ind[q] & ~kLowerFlag;
+                    uint32_t w = (col - (uint32_t)c0) | ((uint32_t)(r - r0) << kPanRowShift) |
+                                 (P.ind[q] & kLowerFlag);
+                    if (q == jb) w |= kPanFirst;
+                    if (q == j - 1) w |= kPanLast;
+                    if (P.diag_pos[r] == q) Q.diag_pos[r] = (int64_t)Q.word.size();
+                    Q.word.push_back(w);
+                    Q.src.push_back(q);
+                }
+                cur[r - r0] = j;
+            }
+            const int64_t e1 = (int64_t)Q.word.size();
+            if (e1 >= (int64_t)INT32_MAX) return false;
+            Q.chunk.push_back(c0);
+            Q.chunk.push_back(c1);
+            Q.chunk.push_back((int)e0);
+            Q.chunk.push_back((int)e1);
+            Q.x_bytes += (double)(c1 - c0) * (double)vsize;
+            i = k;
+        }
+        for (int r = r0; r < r1; r++)
+            if (cur[r - r0] != P.ptr[r + 1]) return false;  // not reached with sorted rows
+        Q.pchunk[p + 1] = (int)(Q.chunk.size() / 4);
+    }
+    Q.nslots = (int64_t)Q.word.size();
+    (void)n;
+    // the windows replace per-entry gathers; they must not cost more bytes
+    // than the entries they serve
+    return Q.x_bytes <= 1.5 * (double)nnz * (double)(4 + vsize);
+}
+
+template <typename T>
+static __global__ void k_panel_scatter(const int *pchunk, const int4 *chunk, const uint32_t *word, int m,
+                                       const int *ptr, const T *src, T *dst)
+{
+    // Positions of the caller's stored entries in the panel arrays: one
+    // work-group per panel walks its chunks in order, counting per row the
+    // entries already placed (a row's k-th entry lands in the chunk whose
+    // window holds its column).
+    __shared__ int cnt[kPanelRows], segs[kPanelRows];
+    const int p = blockIdx.x, r0 = p * kPanelRows, t = threadIdx.x;
+    for (int i = t; i < kPanelRows; i += kBlock) cnt[i] = 0;
+    for (int cc = pchunk[p]; cc < pchunk[p + 1]; cc++) {
+        const int4 ci = chunk[cc];
+        __syncthreads();
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            if (w & kPanFirst) segs[(w >> kPanRowShift) & kPanRowMask] = e;
+        }
+        __syncthreads();
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const int lr = (word[e] >> kPanRowShift) & kPanRowMask;
+            dst[e] = src[ptr[r0 + lr] + cnt[lr] + (e - segs[lr])];
+        }
+        __syncthreads();
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            if (w & kPanLast) {
+                const int lr = (w >> kPanRowShift) & kPanRowMask;
+                cnt[lr] += e - segs[lr] + 1;
+            }
+        }
+    }
+    (void)m;
+}
+
+template <typename T>
+static int upload_panels(shk_csr A, const HostPlan &P, const PanelPlan &Q, const T *val_host)
+{
+    const int m = A->m;
+    const int64_t ns = Q.nslots;
+    const int64_t pad = A->budget + 16;  // k_panel streams whole budget windows
+    const int nchunk = (int)(Q.chunk.size() / 4);
+    A->nslots = ns;
+    A->npanel = Q.npanel;
+    A->nchunk = nchunk;
+    std::vector<uint32_t> word(ns + pad, 0u);
+    std::copy(Q.word.begin(), Q.word.end(), word.begin());
+    hipStream_t s = A->ctx->stream;
+    SHK_HIP(hipMalloc(&A->ptr, sizeof(int) * (m + 1)));
+    SHK_HIP(hipMemcpyAsync(A->ptr, P.ptr.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, s));
+    SHK_HIP(hipMalloc(&A->ind, sizeof(uint32_t) * (ns + pad)));
+    SHK_HIP(hipMalloc(&A->val, sizeof(T) * (ns + pad)));
+    SHK_HIP(hipMalloc(&A->diag, sizeof(T) * std::max(m, 1)));
+    SHK_HIP(hipMalloc(&A->pchunk, sizeof(int) * (Q.npanel + 1)));
+    SHK_HIP(hipMalloc(&A->chunk, sizeof(int) * std::max<size_t>(Q.chunk.size(), 4)));
+    SHK_HIP(hipMalloc(&A->map_diag, sizeof(int64_t) * std::max(m, 1)));
+    SHK_HIP(hipMemcpyAsync(A->ind, word.data(), sizeof(uint32_t) * (ns + pad), hipMemcpyHostToDevice, s));
+    SHK_HIP(hipMemcpyAsync(A->pchunk, Q.pchunk.data(), sizeof(int) * (Q.npanel + 1), hipMemcpyHostToDevice, s));
+    if (nchunk)
+        SHK_HIP(hipMemcpyAsync(A->chunk, Q.chunk.data(), sizeof(int) * Q.chunk.size(), hipMemcpyHostToDevice, s));
+    SHK_HIP(hipMemcpyAsync(A->map_diag, Q.diag_pos.data(), sizeof(int64_t) * m, hipMemcpyHostToDevice, s));
+    std::vector<T> val(ns + pad, (T)0);
+    if (val_host)
+        for (int64_t q = 0; q < ns; q++)
+            if (Q.src[q] >= 0) val[q] = val_host[Q.src[q]];
+    SHK_HIP(hipMemcpyAsync(A->val, val.data(), sizeof(T) * (ns + pad), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gather_diag<T>, dim3(red_blocks(m)), dim3(kBlock), 0, s, m,
+                       (const int64_t *)A->map_diag, (const T *)A->val, (T *)A->diag);
+    SHK_HIP(hipGetLastError());
+    SHK_HIP(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
 template <typename T>
 static int upload(shk_csr A, const HostPlan &P, const T *val_host)
 {
@@ -190,7 +362,12 @@ template <typename T>
 static int set_values_dev(shk_csr A, const T *dval)
 {
     hipStream_t s = A->ctx->stream;
-    if (A->storage == SHK_ROWSYMSPARSE) {
+    if (A->layout == SHK_LAYOUT_PANELS) {
+        if (A->npanel)
+            hipLaunchKernelGGL(k_panel_scatter<T>, dim3(A->npanel), dim3(kBlock), 0, s, (const int *)A->pchunk,
+                               (const int4 *)A->chunk, (const uint32_t *)A->ind, A->m, (const int *)A->ptr, dval,
+                               (T *)A->val);
+    } else if (A->storage == SHK_ROWSYMSPARSE) {
         if (A->nnz_stored)
             hipLaunchKernelGGL(k_scatter_vals<T>, dim3(red_blocks(A->nnz_stored)), dim3(kBlock), 0, s,
                                A->nnz_stored, dval, (const int64_t *)A->map_up,
@@ -235,7 +412,17 @@ extern "C" {
 int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n, int64_t nnz,
                    const int *ptr, const int *ind, const void *val, int src_on_device, shk_csr *out)
 {
+    return shk_csr_create_layout(ctx, storage, dtype, m, n, nnz, ptr, ind, val, src_on_device, SHK_LAYOUT_AUTO,
+                                 out);
+}
+
+int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n, int64_t nnz,
+                          const int *ptr, const int *ind, const void *val, int src_on_device, int want,
+                          shk_csr *out)
+{
     SHK_CHECK(ctx && out, SHK_ERR_ARG, "shk_csr_create: null argument");
+    SHK_CHECK(want == SHK_LAYOUT_AUTO || want == SHK_LAYOUT_ROWS || want == SHK_LAYOUT_PANELS, SHK_ERR_ARG,
+              "shk_csr_create: bad layout");
     *out = nullptr;
     SHK_CHECK(m >= 0 && n >= 0 && nnz >= 0, SHK_ERR_ARG, "shk_csr_create: negative size");
     SHK_CHECK(storage >= 0 && storage <= 2, SHK_ERR_ARG, "shk_csr_create: bad storage");
@@ -262,7 +449,17 @@ int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int
     const int budget = dtype == SHK_F64 ? budget_of<double>() : budget_of<float>();
     const int align = dtype == SHK_F64 ? VecOf<double>::V : VecOf<float>::V;
     SHK_TRY(build_plan(storage, m, n, nnz, hptr.data(), hind.data(), budget, align, P));
+    PanelPlan Q;
+    int layout = SHK_LAYOUT_ROWS;
+    if (want != SHK_LAYOUT_ROWS) {
+        const int win = dtype == SHK_F64 ? panel_win<double>() : panel_win<float>();
+        if (plan_panels(m, n, P, budget, win, align, vs, Q)) layout = SHK_LAYOUT_PANELS;
+        else SHK_CHECK(want != SHK_LAYOUT_PANELS, SHK_ERR_ARG,
+                       "shk_csr_create: the panel layout needs RowSparse / ArrayRowSparse rows with sorted "
+                       "columns near the diagonal");
+    }
     shk_csr A = new shk_csr_s();
+    A->layout = layout;
     A->ctx = ctx;
     A->storage = storage;
     A->dtype = dtype;
@@ -272,11 +469,21 @@ int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int
     A->nnz_dev = (int64_t)P.ind.size();
     A->budget = budget;
     A->max_row = P.max_row;
-    A->nblk = (int)P.blk.size() - 1;
-    A->xcd_q = (A->nblk + kNumXcd - 1) / kNumXcd;
-    A->nblk_launch = A->xcd_q * kNumXcd;
-    int st = dtype == SHK_F64 ? upload<double>(A, P, (const double *)hval.data())
+    int st;
+    if (layout == SHK_LAYOUT_PANELS) {
+        A->nblk = Q.npanel;  // one work-group per panel
+        A->xcd_q = (A->nblk + kNumXcd - 1) / kNumXcd;
+        A->nblk_launch = A->xcd_q * kNumXcd;
+        st = dtype == SHK_F64 ? upload_panels<double>(A, P, Q, (const double *)hval.data())
+                              : upload_panels<float>(A, P, Q, (const float *)hval.data());
+    } else {
+        A->nblk = (int)P.blk.size() - 1;
+        A->xcd_q = (A->nblk + kNumXcd - 1) / kNumXcd;
+        A->nblk_launch = A->xcd_q * kNumXcd;
+        A->nslots = A->nnz_dev;
+        st = dtype == SHK_F64 ? upload<double>(A, P, (const double *)hval.data())
                               : upload<float>(A, P, (const float *)hval.data());
+    }
     if (st != SHK_OK) {
         shk_csr_destroy(A);
         return st;
@@ -299,6 +506,8 @@ int shk_csr_destroy(shk_csr A)
     hipFree(A->map_up);
     hipFree(A->map_lo);
     hipFree(A->map_diag);
+    hipFree(A->pchunk);
+    hipFree(A->chunk);
     delete A;
     return SHK_OK;
 }
@@ -322,6 +531,13 @@ int shk_csr_set_values(shk_ctx ctx, shk_csr A, const void *val, int src_on_devic
         hipFree(tmp);
     }
     return st;
+}
+
+int shk_csr_layout(shk_csr A, int *layout)
+{
+    SHK_CHECK(A && layout, SHK_ERR_ARG, "null argument");
+    *layout = A->layout;
+    return SHK_OK;
 }
 
 int shk_csr_info(shk_csr A, int *m, int *n, int64_t *nnz_stored, int64_t *nnz_dev, int *dtype,

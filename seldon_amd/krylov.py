@@ -83,7 +83,10 @@ class Matrix:
     storage: "row" (RowSparse), "sym" (RowSymSparse: upper triangle given),
     "array" (ArrayRowSparse, rows laid end to end)."""
 
-    def __init__(self, csr, storage: str | None = None, dtype=torch.float64, ctx: Context | None = None):
+    LAYOUTS = {"auto": 0, "rows": 1, "panels": 2}
+
+    def __init__(self, csr, storage: str | None = None, dtype=torch.float64, ctx: Context | None = None,
+                 layout: str = "auto"):
         self.ctx = ctx or Context.default()
         if storage is None:
             storage = "sym" if getattr(csr, "sym", False) else "row"
@@ -95,10 +98,19 @@ class Matrix:
         val = np.ascontiguousarray(np.asarray(csr.val), _NP[dtype])
         self.nnz = int(ptr[-1])
         h = ctypes.c_void_p()
-        L.check(L.load().shk_csr_create(self.ctx.h, _STORAGE[storage], _DT[dtype], self.m, self.n, self.nnz,
-                                        ptr.ctypes.data_as(ctypes.c_void_p), ind.ctypes.data_as(ctypes.c_void_p),
-                                        val.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(h)))
+        L.check(L.load().shk_csr_create_layout(self.ctx.h, _STORAGE[storage], _DT[dtype], self.m, self.n,
+                                               self.nnz, ptr.ctypes.data_as(ctypes.c_void_p),
+                                               ind.ctypes.data_as(ctypes.c_void_p),
+                                               val.ctypes.data_as(ctypes.c_void_p), 0, self.LAYOUTS[layout],
+                                               ctypes.byref(h)))
         self.h = h
+
+    @property
+    def layout(self) -> str:
+        """The device layout chosen at creation: "rows" or "panels"."""
+        v = ctypes.c_int()
+        L.check(L.load().shk_csr_layout(self.h, ctypes.byref(v)))
+        return {1: "rows", 2: "panels"}[v.value]
 
     def GetM(self):
         return self.m

@@ -27,7 +27,7 @@ def test_batched_matches_oracle_per_system(gpu, group):
     from seldon_amd import krylov as K
     from seldon_amd.batched import cg_batched
     A, row_off, b, systems = _small_batch()
-    Ag = K.Matrix(A)
+    Ag = K.Matrix(A, layout="rows")
     bg = torch.tensor(b, device="cuda")
     x = torch.full_like(bg, 7.0)
     its, codes = cg_batched(Ag, row_off, x, bg, tol=1e-8, max_iter=5000, group=group, use_graph=False)
@@ -47,7 +47,7 @@ def test_batched_graph_equals_stream_and_max_iter(gpu):
     from seldon_amd import krylov as K
     from seldon_amd.batched import cg_batched
     A, row_off, b, systems = _small_batch()
-    Ag = K.Matrix(A)
+    Ag = K.Matrix(A, layout="rows")
     bg = torch.tensor(b, device="cuda")
     x1, x2 = torch.zeros_like(bg), torch.zeros_like(bg)
     i1, c1 = cg_batched(Ag, row_off, x1, bg, group=256, use_graph=False)
@@ -67,8 +67,8 @@ def test_batched_ir_reaches_fp64_tolerance(gpu):
     from seldon_amd import krylov as K
     from seldon_amd.batched import cg_batched, cg_batched_ir
     A, row_off, b, systems = _small_batch()
-    A64 = K.Matrix(A)
-    A32 = K.Matrix(A, dtype=torch.float32)
+    A64 = K.Matrix(A, layout="rows")
+    A32 = K.Matrix(A, dtype=torch.float32, layout="rows")
     bg = torch.tensor(b, device="cuda")
     x = torch.zeros_like(bg)
     its, codes = cg_batched_ir(A64, A32, row_off, x, bg, tol=1e-10, inner_tol=1e-4, group=256)
@@ -90,7 +90,7 @@ def test_autotune_picks_a_candidate(gpu):
     from seldon_amd import krylov as K
     from seldon_amd.batched import CANDIDATES, autotune
     A, row_off, b, systems = _small_batch()
-    Ag = K.Matrix(A)
+    Ag = K.Matrix(A, layout="rows")
     bg = torch.tensor(b, device="cuda")
     g, entry = autotune(Ag, row_off, bg, use_cache=False)
     assert g in CANDIDATES and set(entry["seconds"]) == {str(c) for c in CANDIDATES}

@@ -22,9 +22,10 @@ def _same(a, b):
     return np.array_equal(a, b, equal_nan=True)
 
 
+@pytest.mark.parametrize("layout", ["rows", "auto"])
 @pytest.mark.parametrize("st", ["row", "sym", "array"])
 @pytest.mark.parametrize("c", ["d", "f"])
-def test_spmv_golden_fixture(gpu, st, c):
+def test_spmv_golden_fixture(gpu, st, c, layout):
     from seldon_amd import krylov as K
     fx = load_fixture("spmv_powerlaw300.npz")
     sp = fx["spec"]
@@ -32,7 +33,7 @@ def test_spmv_golden_fixture(gpu, st, c):
     M = wl.upper(A) if st == "sym" else A
     tdt, ndt = DT[c]
     x = wl.uniform(A.m, sp["xseed"]); y = wl.uniform(A.m, sp["yseed"]); y[sp["nan_at"]] = np.nan
-    Ag = K.Matrix(M, storage=st, dtype=tdt)
+    Ag = K.Matrix(M, storage=st, dtype=tdt, layout=layout)
     xg = _dev(x, ndt)
     for k, (al, be) in enumerate(fx["alpha_beta"]):
         yg = _dev(y, ndt)
@@ -59,9 +60,10 @@ def _cases():
     yield "emptyrows", wl.CSR(A.m, A.n, ptr2, A.ind[keep].copy(), A.val[keep].copy())
 
 
+@pytest.mark.parametrize("layout", ["rows", "auto"])
 @pytest.mark.parametrize("st", ["row", "sym", "array"])
 @pytest.mark.parametrize("c", ["d", "f"])
-def test_spmv_bitwise_vs_oracle(gpu, st, c):
+def test_spmv_bitwise_vs_oracle(gpu, st, c, layout):
     from oracle import oracle as orc
     from seldon_amd import krylov as K
     tdt, ndt = DT[c]
@@ -69,7 +71,11 @@ def test_spmv_bitwise_vs_oracle(gpu, st, c):
         M = wl.upper(A) if st == "sym" else A
         x = wl.uniform(A.n, 21)
         y = wl.uniform(A.m, 22)
-        Ag = K.Matrix(M, storage=st, dtype=tdt)
+        Ag = K.Matrix(M, storage=st, dtype=tdt, layout=layout)
+        if layout == "auto" and st != "sym":
+            # banded / stencil rows with sorted columns take the panel layout
+            assert Ag.layout == "panels", name
+        print(f"[spmv] {name} {st} {c} {Ag.layout}", flush=True)
         xg = _dev(x, ndt)
         for al, be in ((1.0, 0.0), (-1.0, 1.0), (0.7, -0.3)):
             ref = orc.mltadd(ndt(al), M, x.astype(ndt), ndt(be), y.astype(ndt), storage=st, dtype=ndt)
@@ -119,3 +125,62 @@ def test_mlt_dot_fused(gpu):
     y = orc.mlt(A, x)
     assert np.array_equal(yg.cpu().numpy(), y)
     assert abs(d - orc.dot(x, y)) <= 1e-13 * abs(orc.dot(np.abs(x), np.abs(y)))
+
+
+def test_panel_layout_choice_and_fallback(gpu):
+    """Rows with unsorted columns, or columns spread far from the diagonal,
+    keep the row layout (and refuse an explicit panel request); results stay
+    bitwise either way."""
+    from oracle import oracle as orc
+    from seldon_amd import _lib as L
+    from seldon_amd import krylov as K
+    A = wl.laplace2d(40)
+    # reverse the columns of every third row: Seldon's row sum order follows
+    # the stored order, which the panel walk could not keep
+    ind, val = A.ind.copy(), A.val.copy()
+    for r in range(0, A.m, 3):
+        a, b = A.ptr[r], A.ptr[r + 1]
+        ind[a:b] = ind[a:b][::-1].copy()
+        val[a:b] = val[a:b][::-1].copy()
+    U = wl.CSR(A.m, A.n, A.ptr.copy(), ind, val)
+    Ug = K.Matrix(U)
+    assert Ug.layout == "rows"
+    with pytest.raises(L.ShkError):
+        K.Matrix(U, layout="panels")
+    x = wl.uniform(A.n, 4)
+    yg = torch.empty(A.m, dtype=torch.float64, device="cuda")
+    K.Mlt(Ug, _dev(x, np.float64), yg)
+    assert _same(yg.cpu().numpy(), orc.mlt(U, x))
+    # columns uniform over the whole range: x windows would cost more than
+    # the entries, so AUTO keeps the row layout
+    R = wl.powerlaw(30000, 5, kmin=4, kmax=40, half_width=30000)
+    assert K.Matrix(R).layout == "rows"
+
+
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_panel_rectangular_and_values_update(gpu, c):
+    """Panels on a rectangular matrix, m not a multiple of the panel height,
+    then new values through set_values (the device scatter into the panel
+    arrays) -- bitwise against the oracle both times."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    tdt, ndt = DT[c]
+    A = wl.powerlaw(5000, 11, kmin=2, kmax=300, half_width=700)
+    keep = A.ind < 4200
+    lens = np.array([keep[A.ptr[i]:A.ptr[i + 1]].sum() for i in range(A.m)])
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    R = wl.CSR(A.m, 4200, ptr, A.ind[keep].copy(), A.val[keep].copy())
+    Rg = K.Matrix(R, dtype=tdt, layout="panels")
+    x = wl.uniform(R.n, 6)
+    xg = _dev(x, ndt)
+    yg = torch.empty(R.m, dtype=tdt, device="cuda")
+    K.Mlt(Rg, xg, yg)
+    assert _same(yg.cpu().numpy(), orc.mlt(R, x.astype(ndt), dtype=ndt))
+    v2 = wl.normal(R.nnz, 8)
+    R2 = wl.CSR(R.m, R.n, R.ptr, R.ind, v2)
+    Rg.set_values(_dev(v2, ndt))
+    K.Mlt(Rg, xg, yg)
+    assert _same(yg.cpu().numpy(), orc.mlt(R2, x.astype(ndt), dtype=ndt))
+    Rg.set_values(v2.astype(ndt))  # host source
+    K.Mlt(Rg, xg, yg)
+    assert _same(yg.cpu().numpy(), orc.mlt(R2, x.astype(ndt), dtype=ndt))
