@@ -37,8 +37,8 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(int nblk, int xcd_q, const int
 // at the caller's stored position ptr[row] + k.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sddmm_panel(const int *pchunk, const int4 *chunk,
-                                                        const uint32_t *word, const int *ptr, const T *lam,
-                                                        const T *x, T *g)
+                                                        const uint32_t *word, const unsigned short *slot_row,
+                                                        const int *ptr, const T *lam, const T *x, T *g)
 {
     __shared__ int cnt[kPanelRows], segs[kPanelRows];
     const int p = blockIdx.x, r0 = p * kPanelRows, t = threadIdx.x;
@@ -54,8 +54,9 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_panel(const int *pchunk, const
         for (int e = ci.z + t; e < ci.w; e += kBlock) {
             const uint32_t w = word[e];
             const int lr = (w >> kPanRowShift) & kPanRowMask;
+            const int row = r0 + slot_row[(size_t)p * kPanelRows + lr];
             const int col = ci.x + (int)(w & kPanColMask);
-            g[ptr[r0 + lr] + cnt[lr] + (e - segs[lr])] = -(lam[r0 + lr] * x[col]);
+            g[ptr[row] + cnt[lr] + (e - segs[lr])] = -(lam[row] * x[col]);
         }
         __syncthreads();
         for (int e = ci.z + t; e < ci.w; e += kBlock) {
@@ -93,7 +94,7 @@ static int sddmm_typed(shk_ctx ctx, shk_csr A, const T *lam, const T *x, T *g)
         if (A->npanel)
             hipLaunchKernelGGL(k_sddmm_panel<T>, dim3(A->npanel), dim3(kBlock), 0, ctx->stream,
                                (const int *)A->pchunk, (const int4 *)A->chunk, (const uint32_t *)A->ind,
-                               (const int *)A->ptr, lam, x, G);
+                               (const unsigned short *)A->slot_row, (const int *)A->ptr, lam, x, G);
     } else if (A->nblk > 0)
         hipLaunchKernelGGL(k_sddmm<T>, dim3(A->nblk_launch), dim3(kBlock), 0, ctx->stream, A->nblk, A->xcd_q,
                            (const int4 *)A->blk_info, (const int *)A->ptr, (const uint32_t *)A->ind, lam, x, G);

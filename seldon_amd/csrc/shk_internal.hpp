@@ -112,6 +112,7 @@ struct shk_csr_s {
     int npanel = 0, nchunk = 0;
     int *pchunk = nullptr;    // [npanel+1] first chunk of each panel
     int *chunk = nullptr;     // [4*nchunk] {c0, c1, e0, e1} per chunk
+    unsigned short *slot_row = nullptr;  // [npanel*1024] panel-local row of each row slot
 };
 
 namespace shk {

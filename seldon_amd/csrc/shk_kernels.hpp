@@ -69,6 +69,7 @@ struct SpmvArgs {
     // panel layout
     int npanel;
     const int *pchunk;   // [npanel+1] first chunk of each panel
+    const unsigned short *slot_row;  // panel-local row of each row slot
 };
 
 // Entries seq_sum may read past the end of a row (its look-ahead window);
@@ -132,23 +133,26 @@ struct VecOf<float> {
 // The kernel issues block i's gathers, then block i+W's streaming loads, and
 // only then consumes the gathers: vector loads retire in order, so this way
 // waiting on the gathers never waits on the next block's loads.
-template <typename T, int BUDGET>
+template <typename T, int BUDGET, int NT = kBlock>
 struct SpmvRegs {
     static constexpr int V = VecOf<T>::V;
-    static constexpr int K = BUDGET / (kBlock * V);
+    static constexpr int K = BUDGET / (NT * V);
     typename VecOf<T>::IV iv[K];
     typename VecOf<T>::FV fv[K];
 };
 
-template <typename T, int BUDGET>
-__device__ __forceinline__ void spmv_load(const SpmvArgs<T> &a, int js0, SpmvRegs<T, BUDGET> &R)
+// live = false loads entry 0 in every lane instead (one cache line): the
+// loads are never predicated, so the wait counters stay exact.
+template <typename T, int BUDGET, int NT = kBlock>
+__device__ __forceinline__ void spmv_load(const SpmvArgs<T> &a, int js0, SpmvRegs<T, BUDGET, NT> &R,
+                                          bool live = true)
 {
     typedef VecOf<T> VT;
-    constexpr int V = VT::V, K = SpmvRegs<T, BUDGET>::K;
+    constexpr int V = VT::V, K = SpmvRegs<T, BUDGET, NT>::K;
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const int e = js0 + (k * kBlock + t) * V;
+        const int e = live ? js0 + (k * NT + t) * V : 0;
         R.iv[k] = __builtin_nontemporal_load((const typename VT::IV *)(a.ind + e));
         R.fv[k] = __builtin_nontemporal_load((const typename VT::FV *)(a.val + e));
     }
@@ -393,22 +397,27 @@ __global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T>
 
 // ------------------------------------------------------- panel layout ----
 // Rows are grouped in panels of kPanelRows consecutive rows, one work-group
-// per panel, each thread owning kPanelRpt rows (t, t+256, ...) and their
-// running sums in registers.  A panel's entries are stored as a sequence of
-// chunks, each chunk holding, row after row, the entries of the panel whose
-// columns fall in one window [c0, c1) of at most panel_win<T>() columns, at
-// most `budget` entries in all.  Seldon's rows have sorted columns, so
-// walking the chunks in order visits every row's entries in stored order:
-// the row sums stay sequential (Functions_MatVect.cxx:424-430) and bitwise
-// equal to the CPU's.  Per chunk the x window is staged in LDS with coalesced
-// loads, so the products read x from LDS instead of gathering one cache line
-// per entry from L2.  Each entry's 32-bit word packs its window-local column
-// (13 bits), its panel-local row (10 bits), "first/last entry of the row in
-// this chunk" flags, and kLowerFlag (ArrayRowSparse: a direct term).  The
-// next chunk's entries and x window are loaded into registers while the
-// current one is summed.
+// of NT threads per panel, each thread owning kPanelRows / NT rows (one, in
+// the default launch) and their running sums in registers.  A panel's
+// entries are stored as a sequence of chunks, each chunk holding, row after
+// row, the entries of the panel whose columns fall in one window [c0, c1) of
+// at most panel_win<T>() columns, at most `budget` entries in all.  Seldon's
+// rows have sorted columns, so walking the chunks in order visits every row's
+// entries in stored order: the row sums stay sequential
+// (Functions_MatVect.cxx:424-430) and bitwise equal to the CPU's.
+//
+// Per chunk the x window is staged in LDS with coalesced loads, and the
+// products read x from LDS: the row-block kernel's per-entry gathers each
+// cost a 128-byte L2-to-L1 line for 8 useful bytes, which bounded it far
+// below the HBM roofline on matrices whose columns spread over a wide band.
+// Row slot s of a panel holds the panel's s-th longest row, so the long rows
+// share wavefronts and the waves of short rows finish their sums early.
+// Each entry's 32-bit word packs its window-local column (13 bits), its row
+// slot (10 bits), "first / last entry of the row in this chunk" flags, and
+// kLowerFlag (ArrayRowSparse: a direct term).  The entries of the next two
+// chunks and the next x window are in flight while a chunk is summed.
 constexpr int kPanelRows = 1024;
-constexpr int kPanelRpt = kPanelRows / kBlock;
+constexpr int kPanelThreads = 1024;  // work-group of the panel kernel: a thread per row
 constexpr uint32_t kPanColMask = 0x1FFFu;
 constexpr int kPanRowShift = 13;
 constexpr uint32_t kPanRowMask = 0x3FFu;
@@ -424,152 +433,167 @@ struct alignas(16) PanelSmem {
     unsigned short seg_s[kPanelRows], seg_e[kPanelRows];
 };
 
-template <typename T>
+template <typename T, int NT>
 struct PanelX {
-    static constexpr int XK = panel_win<T>() / kBlock;
+    static constexpr int XK = panel_win<T>() / NT;
     T v[XK];
 };
 
-// x[c0 + i] for i < w (an index past w re-reads x[c0]: in bounds, unused)
-template <typename T>
-__device__ __forceinline__ void panel_xload(const T *x, int4 ci, PanelX<T> &X)
+// x[c0 + i] for i < w (an index past w re-reads x[c0]: in bounds, unused);
+// live = false reads x[0] in every lane (never predicated, as spmv_load)
+template <typename T, int NT>
+__device__ __forceinline__ void panel_xload(const T *x, int4 ci, PanelX<T, NT> &X, bool live = true)
 {
     const int t = threadIdx.x, w = ci.y - ci.x;
 #pragma unroll
-    for (int j = 0; j < PanelX<T>::XK; j++) {
-        const int i = j * kBlock + t;
-        X.v[j] = x[ci.x + (i < w ? i : 0)];
+    for (int j = 0; j < PanelX<T, NT>::XK; j++) {
+        const int i = j * NT + t;
+        X.v[j] = x[live ? ci.x + (i < w ? i : 0) : 0];
     }
 }
 
-template <typename T, int MODE, int BUDGET, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_panel(SpmvArgs<T> a, const int4 *__restrict__ chunk, Fin fin)
+// One chunk.  On entry R holds the chunk's entries and X its x window.  The
+// loads of the next chunk's window (into X) and of the chunk two ahead (into
+// R2, a third register set) are issued first, together: every load then
+// waits behind a whole chunk of work, and the wait counters, which retire
+// loads in issue order, never hold this chunk's reads behind newer ones.
+// The caller rotates the three register sets.
+template <typename T, int MODE, int BUDGET, int NT>
+__device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BUDGET> &sm, int4 ci,
+                                           const SpmvRegs<T, BUDGET, NT> &R, SpmvRegs<T, BUDGET, NT> &R2,
+                                           PanelX<T, NT> &X, bool xnext, int4 cx, bool rnext, int4 c2,
+                                           T (&acc)[kPanelRows / NT])
+{
+    constexpr int RPT = kPanelRows / NT;
+    constexpr int V = VecOf<T>::V, K = BUDGET / (NT * V);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < PanelX<T, NT>::XK; j++) sm.xs[j * NT + t] = X.v[j];
+    __builtin_amdgcn_sched_barrier(0);
+    panel_xload<T, NT>(a.x, cx, X, xnext);
+    spmv_load<T, BUDGET, NT>(a, c2.z, R2, rnext);
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();  // x window in; the previous chunk's sums are done
+    // products a_j * x_col(j) into LDS, row segments marked; the x reads are
+    // all issued before any LDS write
+    T xv[K][V];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const bool in = ci.z + (k * NT + t) * V + v < ci.w;
+            xv[k][v] = sm.xs[in ? (R.iv[k][v] & kPanColMask) : 0u];
+        }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        typename VecOf<T>::FV pv;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const bool in = ci.z + (k * NT + t) * V + v < ci.w;
+            const T val = R.fv[k][v];
+            // direct terms multiply alpha into the value first:
+            // `alpha * data[j] * X(i)` (Functions_MatrixArray.cxx:571)
+            const T pr = (MODE == kArray) ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
+            pv[v] = in ? pr : (T)0;
+        }
+        *(typename VecOf<T>::FV *)(sm.prod + (k * NT + t) * V) = pv;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const int el = (k * NT + t) * V + v;
+            const bool in = ci.z + el < ci.w;
+            const uint32_t w = R.iv[k][v];
+            const int lr = (w >> kPanRowShift) & kPanRowMask;
+            if (in && (w & kPanFirst)) sm.seg_s[lr] = (unsigned short)el;
+            if (in && (w & kPanLast)) sm.seg_e[lr] = (unsigned short)(el + 1);
+        }
+    __syncthreads();
+    // each thread adds its rows' segments in order, slot by slot (a slot's
+    // loop runs as long as its longest segment in the wave), four products
+    // per step read before they are added
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const int rr = t + i * NT;
+        const int s0 = sm.seg_s[rr];
+        const int n0 = (int)sm.seg_e[rr] - s0;
+        for (int j = 0; j < n0; j += 4) {
+            T pv[4];
+#pragma unroll
+            for (int h = 0; h < 4; h++) pv[h] = sm.prod[(j + h < n0) ? s0 + j + h : 0];
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const T sum = acc[i] + pv[h];
+                acc[i] = (j + h < n0) ? sum : acc[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const int rr = t + i * NT;
+        sm.seg_s[rr] = 0;
+        sm.seg_e[rr] = 0;
+    }
+}
+
+template <typename T, int MODE, int BUDGET, typename Fin, int NT = kPanelThreads, int MINW = 2>
+__global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *__restrict__ chunk, Fin fin)
 {
     if (a.skip && *a.skip) return;
     __shared__ PanelSmem<T, BUDGET> sm;
-    constexpr int V = VecOf<T>::V, K = BUDGET / (kBlock * V);
+    constexpr int RPT = kPanelRows / NT;
     const int p = xcd_logical_block(a.xcd_q);
     const int t = threadIdx.x;
     T c[2] = {(T)0, (T)0};
     if (p < a.npanel) {
         const int r0 = p * kPanelRows;
         const int nr = min(kPanelRows, a.m - r0);
-        T acc[kPanelRpt], yin[kPanelRpt];
+        T acc[RPT], yin[RPT];
+        int row[RPT];
+        const unsigned short *srow = a.slot_row + (size_t)p * kPanelRows;
 #pragma unroll
-        for (int i = 0; i < kPanelRpt; i++) {
-            const int rr = t + i * kBlock;
-            yin[i] = (!a.mlt && rr < nr) ? a.y_in[r0 + rr] : (T)0;
+        for (int i = 0; i < RPT; i++) {
+            const int rr = t + i * NT;
+            row[i] = rr < nr ? r0 + srow[rr] : -1;
+            yin[i] = (!a.mlt && row[i] >= 0) ? a.y_in[row[i]] : (T)0;
             sm.seg_s[rr] = 0;
             sm.seg_e[rr] = 0;
         }
 #pragma unroll
-        for (int i = 0; i < kPanelRpt; i++) {
+        for (int i = 0; i < RPT; i++) {
             // ArrayRowSparse adds the direct terms onto beta * y (zero-filled
             // when beta is 0); RowSparse sums a temp first
             if (MODE == kArray) acc[i] = a.mlt ? (T)0 : ((a.beta == (T)0) ? (T)0 : yin[i] * a.beta);
             else acc[i] = (T)0;
         }
         const int cb = a.pchunk[p], ce = a.pchunk[p + 1];
-        int4 ci = make_int4(0, 0, 0, 0);
-        SpmvRegs<T, BUDGET> R;
-        PanelX<T> X;
         if (cb < ce) {
-            ci = chunk[cb];
-            spmv_load<T, BUDGET>(a, ci.z, R);
-            panel_xload<T>(a.x, ci, X);
-        }
-        for (int cc = cb; cc < ce; cc++) {
-            // this chunk's x window into LDS (its loads were issued one chunk
-            // ago), then the next chunk's loads, which have the whole of this
-            // chunk's work to arrive (the barriers pin the order)
-#pragma unroll
-            for (int j = 0; j < PanelX<T>::XK; j++) sm.xs[j * kBlock + t] = X.v[j];
-            __builtin_amdgcn_sched_barrier(0);
-            const bool more = cc + 1 < ce;
-            const int4 cn = more ? chunk[cc + 1] : ci;
-            SpmvRegs<T, BUDGET> Rn;
-            PanelX<T> Xn;
-            if (more) {
-                spmv_load<T, BUDGET>(a, cn.z, Rn);
-                panel_xload<T>(a.x, cn, Xn);
+            // chunks cc, cc+1 in RA, RB; cc+2 goes to RC; the sets rotate
+            SpmvRegs<T, BUDGET, NT> RA, RB, RC;
+            PanelX<T, NT> X;
+            int4 c0 = chunk[cb];
+            int4 c1 = cb + 1 < ce ? chunk[cb + 1] : c0;
+            spmv_load<T, BUDGET, NT>(a, c0.z, RA);
+            panel_xload<T, NT>(a.x, c0, X);
+            spmv_load<T, BUDGET, NT>(a, c1.z, RB, cb + 1 < ce);
+            for (int cc = cb;;) {
+                const int4 c2 = cc + 2 < ce ? chunk[cc + 2] : c0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, c0, RA, RC, X, cc + 1 < ce, c1, cc + 2 < ce, c2, acc);
+                if (++cc >= ce) break;
+                const int4 c3 = cc + 2 < ce ? chunk[cc + 2] : c0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, c1, RB, RA, X, cc + 1 < ce, c2, cc + 2 < ce, c3, acc);
+                if (++cc >= ce) break;
+                const int4 c4 = cc + 2 < ce ? chunk[cc + 2] : c0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, c2, RC, RB, X, cc + 1 < ce, c3, cc + 2 < ce, c4, acc);
+                if (++cc >= ce) break;
+                c0 = c3;
+                c1 = c4;
             }
-            __builtin_amdgcn_sched_barrier(0);
-            __syncthreads();  // x window in; the previous chunk's sums are done
-            // products a_j * x_col(j) into LDS, row segments marked; the x
-            // reads are all issued before any LDS write
-            T xv[K][V];
-#pragma unroll
-            for (int k = 0; k < K; k++)
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    const bool in = ci.z + (k * kBlock + t) * V + v < ci.w;
-                    xv[k][v] = sm.xs[in ? (R.iv[k][v] & kPanColMask) : 0u];
-                }
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                typename VecOf<T>::FV pv;
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    const bool in = ci.z + (k * kBlock + t) * V + v < ci.w;
-                    const T val = R.fv[k][v];
-                    // direct terms multiply alpha into the value first:
-                    // `alpha * data[j] * X(i)` (Functions_MatrixArray.cxx:571)
-                    const T pr = (MODE == kArray) ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
-                    pv[v] = in ? pr : (T)0;
-                }
-                *(typename VecOf<T>::FV *)(sm.prod + (k * kBlock + t) * V) = pv;
-            }
-#pragma unroll
-            for (int k = 0; k < K; k++)
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    const int el = (k * kBlock + t) * V + v;
-                    const bool in = ci.z + el < ci.w;
-                    const uint32_t w = R.iv[k][v];
-                    const int lr = (w >> kPanRowShift) & kPanRowMask;
-                    if (in && (w & kPanFirst)) sm.seg_s[lr] = (unsigned short)el;
-                    if (in && (w & kPanLast)) sm.seg_e[lr] = (unsigned short)(el + 1);
-                }
-            __syncthreads();
-            // each thread adds its rows' segments, in order, its rows side by
-            // side (reads first, then the adds)
-            int s0[kPanelRpt], n0[kPanelRpt], nmax = 0;
-#pragma unroll
-            for (int i = 0; i < kPanelRpt; i++) {
-                const int rr = t + i * kBlock;
-                s0[i] = sm.seg_s[rr];
-                n0[i] = (int)sm.seg_e[rr] - s0[i];
-                nmax = max(nmax, n0[i]);
-            }
-            for (int j = 0; j < nmax; j += 2) {
-                T pv[2][kPanelRpt];
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-#pragma unroll
-                    for (int i = 0; i < kPanelRpt; i++)
-                        pv[h][i] = sm.prod[(j + h < n0[i]) ? s0[i] + j + h : 0];
-#pragma unroll
-                for (int h = 0; h < 2; h++)
-#pragma unroll
-                    for (int i = 0; i < kPanelRpt; i++) {
-                        const T sum = acc[i] + pv[h][i];
-                        acc[i] = (j + h < n0[i]) ? sum : acc[i];
-                    }
-            }
-#pragma unroll
-            for (int i = 0; i < kPanelRpt; i++) {
-                const int rr = t + i * kBlock;
-                sm.seg_s[rr] = 0;
-                sm.seg_e[rr] = 0;
-            }
-            ci = cn;
-            R = Rn;
-            X = Xn;
         }
 #pragma unroll
-        for (int i = 0; i < kPanelRpt; i++) {
-            const int rr = t + i * kBlock;
-            if (rr < nr) {
+        for (int i = 0; i < RPT; i++) {
+            if (row[i] >= 0) {
                 T y;
                 if (MODE == kArray) {
                     y = acc[i];
@@ -577,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_panel(SpmvArgs<T> a, const int4 *__r
                     y = a.mlt ? (T)0 : yin[i] * a.beta;
                     y += a.alpha * acc[i];
                 }
-                spmv_epilogue<T, MODE>(a, r0 + rr, y, c);
+                spmv_epilogue<T, MODE>(a, row[i], y, c);
             }
         }
     }
@@ -606,6 +630,7 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     a.val = (const T *)A->val;
     a.npanel = A->npanel;
     a.pchunk = A->pchunk;
+    a.slot_row = A->slot_row;
     if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
         SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * (size_t)A->nblk_launch));
         a.partials = (T *)ctx->red.partials;
@@ -614,7 +639,7 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     if (A->m == 0) return SHK_OK;
     if (A->layout == SHK_LAYOUT_PANELS) {
         constexpr int PB = budget_of<T>();
-        dim3 grid(A->nblk_launch), block(kBlock);
+        dim3 grid(A->nblk_launch), block(kPanelThreads);
         const int4 *chunk = (const int4 *)A->chunk;
         if (A->storage == SHK_ARRAYROWSPARSE)
             hipLaunchKernelGGL((k_panel<T, kArray, PB, Fin>), grid, block, 0, ctx->stream, a, chunk, fin);

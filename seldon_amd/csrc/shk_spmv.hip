@@ -133,6 +133,7 @@ static int build_plan(int storage, int m, int n, int64_t nnz, const int *ptr, co
 // columns being sorted).  Chunk starts are aligned to the load width.
 struct PanelPlan {
     int npanel = 0;
+    std::vector<unsigned short> slot_row;  // [npanel*kPanelRows] panel-local row of each slot
     std::vector<int> pchunk;        // [npanel+1]
     std::vector<int> chunk;         // 4 per chunk: c0, c1, e0, e1
     std::vector<uint32_t> word;     // packed entry words
@@ -161,9 +162,22 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
     Q.word.reserve(nnz + nnz / 8 + 64);
     Q.src.reserve(nnz + nnz / 8 + 64);
     Q.diag_pos.assign(m, -1);
-    std::vector<int> cols, cur(kPanelRows);
+    Q.slot_row.assign((size_t)Q.npanel * kPanelRows, 0);
+    std::vector<int> cols, cur(kPanelRows), slot_of(kPanelRows), order(kPanelRows);
     for (int p = 0; p < Q.npanel; p++) {
         const int r0 = p * kPanelRows, r1 = std::min(m, r0 + kPanelRows);
+        // Row slots by length, longest first: the panel kernel gives slot s
+        // to one thread, so the panel's long rows land in the same wavefronts
+        // and the waves of short rows finish their sums early.
+        const int nr = r1 - r0;
+        for (int k = 0; k < nr; k++) order[k] = k;
+        std::stable_sort(order.begin(), order.begin() + nr, [&](int a, int b) {
+            return P.ptr[r0 + a + 1] - P.ptr[r0 + a] > P.ptr[r0 + b + 1] - P.ptr[r0 + b];
+        });
+        for (int k = 0; k < nr; k++) {
+            slot_of[order[k]] = k;
+            Q.slot_row[(size_t)p * kPanelRows + k] = (unsigned short)order[k];
+        }
         cols.assign(P.ind.begin() + P.ptr[r0], P.ind.begin() + P.ptr[r1]);
         for (int &c : cols) c &= (int)~kLowerFlag;
         std::sort(cols.begin(), cols.end());
@@ -192,7 +206,7 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
                 while (j < P.ptr[r + 1] && (int)(P.ind[j] & ~kLowerFlag) < c1) j++;
                 for (int q = jb; q < j; q++) {
                     const uint32_t col = P.ind[q] & ~kLowerFlag;
-                    uint32_t w = (col - (uint32_t)c0) | ((uint32_t)(r - r0) << kPanRowShift) |
+                    uint32_t w = (col - (uint32_t)c0) | ((uint32_t)slot_of[r - r0] << kPanRowShift) |
                                  (P.ind[q] & kLowerFlag);
                     if (q == jb) w |= kPanFirst;
                     if (q == j - 1) w |= kPanLast;
@@ -223,8 +237,8 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
 }
 
 template <typename T>
-static __global__ void k_panel_scatter(const int *pchunk, const int4 *chunk, const uint32_t *word, int m,
-                                       const int *ptr, const T *src, T *dst)
+static __global__ void k_panel_scatter(const int *pchunk, const int4 *chunk, const uint32_t *word,
+                                       const unsigned short *slot_row, const int *ptr, const T *src, T *dst)
 {
     // Positions of the caller's stored entries in the panel arrays: one
     // work-group per panel walks its chunks in order, counting per row the
@@ -243,7 +257,7 @@ static __global__ void k_panel_scatter(const int *pchunk, const int4 *chunk, con
         __syncthreads();
         for (int e = ci.z + t; e < ci.w; e += kBlock) {
             const int lr = (word[e] >> kPanRowShift) & kPanRowMask;
-            dst[e] = src[ptr[r0 + lr] + cnt[lr] + (e - segs[lr])];
+            dst[e] = src[ptr[r0 + slot_row[(size_t)p * kPanelRows + lr]] + cnt[lr] + (e - segs[lr])];
         }
         __syncthreads();
         for (int e = ci.z + t; e < ci.w; e += kBlock) {
@@ -254,7 +268,6 @@ static __global__ void k_panel_scatter(const int *pchunk, const int4 *chunk, con
             }
         }
     }
-    (void)m;
 }
 
 template <typename T>
@@ -276,6 +289,9 @@ static int upload_panels(shk_csr A, const HostPlan &P, const PanelPlan &Q, const
     SHK_HIP(hipMalloc(&A->val, sizeof(T) * (ns + pad)));
     SHK_HIP(hipMalloc(&A->diag, sizeof(T) * std::max(m, 1)));
     SHK_HIP(hipMalloc(&A->pchunk, sizeof(int) * (Q.npanel + 1)));
+    SHK_HIP(hipMalloc(&A->slot_row, sizeof(unsigned short) * Q.slot_row.size()));
+    SHK_HIP(hipMemcpyAsync(A->slot_row, Q.slot_row.data(), sizeof(unsigned short) * Q.slot_row.size(),
+                           hipMemcpyHostToDevice, s));
     SHK_HIP(hipMalloc(&A->chunk, sizeof(int) * std::max<size_t>(Q.chunk.size(), 4)));
     SHK_HIP(hipMalloc(&A->map_diag, sizeof(int64_t) * std::max(m, 1)));
     SHK_HIP(hipMemcpyAsync(A->ind, word.data(), sizeof(uint32_t) * (ns + pad), hipMemcpyHostToDevice, s));
@@ -365,8 +381,8 @@ static int set_values_dev(shk_csr A, const T *dval)
     if (A->layout == SHK_LAYOUT_PANELS) {
         if (A->npanel)
             hipLaunchKernelGGL(k_panel_scatter<T>, dim3(A->npanel), dim3(kBlock), 0, s, (const int *)A->pchunk,
-                               (const int4 *)A->chunk, (const uint32_t *)A->ind, A->m, (const int *)A->ptr, dval,
-                               (T *)A->val);
+                               (const int4 *)A->chunk, (const uint32_t *)A->ind,
+                               (const unsigned short *)A->slot_row, (const int *)A->ptr, dval, (T *)A->val);
     } else if (A->storage == SHK_ROWSYMSPARSE) {
         if (A->nnz_stored)
             hipLaunchKernelGGL(k_scatter_vals<T>, dim3(red_blocks(A->nnz_stored)), dim3(kBlock), 0, s,
@@ -508,6 +524,7 @@ int shk_csr_destroy(shk_csr A)
     hipFree(A->map_diag);
     hipFree(A->pchunk);
     hipFree(A->chunk);
+    hipFree(A->slot_row);
     delete A;
     return SHK_OK;
 }
