@@ -17,7 +17,7 @@ on the same matrix as `cpu_baseline`.
 Also: the 128^3 convection-diffusion BiCgStab / Gmres(30) solves and the
 512-system batched Cg (FP64 and FP32+IR), each timed to its tolerance.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--quick] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 10] [--quick] [--no-cpu]
                        [--no-solve] [--no-extra] [--probe]
 """
 import argparse
@@ -51,8 +51,8 @@ def traffic_from_profiles(name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="smaller matrix (1M rows)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")

@@ -61,7 +61,7 @@ typedef enum { SHK_PREC_IDENTITY = 0, SHK_PREC_JACOBI = 1 } shk_precond;
 
 /* Device layout of a matrix (an internal choice; results are identical).
  *   ROWS   -- the caller's CSR order, cut into row blocks;
- *   PANELS -- panels of 1024 rows whose entries are regrouped by column
+ *   PANELS -- panels of 2048 rows whose entries are regrouped by column
  *             windows so the x entries they read are staged on chip; for
  *             RowSparse / ArrayRowSparse with sorted columns near the
  *             diagonal (banded, stencil and FEM matrices in natural order);

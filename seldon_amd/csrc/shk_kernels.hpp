@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T>
 
 // ------------------------------------------------------- panel layout ----
 // Rows are grouped in panels of kPanelRows consecutive rows, one work-group
-// of NT threads per panel, each thread owning kPanelRows / NT rows (one, in
+// of NT threads per panel, each thread owning kPanelRows / NT rows (two, in
 // the default launch) and their running sums in registers.  A panel's
 // entries are stored as a sequence of chunks, each chunk holding, row after
 // row, the entries of the panel whose columns fall in one window [c0, c1) of
@@ -413,16 +413,16 @@ __global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T>
 // Row slot s of a panel holds the panel's s-th longest row, so the long rows
 // share wavefronts and the waves of short rows finish their sums early.
 // Each entry's 32-bit word packs its window-local column (13 bits), its row
-// slot (10 bits), "first / last entry of the row in this chunk" flags, and
+// slot (11 bits), "first / last entry of the row in this chunk" flags, and
 // kLowerFlag (ArrayRowSparse: a direct term).  The entries of the next two
 // chunks and the next x window are in flight while a chunk is summed.
-constexpr int kPanelRows = 1024;
-constexpr int kPanelThreads = 1024;  // work-group of the panel kernel: a thread per row
+constexpr int kPanelRows = 2048;
+constexpr int kPanelThreads = 1024;  // work-group of the panel kernel: two rows per thread
 constexpr uint32_t kPanColMask = 0x1FFFu;
 constexpr int kPanRowShift = 13;
-constexpr uint32_t kPanRowMask = 0x3FFu;
-constexpr uint32_t kPanFirst = 1u << 23;
-constexpr uint32_t kPanLast = 1u << 24;
+constexpr uint32_t kPanRowMask = 0x7FFu;
+constexpr uint32_t kPanFirst = 1u << 24;
+constexpr uint32_t kPanLast = 1u << 25;
 template <typename T>
 constexpr int panel_win() { return sizeof(T) == 8 ? 3072 : 6144; }  // 24 KiB of x
 

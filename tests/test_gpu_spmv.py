@@ -153,8 +153,9 @@ def test_panel_layout_choice_and_fallback(gpu):
     assert _same(yg.cpu().numpy(), orc.mlt(U, x))
     # columns uniform over the whole range: x windows would cost more than
     # the entries, so AUTO keeps the row layout
-    R = wl.powerlaw(30000, 5, kmin=4, kmax=40, half_width=30000)
+    R = wl.powerlaw(100000, 5, kmin=4, kmax=40, half_width=100000)
     assert K.Matrix(R).layout == "rows"
+    assert K.Matrix(R, dtype=torch.float32).layout == "rows"
 
 
 @pytest.mark.parametrize("c", ["d", "f"])
