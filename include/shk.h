@@ -63,8 +63,9 @@ typedef enum { SHK_PREC_IDENTITY = 0, SHK_PREC_JACOBI = 1 } shk_precond;
  *   ROWS   -- the caller's CSR order, cut into row blocks;
  *   PANELS -- panels of 2048 rows whose entries are regrouped by column
  *             windows so the x entries they read are staged on chip; for
- *             RowSparse / ArrayRowSparse with sorted columns near the
- *             diagonal (banded, stencil and FEM matrices in natural order);
+ *             rows with sorted columns near the diagonal (banded, stencil
+ *             and FEM matrices in natural order; RowSymSparse upper parts
+ *             and their transposes are walked one after the other);
  *   AUTO   -- PANELS where it applies, else ROWS. */
 typedef enum { SHK_LAYOUT_AUTO = 0, SHK_LAYOUT_ROWS = 1, SHK_LAYOUT_PANELS = 2 } shk_layout;
 

@@ -12,9 +12,9 @@ int dot_impl(shk_ctx ctx, int64_t n, const T *x, const T *y, const T *u1, T alph
     // result[0] = sum u0*v, result[1] = sum u1*v with v = y (+ alpha ax)
     const int nb = red_blocks(n);
     SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * nb));
-    hipLaunchKernelGGL((k_axpy_dot<T, FinStore<T>>), dim3(nb), dim3(kBlock), 0, ctx->stream, n, alpha,
-                       ax, ay ? ay : (T *)y, x, u1, (T *)ctx->red.partials, ctx->red.ticket,
-                       (const int *)nullptr, FinStore<T>{dres});
+    hipLaunchKernelGGL(k_axpy_dot<T>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, alpha, ax,
+                       ay ? ay : (T *)y, x, u1, (T *)ctx->red.partials, (const int *)nullptr);
+    grid_finish<T>(ctx->stream, (const T *)ctx->red.partials, nb, nullptr, FinStore<T>{dres});
     SHK_HIP(hipGetLastError());
     return SHK_OK;
 }

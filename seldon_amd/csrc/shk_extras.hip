@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_panel(const int *pchunk, const
             const uint32_t w = word[e];
             const int lr = (w >> kPanRowShift) & kPanRowMask;
             const int row = r0 + slot_row[(size_t)p * kPanelRows + lr];
-            const int col = ci.x + (int)(w & kPanColMask);
+            const int col = chunk_c0(ci) + (int)(w & kPanColMask);
             g[ptr[row] + cnt[lr] + (e - segs[lr])] = -(lam[row] * x[col]);
         }
         __syncthreads();
@@ -65,6 +65,24 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_panel(const int *pchunk, const
                 const int lr = (w >> kPanRowShift) & kPanRowMask;
                 cnt[lr] += e - segs[lr] + 1;
             }
+        }
+    }
+}
+
+// Panel layout, RowSymSparse: the per-slot terms G for the fold below, at
+// the panel positions the maps point to.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sddmm_panel_slots(const int *pchunk, const int4 *chunk,
+                                                              const uint32_t *word, const unsigned short *slot_row,
+                                                              const T *lam, const T *x, T *G)
+{
+    const int p = blockIdx.x, r0 = p * kPanelRows, t = threadIdx.x;
+    for (int cc = pchunk[p]; cc < pchunk[p + 1]; cc++) {
+        const int4 ci = chunk[cc];
+        for (int e = ci.z + t; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            const int row = r0 + slot_row[(size_t)p * kPanelRows + ((w >> kPanRowShift) & kPanRowMask)];
+            G[e] = -(lam[row] * x[chunk_c0(ci) + (int)(w & kPanColMask)]);
         }
     }
 }
@@ -90,7 +108,12 @@ static int sddmm_typed(shk_ctx ctx, shk_csr A, const T *lam, const T *x, T *g)
         SHK_TRY(ensure_ws2(ctx, sizeof(T) * (size_t)std::max<int64_t>(A->nslots, 1)));
         G = (T *)ctx->ws2;
     }
-    if (A->layout == SHK_LAYOUT_PANELS) {
+    if (A->layout == SHK_LAYOUT_PANELS && A->storage == SHK_ROWSYMSPARSE) {
+        if (A->npanel)
+            hipLaunchKernelGGL(k_sddmm_panel_slots<T>, dim3(A->npanel), dim3(kBlock), 0, ctx->stream,
+                               (const int *)A->pchunk, (const int4 *)A->chunk, (const uint32_t *)A->ind,
+                               (const unsigned short *)A->slot_row, lam, x, G);
+    } else if (A->layout == SHK_LAYOUT_PANELS) {
         if (A->npanel)
             hipLaunchKernelGGL(k_sddmm_panel<T>, dim3(A->npanel), dim3(kBlock), 0, ctx->stream,
                                (const int *)A->pchunk, (const int4 *)A->chunk, (const uint32_t *)A->ind,

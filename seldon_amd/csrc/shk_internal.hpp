@@ -51,7 +51,7 @@ inline size_t dsize(int dt) { return dt == SHK_F32 ? 4 : 8; }
 // Device-side scalar workspace for one in-flight reduction family.
 struct RedSlot {
     void *partials = nullptr; // [nv][nblocks] of T
-    unsigned *ticket = nullptr;
+    unsigned *ticket = nullptr; // counters of the batched solver's work queue
     size_t cap_bytes = 0;
 };
 
@@ -63,7 +63,7 @@ struct shk_ctx_s {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     char arch[64] = {0};
-    // reduction workspace: one partial buffer + ticket per concurrent user
+    // reduction workspace: block partials of the split reductions
     shk::RedSlot red;
     // small device scratch for scalar results and solver state
     void *scalars = nullptr;      // 4 KiB
@@ -161,38 +161,47 @@ __device__ __forceinline__ void block_sum(T (&v)[NV], T *lds /*[NV*16]*/)
     __syncthreads();
 }
 
-// Grid reduction epilogue: every block of the launch calls this once with its
-// per-thread contributions.  Block sums go to partials[k*nblocks + b]; the
-// last block to arrive (ticket) sums them in a fixed order and returns true in
-// all its threads with the totals in out[] (valid in thread 0).  The ticket is
-// reset by that block, so the workspace needs no clearing between launches.
+// Grid reductions are split in two launches.  Every block of the producing
+// kernel calls block_partial once: its block sum goes to
+// partials[k*nblocks + b].  The stream then runs k_grid_finish, one block
+// that adds the partials in a fixed order and hands the totals to the
+// finalize functor.  (A last-block-done ticket would need a device-scope
+// release per block, which on this multi-XCD part writes back the XCD's L2
+// each time: it cost ~27 ns per block, serialised, far more than the launch
+// of the second kernel.)  Results are reproducible run to run: the grid of
+// every producer is a function of the problem size only.
 template <typename T, int NV>
-__device__ __forceinline__ bool grid_sum(T (&v)[NV], T *partials, unsigned *ticket,
-                                         int nblocks, int bidx, T (&out)[NV])
+__device__ __forceinline__ void block_partial(T (&v)[NV], T *partials, int nblocks, int bidx)
 {
     __shared__ T lds[NV * 16];
-    __shared__ int is_last;
     block_sum<T, NV>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) partials[(size_t)k * nblocks + bidx] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        is_last = (t == (unsigned)(nblocks - 1));
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+}
+
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_grid_finish(const T *partials, int nblocks, const int *skip, Fin fin)
+{
+    if (skip && *skip) return;
+    __shared__ T lds[2 * 16];
+    T out[2];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
+    for (int k = 0; k < 2; k++) {
         T s = 0;
-        const volatile T *pk = partials + (size_t)k * nblocks;
-        for (int i = threadIdx.x; i < nblocks; i += blockDim.x) s += pk[i];
+        for (int i = threadIdx.x; i < nblocks; i += kBlock) s += partials[(size_t)k * nblocks + i];
         out[k] = s;
     }
-    block_sum<T, NV>(out, lds);
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
+    block_sum<T, 2>(out, lds);
+    if (threadIdx.x == 0) fin(out);
+}
+
+// The second launch of a split grid reduction (see block_partial).
+template <typename T, typename Fin>
+inline void grid_finish(hipStream_t st, const T *partials, int nblocks, const int *skip, Fin fin)
+{
+    hipLaunchKernelGGL((k_grid_finish<T, Fin>), dim3(1), dim3(kBlock), 0, st, partials, nblocks, skip, fin);
 }
 
 // number of blocks used by the grid-stride BLAS-1 reductions (fixed per n)

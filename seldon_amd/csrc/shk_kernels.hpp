@@ -63,8 +63,7 @@ struct SpmvArgs {
     const T *diag;     // Jacobi epilogue: out = y / diag (null = none)
     int dot0, dot1;    // DotKind for the two partial sums
     const T *u0, *u1;  // dot vectors
-    T *partials;
-    unsigned *ticket;
+    T *partials;       // [2][grid] block sums of the two dots
     const int *skip;   // device flag: nonzero = return at once (solver done)
     // panel layout
     int npanel;
@@ -337,8 +336,8 @@ __device__ __forceinline__ void spmv_long_row(const SpmvArgs<T> &a, int4 bi, T *
 // they gather are shared through one L2.  While a work-group stages and sums
 // block i, the streaming loads and row metadata of block i+W are already in
 // flight, and the block table entry of block i+2W is being fetched.
-template <typename T, int MODE, int BUDGET, typename Fin>
-__global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T> a, const int4 *__restrict__ blk_info, Fin fin)
+template <typename T, int MODE, int BUDGET>
+__global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T> a, const int4 *__restrict__ blk_info)
 {
     if (a.skip && *a.skip) return;
     __shared__ SpmvSmem<T, BUDGET> sm;
@@ -387,12 +386,7 @@ __global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T>
         M = Mn;
         lb = lbn;
     }
-    if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
-        T out[2];
-        if (grid_sum<T, 2>(c, a.partials, a.ticket, gridDim.x, blockIdx.x, out)) {
-            if (threadIdx.x == 0) fin(out);
-        }
-    }
+    if (a.dot0 != kDotNone || a.dot1 != kDotNone) block_partial<T, 2>(c, a.partials, gridDim.x, blockIdx.x);
 }
 
 // ------------------------------------------------------- panel layout ----
@@ -404,7 +398,12 @@ __global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T>
 // at most panel_win<T>() columns, at most `budget` entries in all.  Seldon's
 // rows have sorted columns, so walking the chunks in order visits every row's
 // entries in stored order: the row sums stay sequential
-// (Functions_MatVect.cxx:424-430) and bitwise equal to the CPU's.
+// (Functions_MatVect.cxx:424-430) and bitwise equal to the CPU's.  For
+// RowSymSparse the chunks of the upper parts come first (the temp sums),
+// then, from the chunk flagged kChunkDirect on, those of the transposed
+// parts; at that chunk each row's y = beta y + alpha temp is formed and the
+// direct terms `(alpha * data[j]) * X(i)` follow (Functions_MatVect.cxx:
+// 553-563).
 //
 // Per chunk the x window is staged in LDS with coalesced loads, and the
 // products read x from LDS: the row-block kernel's per-entry gathers each
@@ -423,8 +422,21 @@ constexpr int kPanRowShift = 13;
 constexpr uint32_t kPanRowMask = 0x7FFu;
 constexpr uint32_t kPanFirst = 1u << 24;
 constexpr uint32_t kPanLast = 1u << 25;
+// chunk descriptor {c0 | flag, c1, e0, e1}: the flag marks a RowSymSparse
+// panel's first chunk of transposed parts
+constexpr uint32_t kChunkDirect = 1u << 31;
+__device__ __forceinline__ int chunk_c0(int4 ci) { return (int)((uint32_t)ci.x & ~kChunkDirect); }
 template <typename T>
 constexpr int panel_win() { return sizeof(T) == 8 ? 3072 : 6144; }  // 24 KiB of x
+#ifndef SHK_PANEL_BUDGET_F64
+#define SHK_PANEL_BUDGET_F64 2048
+#endif
+#ifndef SHK_PANEL_BUDGET_F32
+#define SHK_PANEL_BUDGET_F32 4096
+#endif
+// entries per chunk (LDS products: 16 KiB)
+template <typename T>
+constexpr int panel_budget() { return sizeof(T) == 8 ? SHK_PANEL_BUDGET_F64 : SHK_PANEL_BUDGET_F32; }
 
 template <typename T, int BUDGET>
 struct alignas(16) PanelSmem {
@@ -444,11 +456,11 @@ struct PanelX {
 template <typename T, int NT>
 __device__ __forceinline__ void panel_xload(const T *x, int4 ci, PanelX<T, NT> &X, bool live = true)
 {
-    const int t = threadIdx.x, w = ci.y - ci.x;
+    const int t = threadIdx.x, c0 = chunk_c0(ci), w = ci.y - c0;
 #pragma unroll
     for (int j = 0; j < PanelX<T, NT>::XK; j++) {
         const int i = j * NT + t;
-        X.v[j] = x[live ? ci.x + (i < w ? i : 0) : 0];
+        X.v[j] = x[live ? c0 + (i < w ? i : 0) : 0];
     }
 }
 
@@ -462,7 +474,7 @@ template <typename T, int MODE, int BUDGET, int NT>
 __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BUDGET> &sm, int4 ci,
                                            const SpmvRegs<T, BUDGET, NT> &R, SpmvRegs<T, BUDGET, NT> &R2,
                                            PanelX<T, NT> &X, bool xnext, int4 cx, bool rnext, int4 c2,
-                                           T (&acc)[kPanelRows / NT])
+                                           T (&acc)[kPanelRows / NT], const T (&y0)[kPanelRows / NT])
 {
     constexpr int RPT = kPanelRows / NT;
     constexpr int V = VecOf<T>::V, K = BUDGET / (NT * V);
@@ -492,8 +504,10 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
             const bool in = ci.z + (k * NT + t) * V + v < ci.w;
             const T val = R.fv[k][v];
             // direct terms multiply alpha into the value first:
-            // `alpha * data[j] * X(i)` (Functions_MatrixArray.cxx:571)
-            const T pr = (MODE == kArray) ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
+            // `alpha * data[j] * X(i)` (Functions_MatrixArray.cxx:571,
+            // Functions_MatVect.cxx:563)
+            const bool direct = MODE == kArray || (MODE == kSym && (R.iv[k][v] & kLowerFlag));
+            const T pr = direct ? (a.alpha * val) * xv[k][v] : val * xv[k][v];
             pv[v] = in ? pr : (T)0;
         }
         *(typename VecOf<T>::FV *)(sm.prod + (k * NT + t) * V) = pv;
@@ -510,6 +524,16 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
             if (in && (w & kPanLast)) sm.seg_e[lr] = (unsigned short)(el + 1);
         }
     __syncthreads();
+    if (MODE == kSym && ((uint32_t)ci.x & kChunkDirect)) {
+        // the upper parts are done: y = beta y + alpha temp, then the direct
+        // terms (Functions_MatVect.cxx:553-563)
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            T y = y0[i];
+            y += a.alpha * acc[i];
+            acc[i] = y;
+        }
+    }
     // each thread adds its rows' segments in order, slot by slot (a slot's
     // loop runs as long as its longest segment in the wave), four products
     // per step read before they are added
@@ -537,8 +561,8 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
     }
 }
 
-template <typename T, int MODE, int BUDGET, typename Fin, int NT = kPanelThreads, int MINW = 2>
-__global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *__restrict__ chunk, Fin fin)
+template <typename T, int MODE, int BUDGET, int NT = kPanelThreads, int MINW = 2>
+__global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *__restrict__ chunk)
 {
     if (a.skip && *a.skip) return;
     __shared__ PanelSmem<T, BUDGET> sm;
@@ -560,13 +584,16 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
             sm.seg_s[rr] = 0;
             sm.seg_e[rr] = 0;
         }
+        T y0[RPT];
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             // ArrayRowSparse adds the direct terms onto beta * y (zero-filled
-            // when beta is 0); RowSparse sums a temp first
-            if (MODE == kArray) acc[i] = a.mlt ? (T)0 : ((a.beta == (T)0) ? (T)0 : yin[i] * a.beta);
-            else acc[i] = (T)0;
+            // when beta is 0); RowSparse and RowSymSparse sum a temp first
+            if (MODE == kArray) y0[i] = a.mlt ? (T)0 : ((a.beta == (T)0) ? (T)0 : yin[i] * a.beta);
+            else y0[i] = a.mlt ? (T)0 : yin[i] * a.beta;
+            acc[i] = MODE == kArray ? y0[i] : (T)0;
         }
+        bool direct = false;  // RowSymSparse: y formed, transposed parts started
         const int cb = a.pchunk[p], ce = a.pchunk[p + 1];
         if (cb < ce) {
             // chunks cc, cc+1 in RA, RB; cc+2 goes to RC; the sets rotate
@@ -579,13 +606,16 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
             spmv_load<T, BUDGET, NT>(a, c1.z, RB, cb + 1 < ce);
             for (int cc = cb;;) {
                 const int4 c2 = cc + 2 < ce ? chunk[cc + 2] : c0;
-                panel_step<T, MODE, BUDGET, NT>(a, sm, c0, RA, RC, X, cc + 1 < ce, c1, cc + 2 < ce, c2, acc);
+                direct |= ((uint32_t)c0.x & kChunkDirect) != 0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, c0, RA, RC, X, cc + 1 < ce, c1, cc + 2 < ce, c2, acc, y0);
                 if (++cc >= ce) break;
                 const int4 c3 = cc + 2 < ce ? chunk[cc + 2] : c0;
-                panel_step<T, MODE, BUDGET, NT>(a, sm, c1, RB, RA, X, cc + 1 < ce, c2, cc + 2 < ce, c3, acc);
+                direct |= ((uint32_t)c1.x & kChunkDirect) != 0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, c1, RB, RA, X, cc + 1 < ce, c2, cc + 2 < ce, c3, acc, y0);
                 if (++cc >= ce) break;
                 const int4 c4 = cc + 2 < ce ? chunk[cc + 2] : c0;
-                panel_step<T, MODE, BUDGET, NT>(a, sm, c2, RC, RB, X, cc + 1 < ce, c3, cc + 2 < ce, c4, acc);
+                direct |= ((uint32_t)c2.x & kChunkDirect) != 0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, c2, RC, RB, X, cc + 1 < ce, c3, cc + 2 < ce, c4, acc, y0);
                 if (++cc >= ce) break;
                 c0 = c3;
                 c1 = c4;
@@ -595,22 +625,17 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
         for (int i = 0; i < RPT; i++) {
             if (row[i] >= 0) {
                 T y;
-                if (MODE == kArray) {
+                if (MODE == kArray || (MODE == kSym && direct)) {
                     y = acc[i];
                 } else {
-                    y = a.mlt ? (T)0 : yin[i] * a.beta;
+                    y = y0[i];
                     y += a.alpha * acc[i];
                 }
                 spmv_epilogue<T, MODE>(a, row[i], y, c);
             }
         }
     }
-    if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
-        T out[2];
-        if (grid_sum<T, 2>(c, a.partials, a.ticket, gridDim.x, blockIdx.x, out)) {
-            if (threadIdx.x == 0) fin(out);
-        }
-    }
+    if (a.dot0 != kDotNone || a.dot1 != kDotNone) block_partial<T, 2>(c, a.partials, gridDim.x, blockIdx.x);
 }
 
 // ------------------------------------------------------------ launch ----
@@ -631,20 +656,23 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     a.npanel = A->npanel;
     a.pchunk = A->pchunk;
     a.slot_row = A->slot_row;
-    if (a.dot0 != kDotNone || a.dot1 != kDotNone) {
+    const bool dots = a.dot0 != kDotNone || a.dot1 != kDotNone;
+    if (dots) {
         SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * (size_t)A->nblk_launch));
         a.partials = (T *)ctx->red.partials;
-        a.ticket = ctx->red.ticket;
     }
     if (A->m == 0) return SHK_OK;
     if (A->layout == SHK_LAYOUT_PANELS) {
-        constexpr int PB = budget_of<T>();
+        constexpr int PB = panel_budget<T>();
         dim3 grid(A->nblk_launch), block(kPanelThreads);
         const int4 *chunk = (const int4 *)A->chunk;
         if (A->storage == SHK_ARRAYROWSPARSE)
-            hipLaunchKernelGGL((k_panel<T, kArray, PB, Fin>), grid, block, 0, ctx->stream, a, chunk, fin);
+            hipLaunchKernelGGL((k_panel<T, kArray, PB>), grid, block, 0, ctx->stream, a, chunk);
+        else if (A->storage == SHK_ROWSYMSPARSE)
+            hipLaunchKernelGGL((k_panel<T, kSym, PB>), grid, block, 0, ctx->stream, a, chunk);
         else
-            hipLaunchKernelGGL((k_panel<T, kCsr, PB, Fin>), grid, block, 0, ctx->stream, a, chunk, fin);
+            hipLaunchKernelGGL((k_panel<T, kCsr, PB>), grid, block, 0, ctx->stream, a, chunk);
+        if (dots) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
         SHK_HIP(hipGetLastError());
         return SHK_OK;
     }
@@ -654,15 +682,16 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     dim3 grid(per_xcd * kNumXcd), block(kBlock);
     switch (A->storage) {
     case SHK_ROWSPARSE:
-        hipLaunchKernelGGL((k_spmv<T, kCsr, B, Fin>), grid, block, 0, ctx->stream, a, a.blk_info, fin);
+        hipLaunchKernelGGL((k_spmv<T, kCsr, B>), grid, block, 0, ctx->stream, a, a.blk_info);
         break;
     case SHK_ROWSYMSPARSE:
-        hipLaunchKernelGGL((k_spmv<T, kSym, B, Fin>), grid, block, 0, ctx->stream, a, a.blk_info, fin);
+        hipLaunchKernelGGL((k_spmv<T, kSym, B>), grid, block, 0, ctx->stream, a, a.blk_info);
         break;
     default:
-        hipLaunchKernelGGL((k_spmv<T, kArray, B, Fin>), grid, block, 0, ctx->stream, a, a.blk_info, fin);
+        hipLaunchKernelGGL((k_spmv<T, kArray, B>), grid, block, 0, ctx->stream, a, a.blk_info);
         break;
     }
+    if (dots) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
     SHK_HIP(hipGetLastError());
     return SHK_OK;
 }
@@ -718,11 +747,11 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int64_t n, const T *r, const 
 }
 
 // Generic fused reduction: optional y += alpha x first (alpha == 0 skips, as
-// Add does), then sum of u0*v and (optional) u1*v over v = y.
-template <typename T, typename Fin>
+// Add does), then block sums of u0*v and (optional) u1*v over v = y; the
+// totals come from grid_finish on the same stream.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy_dot(int64_t n, T alpha, const T *x, T *y,
-                                                     const T *u0, const T *u1, T *partials,
-                                                     unsigned *ticket, const int *skip, Fin fin)
+                                                     const T *u0, const T *u1, T *partials, const int *skip)
 {
     if (skip && *skip) return;
     T c[2] = {(T)0, (T)0};
@@ -736,10 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_axpy_dot(int64_t n, T alpha, const T
         c[0] += u0[i] * v;
         if (u1) c[1] += u1[i] * v;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out)) {
-        if (threadIdx.x == 0) fin(out);
-    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 }  // namespace shk

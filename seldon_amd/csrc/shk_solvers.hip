@@ -13,14 +13,18 @@
 
 namespace shk {
 
+// The vector arguments of these kernels are distinct buffers (__restrict__),
+// so an unrolled loop keeps several elements' loads in flight; each thread
+// still accumulates its elements in increasing index order.
 #define GRID_STRIDE(i, n) \
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (n); i += (int64_t)gridDim.x * kBlock)
+    _Pragma("unroll 4") for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (n); \
+                             i += (int64_t)gridDim.x * kBlock)
 
 // ---------------------------------------------------------------- Cg ----
 // Init: x = 0 (x.Zero(), Cg.cxx:67) and r = b (Copy, :63); sum r^2.
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_cg_init0(int64_t n, const T *b, T *x, T *r,
-                                                     KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init0(int64_t n, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ r,
+                                                     KState<T> *st, T *partials)
 {
     if (st->skip) return;
     T c[2] = {(T)0, (T)0};
@@ -30,14 +34,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_init0(int64_t n, const T *b, T *x
         r[i] = v;
         c[0] += v * v;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // z = M^{-1} r (Jacobi), sum r*z  (Cg.cxx:75-78)
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_cg_z(int64_t n, const T *r, const T *d, T *z,
-                                                 KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_z(int64_t n, const T *__restrict__ r, const T *__restrict__ d, T *__restrict__ z,
+                                                 KState<T> *st, T *partials)
 {
     if (st->skip) return;
     T c[2] = {(T)0, (T)0};
@@ -47,13 +50,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_z(int64_t n, const T *r, const T 
         z[i] = zv;
         c[0] += rv * zv;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // First ? p = z : { p *= beta; p += 1 * z }   (Cg.cxx:86-94)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *z, T *p, const KState<T> *st)
+__global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *__restrict__ z, T *__restrict__ p, const KState<T> *st)
 {
     if (st->skip) return;
     const bool first = st->nb_iter == 0;
@@ -70,9 +72,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *z, T *p, co
 }
 
 // x += alpha p; r += (-alpha) q; sum r^2   (Cg.cxx:107-108, then Finished)
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *p, const T *q, T *x, T *r,
-                                                  KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *__restrict__ p, const T *__restrict__ q, T *__restrict__ x, T *__restrict__ r,
+                                                  KState<T> *st, T *partials)
 {
     if (st->skip) return;
     const T alpha = st->alpha, malpha = -st->alpha;
@@ -87,15 +89,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *p, const T
         }
         c[0] += rv * rv;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // ---------------------------------------------------------- BiCgStab ----
 // [x = 0, r = b]; rtilde = r; sums r^2 and rtilde*r  (BiCgStab.cxx:61-71)
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_bi_init(int64_t n, const T *b, T *x, T *r, T *rt, int zero_x,
-                                                    KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_init(int64_t n, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ r, T *__restrict__ rt, int zero_x,
+                                                    KState<T> *st, T *partials)
 {
     if (st->skip) return;
     T c[2] = {(T)0, (T)0};
@@ -112,14 +113,13 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(int64_t n, const T *b, T *x,
         c[0] += v * v;
         c[1] += v * v;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // p update (:81-96) and phat = M^{-1} p (:98)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const T *r, const T *v, T *p, const T *d,
-                                                 T *phat, const KState<T> *st)
+__global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const T *__restrict__ r, const T *__restrict__ v, T *__restrict__ p, const T *__restrict__ d,
+                                                 T *__restrict__ phat, const KState<T> *st)
 {
     if (st->skip) return;
     const bool first = st->nb_iter == 0;
@@ -141,10 +141,9 @@ __global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const T *r, const T 
 }
 
 // s = r; s += (-alpha) v (:111-112); shat = M^{-1} s (:124); sum s^2
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *r, const T *v, T *s, const T *d,
-                                                 T *shat, KState<T> *st, T *partials, unsigned *ticket,
-                                                 Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *__restrict__ r, const T *__restrict__ v, T *__restrict__ s, const T *__restrict__ d,
+                                                 T *__restrict__ shat, KState<T> *st, T *partials)
 {
     if (st->skip) return;
     const T malpha = -st->alpha;
@@ -157,13 +156,12 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *r, const T 
         if (d) shat[i] = sv / d[i];
         c[0] += sv * sv;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // Finished(s) ended the solve in this slot: x += alpha phat (:116-120)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bi_fix(int64_t n, const T *phat, T *x, const KState<T> *st,
+__global__ __launch_bounds__(kBlock) void k_bi_fix(int64_t n, const T *__restrict__ phat, T *__restrict__ x, const KState<T> *st,
                                                    int slot)
 {
     if (st->stop_slot != slot) return;
@@ -174,10 +172,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_fix(int64_t n, const T *phat, T *
 
 // x += alpha phat; x += omega shat; r = s; r += (-omega) t (:132-137);
 // sums r^2, rtilde*r for the next loop head
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *phat, const T *shat, const T *s,
-                                                  const T *t, const T *rt, T *x, T *r, KState<T> *st,
-                                                  T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *__restrict__ phat, const T *__restrict__ shat, const T *__restrict__ s,
+                                                  const T *__restrict__ t, const T *__restrict__ rt, T *__restrict__ x, T *__restrict__ r, KState<T> *st,
+                                                  T *partials)
 {
     if (st->skip) return;
     const T alpha = st->alpha, omega = st->omega, momega = -st->omega;
@@ -194,15 +192,14 @@ __global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *phat, cons
         c[0] += rv * rv;
         c[1] += rt[i] * rv;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_gm_init0(int64_t n, const T *b, const T *d, T *x, T *r,
-                                                     KState<T> *st, T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_init0(int64_t n, const T *__restrict__ b, const T *__restrict__ d, T *__restrict__ x, T *__restrict__ r,
+                                                     KState<T> *st, T *partials)
 {
     if (st->skip) return;
     T c[2] = {(T)0, (T)0};
@@ -212,13 +209,12 @@ __global__ __launch_bounds__(kBlock) void k_gm_init0(int64_t n, const T *b, cons
         r[i] = v;
         c[0] += v * v;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // V_0 = r * (1/beta)  (:105-106)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gm_v0(int64_t n, const T *r, T *v0, const KState<T> *st)
+__global__ __launch_bounds__(kBlock) void k_gm_v0(int64_t n, const T *__restrict__ r, T *__restrict__ v0, const KState<T> *st)
 {
     if (st->cyc_skip) return;
     const T sc = st->gm_scale;
@@ -226,10 +222,9 @@ __global__ __launch_bounds__(kBlock) void k_gm_v0(int64_t n, const T *r, T *v0, 
 }
 
 // MGS step k >= 1 of Arnoldi step i: w += (-H(k-1,i)) V_{k-1}; H(k,i) = (V_k, w)
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *vprev, const T *vk, T *w,
-                                                   int hpos, KState<T> *st, T *partials, unsigned *ticket,
-                                                   Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *__restrict__ vprev, const T *__restrict__ vk, T *__restrict__ w,
+                                                   int hpos, KState<T> *st, T *partials)
 {
     if (st->skip) return;
     const T a = (T)(-st->H[hpos]);
@@ -243,14 +238,13 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *vprev, co
         }
         c[0] += vk[j] * wv;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // last MGS update w += (-H(i,i)) V_i, then hi_ip1 = |w| (:131-135)
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_gm_last(int64_t n, const T *vi, T *w, int hpos, KState<T> *st,
-                                                    T *partials, unsigned *ticket, Fin fin)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_last(int64_t n, const T *__restrict__ vi, T *__restrict__ w, int hpos, KState<T> *st,
+                                                    T *partials)
 {
     if (st->skip) return;
     const T a = (T)(-st->H[hpos]);
@@ -264,14 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_gm_last(int64_t n, const T *vi, T *w
         }
         c[0] += wv * wv;
     }
-    T out[2];
-    if (grid_sum<T, 2>(c, partials, ticket, gridDim.x, blockIdx.x, out) && threadIdx.x == 0) fin(out);
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // V_{i+1} = w, scaled by 1/hi when hi != 0 (:136-140); runs before the
 // cycle-done test of the step has any effect (V_{i+1} is unused after it)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gm_vnext(int64_t n, const T *w, T *vnext, const KState<T> *st)
+__global__ __launch_bounds__(kBlock) void k_gm_vnext(int64_t n, const T *__restrict__ w, T *__restrict__ vnext, const KState<T> *st)
 {
     if (st->skip) return;
     const T sc = st->gm_scale;
@@ -296,7 +289,7 @@ __global__ void k_gm_solve(KState<T> *st)
 
 // x += sum_k s(k) V_k, Adds in k order per element (:169-170)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gm_xupd(int64_t n, const T *V, T *x, const KState<T> *st)
+__global__ __launch_bounds__(kBlock) void k_gm_xupd(int64_t n, const T *__restrict__ V, T *__restrict__ x, const KState<T> *st)
 {
     if (st->cyc_skip) return;
     __shared__ T coef[1024];
@@ -328,7 +321,6 @@ struct Solve {
     double *dhist = nullptr;
     int *dhist_it = nullptr;
     T *partials;
-    unsigned *ticket;
     char *ws;
     size_t off = 0;
 
@@ -353,7 +345,6 @@ struct Solve {
         ws = (char *)ctx->ws;
         off = 0;
         partials = (T *)ctx->red.partials;
-        ticket = ctx->red.ticket;
         dst = (KState<T> *)ctx->state;
         hst = (KState<T> *)ctx->state_host;
         if (hist_cap) {
@@ -417,6 +408,14 @@ struct Solve {
     {
         return launch_spmv<T, Fin>(A, a, fin);
     }
+
+    // Second half of the split reduction of the kernel just enqueued (which
+    // skipped itself, and wrote no partials, when the solve had ended).
+    template <typename Fin>
+    void fin(Fin f)
+    {
+        grid_finish<T>(ctx->stream, partials, nb, (const int *)&dst->skip, f);
+    }
 };
 
 constexpr int kChunk = 16;
@@ -432,14 +431,14 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
     hipStream_t st = ctx->stream;
     dim3 g(S.nb), blk(kBlock);
     // Init(b) (Cg.cxx:58)
-    hipLaunchKernelGGL((k_axpy_dot<T, FinInitNorm<T>>), g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
-                       (const T *)nullptr, S.partials, S.ticket, (const int *)&S.dst->skip,
-                       FinInitNorm<T>{S.dst});
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
     // r = b - A x or x = 0, r = b; then the first Finished(r)
     FinCgTop<T> top0{S.dst, 0, jac ? 0 : 1};
     if (it->init_guess_null) {
-        hipLaunchKernelGGL((k_cg_init0<T, FinCgTop<T>>), g, blk, 0, st, n, b, x, r, S.dst, S.partials, S.ticket,
-                           top0);
+        hipLaunchKernelGGL(k_cg_init0<T>, g, blk, 0, st, n, b, x, r, S.dst, S.partials);
+        S.fin(top0);
     } else {
         SpmvArgs<T> a{};
         a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1;
@@ -451,16 +450,17 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
     for (int64_t done_its = 0; done_its <= limit; done_its += kChunk) {
         for (int k = 0; k < kChunk; k++) {
-            if (jac)
-                hipLaunchKernelGGL((k_cg_z<T, FinCgRho<T>>), g, blk, 0, st, n, r, dg, z, S.dst, S.partials,
-                                   S.ticket, FinCgRho<T>{S.dst});
+            if (jac) {
+                hipLaunchKernelGGL(k_cg_z<T>, g, blk, 0, st, n, r, dg, z, S.dst, S.partials);
+                S.fin(FinCgRho<T>{S.dst});
+            }
             hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, st, n, z, p, S.dst);
             SpmvArgs<T> a{};
             a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = p; a.skip = &S.dst->skip;
             SHK_TRY(S.spmv(a, FinCgDelta<T>{S.dst}));
-            hipLaunchKernelGGL((k_cg_xr<T, FinCgTop<T>>), g, blk, 0, st, n, p, q, x, r, S.dst, S.partials,
-                               S.ticket, FinCgTop<T>{S.dst, 1, jac ? 0 : 1});
+            hipLaunchKernelGGL(k_cg_xr<T>, g, blk, 0, st, n, p, q, x, r, S.dst, S.partials);
+            S.fin(FinCgTop<T>{S.dst, 1, jac ? 0 : 1});
         }
         SHK_HIP(hipGetLastError());
         SHK_TRY(S.read_state());
@@ -482,16 +482,17 @@ int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iterat
     const T *dg = jac ? (const T *)A->diag : nullptr;
     hipStream_t st = ctx->stream;
     dim3 g(S.nb), blk(kBlock);
-    hipLaunchKernelGGL((k_axpy_dot<T, FinInitNorm<T>>), g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
-                       (const T *)nullptr, S.partials, S.ticket, (const int *)&S.dst->skip,
-                       FinInitNorm<T>{S.dst});
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
     if (!it->init_guess_null) {
         SpmvArgs<T> a{};
         a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
         SHK_TRY(S.spmv(a, FinNone{}));
     }
-    hipLaunchKernelGGL((k_bi_init<T, FinBiTop<T>>), g, blk, 0, st, n, b, x, r, rt, it->init_guess_null ? 1 : 0,
-                       S.dst, S.partials, S.ticket, FinBiTop<T>{S.dst, 0});
+    hipLaunchKernelGGL(k_bi_init<T>, g, blk, 0, st, n, b, x, r, rt, it->init_guess_null ? 1 : 0,
+                       S.dst, S.partials);
+    S.fin(FinBiTop<T>{S.dst, 0});
     SHK_HIP(hipGetLastError());
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
     int slot = 0;
@@ -502,15 +503,16 @@ int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iterat
             a.x = phat; a.y_out = v; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = rt; a.skip = &S.dst->skip;
             SHK_TRY(S.spmv(a, FinBiSigma<T>{S.dst}));
-            hipLaunchKernelGGL((k_bi_s<T, FinBiS<T>>), g, blk, 0, st, n, r, v, s, dg, shat, S.dst, S.partials,
-                               S.ticket, FinBiS<T>{S.dst, slot});
+            hipLaunchKernelGGL(k_bi_s<T>, g, blk, 0, st, n, r, v, s, dg, shat, S.dst, S.partials);
+            S.fin(FinBiS<T>{S.dst, slot});
             hipLaunchKernelGGL(k_bi_fix<T>, g, blk, 0, st, n, phat, x, S.dst, slot);
             SpmvArgs<T> b2{};
             b2.x = shat; b2.y_out = t; b2.alpha = (T)1; b2.mlt = 1;
             b2.dot0 = kDotVec; b2.u0 = s; b2.dot1 = kDotSelf; b2.skip = &S.dst->skip;
             SHK_TRY(S.spmv(b2, FinBiOmega<T>{S.dst}));
-            hipLaunchKernelGGL((k_bi_xr<T, FinBiTop<T>>), g, blk, 0, st, n, phat, shat, s, t, rt, x, r, S.dst,
-                               S.partials, S.ticket, FinBiTop<T>{S.dst, 1});
+            hipLaunchKernelGGL(k_bi_xr<T>, g, blk, 0, st, n, phat, shat, s, t, rt, x, r, S.dst,
+                               S.partials);
+            S.fin(FinBiTop<T>{S.dst, 1});
         }
         SHK_HIP(hipGetLastError());
         SHK_TRY(S.read_state());
@@ -550,8 +552,8 @@ int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
     auto Vk = [&](int k) { return V + (size_t)k * n; };
     // residual, preconditioning, beta, outer.Init (Gmres.cxx:82-95)
     if (it->init_guess_null) {
-        hipLaunchKernelGGL((k_gm_init0<T, FinGmInit<T>>), g, blk, 0, st, n, b, dg, x, r, S.dst, S.partials,
-                           S.ticket, FinGmInit<T>{S.dst});
+        hipLaunchKernelGGL(k_gm_init0<T>, g, blk, 0, st, n, b, dg, x, r, S.dst, S.partials);
+        S.fin(FinGmInit<T>{S.dst});
     } else {
         SpmvArgs<T> a{};
         a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.diag = dg;
@@ -572,12 +574,14 @@ int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
                 a.x = Vk(i); a.y_out = w; a.alpha = (T)1; a.mlt = 1; a.diag = dg;
                 a.dot0 = kDotVec; a.u0 = Vk(0); a.skip = &S.dst->skip;
                 SHK_TRY(S.spmv(a, FinGmH<T>{S.dst, 0, i}));
-                for (int k = 1; k <= i; k++)
-                    hipLaunchKernelGGL((k_gm_mgs<T, FinGmH<T>>), g, blk, 0, st, n, (const T *)Vk(k - 1),
-                                       (const T *)Vk(k), w, i * (m + 1) + (k - 1), S.dst, S.partials, S.ticket,
-                                       FinGmH<T>{S.dst, k, i});
-                hipLaunchKernelGGL((k_gm_last<T, FinGmStep<T>>), g, blk, 0, st, n, (const T *)Vk(i), w,
-                                   i * (m + 1) + i, S.dst, S.partials, S.ticket, FinGmStep<T>{S.dst, i});
+                for (int k = 1; k <= i; k++) {
+                    hipLaunchKernelGGL(k_gm_mgs<T>, g, blk, 0, st, n, (const T *)Vk(k - 1),
+                                       (const T *)Vk(k), w, i * (m + 1) + (k - 1), S.dst, S.partials);
+                    S.fin(FinGmH<T>{S.dst, k, i});
+                }
+                hipLaunchKernelGGL(k_gm_last<T>, g, blk, 0, st, n, (const T *)Vk(i), w,
+                                   i * (m + 1) + i, S.dst, S.partials);
+                S.fin(FinGmStep<T>{S.dst, i});
                 hipLaunchKernelGGL(k_gm_vnext<T>, g, blk, 0, st, n, (const T *)w, Vk(i + 1), S.dst);
             }
             SHK_HIP(hipGetLastError());
@@ -676,10 +680,11 @@ static int ir_impl(shk_ctx ctx, shk_csr A64, shk_csr A32, double *x, const doubl
     double bn2 = 0;
     {
         const int nb = red_blocks(n);
-        hipLaunchKernelGGL((k_axpy_dot<double, FinStore<double>>), dim3(nb), dim3(kBlock), 0, ctx->stream, n, 0.0,
+        hipLaunchKernelGGL(k_axpy_dot<double>, dim3(nb), dim3(kBlock), 0, ctx->stream, n, 0.0,
                            (const double *)nullptr, (double *)b, b, (const double *)nullptr,
-                           (double *)ctx->red.partials, ctx->red.ticket, (const int *)nullptr,
-                           FinStore<double>{(double *)ctx->scalars});
+                           (double *)ctx->red.partials, (const int *)nullptr);
+        grid_finish<double>(ctx->stream, (const double *)ctx->red.partials, nb, nullptr,
+                            FinStore<double>{(double *)ctx->scalars});
         SHK_HIP(hipGetLastError());
         SHK_HIP(hipMemcpyAsync(&bn2, ctx->scalars, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         SHK_HIP(hipStreamSynchronize(ctx->stream));

@@ -149,10 +149,19 @@ struct PanelPlan {
 static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, int align, size_t vsize,
                         PanelPlan &Q)
 {
-    if (m == 0 || !P.mid.empty()) return false;
-    for (int r = 0; r < m; r++)
-        for (int j = P.ptr[r] + 1; j < P.ptr[r + 1]; j++)
-            if ((P.ind[j] & ~kLowerFlag) < (P.ind[j - 1] & ~kLowerFlag)) return false;
+    if (m == 0) return false;
+    // RowSymSparse rows are [upper part ; transposed part] (mid[r] splits
+    // them): the chunks of a panel walk the upper parts first, then -- from
+    // the chunk flagged kChunkDirect on -- the transposed parts, each part
+    // column-sorted, so every expanded row is still visited in stored order.
+    const bool sym = !P.mid.empty();
+    auto part_lo = [&](int r, int pass) { return pass == 0 ? P.ptr[r] : P.mid[r]; };
+    auto part_hi = [&](int r, int pass) { return pass == 0 ? (sym ? P.mid[r] : P.ptr[r + 1]) : P.ptr[r + 1]; };
+    const int npass = sym ? 2 : 1;
+    for (int pass = 0; pass < npass; pass++)
+        for (int r = 0; r < m; r++)
+            for (int j = part_lo(r, pass) + 1; j < part_hi(r, pass); j++)
+                if ((P.ind[j] & ~kLowerFlag) < (P.ind[j - 1] & ~kLowerFlag)) return false;
     const int64_t nnz = P.ptr[m];
     Q.npanel = (m + kPanelRows - 1) / kPanelRows;
     Q.pchunk.assign(Q.npanel + 1, 0);
@@ -178,55 +187,60 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
             slot_of[order[k]] = k;
             Q.slot_row[(size_t)p * kPanelRows + k] = (unsigned short)order[k];
         }
-        cols.assign(P.ind.begin() + P.ptr[r0], P.ind.begin() + P.ptr[r1]);
-        for (int &c : cols) c &= (int)~kLowerFlag;
-        std::sort(cols.begin(), cols.end());
-        for (int r = r0; r < r1; r++) cur[r - r0] = P.ptr[r];
-        size_t i = 0;
-        while (i < cols.size()) {
-            const int c0 = cols[i];
-            size_t k = i;
-            // whole columns only, at most `budget` entries and `win` columns
-            while (k < cols.size() && cols[k] - c0 < win) {
-                size_t k2 = k;
-                while (k2 < cols.size() && cols[k2] == cols[k]) k2++;
-                if ((int64_t)(k2 - i) > budget) break;
-                k = k2;
-            }
-            if (k == i) return false;  // one column with more entries than a chunk holds
-            const int c1 = cols[k - 1] + 1;
-            while ((int64_t)Q.word.size() % align) {
-                Q.word.push_back(0u);
-                Q.src.push_back(-1);
-            }
-            const int64_t e0 = (int64_t)Q.word.size();
-            for (int r = r0; r < r1; r++) {
-                int j = cur[r - r0];
-                const int jb = j;
-                while (j < P.ptr[r + 1] && (int)(P.ind[j] & ~kLowerFlag) < c1) j++;
-                for (int q = jb; q < j; q++) {
-                    const uint32_t col = P.ind[q] & ~kLowerFlag;
-                    uint32_t w = (col - (uint32_t)c0) | ((uint32_t)slot_of[r - r0] << kPanRowShift) |
-                                 (P.ind[q] & kLowerFlag);
-                    if (q == jb) w |= kPanFirst;
-                    if (q == j - 1) w |= kPanLast;
-                    if (P.diag_pos[r] == q) Q.diag_pos[r] = (int64_t)Q.word.size();
-                    Q.word.push_back(w);
-                    Q.src.push_back(q);
+        for (int pass = 0; pass < npass; pass++) {
+            cols.clear();
+            for (int r = r0; r < r1; r++)
+                for (int j = part_lo(r, pass); j < part_hi(r, pass); j++) cols.push_back((int)(P.ind[j] & ~kLowerFlag));
+            std::sort(cols.begin(), cols.end());
+            for (int r = r0; r < r1; r++) cur[r - r0] = part_lo(r, pass);
+            bool first_chunk = true;
+            size_t i = 0;
+            while (i < cols.size()) {
+                const int c0 = cols[i];
+                size_t k = i;
+                // whole columns only, at most `budget` entries and `win` columns
+                while (k < cols.size() && cols[k] - c0 < win) {
+                    size_t k2 = k;
+                    while (k2 < cols.size() && cols[k2] == cols[k]) k2++;
+                    if ((int64_t)(k2 - i) > budget) break;
+                    k = k2;
                 }
-                cur[r - r0] = j;
+                if (k == i) return false;  // one column with more entries than a chunk holds
+                const int c1 = cols[k - 1] + 1;
+                while ((int64_t)Q.word.size() % align) {
+                    Q.word.push_back(0u);
+                    Q.src.push_back(-1);
+                }
+                const int64_t e0 = (int64_t)Q.word.size();
+                for (int r = r0; r < r1; r++) {
+                    int j = cur[r - r0];
+                    const int jb = j;
+                    while (j < part_hi(r, pass) && (int)(P.ind[j] & ~kLowerFlag) < c1) j++;
+                    for (int q = jb; q < j; q++) {
+                        const uint32_t col = P.ind[q] & ~kLowerFlag;
+                        uint32_t w = (col - (uint32_t)c0) | ((uint32_t)slot_of[r - r0] << kPanRowShift) |
+                                     (P.ind[q] & kLowerFlag);
+                        if (q == jb) w |= kPanFirst;
+                        if (q == j - 1) w |= kPanLast;
+                        if (P.diag_pos[r] == q) Q.diag_pos[r] = (int64_t)Q.word.size();
+                        Q.word.push_back(w);
+                        Q.src.push_back(q);
+                    }
+                    cur[r - r0] = j;
+                }
+                const int64_t e1 = (int64_t)Q.word.size();
+                if (e1 >= (int64_t)INT32_MAX) return false;
+                Q.chunk.push_back((int)((uint32_t)c0 | (pass == 1 && first_chunk ? kChunkDirect : 0u)));
+                Q.chunk.push_back(c1);
+                Q.chunk.push_back((int)e0);
+                Q.chunk.push_back((int)e1);
+                Q.x_bytes += (double)(c1 - c0) * (double)vsize;
+                first_chunk = false;
+                i = k;
             }
-            const int64_t e1 = (int64_t)Q.word.size();
-            if (e1 >= (int64_t)INT32_MAX) return false;
-            Q.chunk.push_back(c0);
-            Q.chunk.push_back(c1);
-            Q.chunk.push_back((int)e0);
-            Q.chunk.push_back((int)e1);
-            Q.x_bytes += (double)(c1 - c0) * (double)vsize;
-            i = k;
+            for (int r = r0; r < r1; r++)
+                if (cur[r - r0] != part_hi(r, pass)) return false;  // not reached with sorted parts
         }
-        for (int r = r0; r < r1; r++)
-            if (cur[r - r0] != P.ptr[r + 1]) return false;  // not reached with sorted rows
         Q.pchunk[p + 1] = (int)(Q.chunk.size() / 4);
     }
     Q.nslots = (int64_t)Q.word.size();
@@ -300,9 +314,31 @@ static int upload_panels(shk_csr A, const HostPlan &P, const PanelPlan &Q, const
         SHK_HIP(hipMemcpyAsync(A->chunk, Q.chunk.data(), sizeof(int) * Q.chunk.size(), hipMemcpyHostToDevice, s));
     SHK_HIP(hipMemcpyAsync(A->map_diag, Q.diag_pos.data(), sizeof(int64_t) * m, hipMemcpyHostToDevice, s));
     std::vector<T> val(ns + pad, (T)0);
+    const bool sym = !P.mid.empty();
     if (val_host)
         for (int64_t q = 0; q < ns; q++)
-            if (Q.src[q] >= 0) val[q] = val_host[Q.src[q]];
+            if (Q.src[q] >= 0) val[q] = val_host[sym ? P.src_of[Q.src[q]] : Q.src[q]];
+    if (sym) {
+        // stored entry -> panel position of its upper copy and of its
+        // transposed copy, for value updates and the gradient fold
+        std::vector<int64_t> up(std::max<int64_t>(A->nnz_stored, 1), -1), lo(std::max<int64_t>(A->nnz_stored, 1), -1);
+        std::vector<int> row_of(P.ind.size());
+        for (int r = 0; r < m; r++)
+            for (int q = P.ptr[r]; q < P.ptr[r + 1]; q++) row_of[q] = r;
+        for (int64_t pos = 0; pos < ns; pos++) {
+            const int64_t q = Q.src[pos];
+            if (q < 0) continue;
+            if (q < P.mid[row_of[q]]) up[P.src_of[q]] = pos;
+            else lo[P.src_of[q]] = pos;
+        }
+        SHK_HIP(hipMalloc(&A->mid, sizeof(int) * std::max(m, 1)));
+        if (m) SHK_HIP(hipMemcpyAsync(A->mid, P.mid.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+        SHK_HIP(hipMalloc(&A->map_up, sizeof(int64_t) * up.size()));
+        SHK_HIP(hipMalloc(&A->map_lo, sizeof(int64_t) * lo.size()));
+        SHK_HIP(hipMemcpyAsync(A->map_up, up.data(), sizeof(int64_t) * up.size(), hipMemcpyHostToDevice, s));
+        SHK_HIP(hipMemcpyAsync(A->map_lo, lo.data(), sizeof(int64_t) * lo.size(), hipMemcpyHostToDevice, s));
+        SHK_HIP(hipStreamSynchronize(s));
+    }
     SHK_HIP(hipMemcpyAsync(A->val, val.data(), sizeof(T) * (ns + pad), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_gather_diag<T>, dim3(red_blocks(m)), dim3(kBlock), 0, s, m,
                        (const int64_t *)A->map_diag, (const T *)A->val, (T *)A->diag);
@@ -378,7 +414,7 @@ template <typename T>
 static int set_values_dev(shk_csr A, const T *dval)
 {
     hipStream_t s = A->ctx->stream;
-    if (A->layout == SHK_LAYOUT_PANELS) {
+    if (A->layout == SHK_LAYOUT_PANELS && A->storage != SHK_ROWSYMSPARSE) {
         if (A->npanel)
             hipLaunchKernelGGL(k_panel_scatter<T>, dim3(A->npanel), dim3(kBlock), 0, s, (const int *)A->pchunk,
                                (const int4 *)A->chunk, (const uint32_t *)A->ind,
@@ -469,10 +505,10 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
     int layout = SHK_LAYOUT_ROWS;
     if (want != SHK_LAYOUT_ROWS) {
         const int win = dtype == SHK_F64 ? panel_win<double>() : panel_win<float>();
-        if (plan_panels(m, n, P, budget, win, align, vs, Q)) layout = SHK_LAYOUT_PANELS;
+        const int pbudget = dtype == SHK_F64 ? panel_budget<double>() : panel_budget<float>();
+        if (plan_panels(m, n, P, pbudget, win, align, vs, Q)) layout = SHK_LAYOUT_PANELS;
         else SHK_CHECK(want != SHK_LAYOUT_PANELS, SHK_ERR_ARG,
-                       "shk_csr_create: the panel layout needs RowSparse / ArrayRowSparse rows with sorted "
-                       "columns near the diagonal");
+                       "shk_csr_create: the panel layout needs rows with sorted columns near the diagonal");
     }
     shk_csr A = new shk_csr_s();
     A->layout = layout;
@@ -483,7 +519,8 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
     A->n = n;
     A->nnz_stored = nnz;
     A->nnz_dev = (int64_t)P.ind.size();
-    A->budget = budget;
+    A->budget = layout == SHK_LAYOUT_PANELS ? (dtype == SHK_F64 ? panel_budget<double>() : panel_budget<float>())
+                                            : budget;
     A->max_row = P.max_row;
     int st;
     if (layout == SHK_LAYOUT_PANELS) {

@@ -93,8 +93,10 @@ def test_solver_matches_seldon(gpu, fname):
         hi = max(int(fx["nb_iter"]), int(fx["other_nb_iter"]))
         slack = max(hi - lo, 2)
         assert lo - slack <= it.nb_iter <= hi + slack, (it.nb_iter, lo, hi)
-        # before Seldon's own builds drift apart, the GPU tracks them
-        agree = np.nonzero(spread > 1e-8)[0]
+        # BiCgStab amplifies rounding exponentially: while Seldon's own two
+        # builds still agree to 1e-10, the GPU (fixed-order tree reductions,
+        # a third valid summation order) tracks them within 10x their spread
+        agree = np.nonzero(spread > 1e-10)[0]
         k = int(agree[0]) if agree.size else n
         assert np.all(rel[:k] <= tol[:k]), (int(np.argmax(rel[:k] > tol[:k])), k)
     else:
