@@ -413,8 +413,9 @@ __global__ __launch_bounds__(kBlock, SHK_SPMV_WG_PER_CU) void k_spmv(SpmvArgs<T>
 // share wavefronts and the waves of short rows finish their sums early.
 // Each entry's 32-bit word packs its window-local column (13 bits), its row
 // slot (11 bits), "first / last entry of the row in this chunk" flags, and
-// kLowerFlag (ArrayRowSparse: a direct term).  The entries of the next two
-// chunks and the next x window are in flight while a chunk is summed.
+// kLowerFlag (ArrayRowSparse: a direct term).  The entries of the next
+// panel_sets() - 1 chunks and the next panel_xlead() x windows are in flight
+// while a chunk is summed.
 constexpr int kPanelRows = 2048;
 constexpr int kPanelThreads = 1024;  // work-group of the panel kernel: two rows per thread
 constexpr uint32_t kPanColMask = 0x1FFFu;
@@ -429,12 +430,36 @@ __device__ __forceinline__ int chunk_c0(int4 ci) { return (int)((uint32_t)ci.x &
 template <typename T>
 constexpr int panel_win() { return sizeof(T) == 8 ? 3072 : 6144; }  // 24 KiB of x
 #ifndef SHK_PANEL_BUDGET_F64
-#define SHK_PANEL_BUDGET_F64 2048
+#define SHK_PANEL_BUDGET_F64 4096
 #endif
 #ifndef SHK_PANEL_BUDGET_F32
 #define SHK_PANEL_BUDGET_F32 4096
 #endif
-// entries per chunk (LDS products: 16 KiB)
+// register sets of chunk entries in the panel kernel's pipeline: loads run
+// panel_sets() - 1 chunks ahead of the sums
+#ifndef SHK_PANEL_SETS_F64
+#define SHK_PANEL_SETS_F64 2
+#endif
+#ifndef SHK_PANEL_SETS_F32
+#define SHK_PANEL_SETS_F32 3
+#endif
+// waves per SIMD the panel kernel is compiled for: 8 = two 1024-thread
+// groups per CU, which caps it at 64 VGPRs
+#ifndef SHK_PANEL_MINW
+#define SHK_PANEL_MINW 8
+#endif
+template <typename T>
+constexpr int panel_sets() { return sizeof(T) == 8 ? SHK_PANEL_SETS_F64 : SHK_PANEL_SETS_F32; }
+// x windows in flight (register sets of the window loads)
+#ifndef SHK_PANEL_XLEAD_F64
+#define SHK_PANEL_XLEAD_F64 1
+#endif
+#ifndef SHK_PANEL_XLEAD_F32
+#define SHK_PANEL_XLEAD_F32 1
+#endif
+template <typename T>
+constexpr int panel_xlead() { return sizeof(T) == 8 ? SHK_PANEL_XLEAD_F64 : SHK_PANEL_XLEAD_F32; }
+// entries per chunk (LDS products: 32 KiB FP64, 16 KiB FP32)
 template <typename T>
 constexpr int panel_budget() { return sizeof(T) == 8 ? SHK_PANEL_BUDGET_F64 : SHK_PANEL_BUDGET_F32; }
 
@@ -464,6 +489,17 @@ __device__ __forceinline__ void panel_xload(const T *x, int4 ci, PanelX<T, NT> &
     }
 }
 
+// beta * y_in for one row, the value y starts from: Seldon's Mlt(beta, Y)
+// before the sums (Functions_MatVect.cxx:414); ArrayRowSparse zero-fills Y
+// when beta is 0 (Functions_MatrixArray.cxx:548-551).  Read where it is
+// used, so the panel kernel holds no registers for it meanwhile.
+template <typename T, int MODE>
+__device__ __forceinline__ T panel_y0(const SpmvArgs<T> &a, int row)
+{
+    if (a.mlt || row < 0 || (MODE == kArray && a.beta == (T)0)) return (T)0;
+    return a.y_in[row] * a.beta;
+}
+
 // One chunk.  On entry R holds the chunk's entries and X its x window.  The
 // loads of the next chunk's window (into X) and of the chunk two ahead (into
 // R2, a third register set) are issued first, together: every load then
@@ -474,7 +510,7 @@ template <typename T, int MODE, int BUDGET, int NT>
 __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BUDGET> &sm, int4 ci,
                                            const SpmvRegs<T, BUDGET, NT> &R, SpmvRegs<T, BUDGET, NT> &R2,
                                            PanelX<T, NT> &X, bool xnext, int4 cx, bool rnext, int4 c2,
-                                           T (&acc)[kPanelRows / NT], const T (&y0)[kPanelRows / NT])
+                                           T (&acc)[kPanelRows / NT], const int (&row)[kPanelRows / NT])
 {
     constexpr int RPT = kPanelRows / NT;
     constexpr int V = VecOf<T>::V, K = BUDGET / (NT * V);
@@ -529,7 +565,7 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
         // terms (Functions_MatVect.cxx:553-563)
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
-            T y = y0[i];
+            T y = panel_y0<T, MODE>(a, row[i]);
             y += a.alpha * acc[i];
             acc[i] = y;
         }
@@ -561,7 +597,7 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
     }
 }
 
-template <typename T, int MODE, int BUDGET, int NT = kPanelThreads, int MINW = 2>
+template <typename T, int MODE, int BUDGET, int NT = kPanelThreads, int MINW = SHK_PANEL_MINW>
 __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *__restrict__ chunk)
 {
     if (a.skip && *a.skip) return;
@@ -573,52 +609,67 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
     if (p < a.npanel) {
         const int r0 = p * kPanelRows;
         const int nr = min(kPanelRows, a.m - r0);
-        T acc[RPT], yin[RPT];
+        T acc[RPT];
         int row[RPT];
         const unsigned short *srow = a.slot_row + (size_t)p * kPanelRows;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             const int rr = t + i * NT;
             row[i] = rr < nr ? r0 + srow[rr] : -1;
-            yin[i] = (!a.mlt && row[i] >= 0) ? a.y_in[row[i]] : (T)0;
+            // ArrayRowSparse adds the direct terms onto beta * y; RowSparse
+            // and RowSymSparse sum a temp first
+            acc[i] = MODE == kArray ? panel_y0<T, MODE>(a, row[i]) : (T)0;
             sm.seg_s[rr] = 0;
             sm.seg_e[rr] = 0;
-        }
-        T y0[RPT];
-#pragma unroll
-        for (int i = 0; i < RPT; i++) {
-            // ArrayRowSparse adds the direct terms onto beta * y (zero-filled
-            // when beta is 0); RowSparse and RowSymSparse sum a temp first
-            if (MODE == kArray) y0[i] = a.mlt ? (T)0 : ((a.beta == (T)0) ? (T)0 : yin[i] * a.beta);
-            else y0[i] = a.mlt ? (T)0 : yin[i] * a.beta;
-            acc[i] = MODE == kArray ? y0[i] : (T)0;
         }
         bool direct = false;  // RowSymSparse: y formed, transposed parts started
         const int cb = a.pchunk[p], ce = a.pchunk[p + 1];
         if (cb < ce) {
-            // chunks cc, cc+1 in RA, RB; cc+2 goes to RC; the sets rotate
-            SpmvRegs<T, BUDGET, NT> RA, RB, RC;
-            PanelX<T, NT> X;
-            int4 c0 = chunk[cb];
-            int4 c1 = cb + 1 < ce ? chunk[cb + 1] : c0;
-            spmv_load<T, BUDGET, NT>(a, c0.z, RA);
-            panel_xload<T, NT>(a.x, c0, X);
-            spmv_load<T, BUDGET, NT>(a, c1.z, RB, cb + 1 < ce);
-            for (int cc = cb;;) {
-                const int4 c2 = cc + 2 < ce ? chunk[cc + 2] : c0;
-                direct |= ((uint32_t)c0.x & kChunkDirect) != 0;
-                panel_step<T, MODE, BUDGET, NT>(a, sm, c0, RA, RC, X, cc + 1 < ce, c1, cc + 2 < ce, c2, acc, y0);
-                if (++cc >= ce) break;
-                const int4 c3 = cc + 2 < ce ? chunk[cc + 2] : c0;
-                direct |= ((uint32_t)c1.x & kChunkDirect) != 0;
-                panel_step<T, MODE, BUDGET, NT>(a, sm, c1, RB, RA, X, cc + 1 < ce, c2, cc + 2 < ce, c3, acc, y0);
-                if (++cc >= ce) break;
-                const int4 c4 = cc + 2 < ce ? chunk[cc + 2] : c0;
-                direct |= ((uint32_t)c2.x & kChunkDirect) != 0;
-                panel_step<T, MODE, BUDGET, NT>(a, sm, c2, RC, RB, X, cc + 1 < ce, c3, cc + 2 < ce, c4, acc, y0);
-                if (++cc >= ce) break;
-                c0 = c3;
-                c1 = c4;
+            // NS register sets of entries rotate: chunk cc + u is in R[u]
+            // and, while it is summed, chunk cc + u + NS - 1 loads into the
+            // set freed by the previous chunk.  The x windows run XL chunks
+            // ahead in XL sets: a window is waited for before it is staged,
+            // and the wait counters retire loads in issue order, so the
+            // entry loads stay in flight across that wait only when they
+            // were issued after it.  The loop has one exit (full rounds of
+            // NS chunks) and the last chunks run after it: a loop left from
+            // the middle of its body is rebuilt by the compiler with paths
+            // that carry in-flight loads back to its head, where it then
+            // waits for every load.  Chunk descriptors are read a step
+            // before their loads are issued (index clamped, no branch).
+            constexpr int NS = panel_sets<T>(), XL = panel_xlead<T>();
+            static_assert(NS % XL == 0 && XL < NS, "x window sets must divide the round");
+            SpmvRegs<T, BUDGET, NT> R[NS];
+            PanelX<T, NT> X[XL];
+            auto desc = [&](int k) { return chunk[k < ce ? k : ce - 1]; };
+            int4 d[NS];
+#pragma unroll
+            for (int j = 0; j < NS; j++) d[j] = desc(cb + j);
+            spmv_load<T, BUDGET, NT>(a, d[0].z, R[0]);
+#pragma unroll
+            for (int j = 0; j < XL; j++) panel_xload<T, NT>(a.x, d[j], X[j], cb + j < ce);
+#pragma unroll
+            for (int j = 1; j < NS - 1; j++) spmv_load<T, BUDGET, NT>(a, d[j].z, R[j], cb + j < ce);
+            int cc = cb;
+            for (; cc + NS <= ce; cc += NS) {
+#pragma unroll
+                for (int u = 0; u < NS; u++) {
+                    const int4 dn = desc(cc + u + NS);
+                    direct |= ((uint32_t)d[u].x & kChunkDirect) != 0;
+                    panel_step<T, MODE, BUDGET, NT>(a, sm, d[u], R[u], R[(u + NS - 1) % NS], X[u % XL],
+                                                    cc + u + XL < ce, d[(u + XL) % NS], cc + u + NS - 1 < ce,
+                                                    d[(u + NS - 1) % NS], acc, row);
+                    d[u] = dn;
+                }
+            }
+            // fewer than NS chunks left, in R[0], R[1], ...
+            const int left = ce - cc;
+#pragma unroll
+            for (int u = 0; u < NS - 1; u++) {
+                if (u >= left) break;
+                direct |= ((uint32_t)d[u].x & kChunkDirect) != 0;
+                panel_step<T, MODE, BUDGET, NT>(a, sm, d[u], R[u], R[(u + NS - 1) % NS], X[u % XL], u + XL < left,
+                                                d[(u + XL) % NS], false, d[u], acc, row);
             }
         }
 #pragma unroll
@@ -628,7 +679,7 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
                 if (MODE == kArray || (MODE == kSym && direct)) {
                     y = acc[i];
                 } else {
-                    y = y0[i];
+                    y = panel_y0<T, MODE>(a, row[i]);
                     y += a.alpha * acc[i];
                 }
                 spmv_epilogue<T, MODE>(a, row[i], y, c);
