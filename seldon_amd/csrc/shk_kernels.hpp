@@ -463,11 +463,22 @@ constexpr int panel_xlead() { return sizeof(T) == 8 ? SHK_PANEL_XLEAD_F64 : SHK_
 template <typename T>
 constexpr int panel_budget() { return sizeof(T) == 8 ? SHK_PANEL_BUDGET_F64 : SHK_PANEL_BUDGET_F32; }
 
+// The staged x window (32-bit halves in two planes were tried for FP64: the
+// scattered 64-bit reads are faster).
+template <typename T>
+struct PanelXs {
+    T v[panel_win<T>()];
+    __device__ __forceinline__ void put(int i, T x) { v[i] = x; }
+    __device__ __forceinline__ T get(uint32_t i) const { return v[i]; }
+};
+
+// seg[s]: the row in slot s has its entries of the current chunk at
+// [seg & 0xffff, seg >> 16) of prod (both halves 0: none)
 template <typename T, int BUDGET>
 struct alignas(16) PanelSmem {
-    T xs[panel_win<T>()];
+    PanelXs<T> xs;
     T prod[BUDGET];
-    unsigned short seg_s[kPanelRows], seg_e[kPanelRows];
+    unsigned seg[kPanelRows + 4];  // [kPanelRows]: spare word for masked-off stores
 };
 
 template <typename T, int NT>
@@ -501,11 +512,11 @@ __device__ __forceinline__ T panel_y0(const SpmvArgs<T> &a, int row)
 }
 
 // One chunk.  On entry R holds the chunk's entries and X its x window.  The
-// loads of the next chunk's window (into X) and of the chunk two ahead (into
-// R2, a third register set) are issued first, together: every load then
-// waits behind a whole chunk of work, and the wait counters, which retire
+// loads of a later chunk's window (into X) and of a later chunk's entries
+// (into R2, a free register set) are issued first, together: every load then
+// waits behind whole chunks of work, and the wait counters, which retire
 // loads in issue order, never hold this chunk's reads behind newer ones.
-// The caller rotates the three register sets.
+// The caller rotates the register sets.
 template <typename T, int MODE, int BUDGET, int NT>
 __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BUDGET> &sm, int4 ci,
                                            const SpmvRegs<T, BUDGET, NT> &R, SpmvRegs<T, BUDGET, NT> &R2,
@@ -516,7 +527,7 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
     constexpr int V = VecOf<T>::V, K = BUDGET / (NT * V);
     const int t = threadIdx.x;
 #pragma unroll
-    for (int j = 0; j < PanelX<T, NT>::XK; j++) sm.xs[j * NT + t] = X.v[j];
+    for (int j = 0; j < PanelX<T, NT>::XK; j++) sm.xs.put(j * NT + t, X.v[j]);
     __builtin_amdgcn_sched_barrier(0);
     panel_xload<T, NT>(a.x, cx, X, xnext);
     spmv_load<T, BUDGET, NT>(a, c2.z, R2, rnext);
@@ -530,7 +541,7 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const bool in = ci.z + (k * NT + t) * V + v < ci.w;
-            xv[k][v] = sm.xs[in ? (R.iv[k][v] & kPanColMask) : 0u];
+            xv[k][v] = sm.xs.get(in ? (R.iv[k][v] & kPanColMask) : 0u);
         }
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -556,8 +567,12 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
             const bool in = ci.z + el < ci.w;
             const uint32_t w = R.iv[k][v];
             const int lr = (w >> kPanRowShift) & kPanRowMask;
-            if (in && (w & kPanFirst)) sm.seg_s[lr] = (unsigned short)el;
-            if (in && (w & kPanLast)) sm.seg_e[lr] = (unsigned short)(el + 1);
+            // unconditional stores (entries that are not a segment's first or
+            // last write to a spare word): no per-store exec masking
+            unsigned short *half = (unsigned short *)sm.seg;
+            const bool f = in && (w & kPanFirst), l = in && (w & kPanLast);
+            half[f ? 2 * lr : 2 * kPanelRows] = (unsigned short)el;
+            half[l ? 2 * lr + 1 : 2 * kPanelRows + 1] = (unsigned short)(el + 1);
         }
     __syncthreads();
     if (MODE == kSym && ((uint32_t)ci.x & kChunkDirect)) {
@@ -576,8 +591,9 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         const int rr = t + i * NT;
-        const int s0 = sm.seg_s[rr];
-        const int n0 = (int)sm.seg_e[rr] - s0;
+        const unsigned sw = sm.seg[rr];
+        const int s0 = (int)(sw & 0xffffu);
+        const int n0 = (int)(sw >> 16) - s0;
         for (int j = 0; j < n0; j += 4) {
             T pv[4];
 #pragma unroll
@@ -590,11 +606,7 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
         }
     }
 #pragma unroll
-    for (int i = 0; i < RPT; i++) {
-        const int rr = t + i * NT;
-        sm.seg_s[rr] = 0;
-        sm.seg_e[rr] = 0;
-    }
+    for (int i = 0; i < RPT; i++) sm.seg[t + i * NT] = 0u;
 }
 
 template <typename T, int MODE, int BUDGET, int NT = kPanelThreads, int MINW = SHK_PANEL_MINW>
@@ -619,8 +631,7 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
             // ArrayRowSparse adds the direct terms onto beta * y; RowSparse
             // and RowSymSparse sum a temp first
             acc[i] = MODE == kArray ? panel_y0<T, MODE>(a, row[i]) : (T)0;
-            sm.seg_s[rr] = 0;
-            sm.seg_e[rr] = 0;
+            sm.seg[rr] = 0u;
         }
         bool direct = false;  // RowSymSparse: y formed, transposed parts started
         const int cb = a.pchunk[p], ce = a.pchunk[p + 1];
