@@ -11,6 +11,8 @@
 //     (computation/solver/iterative/Iterative.hxx:47-108, Iterative.cxx:81-354);
 //   * Preconditioner_Base (Iterative.hxx:25-39);
 //   * the exception classes of share/Errors.hxx used on this path;
+//   * the transpose tags SeldonTrans / SeldonNoTrans / SeldonConjTrans
+//     (share/MatrixFlag.hxx:32-80);
 //   * a host dense vector Vector<T, VectFull> and host CSR
 //     Matrix<T, General, RowSparse> / Matrix<T, Symmetric, RowSymSparse>
 //     to build systems and move them to and from the GPU.
@@ -52,6 +54,38 @@ namespace Seldon
 
   template <class T>
   class MallocAlloc {};
+
+  // transpose tags (share/MatrixFlag.hxx:32-80): the global objects
+  // SeldonTrans / SeldonNoTrans / SeldonConjTrans select the transposed
+  // products; status 0: Trans, 1: NoTrans, 2: ConjTrans
+  class SeldonTranspose
+  {
+  protected:
+    int status_;
+  public:
+    explicit SeldonTranspose(int status) : status_(status) {}
+    bool Trans() const { return status_ == 0; }
+    bool NoTrans() const { return status_ == 1; }
+    bool ConjTrans() const { return status_ == 2; }
+  };
+  class class_SeldonTrans : public SeldonTranspose
+  {
+  public:
+    class_SeldonTrans() : SeldonTranspose(0) {}
+  };
+  class class_SeldonNoTrans : public SeldonTranspose
+  {
+  public:
+    class_SeldonNoTrans() : SeldonTranspose(1) {}
+  };
+  class class_SeldonConjTrans : public SeldonTranspose
+  {
+  public:
+    class_SeldonConjTrans() : SeldonTranspose(2) {}
+  };
+  static const class_SeldonTrans SeldonTrans;
+  static const class_SeldonNoTrans SeldonNoTrans;
+  static const class_SeldonConjTrans SeldonConjTrans;
 
   // ------------------------------------------------------------ errors --
   class Error

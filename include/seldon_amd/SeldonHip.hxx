@@ -21,6 +21,10 @@
 // when the preconditioner is Preconditioner_Base (identity) or
 // JacobiPreconditioner.  Every call goes through the C ABI of include/shk.h.
 //
+// The transposed products (MltAdd / Mlt with SeldonTrans) and Swap are
+// overloaded too, so the transpose-using solvers (BiCg, Qmr, Cgne, Lsqr)
+// also run on the GPU storages through Seldon's own templates.
+//
 // Names: VectHip, RowSparseHip, RowSymSparseHip, ArrayRowSparseHip are the
 // device counterparts of Seldon's VectFull, RowSparse, RowSymSparse and
 // ArrayRowSparse storages (share/Storage.hxx:56,82,115,138).
@@ -216,6 +220,12 @@ namespace Seldon
       h.Reallocate(m_);
       Download(h.GetData());
     }
+    // exchange storage with Y (Swap)
+    void SwapStorage(Vector& Y)
+    {
+      std::swap(m_, Y.m_);
+      std::swap(data_, Y.data_);
+    }
     // element read for inspection (one transfer)
     T GetElement(int i) const
     {
@@ -351,6 +361,55 @@ namespace Seldon
     if (X.GetM() != M.GetN() || Y.GetM() != M.GetM())
       throw WrongDim("Mlt(M, X, Y)", "sizes do not match");
     HipCheck(shk_mlt(HipContext::Get().Handle(), M.GetHandle(), X.GetData(), Y.GetData()), "Mlt");
+  }
+
+  // MltAdd(alpha, SeldonTrans, M, X, beta, Y): Y <- beta Y + alpha M^T X
+  // (Functions_MatVect.cxx:729-758 RowSparse, :918-927 RowSymSparse;
+  // Functions_MatrixArray.cxx:591-632 ArrayRowSparse).  SeldonConjTrans is
+  // the same for real values; SeldonNoTrans is the plain product.
+  template <class T0, class T1, class Prop1, class Storage1, class Allocator1, class T2, class Allocator2,
+            class T3, class T4, class Allocator4>
+  typename std::enable_if<IsHipSparse<Storage1>::value>::type
+  MltAdd(const T0 alpha, const SeldonTranspose& Trans, const Matrix<T1, Prop1, Storage1, Allocator1>& M,
+         const Vector<T2, VectHip, Allocator2>& X, const T3 beta, Vector<T4, VectHip, Allocator4>& Y)
+  {
+    if (Trans.NoTrans())
+      {
+        MltAdd(alpha, M, X, beta, Y);
+        return;
+      }
+    if (X.GetM() != M.GetM() || Y.GetM() != M.GetN())
+      throw WrongDim("MltAdd(alpha, SeldonTrans, M, X, beta, Y)", "sizes do not match");
+    HipCheck(shk_spmv_trans(HipContext::Get().Handle(), M.GetHandle(), double(alpha), X.GetData(),
+                            double(beta), Y.GetData()),
+             "MltAdd(SeldonTrans)");
+  }
+
+  // Mlt(SeldonTrans, M, X, Y) (Functions_MatVect.cxx:123-130): Y = M^T X,
+  // summed per column of M in increasing row order like Seldon's generic loop
+  // (:1307-1313); SeldonNoTrans is Mlt(M, X, Y).
+  template <class T0, class Prop0, class Storage0, class Allocator0, class T1, class Allocator1, class T2,
+            class Allocator2>
+  typename std::enable_if<IsHipSparse<Storage0>::value>::type
+  Mlt(const SeldonTranspose& Trans, const Matrix<T0, Prop0, Storage0, Allocator0>& M,
+      const Vector<T1, VectHip, Allocator1>& X, Vector<T2, VectHip, Allocator2>& Y)
+  {
+    if (Trans.NoTrans())
+      {
+        Mlt(M, X, Y);
+        return;
+      }
+    if (X.GetM() != M.GetM() || Y.GetM() != M.GetN())
+      throw WrongDim("Mlt(SeldonTrans, M, X, Y)", "sizes do not match");
+    HipCheck(shk_mlt_trans(HipContext::Get().Handle(), M.GetHandle(), X.GetData(), Y.GetData()),
+             "Mlt(SeldonTrans)");
+  }
+
+  // Swap(X, Y): exchanges the two vectors' storage (Functions_Vector.cxx:295-304)
+  template <class T, class Allocator>
+  void Swap(Vector<T, VectHip, Allocator>& X, Vector<T, VectHip, Allocator>& Y)
+  {
+    X.SwapStorage(Y);
   }
 
   // Mlt(alpha, X): X *= alpha (Functions_Vector.cxx:69-74)

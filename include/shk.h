@@ -134,6 +134,19 @@ int shk_spmv_out(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double be
                  const void *dy_in, void *dy_out);
 /* Mlt(M, X, Y): Y.Fill(0); MltAdd(1, M, X, 0, Y) (Functions_MatVect.cxx:72-81). */
 int shk_mlt(shk_ctx ctx, shk_csr A, const void *dx, void *dy);
+/* MltAdd(alpha, SeldonTrans, A, X, beta, Y): Y <- beta*Y + alpha*A^T*X
+ *   RowSparse      Functions_MatVect.cxx:729-758 (Mlt(beta, Y), then
+ *                  Y(ind[j]) += alpha * data[j] * X(i), rows i ascending)
+ *   ArrayRowSparse matrix_sparse/Functions_MatrixArray.cxx:591-632
+ *                  (beta == 0 zero-fills)
+ *   RowSymSparse   forwards to shk_spmv (Functions_MatVect.cxx:918-927)
+ * X has A's row count, Y its column count.  The first call builds A^T on
+ * the device (kept with A; rebuilt after shk_csr_set_values).
+ * SeldonConjTrans is the same for real values. */
+int shk_spmv_trans(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double beta, void *dy);
+/* Mlt(SeldonTrans, M, X, Y): Y.Fill(0); MltAdd(1, SeldonTrans, M, X, 0, Y)
+ * (Functions_MatVect.cxx:123-130). */
+int shk_mlt_trans(shk_ctx ctx, shk_csr A, const void *dx, void *dy);
 /* Fused: Y = A*X and returns (X, Y) -- the CG step q = A p, (p, q)
  * (computation/solver/iterative/Cg.cxx:97-98). */
 int shk_mlt_dot(shk_ctx ctx, shk_csr A, const void *dx, void *dy, double *result);

@@ -65,6 +65,8 @@ def lib():
         for sfx, T in (("d", ctypes.c_double), ("s", ctypes.c_float)):
             getattr(L, f"orc_{sfx}_mltadd").argtypes = [T, ctypes.POINTER(OrcMat), vp, T, vp]
             getattr(L, f"orc_{sfx}_mlt").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
+            getattr(L, f"orc_{sfx}_mltadd_trans").argtypes = [T, ctypes.POINTER(OrcMat), vp, T, vp]
+            getattr(L, f"orc_{sfx}_mlt_trans").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
             getattr(L, f"orc_{sfx}_dot").argtypes = [ctypes.c_int, vp, vp]
             getattr(L, f"orc_{sfx}_dot").restype = T
             getattr(L, f"orc_{sfx}_norm2").argtypes = [ctypes.c_int, vp]
@@ -122,6 +124,27 @@ def mlt(A, x, storage=None, dtype=np.float64):
     x = np.ascontiguousarray(x, dtype)
     y = np.empty(A.m, dtype)
     getattr(lib(), f"orc_{_sfx(dtype)}_mlt")(ctypes.byref(M.s), _p(x), _p(y))
+    return y
+
+
+def mltadd_trans(alpha, A, x, beta, y, storage=None, dtype=np.float64):
+    """MltAdd(alpha, SeldonTrans, A, x, beta, y): x has A.m entries, y A.n
+    (A.m for RowSymSparse)."""
+    M = _Mat(A, storage_of(A, storage), dtype)
+    x = np.ascontiguousarray(x, dtype)
+    y = np.array(y, dtype=dtype, copy=True)
+    getattr(lib(), f"orc_{_sfx(dtype)}_mltadd_trans")(alpha, ctypes.byref(M.s), _p(x), beta, _p(y))
+    return y
+
+
+def mlt_trans(A, x, storage=None, dtype=np.float64):
+    """Mlt(SeldonTrans, A, x, y): Seldon's generic loop, which reads A(j, i)
+    for every (j, i): O(m n), for small and medium test sizes."""
+    st = storage_of(A, storage)
+    M = _Mat(A, st, dtype)
+    x = np.ascontiguousarray(x, dtype)
+    y = np.empty(A.m if st == ROWSYMSPARSE else A.n, dtype)
+    getattr(lib(), f"orc_{_sfx(dtype)}_mlt_trans")(ctypes.byref(M.s), _p(x), _p(y))
     return y
 
 

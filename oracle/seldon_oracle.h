@@ -65,6 +65,9 @@ void orc_iter_init(orc_iter *it, int max_iter, double tol);
     void orc_##SFX##_mltadd(T alpha, const orc_mat *A, const T *x, T beta,     \
                             T *y);                                             \
     void orc_##SFX##_mlt(const orc_mat *A, const T *x, T *y);                  \
+    void orc_##SFX##_mltadd_trans(T alpha, const orc_mat *A, const T *x,       \
+                                  T beta, T *y);                               \
+    void orc_##SFX##_mlt_trans(const orc_mat *A, const T *x, T *y);            \
     T orc_##SFX##_dot(int n, const T *x, const T *y);                          \
     T orc_##SFX##_norm2(int n, const T *x);                                    \
     void orc_##SFX##_add(int n, T alpha, const T *x, T *y);                    \

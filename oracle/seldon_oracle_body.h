@@ -117,6 +117,88 @@ void FN(mlt)(const orc_mat *A, const T *x, T *y)
     FN(mltadd)((T)1, A, x, (T)0, y);
 }
 
+/* MltAdd(alpha, SeldonTrans, M, X, beta, Y): Y <- beta Y + alpha M^T X.
+ * RowSparse: computation/basic_functions/Functions_MatVect.cxx:729-758
+ * (Mlt(beta, Y), then the scatter over the rows in order).  ArrayRowSparse:
+ * matrix_sparse/Functions_MatrixArray.cxx:591-632.  RowSymSparse forwards
+ * to the plain product (Functions_MatVect.cxx:918-927).  x has m entries,
+ * y has n. */
+void FN(mltadd_trans)(T alpha, const orc_mat *A, const T *x, T beta, T *y)
+{
+    const int m = A->m, n = A->n;
+    const int *ptr = A->ptr, *ind = A->ind;
+    const T *data = (const T *)A->val;
+    if (A->storage == ORC_ROWSYMSPARSE) {
+        FN(mltadd)(alpha, A, x, beta, y);
+    } else if (A->storage == ORC_ROWSPARSE) {
+        FN(scal)(n, beta, y);
+        for (int i = 0; i < m; i++)
+            for (int j = ptr[i]; j < ptr[i + 1]; j++) y[ind[j]] += alpha * data[j] * x[i];
+    } else {
+        if (beta == (T)0) FN(fill)(n, (T)0, y);
+        else FN(scal)(n, beta, y);
+        if (alpha == (T)1) {
+            for (int i = 0; i < m; i++)
+                for (int j = ptr[i]; j < ptr[i + 1]; j++) y[ind[j]] += data[j] * x[i];
+        } else {
+            for (int i = 0; i < m; i++)
+                for (int j = ptr[i]; j < ptr[i + 1]; j++) y[ind[j]] += alpha * data[j] * x[i];
+        }
+    }
+}
+
+/* Mlt(SeldonTrans, M, X, Y): Functions_MatVect.cxx:123-130 fills Y with 0
+ * and calls MltAdd(1, Trans, M, X, 0, Y) with Trans a SeldonTranspose
+ * reference, which selects the generic MltAdd of :1279-1316 (the
+ * storage-specific transposed products take the class_SeldonTrans tag):
+ * Y.Fill(0), then for every column i, temp = sum over j = 0 .. m-1 of
+ * M(j, i) * X(j) -- absent entries read as 0 (Matrix_Sparse / SymSparse
+ * operator(), RowSymSparse swapping to the stored upper entry) -- and
+ * Y(i) += 1 * temp. */
+void FN(mlt_trans)(const orc_mat *A, const T *x, T *y)
+{
+    const int ma = A->m, na = A->storage == ORC_ROWSYMSPARSE ? A->m : A->n;
+    const int *ptr = A->ptr, *ind = A->ind;
+    const T *data = (const T *)A->val;
+    const int sym = A->storage == ORC_ROWSYMSPARSE;
+    /* the matrix's entries M(j, i) as (j, i, v), the symmetric ones mirrored */
+    long nz = 0;
+    for (int r = 0; r < ma; r++)
+        for (int k = ptr[r]; k < ptr[r + 1]; k++) nz += (sym && ind[k] != r) ? 2 : 1;
+    int *er = (int *)malloc(sizeof(int) * (size_t)(nz + 1)), *ec = (int *)malloc(sizeof(int) * (size_t)(nz + 1));
+    T *ev = (T *)malloc(sizeof(T) * (size_t)(nz + 1));
+    long q = 0;
+    for (int r = 0; r < ma; r++)
+        for (int k = ptr[r]; k < ptr[r + 1]; k++) {
+            er[q] = r; ec[q] = ind[k]; ev[q] = data[k]; q++;
+            if (sym && ind[k] != r) { er[q] = ind[k]; ec[q] = r; ev[q] = data[k]; q++; }
+        }
+    /* column lists in increasing j: stable counting sorts by j, then by i */
+    long *cnt = (long *)calloc((size_t)(ma > na ? ma : na) + 2, sizeof(long));
+    long *t1 = (long *)malloc(sizeof(long) * (size_t)(nz + 1)), *t2 = (long *)malloc(sizeof(long) * (size_t)(nz + 1));
+    for (long k = 0; k < nz; k++) cnt[er[k] + 1]++;
+    for (int r = 0; r < ma; r++) cnt[r + 1] += cnt[r];
+    for (long k = 0; k < nz; k++) t1[cnt[er[k]]++] = k;
+    memset(cnt, 0, sizeof(long) * ((size_t)(ma > na ? ma : na) + 2));
+    for (long k = 0; k < nz; k++) cnt[ec[k] + 1]++;
+    for (int c = 0; c < na; c++) cnt[c + 1] += cnt[c];
+    long *cptr = (long *)malloc(sizeof(long) * ((size_t)na + 1));
+    for (int c = 0; c <= na; c++) cptr[c] = cnt[c];
+    for (long k = 0; k < nz; k++) { long e = t1[k]; t2[cnt[ec[e]]++] = e; }
+    FN(fill)(na, (T)0, y);
+    for (int i = 0; i < na; i++) {
+        T temp = 0;
+        long k = cptr[i];
+        for (int j = 0; j < ma; j++) {
+            T mji = 0;
+            if (k < cptr[i + 1] && er[t2[k]] == j) mji = ev[t2[k++]];
+            temp += mji * x[j];
+        }
+        y[i] += (T)1 * temp;
+    }
+    free(er); free(ec); free(ev); free(cnt); free(t1); free(t2); free(cptr);
+}
+
 /* Diagonal for the Jacobi preconditioner (an entry with ind == row; 0 when
  * the row stores none). */
 void FN(jacobi_diag)(const orc_mat *A, T *diag)

@@ -14,6 +14,8 @@
 // Usage:
 //   seldon_ref spmv  <row|sym|array> <d|f> A x y alpha beta out
 //   seldon_ref mlt   <row|sym|array> <d|f> A x out
+//   seldon_ref spmvt <row|sym|array> <d|f> A x y alpha beta out   (SeldonTrans)
+//   seldon_ref mltt  <row|sym|array> <d|f> A x out                 (SeldonTrans)
 //   seldon_ref solve <cg|bicgstab|gmres> <row|sym|array> <id|jacobi>
 //                    A b x0|- tol maxit restart out
 // The Jacobi preconditioner does not exist in the reference; it is defined
@@ -85,8 +87,28 @@ static void to_array(const Matrix<T, General, RowSparse>& A,
     }
 }
 
+// Y <- the product selected by (add, trans) on A: Mlt / MltAdd, plain or
+// with SeldonTrans (Functions_MatVect.cxx:123-130, :729-758, :918-927;
+// Functions_MatrixArray.cxx:591-632).
+template <class Mat, class T>
+static void product(bool add, bool trans, const Mat& A, const Vector<T>& x, T alpha, T beta, Vector<T>& y,
+                    int out_len)
+{
+  if (!add)
+    {
+      y.Reallocate(out_len);
+      y.Fill(T(0));
+      if (trans) Mlt(SeldonTrans, A, x, y);
+      else Mlt(A, x, y);
+    }
+  else if (trans)
+    MltAdd(alpha, SeldonTrans, A, x, beta, y);
+  else
+    MltAdd(alpha, A, x, beta, y);
+}
+
 template <class T>
-static int run_spmv(bool add, const string& st, char** a)
+static int run_spmv(bool add, const string& st, char** a, bool trans = false)
 {
   Vector<T> x, y;
   x.Read(a[1]);
@@ -108,25 +130,21 @@ static int run_spmv(bool add, const string& st, char** a)
     {
       Matrix<T, Symmetric, RowSymSparse> A;
       A.Read(a[0]);
-      if (!add) { y.Reallocate(A.GetM()); y.Fill(T(0)); Mlt(A, x, y); }
-      else MltAdd(alpha, A, x, beta, y);
+      product(add, trans, A, x, alpha, beta, y, A.GetM());
     }
   else
     {
       Matrix<T, General, RowSparse> A;
       A.Read(a[0]);
+      const int len = trans ? A.GetN() : A.GetM();
       if (st == "array")
         {
           Matrix<T, General, ArrayRowSparse> B;
           to_array(A, B);
-          if (!add) { y.Reallocate(B.GetM()); y.Fill(T(0)); Mlt(B, x, y); }
-          else MltAdd(alpha, B, x, beta, y);
+          product(add, trans, B, x, alpha, beta, y, len);
         }
       else
-        {
-          if (!add) { y.Reallocate(A.GetM()); y.Fill(T(0)); Mlt(A, x, y); }
-          else MltAdd(alpha, A, x, beta, y);
-        }
+        product(add, trans, A, x, alpha, beta, y, len);
     }
   y.Write(out);
   return 0;
@@ -219,6 +237,12 @@ int main(int argc, char** argv)
       if (argc >= 7 && string(argv[1]) == "mlt")
         return string(argv[3]) == "f" ? run_spmv<float>(false, argv[2], argv + 4)
                                       : run_spmv<double>(false, argv[2], argv + 4);
+      if (argc >= 10 && string(argv[1]) == "spmvt")
+        return string(argv[3]) == "f" ? run_spmv<float>(true, argv[2], argv + 4, true)
+                                      : run_spmv<double>(true, argv[2], argv + 4, true);
+      if (argc >= 7 && string(argv[1]) == "mltt")
+        return string(argv[3]) == "f" ? run_spmv<float>(false, argv[2], argv + 4, true)
+                                      : run_spmv<double>(false, argv[2], argv + 4, true);
       if (argc >= 12 && string(argv[1]) == "solve")
         return run_solve(argv + 2);
     }

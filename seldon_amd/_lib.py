@@ -65,6 +65,8 @@ SIGNATURES = [
     ("shk_spmv", _I, [_P, _P, _D, _P, _D, _P]),
     ("shk_spmv_out", _I, [_P, _P, _D, _P, _D, _P, _P]),
     ("shk_mlt", _I, [_P, _P, _P, _P]),
+    ("shk_spmv_trans", _I, [_P, _P, _D, _P, _D, _P]),
+    ("shk_mlt_trans", _I, [_P, _P, _P, _P]),
     ("shk_mlt_dot", _I, [_P, _P, _P, _P, _PD]),
     ("shk_dot", _I, [_P, _I, _I64, _P, _P, _PD]),
     ("shk_dot_device", _I, [_P, _I, _I64, _P, _P, _P]),

@@ -112,7 +112,11 @@ struct shk_csr_s {
     int npanel = 0, nchunk = 0;
     int *pchunk = nullptr;    // [npanel+1] first chunk of each panel
     int *chunk = nullptr;     // [4*nchunk] {c0, c1, e0, e1} per chunk
-    unsigned short *slot_row = nullptr;  // [npanel*1024] panel-local row of each row slot
+    unsigned short *slot_row = nullptr;  // [npanel*kPanelRows] panel-local row of each row slot
+    // A^T for the transposed products (RowSparse, ArrayRowSparse), built on
+    // first use as an ArrayRowSparse matrix whose rows list, for each column
+    // c of A, the entries (i, c) in increasing i
+    shk_csr_s *trans = nullptr;
 };
 
 namespace shk {

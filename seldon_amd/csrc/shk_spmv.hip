@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <vector>
 
 #include "shk_kernels.hpp"
 
@@ -448,6 +449,161 @@ static int spmv_typed(shk_csr A, int mlt, T alpha, const void *dx, T beta, const
     return launch_spmv<T>(A, a, FinNone{});
 }
 
+// The device rows as coordinates (row, col, val): RowSymSparse rows are the
+// expanded ones, so this is the full symmetric matrix.  Row layout: the
+// device ptr walks ind/val; panel layout: every chunk entry knows its row
+// slot and window-local column.
+template <typename T>
+static __global__ void k_panel_coo(const int *pchunk, const int4 *chunk, const uint32_t *word,
+                                   const unsigned short *slot_row, const T *val, int *row_out, int *col_out,
+                                   T *val_out)
+{
+    const int p = blockIdx.x, r0 = p * kPanelRows;
+    for (int cc = pchunk[p]; cc < pchunk[p + 1]; cc++) {
+        const int4 ci = chunk[cc];
+        for (int e = ci.z + (int)threadIdx.x; e < ci.w; e += kBlock) {
+            const uint32_t w = word[e];
+            row_out[e] = r0 + slot_row[(size_t)p * kPanelRows + ((w >> kPanRowShift) & kPanRowMask)];
+            col_out[e] = chunk_c0(ci) + (int)(w & kPanColMask);
+            val_out[e] = val[e];
+        }
+    }
+}
+
+template <typename T>
+static int coo_download(shk_csr A, std::vector<int> &row, std::vector<int> &col, std::vector<T> &val)
+{
+    shk_ctx ctx = A->ctx;
+    if (A->layout == SHK_LAYOUT_PANELS) {
+        const int64_t ns = A->nslots;
+        std::vector<int> r((size_t)ns), c((size_t)ns);
+        std::vector<T> v((size_t)ns);
+        int *drow = nullptr, *dcol = nullptr;
+        T *dval = nullptr;
+        hipError_t e0 = hipMalloc(&drow, sizeof(int) * (size_t)std::max<int64_t>(ns, 1));
+        hipError_t e1 = hipMalloc(&dcol, sizeof(int) * (size_t)std::max<int64_t>(ns, 1));
+        hipError_t e2 = hipMalloc(&dval, sizeof(T) * (size_t)std::max<int64_t>(ns, 1));
+        hipError_t e3 = hipSuccess, e4 = hipSuccess, e5 = hipSuccess, e6 = hipSuccess, e7 = hipSuccess;
+        if (e0 == hipSuccess && e1 == hipSuccess && e2 == hipSuccess && ns > 0) {
+            // positions outside every chunk (alignment padding) keep row -1
+            e3 = hipMemsetAsync(drow, 0xff, sizeof(int) * (size_t)ns, ctx->stream);
+            if (A->npanel)
+                hipLaunchKernelGGL(k_panel_coo<T>, dim3(A->npanel), dim3(kBlock), 0, ctx->stream,
+                                   (const int *)A->pchunk, (const int4 *)A->chunk, (const uint32_t *)A->ind,
+                                   (const unsigned short *)A->slot_row, (const T *)A->val, drow, dcol, dval);
+            e4 = hipMemcpyAsync(r.data(), drow, sizeof(int) * (size_t)ns, hipMemcpyDeviceToHost, ctx->stream);
+            e5 = hipMemcpyAsync(c.data(), dcol, sizeof(int) * (size_t)ns, hipMemcpyDeviceToHost, ctx->stream);
+            e6 = hipMemcpyAsync(v.data(), dval, sizeof(T) * (size_t)ns, hipMemcpyDeviceToHost, ctx->stream);
+            e7 = hipStreamSynchronize(ctx->stream);
+        }
+        hipFree(drow);
+        hipFree(dcol);
+        hipFree(dval);
+        SHK_CHECK(e0 == hipSuccess && e1 == hipSuccess && e2 == hipSuccess, SHK_ERR_NOMEM,
+                  "transpose: device memory");
+        SHK_HIP(e3);
+        SHK_HIP(e4);
+        SHK_HIP(e5);
+        SHK_HIP(e6);
+        SHK_HIP(e7);
+        for (int64_t e = 0; e < ns; e++)
+            if (r[e] >= 0) {
+                row.push_back(r[e]);
+                col.push_back(c[e]);
+                val.push_back(v[e]);
+            }
+        return SHK_OK;
+    }
+    const int64_t nd = A->nnz_dev;
+    std::vector<int> ptr((size_t)A->m + 1);
+    std::vector<uint32_t> ind((size_t)nd);
+    val.resize((size_t)nd);
+    SHK_HIP(hipMemcpyAsync(ptr.data(), A->ptr, sizeof(int) * ((size_t)A->m + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (nd > 0) {
+        SHK_HIP(hipMemcpyAsync(ind.data(), A->ind, sizeof(uint32_t) * (size_t)nd, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        SHK_HIP(hipMemcpyAsync(val.data(), A->val, sizeof(T) * (size_t)nd, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    SHK_HIP(hipStreamSynchronize(ctx->stream));
+    row.resize((size_t)nd);
+    col.resize((size_t)nd);
+    for (int i = 0; i < A->m; i++)
+        for (int j = ptr[i]; j < ptr[(size_t)i + 1]; j++) {
+            row[j] = i;
+            col[j] = (int)(ind[j] & ~kLowerFlag);
+        }
+    return SHK_OK;
+}
+
+// CSR with `nmajor` rows from coordinates keyed (major, minor): two stable
+// counting sorts, by minor then by major, so each row lists its entries in
+// increasing minor index.
+template <typename T>
+static void coo_to_csr(int nmajor, int nminor, const std::vector<int> &major, const std::vector<int> &minor,
+                       const std::vector<T> &val, std::vector<int> &ptr, std::vector<int> &ind,
+                       std::vector<T> &out)
+{
+    const size_t nz = major.size();
+    std::vector<int64_t> order(nz), tmp(nz);
+    std::vector<int64_t> cnt((size_t)std::max(nmajor, nminor) + 1);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (size_t k = 0; k < nz; k++) cnt[(size_t)minor[k] + 1]++;
+    for (int i = 0; i < nminor; i++) cnt[(size_t)i + 1] += cnt[i];
+    for (size_t k = 0; k < nz; k++) tmp[cnt[minor[k]]++] = (int64_t)k;
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (size_t k = 0; k < nz; k++) cnt[(size_t)major[k] + 1]++;
+    for (int i = 0; i < nmajor; i++) cnt[(size_t)i + 1] += cnt[i];
+    ptr.assign((size_t)nmajor + 1, 0);
+    for (int i = 0; i <= nmajor; i++) ptr[i] = (int)cnt[i];
+    for (size_t q = 0; q < nz; q++) {
+        const int64_t k = tmp[q];
+        order[cnt[major[k]]++] = k;
+    }
+    ind.resize(nz);
+    out.resize(nz);
+    for (size_t q = 0; q < nz; q++) {
+        ind[q] = minor[order[q]];
+        out[q] = val[order[q]];
+    }
+}
+
+// The matrix behind the transposed products, built on first use:
+// RowSparse / ArrayRowSparse: A^T as an ArrayRowSparse matrix (every term
+// "direct": alpha * a * x added onto beta * y) whose row c lists the entries
+// (i, c) of A in increasing i, the order of Seldon's transposed loops
+// (Functions_MatVect.cxx:753-755, Functions_MatrixArray.cxx:607-613).
+// RowSymSparse: the full symmetric matrix as a RowSparse one with sorted
+// columns, the order of the generic loop behind Mlt(SeldonTrans, ...)
+// (Functions_MatVect.cxx:1307-1313: temp = sum over j of M(j, i) X(j)).
+template <typename T>
+static int build_trans(shk_csr A)
+{
+    std::vector<int> row, col;
+    std::vector<T> val;
+    SHK_TRY(coo_download<T>(A, row, col, val));
+    for (size_t k = 0; k < row.size(); k++)
+        SHK_CHECK(row[k] >= 0 && row[k] < A->m && col[k] >= 0 && col[k] < A->n, SHK_ERR_STATE,
+                  "transpose: entry out of range");
+    std::vector<int> ptr, ind;
+    std::vector<T> out;
+    const bool sym = A->storage == SHK_ROWSYMSPARSE;
+    if (sym) coo_to_csr<T>(A->m, A->n, row, col, val, ptr, ind, out);
+    else coo_to_csr<T>(A->n, A->m, col, row, val, ptr, ind, out);
+    const int64_t nz = (int64_t)ind.size();
+    shk_csr T_ = nullptr;
+    SHK_TRY(shk_csr_create_layout(A->ctx, sym ? SHK_ROWSPARSE : SHK_ARRAYROWSPARSE, (shk_dtype)A->dtype,
+                                  sym ? A->m : A->n, sym ? A->n : A->m, nz, ptr.data(), nz ? ind.data() : nullptr,
+                                  nz ? (const void *)out.data() : nullptr, 0, SHK_LAYOUT_AUTO, &T_));
+    A->trans = T_;
+    return SHK_OK;
+}
+
+static int ensure_trans(shk_csr A)
+{
+    if (A->trans) return SHK_OK;
+    return A->dtype == SHK_F64 ? build_trans<double>(A) : build_trans<float>(A);
+}
+
 }  // namespace shk
 
 using namespace shk;
@@ -548,6 +704,7 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
 int shk_csr_destroy(shk_csr A)
 {
     if (!A) return SHK_OK;
+    if (A->trans) shk_csr_destroy(A->trans);
     if (A->ctx) hipStreamSynchronize(A->ctx->stream);
     hipFree(A->ptr);
     hipFree(A->mid);
@@ -570,6 +727,10 @@ int shk_csr_set_values(shk_ctx ctx, shk_csr A, const void *val, int src_on_devic
 {
     SHK_TRY(check_csr(ctx, A));
     const size_t vs = dsize(A->dtype);
+    if (A->trans) {  // rebuilt from the new values when next needed
+        shk_csr_destroy(A->trans);
+        A->trans = nullptr;
+    }
     if (A->nnz_stored == 0) return SHK_OK;
     const void *dsrc = val;
     void *tmp = nullptr;
@@ -630,6 +791,33 @@ int shk_spmv_out(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double be
     SHK_TRY(check_device_ptr(dy_out, "y_out"));
     if (A->dtype == SHK_F64) return spmv_typed<double>(A, 0, alpha, dx, beta, dy_in, dy_out);
     return spmv_typed<float>(A, 0, (float)alpha, dx, (float)beta, dy_in, dy_out);
+}
+
+int shk_spmv_trans(shk_ctx ctx, shk_csr A, double alpha, const void *dx, double beta, void *dy)
+{
+    SHK_TRY(check_csr(ctx, A));
+    if (A->storage == SHK_ROWSYMSPARSE) return shk_spmv(ctx, A, alpha, dx, beta, dy);
+    if (A->n == 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(dy, "y"));
+    SHK_TRY(ensure_trans(A));
+    if (A->storage == SHK_ROWSPARSE && beta == 0.0) {
+        // RowSparse starts from Mlt(beta, Y), a product even for beta = 0
+        // (Inf and NaN stay NaN); A^T's ArrayRowSparse form would zero-fill
+        SHK_TRY(shk_scal(ctx, (shk_dtype)A->dtype, A->n, 0.0, dy));
+        beta = 1.0;
+    }
+    return shk_spmv(ctx, A->trans, alpha, dx, beta, dy);
+}
+
+int shk_mlt_trans(shk_ctx ctx, shk_csr A, const void *dx, void *dy)
+{
+    SHK_TRY(check_csr(ctx, A));
+    if (A->n == 0) return SHK_OK;
+    SHK_TRY(check_device_ptr(dx, "x"));
+    SHK_TRY(check_device_ptr(dy, "y"));
+    SHK_TRY(ensure_trans(A));
+    return shk_mlt(ctx, A->trans, dx, dy);
 }
 
 int shk_mlt(shk_ctx ctx, shk_csr A, const void *dx, void *dy)

@@ -157,16 +157,56 @@ def _ctx_of(*objs):
 
 # ----------------------------------------------------------- MltAdd / Mlt --
 
-def MltAdd(alpha, A: Matrix, X: torch.Tensor, beta, Y: torch.Tensor):
-    """Y <- beta Y + alpha A X (Functions_MatVect.cxx:398-431, :526-564;
-    Functions_MatrixArray.cxx:534-575)."""
-    L.check(L.load().shk_spmv(A.ctx.h, A.h, float(alpha), _ptr(X), float(beta), _ptr(Y)))
+class SeldonTranspose:
+    """Seldon's transpose tags (share/MatrixFlag.hxx:32-80): pass
+    SeldonTrans / SeldonNoTrans / SeldonConjTrans as the second argument of
+    MltAdd or the first of Mlt.  ConjTrans is Trans for real values."""
+
+    def __init__(self, status: int, name: str):
+        self.status = status  # 0: Trans, 1: NoTrans, 2: ConjTrans (MatrixFlag.hxx:38)
+        self.name = name
+
+    def Trans(self) -> bool:
+        return self.status == 0
+
+    def NoTrans(self) -> bool:
+        return self.status == 1
+
+    def ConjTrans(self) -> bool:
+        return self.status == 2
+
+    def __repr__(self):
+        return self.name
+
+
+SeldonTrans = SeldonTranspose(0, "SeldonTrans")
+SeldonNoTrans = SeldonTranspose(1, "SeldonNoTrans")
+SeldonConjTrans = SeldonTranspose(2, "SeldonConjTrans")
+
+
+def MltAdd(alpha, *args):
+    """MltAdd(alpha, A, X, beta, Y): Y <- beta Y + alpha A X
+    (Functions_MatVect.cxx:398-431, :526-564; Functions_MatrixArray.cxx:534-575).
+    MltAdd(alpha, SeldonTrans, A, X, beta, Y): Y <- beta Y + alpha A^T X
+    (Functions_MatVect.cxx:729-758; Functions_MatrixArray.cxx:591-632;
+    RowSymSparse: the plain product, Functions_MatVect.cxx:918-927)."""
+    if isinstance(args[0], SeldonTranspose):
+        trans, A, X, beta, Y = args
+    else:
+        trans, (A, X, beta, Y) = SeldonNoTrans, args
+    fn = L.load().shk_spmv if trans.NoTrans() else L.load().shk_spmv_trans
+    L.check(fn(A.ctx.h, A.h, float(alpha), _ptr(X), float(beta), _ptr(Y)))
 
 
 def Mlt(*args):
     """Mlt(A, X, Y): Y = A X (Functions_MatVect.cxx:72-81);
+    Mlt(SeldonTrans, A, X, Y): Y = A^T X (Functions_MatVect.cxx:123-130);
     Mlt(alpha, X): X *= alpha (Functions_Vector.cxx:69-74)."""
-    if len(args) == 3:
+    if len(args) == 4:
+        trans, A, X, Y = args
+        fn = L.load().shk_mlt if trans.NoTrans() else L.load().shk_mlt_trans
+        L.check(fn(A.ctx.h, A.h, _ptr(X), _ptr(Y)))
+    elif len(args) == 3:
         A, X, Y = args
         L.check(L.load().shk_mlt(A.ctx.h, A.h, _ptr(X), _ptr(Y)))
     else:

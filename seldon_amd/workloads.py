@@ -181,6 +181,17 @@ def upper(A: CSR) -> CSR:
     return CSR(A.m, A.n, ptr_u, ind_u, val_u, sym=True)
 
 
+def first_columns(A: CSR, ncols: int) -> CSR:
+    """The m x ncols matrix of A's entries in columns [0, ncols), stored order
+    kept (a rectangular case for the transposed products)."""
+    keep = A.ind < ncols
+    rows = np.repeat(np.arange(A.m), np.diff(A.ptr))[keep]
+    ptr = np.zeros(A.m + 1, np.int32)
+    np.add.at(ptr, rows + 1, 1)
+    return CSR(A.m, ncols, np.cumsum(ptr).astype(np.int32), A.ind[keep].astype(np.int32),
+               A.val[keep].astype(np.float64))
+
+
 def apply(A: CSR, x: np.ndarray) -> np.ndarray:
     """y = A x (full CSR only), sequential per-row sums in double."""
     assert not A.sym

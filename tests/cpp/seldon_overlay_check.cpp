@@ -8,6 +8,10 @@
 //   3. Cg on the same GPU storages with a user preconditioner type: Seldon's
 //      generic Cg template, each primitive (Mlt, DotProdConj, Add, Norm2,
 //      Copy, ...) reaching the GPU through the overload set,
+// then, on a 2-D convection-diffusion system (nonsymmetric), the solvers that
+// use the transposed product -- BiCg, Qmr, Cgne, Lsqr -- with Seldon's own
+// templates, once on the CPU storages and once on the GPU storages
+// (Mlt(SeldonTrans, ...), Swap and the rest through the overload set),
 // and prints each residual history (Iteration::Finished lines) after a
 // "== <label>" marker, then "ERRCODE e NBITER n".
 #include "Seldon.hxx"
@@ -27,6 +31,51 @@ public:
   template <class Matrix1, class Vector1>
   void TransSolve(const Matrix1& A, const Vector1& r, Vector1& z) { Solve(A, r, z); }
 };
+
+// upwind convection-diffusion on a g x g grid, natural order, sorted columns:
+// -lap u + h (vx d/dx + vy d/dy) u, rows scaled by h^2
+static void convdiff2d(int g, double vx, double vy, Matrix<double, General, RowSparse>& A)
+{
+  const int n = g * g;
+  const double h = 1.0 / (g + 1);
+  Vector<double> values;
+  Vector<int> ptr(n + 1), ind;
+  std::vector<double> vv;
+  std::vector<int> vi;
+  ptr(0) = 0;
+  for (int iy = 0; iy < g; iy++)
+    for (int ix = 0; ix < g; ix++)
+      {
+        int r = iy * g + ix;
+        if (iy > 0) { vi.push_back(r - g); vv.push_back(-1 - h * vy); }
+        if (ix > 0) { vi.push_back(r - 1); vv.push_back(-1 - h * vx); }
+        vi.push_back(r); vv.push_back(4 + h * (vx + vy));
+        if (ix < g - 1) { vi.push_back(r + 1); vv.push_back(-1); }
+        if (iy < g - 1) { vi.push_back(r + g); vv.push_back(-1); }
+        ptr(r + 1) = (int)vi.size();
+      }
+  values.Reallocate(vv.size());
+  ind.Reallocate(vi.size());
+  for (size_t k = 0; k < vv.size(); k++) { values(k) = vv[k]; ind(k) = vi[k]; }
+  A.SetData(n, n, values, ptr, ind);
+}
+
+template <class Mat, class Vec>
+static void run_trans_solver(const string& which, const string& label, Mat& A, const Vec& b, int maxit, double tol)
+{
+  cout << "== " << which << "_" << label << endl;
+  Vec x(b);
+  x.Fill(0.0);
+  Iteration<double> it(maxit, tol);
+  it.ShowFullHistory();
+  Preconditioner_Base M;
+  int code = 0;
+  if (which == "bicg") code = BiCg(A, x, b, M, it);
+  else if (which == "qmr") code = Qmr(A, x, b, M, it);
+  else if (which == "cgne") code = Cgne(A, x, b, M, it);
+  else code = Lsqr(A, x, b, M, it);
+  cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+}
 
 int main(int argc, char** argv)
 {
@@ -88,6 +137,23 @@ int main(int argc, char** argv)
     UserIdentity M;
     int code = Cg(Ag, x, bg, M, it);
     cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+  }
+  {
+    const int gc = argc > 2 ? atoi(argv[2]) : 24, nc = gc * gc;
+    Matrix<double, General, RowSparse> C;
+    convdiff2d(gc, 20.0, 10.0, C);
+    Vector<double> xc(nc), bc(nc);
+    for (int i = 0; i < nc; i++) xc(i) = std::cos(0.21 * i) - 0.3 * std::sin(0.7 * i);
+    Mlt(C, xc, bc);
+    Matrix<double, General, RowSparseHip, HipAlloc<double> > Cg_(C);
+    Vector<double, VectHip, HipAlloc<double> > bcg(bc);
+    const char* solvers[4] = {"bicg", "qmr", "cgne", "lsqr"};
+    for (int k = 0; k < 4; k++)
+      {
+        const int maxit = k < 2 ? 2000 : 4000;
+        run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
+        run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
+      }
   }
   return 0;
 }

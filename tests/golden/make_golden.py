@@ -185,10 +185,48 @@ def make_spmv():
     print("spmv fixtures:", len(out))
 
 
+def make_spmvt():
+    """MltAdd(alpha, SeldonTrans, M, X, beta, Y) and Mlt(SeldonTrans, M, X, Y)
+    from the reference build, on a 300 x 220 matrix (RowSparse and
+    ArrayRowSparse) and on its square parent's upper triangle
+    (RowSymSparse)."""
+    A = wl.powerlaw(300, 7, kmin=2, kmax=40, half_width=60)
+    R = wl.first_columns(A, 220)
+    U = wl.upper(A)
+    x = wl.uniform(300, 21)
+    y_rect = wl.uniform(220, 22)
+    y_rect[5] = np.nan
+    y_sym = wl.uniform(300, 23)
+    y_sym[5] = np.nan
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for st, M, y in (("row", R, y_rect), ("array", R, y_rect), ("sym", U, y_sym)):
+            for dt, c in ((np.float64, "d"), (np.float32, "f")):
+                orc.write_seldon_csr(f"{tmp}/M.bin", M, dt)
+                orc.write_seldon_vec(f"{tmp}/x.bin", x.astype(dt))
+                orc.write_seldon_vec(f"{tmp}/y.bin", y.astype(dt))
+                for k, (al, be) in enumerate(SPMV_ALPHA_BETA):
+                    orc.run_ref(["spmvt", st, c, f"{tmp}/M.bin", f"{tmp}/x.bin", f"{tmp}/y.bin",
+                                 repr(al), repr(be), f"{tmp}/o.bin"])
+                    out[f"{st}_{c}_{k}"] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
+                orc.run_ref(["mltt", st, c, f"{tmp}/M.bin", f"{tmp}/x.bin", f"{tmp}/o.bin"])
+                out[f"{st}_{c}_mlt"] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
+    np.savez_compressed(os.path.join(OUT, "spmvt_rect300x220.npz"),
+                        spec=json.dumps(dict(n=300, seed=7, kmin=2, kmax=40, half_width=60, ncols=220,
+                                             xseed=21, yseed_rect=22, yseed_sym=23, nan_at=5)),
+                        alpha_beta=np.array(SPMV_ALPHA_BETA), digest=csr_digest(R), **out)
+    print("spmvt fixtures:", len(out))
+
+
 if __name__ == "__main__":
     if not orc.ref_available("generic") or not orc.ref_available("blas"):
         orc.build()
         import subprocess
         subprocess.run(["make", "-s", "-C", orc.HERE, "ref"], check=True)
-    make_spmv()
-    make_solves()
+    which = sys.argv[1:] or ["spmv", "spmvt", "solves"]
+    if "spmv" in which:
+        make_spmv()
+    if "spmvt" in which:
+        make_spmvt()
+    if "solves" in which:
+        make_solves()

@@ -186,3 +186,83 @@ def test_panel_rectangular_and_values_update(gpu, c):
     Rg.set_values(v2.astype(ndt))  # host source
     K.Mlt(Rg, xg, yg)
     assert _same(yg.cpu().numpy(), orc.mlt(R2, x.astype(ndt), dtype=ndt))
+
+
+# ------------------------------------------------ transposed products --
+
+@pytest.mark.parametrize("layout", ["rows", "auto"])
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_spmv_trans_golden_fixture(gpu, st, c, layout):
+    """MltAdd(alpha, SeldonTrans, ..) and Mlt(SeldonTrans, ..) bitwise against
+    the real Seldon build, on a 300 x 220 matrix (RowSymSparse: square)."""
+    from seldon_amd import krylov as K
+    fx = load_fixture("spmvt_rect300x220.npz")
+    sp = fx["spec"]
+    A = wl.powerlaw(sp["n"], sp["seed"], kmin=sp["kmin"], kmax=sp["kmax"], half_width=sp["half_width"])
+    M = wl.upper(A) if st == "sym" else wl.first_columns(A, sp["ncols"])
+    tdt, ndt = DT[c]
+    x = wl.uniform(sp["n"], sp["xseed"])
+    y = wl.uniform(A.m if st == "sym" else sp["ncols"], sp["yseed_sym"] if st == "sym" else sp["yseed_rect"])
+    y[sp["nan_at"]] = np.nan
+    Ag = K.Matrix(M, storage=st, dtype=tdt, layout=layout)
+    xg = _dev(x, ndt)
+    for k, (al, be) in enumerate(fx["alpha_beta"]):
+        yg = _dev(y, ndt)
+        K.MltAdd(al, K.SeldonTrans, Ag, xg, be, yg)
+        assert _same(yg.cpu().numpy(), fx[f"{st}_{c}_{k}"]), (st, c, al, be)
+    yg = torch.full((len(y),), 7.0, dtype=tdt, device="cuda")
+    K.Mlt(K.SeldonTrans, Ag, xg, yg)
+    assert _same(yg.cpu().numpy(), fx[f"{st}_{c}_mlt"])
+
+
+@pytest.mark.parametrize("layout", ["rows", "auto"])
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+def test_spmv_trans_bitwise_vs_oracle(gpu, st, layout):
+    """Transposed products against the oracle on matrices with long and empty
+    rows, a stencil, and a wide rectangular one (A^T is built on the first
+    call, in the layout AUTO picks for it)."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    cases = [(n, A) for n, A in _cases() if n in ("powerlaw20k", "convdiff24", "longrows", "emptyrows")]
+    P = wl.powerlaw(9000, 4, kmin=1, kmax=300, half_width=2500)
+    cases.append(("rect9000x6000", wl.first_columns(P, 6000)))
+    for name, A in cases:
+        if st == "sym" and A.m != A.n:
+            continue
+        M = wl.upper(A) if st == "sym" else A
+        n_out = A.m if st == "sym" else A.n
+        x = wl.uniform(A.m, 31)
+        y = wl.uniform(n_out, 32)
+        Ag = K.Matrix(M, storage=st, dtype=torch.float64, layout=layout)
+        print(f"[spmvt] {name} {st} {Ag.layout}", flush=True)
+        xg = _dev(x, np.float64)
+        for al, be in ((1.0, 0.0), (-1.0, 1.0), (0.7, -0.3)):
+            ref = orc.mltadd_trans(al, M, x, be, y, storage=st)
+            yg = _dev(y, np.float64)
+            K.MltAdd(al, K.SeldonTrans, Ag, xg, be, yg)
+            assert _same(yg.cpu().numpy(), ref), (name, st, al, be)
+        if A.m <= 12000:  # the oracle's Mlt(SeldonTrans) is Seldon's O(m n) loop
+            yg = torch.empty(n_out, dtype=torch.float64, device="cuda")
+            K.Mlt(K.SeldonTrans, Ag, xg, yg)
+            assert _same(yg.cpu().numpy(), orc.mlt_trans(M, x, storage=st)), (name, st, "mlt")
+
+
+def test_spmv_trans_follows_new_values(gpu):
+    """A^T is rebuilt after new values; SeldonNoTrans is the plain product."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    A = wl.first_columns(wl.powerlaw(5000, 6, kmin=2, kmax=50, half_width=800), 4000)
+    x = wl.uniform(A.m, 41)
+    Ag = K.Matrix(A)
+    xg = _dev(x, np.float64)
+    yg = torch.empty(A.n, dtype=torch.float64, device="cuda")
+    K.Mlt(K.SeldonTrans, Ag, xg, yg)
+    assert _same(yg.cpu().numpy(), orc.mltadd_trans(1.0, A, x, 0.0, np.zeros(A.n)))
+    B = wl.CSR(A.m, A.n, A.ptr, A.ind, A.val * -1.5 + 0.25)
+    Ag.set_values(B.val)
+    K.MltAdd(2.0, K.SeldonTrans, Ag, xg, 0.0, yg)
+    assert _same(yg.cpu().numpy(), orc.mltadd_trans(2.0, B, x, 0.0, np.zeros(A.n)))
+    y2 = torch.empty(A.m, dtype=torch.float64, device="cuda")
+    K.Mlt(K.SeldonNoTrans, Ag, _dev(wl.uniform(A.n, 42), np.float64), y2)
+    assert _same(y2.cpu().numpy(), orc.mlt(B, wl.uniform(A.n, 42)))

@@ -3,8 +3,8 @@
 Pins, each chosen so that a plausible slip (a dropped term, a wrong sign or
 index, a transposed operand, a wrong stopping count) fails one of them:
   * the golden fixtures written by the real Seldon build (tests/golden):
-    SpMV outputs on three storages x two precisions x six (alpha, beta),
-    bitwise; Cg / BiCgStab residual histories, iteration counts, error codes
+    SpMV outputs, plain and with SeldonTrans (on a rectangular matrix), on
+    three storages x two precisions x six (alpha, beta), bitwise; Cg / BiCgStab residual histories, iteration counts, error codes
     and solutions, bitwise against the generic-template build; Gmres against
     the MKL-linked build (the only one that compiles) within its recorded
     spread, and bitwise against its own recorded history;
@@ -130,6 +130,84 @@ def test_oracle_bitwise_vs_live_seldon():
             r = orc.solve(meth, A, b, precond=prec, tol=1e-9, max_iter=1000)
             assert np.array_equal(h, r.hist) and code == r.code and nb == r.nb_iter
             assert np.array_equal(orc.read_seldon_vec(f"{tmp}/x.bin"), r.x)
+
+
+def _spmvt_case(fx, dt):
+    sp = fx["spec"]
+    A = wl.powerlaw(sp["n"], sp["seed"], kmin=sp["kmin"], kmax=sp["kmax"], half_width=sp["half_width"])
+    R = wl.first_columns(A, sp["ncols"])
+    x = wl.uniform(sp["n"], sp["xseed"]).astype(dt)
+    yr = wl.uniform(sp["ncols"], sp["yseed_rect"])
+    ys = wl.uniform(sp["n"], sp["yseed_sym"])
+    yr[sp["nan_at"]] = np.nan
+    ys[sp["nan_at"]] = np.nan
+    return A, R, x, yr.astype(dt), ys.astype(dt)
+
+
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_oracle_spmv_trans_matches_seldon_fixture(st, c):
+    """MltAdd(alpha, SeldonTrans, ...) and Mlt(SeldonTrans, ...) of the real
+    Seldon build, on a 300 x 220 matrix (and a symmetric one), bitwise."""
+    fx = load_fixture("spmvt_rect300x220.npz")
+    dt = np.float64 if c == "d" else np.float32
+    A, R, x, yr, ys = _spmvt_case(fx, dt)
+    assert csr_digest(R) == str(fx["digest"])
+    M, y = (wl.upper(A), ys) if st == "sym" else (R, yr)
+    for k, (al, be) in enumerate(fx["alpha_beta"]):
+        got = orc.mltadd_trans(dt(al), M, x, dt(be), y, storage=st, dtype=dt)
+        assert np.array_equal(got, fx[f"{st}_{c}_{k}"], equal_nan=True), (st, c, al, be)
+    assert np.array_equal(orc.mlt_trans(M, x, storage=st, dtype=dt), fx[f"{st}_{c}_mlt"])
+
+
+def test_spmv_trans_fixture_has_teeth():
+    """The transposed product is A^T x (scipy agrees to rounding), it is not
+    the plain product of the transpose's storage order with other sums, and
+    RowSparse keeps beta*NaN where ArrayRowSparse zero-fills."""
+    fx = load_fixture("spmvt_rect300x220.npz")
+    A, R, x, yr, _ = _spmvt_case(fx, np.float64)
+    ref = R.to_scipy().T @ x
+    got = orc.mlt_trans(R, x, storage="row")
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the same sums in a different order (a column-sorted CSR of R^T summed
+    # per row with a temp, as the plain RowSparse product would) round
+    # differently somewhere
+    Rt = R.to_scipy().T.tocsr()
+    T = wl.CSR(R.n, R.m, Rt.indptr.astype(np.int32), Rt.indices.astype(np.int32), Rt.data.copy())
+    plain = orc.mltadd(0.3, T, x, -1.0, yr, storage="row")
+    assert not np.array_equal(plain, fx["row_d_1"], equal_nan=True)
+    row = orc.mltadd_trans(1.0, R, x, 0.0, yr, storage="row")
+    arr = orc.mltadd_trans(1.0, R, x, 0.0, yr, storage="array")
+    nan_at = fx["spec"]["nan_at"]
+    assert np.isnan(row[nan_at]) and not np.isnan(arr[nan_at])
+    # RowSymSparse: MltAdd(.., SeldonTrans, ..) is the plain product, while
+    # Mlt(SeldonTrans, ..) sums each full row in column order -- other sums
+    U = wl.upper(A)
+    assert np.array_equal(orc.mltadd_trans(1.0, U, x, 0.0, np.zeros(A.m), storage="sym"),
+                          orc.mltadd(1.0, U, x, 0.0, np.zeros(A.m), storage="sym"))
+    assert not np.array_equal(orc.mlt_trans(U, x, storage="sym"), orc.mlt(U, x, storage="sym"))
+
+
+@pytest.mark.skipif(not orc.ref_available("generic"), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_trans_bitwise_vs_live_seldon():
+    rng = np.random.default_rng(8)
+    with tempfile.TemporaryDirectory() as tmp:
+        for seed in range(2):
+            A = wl.powerlaw(1500 + 700 * seed, 50 + seed, kmin=1, kmax=80, half_width=400)
+            R = wl.first_columns(A, A.m - 300)
+            x = rng.standard_normal(A.m)
+            for st, M in (("row", R), ("array", R), ("sym", wl.upper(A))):
+                y = rng.standard_normal(A.m if st == "sym" else R.n)
+                orc.write_seldon_csr(f"{tmp}/A.bin", M)
+                orc.write_seldon_vec(f"{tmp}/x.bin", x)
+                orc.write_seldon_vec(f"{tmp}/y.bin", y)
+                orc.run_ref(["spmvt", st, "d", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/y.bin", "-1.3", "0.6",
+                             f"{tmp}/o.bin"])
+                ref = orc.read_seldon_vec(f"{tmp}/o.bin")
+                assert np.array_equal(ref, orc.mltadd_trans(-1.3, M, x, 0.6, y, storage=st))
+                orc.run_ref(["mltt", st, "d", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/o.bin"])
+                ref = orc.read_seldon_vec(f"{tmp}/o.bin")
+                assert np.array_equal(ref, orc.mlt_trans(M, x, storage=st))
 
 
 def test_laplacian_eigenpairs():
