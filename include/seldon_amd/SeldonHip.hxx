@@ -226,6 +226,9 @@ namespace Seldon
       std::swap(m_, Y.m_);
       std::swap(data_, Y.data_);
     }
+    // element read by value, one device-to-host transfer (CoCg.cxx:53 reads
+    // b(0) to form a zero of the value type)
+    T operator()(int i) const { return GetElement(i); }
     // element read for inspection (one transfer)
     T GetElement(int i) const
     {

@@ -9,9 +9,11 @@
 //      generic Cg template, each primitive (Mlt, DotProdConj, Add, Norm2,
 //      Copy, ...) reaching the GPU through the overload set,
 // then, on a 2-D convection-diffusion system (nonsymmetric), the solvers that
-// use the transposed product -- BiCg, Qmr, Cgne, Lsqr -- with Seldon's own
-// templates, once on the CPU storages and once on the GPU storages
-// (Mlt(SeldonTrans, ...), Swap and the rest through the overload set),
+// use the transposed product -- BiCg, Qmr, Cgne, Lsqr -- and Cgs, QCgs,
+// TfQmr, Gcr, BiCgStabl, and on the Laplacian the symmetric ones -- MinRes,
+// Symmlq, CoCg, BiCgcr, QmrSym -- with Seldon's own templates, once on the
+// CPU storages and once on the GPU storages (Mlt(SeldonTrans, ...), Swap and
+// the rest through the overload set),
 // and prints each residual history (Iteration::Finished lines) after a
 // "== <label>" marker, then "ERRCODE e NBITER n".
 #include "Seldon.hxx"
@@ -73,7 +75,17 @@ static void run_trans_solver(const string& which, const string& label, Mat& A, c
   if (which == "bicg") code = BiCg(A, x, b, M, it);
   else if (which == "qmr") code = Qmr(A, x, b, M, it);
   else if (which == "cgne") code = Cgne(A, x, b, M, it);
-  else code = Lsqr(A, x, b, M, it);
+  else if (which == "lsqr") code = Lsqr(A, x, b, M, it);
+  else if (which == "cgs") code = Cgs(A, x, b, M, it);
+  else if (which == "qcgs") code = QCgs(A, x, b, M, it);
+  else if (which == "tfqmr") code = TfQmr(A, x, b, M, it);
+  else if (which == "gcr") code = Gcr(A, x, b, M, it);
+  else if (which == "bicgstabl") code = BiCgStabl(A, x, b, M, it);
+  else if (which == "minres") code = MinRes(A, x, b, M, it);
+  else if (which == "symmlq") code = Symmlq(A, x, b, M, it);
+  else if (which == "cocg") code = CoCg(A, x, b, M, it);
+  else if (which == "bicgcr") code = BiCgcr(A, x, b, M, it);
+  else code = QmrSym(A, x, b, M, it);
   cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
 }
 
@@ -147,12 +159,21 @@ int main(int argc, char** argv)
     Mlt(C, xc, bc);
     Matrix<double, General, RowSparseHip, HipAlloc<double> > Cg_(C);
     Vector<double, VectHip, HipAlloc<double> > bcg(bc);
-    const char* solvers[4] = {"bicg", "qmr", "cgne", "lsqr"};
-    for (int k = 0; k < 4; k++)
+    const char* solvers[9] = {"bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr", "bicgstabl"};
+    for (int k = 0; k < 9; k++)
       {
-        const int maxit = k < 2 ? 2000 : 4000;
+        const int maxit = (k == 2 || k == 3) ? 4000 : 2000;
         run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
         run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
+      }
+  }
+  {
+    // the symmetric methods on the SPD Laplacian above
+    const char* solvers[5] = {"minres", "symmlq", "cocg", "bicgcr", "qmrsym"};
+    for (int k = 0; k < 5; k++)
+      {
+        run_trans_solver(solvers[k], "cpu", A, bh, 2000, 1e-8);
+        run_trans_solver(solvers[k], "gpu", Ag, bg, 2000, 1e-8);
       }
   }
   return 0;
