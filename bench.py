@@ -158,6 +158,8 @@ def main():
     if not args.no_extra:
         del A64, A32, x64, y64, x32, y32
         torch.cuda.empty_cache()
+        result["assembly"] = assembly(K, A, np, torch, cpu=not args.no_cpu)
+        torch.cuda.empty_cache()
         result["convdiff128"] = convdiff_solves(K, wl, np, torch)
         torch.cuda.empty_cache()
         result["batched"] = batched_cg(K, np, torch)
@@ -295,6 +297,43 @@ def batched_cg(K, np, torch):
         log(f"batched {variant}: {dt:.3f}s group {group}, {int(its.sum())} its, {gbs:.0f} GB/s, "
             f"err {err:.2e}, nonzero codes {int((codes != 0).sum())}")
     return out
+
+
+def assembly(K, A, np, torch, cpu=True, reps=5):
+    """ConvertMatrix_from_Coordinates on the device: the power-law matrix's
+    113.2M entries as shuffled FP64 coordinates, to a RowSparse CSR (stable
+    (row, column) order).  Bytes counted: read row, col, val (16 B per entry),
+    write ind and val (12 B) and ptr (4(m+1)); the sort's own passes are on
+    top.  The oracle (Seldon's MergeSort conversion restated in C, one core)
+    is timed on a 1M-entry slice of the same coordinates."""
+    rng = np.random.default_rng(77)
+    row = np.repeat(np.arange(A.m, dtype=np.int32), np.diff(A.ptr))
+    perm = rng.permutation(row.size)
+    rg = torch.from_numpy(row[perm]).cuda()
+    cg = torch.from_numpy(A.ind[perm].astype(np.int32)).cuda()
+    vg = torch.from_numpy(A.val[perm]).cuda()
+    K.ConvertMatrix_from_Coordinates(rg, cg, vg)  # warm-up (workspace, rocPRIM setup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m, n, ptr, ind, out = K.ConvertMatrix_from_Coordinates(rg, cg, vg)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    ok = bool(torch.equal(ptr.cpu(), torch.from_numpy(A.ptr)) and torch.equal(ind.cpu(), torch.from_numpy(A.ind)))
+    nbytes = 28.0 * row.size + 4.0 * (A.m + 1)
+    res = {"entries": int(row.size), "seconds": round(dt, 5), "entries_per_s": round(row.size / dt, 1),
+           "GBs": round(nbytes / dt / 1e9, 2), "matches_original_csr": ok}
+    log(f"assembly: {row.size} coordinates -> CSR in {dt * 1e3:.1f} ms ({row.size / dt / 1e9:.2f} G entries/s), "
+        f"same CSR as generated: {ok}")
+    if cpu:
+        from oracle import oracle as orc
+        k = 1 << 20
+        t0 = time.perf_counter()
+        orc.coo_to_csr(row[perm[:k]], A.ind[perm[:k]], A.val[perm[:k]])
+        ct = time.perf_counter() - t0
+        res["cpu_oracle"] = {"entries": k, "seconds": round(ct, 4), "entries_per_s": round(k / ct, 1), "cores": 1,
+                             "kind": "port"}
+    return res
 
 
 def cpu_baseline(A, xt, bytes64, budget_s=10.0):

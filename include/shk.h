@@ -151,6 +151,20 @@ int shk_mlt_trans(shk_ctx ctx, shk_csr A, const void *dx, void *dy);
  * (computation/solver/iterative/Cg.cxx:97-98). */
 int shk_mlt_dot(shk_ctx ctx, shk_csr A, const void *dx, void *dy, double *result);
 
+/* ---- assembly ----------------------------------------------------------- */
+/* ConvertMatrix_from_Coordinates(IndRow, IndCol, Val, A, index) for RowSparse
+ * (matrix_sparse/Matrix_Conversions.cxx:1288-1335), on the device: the
+ * entries ordered by (row, column), duplicates kept in their input order
+ * (Seldon's MergeSort is stable, vector/Functions_Arrays.cxx:412-480);
+ * m = max(row - index) + 1 but at least m_min (A.GetM() before the call),
+ * n likewise.  Inputs and outputs are device arrays: dptr holds m_cap + 1
+ * ints, dind and dval_out nnz entries.  SHK_ERR_DIM when m > m_cap (*m then
+ * says how many rows are needed); dptr = NULL only computes *m and *n.  The
+ * result feeds shk_csr_create with src_on_device = 1. */
+int shk_coo_to_csr(shk_ctx ctx, shk_dtype dt, int64_t nnz, const int *drow, const int *dcol, const void *dval,
+                   int index, int m_min, int n_min, int m_cap, int *m, int *n, int *dptr, int *dind,
+                   void *dval_out);
+
 /* ---- BLAS-1 (computation/basic_functions/Functions_Vector.cxx) --------- */
 /* DotProd / DotProdConj (:330-346, :391-397): accumulated in the vector type. */
 int shk_dot(shk_ctx ctx, shk_dtype dt, int64_t n, const void *dx, const void *dy, double *result);

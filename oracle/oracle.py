@@ -67,6 +67,8 @@ def lib():
             getattr(L, f"orc_{sfx}_mlt").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
             getattr(L, f"orc_{sfx}_mltadd_trans").argtypes = [T, ctypes.POINTER(OrcMat), vp, T, vp]
             getattr(L, f"orc_{sfx}_mlt_trans").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
+            getattr(L, f"orc_{sfx}_coo_to_csr").argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int,
+                                                            ctypes.c_int, vp, vp, vp]
             getattr(L, f"orc_{sfx}_dot").argtypes = [ctypes.c_int, vp, vp]
             getattr(L, f"orc_{sfx}_dot").restype = T
             getattr(L, f"orc_{sfx}_norm2").argtypes = [ctypes.c_int, vp]
@@ -79,6 +81,8 @@ def lib():
             getattr(L, f"orc_{sfx}_jacobi_diag").argtypes = [ctypes.POINTER(OrcMat), vp]
             getattr(L, f"orc_{sfx}_sddmm").argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
         L.orc_iter_init.argtypes = [ctypes.POINTER(OrcIter), ctypes.c_int, ctypes.c_double]
+        L.orc_coo_dims.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_trsv_upper.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp]
         L.orc_genrot.argtypes = [ctypes.POINTER(ctypes.c_double)] * 4
         L.orc_applyrot.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -146,6 +150,21 @@ def mlt_trans(A, x, storage=None, dtype=np.float64):
     y = np.empty(A.m if st == ROWSYMSPARSE else A.n, dtype)
     getattr(lib(), f"orc_{_sfx(dtype)}_mlt_trans")(ctypes.byref(M.s), _p(x), _p(y))
     return y
+
+
+def coo_to_csr(row, col, val, index=0, m_min=0, n_min=0, dtype=np.float64):
+    """ConvertMatrix_from_Coordinates (RowSparse): returns (m, n, ptr, ind, val)."""
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, dtype)
+    m, n = ctypes.c_int(), ctypes.c_int()
+    lib().orc_coo_dims(row.size, _p(row), _p(col), index, m_min, n_min, ctypes.byref(m), ctypes.byref(n))
+    ptr = np.empty(m.value + 1, np.int32)
+    ind = np.empty(row.size, np.int32)
+    out = np.empty(row.size, dtype)
+    getattr(lib(), f"orc_{_sfx(dtype)}_coo_to_csr")(row.size, _p(row), _p(col), _p(val), index, m.value,
+                                                    _p(ptr), _p(ind), _p(out))
+    return m.value, n.value, ptr, ind, out
 
 
 def dot(x, y):
@@ -242,6 +261,16 @@ def write_seldon_vec(path, x):
     with open(path, "wb") as f:
         np.array([x.size], np.int32).tofile(f)
         np.ascontiguousarray(x).tofile(f)
+
+
+def read_seldon_csr(path, dtype=np.float64):
+    """(m, n, ptr, ind, val) from a Seldon binary CSR."""
+    with open(path, "rb") as f:
+        m, n, nz = np.fromfile(f, np.int32, 3)
+        ptr = np.fromfile(f, np.int32, m + 1)
+        ind = np.fromfile(f, np.int32, nz)
+        val = np.fromfile(f, dtype, nz)
+    return int(m), int(n), ptr, ind, val
 
 
 def read_seldon_vec(path, dtype=np.float64):

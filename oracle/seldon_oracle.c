@@ -10,6 +10,22 @@
 #include <string.h>
 
 /* Iteration(int, tol) constructor: Iterative.cxx:97-109. */
+/* The dimensions ConvertMatrix_from_Coordinates gives the matrix
+ * (matrix_sparse/Matrix_Conversions.cxx:1310-1316): m = GetNormInf(IndRow)
+ * + 1 (the largest absolute shifted index, 0 without entries) but at least
+ * A.GetM(), n likewise. */
+void orc_coo_dims(int nnz, const int *row, const int *col, int index, int m_min, int n_min, int *m, int *n)
+{
+    int rm = 0, cm = 0;
+    for (int i = 0; i < nnz; i++) {
+        int r = abs(row[i] - index), c = abs(col[i] - index);
+        if (r > rm) rm = r;
+        if (c > cm) cm = c;
+    }
+    *m = rm + 1 > m_min ? rm + 1 : m_min;
+    *n = cm + 1 > n_min ? cm + 1 : n_min;
+}
+
 void orc_iter_init(orc_iter *it, int max_iter, double tol)
 {
     memset(it, 0, sizeof(*it));

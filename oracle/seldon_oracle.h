@@ -59,6 +59,7 @@ typedef struct {
 } orc_iter;
 
 void orc_iter_init(orc_iter *it, int max_iter, double tol);
+void orc_coo_dims(int nnz, const int *row, const int *col, int index, int m_min, int n_min, int *m, int *n);
 
 /* ---- family-generated entry points (SFX = d | s, T = double | float) ---- */
 #define ORC_DECLARE(SFX, T)                                                    \
@@ -68,6 +69,9 @@ void orc_iter_init(orc_iter *it, int max_iter, double tol);
     void orc_##SFX##_mltadd_trans(T alpha, const orc_mat *A, const T *x,       \
                                   T beta, T *y);                               \
     void orc_##SFX##_mlt_trans(const orc_mat *A, const T *x, T *y);            \
+    void orc_##SFX##_coo_to_csr(int nnz, const int *row, const int *col,       \
+                                const T *val, int index, int m, int *ptr,      \
+                                int *ind, T *out);                             \
     T orc_##SFX##_dot(int n, const T *x, const T *y);                          \
     T orc_##SFX##_norm2(int n, const T *x);                                    \
     void orc_##SFX##_add(int n, T alpha, const T *x, T *y);                    \

@@ -199,6 +199,77 @@ void FN(mlt_trans)(const orc_mat *A, const T *x, T *y)
     free(er); free(ec); free(ev); free(cnt); free(t1); free(t2); free(cptr);
 }
 
+/* MergeSort(m, n, tab1, tab2, tab3) of vector/Functions_Arrays.cxx:412-480:
+ * bottom-up, runs of inc, the right run's element taken only when strictly
+ * smaller (so the sort is stable); tab1 the keys. */
+static void FN(mergesort3)(int lo, int hi, int *tab1, int *tab2, T *tab3)
+{
+    if (lo >= hi) return;
+    const int len = hi - lo + 1;
+    int *t1 = (int *)malloc(sizeof(int) * (size_t)len), *t2 = (int *)malloc(sizeof(int) * (size_t)len);
+    T *t3 = (T *)malloc(sizeof(T) * (size_t)len);
+    int inc = 1, ind = 0, current, i, j, sup;
+    while (inc < len) {
+        for (i = lo; i <= hi - inc; i += 2 * inc) {
+            ind = i;
+            current = i + inc;
+            sup = i + 2 * inc;
+            if (sup >= hi + 1) sup = hi + 1;
+            j = i;
+            while (j < i + inc) {
+                if (current < sup) {
+                    while (current < sup && tab1[j] > tab1[current]) {
+                        t1[ind - lo] = tab1[current]; t2[ind - lo] = tab2[current]; t3[ind - lo] = tab3[current];
+                        current++; ind++;
+                    }
+                    t1[ind - lo] = tab1[j]; t2[ind - lo] = tab2[j]; t3[ind - lo] = tab3[j];
+                    ind++; j++;
+                    if (j == i + inc)
+                        for (j = current; j < sup; j++) {
+                            t1[ind - lo] = tab1[j]; t2[ind - lo] = tab2[j]; t3[ind - lo] = tab3[j];
+                            ind++;
+                        }
+                } else {
+                    for (current = j; current < i + inc; current++) {
+                        t1[ind - lo] = tab1[current]; t2[ind - lo] = tab2[current]; t3[ind - lo] = tab3[current];
+                        ind++;
+                    }
+                    j = current + 1;
+                }
+            }
+        }
+        for (i = lo; i < ind; i++) { tab1[i] = t1[i - lo]; tab2[i] = t2[i - lo]; tab3[i] = t3[i - lo]; }
+        inc *= 2;
+    }
+    free(t1); free(t2); free(t3);
+}
+
+/* ConvertMatrix_from_Coordinates(IndRow, IndCol, Val, A, index) for RowSparse:
+ * matrix_sparse/Matrix_Conversions.cxx:1288-1335.  Shifts the indices by
+ * `index`, m = max(row) + 1 but at least m_min (A.GetM()), n likewise (the
+ * dimensions come from orc_coo_dims), Sort(IndRow, IndCol, Val) by rows,
+ * counts Ptr, then Sort(Ptr(i), Ptr(i + 1) - 1, IndCol, Val) per row.
+ * ptr has m + 1 entries; ind / out have nnz. */
+void FN(coo_to_csr)(int nnz, const int *row, const int *col, const T *val, int index, int m, int *ptr, int *ind,
+                    T *out)
+{
+    int *r = (int *)malloc(sizeof(int) * (size_t)(nnz + 1));
+    for (int i = 0; i < nnz; i++) {
+        r[i] = row[i] - index;
+        ind[i] = col[i] - index;
+        out[i] = val[i];
+    }
+    FN(mergesort3)(0, nnz - 1, r, ind, out);
+    for (int i = 0; i <= m; i++) ptr[i] = 0;
+    for (int i = 0; i < nnz; i++) ptr[r[i] + 1]++;
+    for (int i = 0; i < m; i++) ptr[i + 1] += ptr[i];
+    /* Sort(Ptr(i), Ptr(i + 1) - 1, IndCol, Val): the two-array MergeSort
+     * (:337-410) is the same merge on (IndCol, Val); the row array is
+     * carried along here unchanged (constant within the row) */
+    for (int i = 0; i < m; i++) FN(mergesort3)(ptr[i], ptr[i + 1] - 1, ind, r, out);
+    free(r);
+}
+
 /* Diagonal for the Jacobi preconditioner (an entry with ind == row; 0 when
  * the row stores none). */
 void FN(jacobi_diag)(const orc_mat *A, T *diag)

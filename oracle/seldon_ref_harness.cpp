@@ -16,6 +16,9 @@
 //   seldon_ref mlt   <row|sym|array> <d|f> A x out
 //   seldon_ref spmvt <row|sym|array> <d|f> A x y alpha beta out   (SeldonTrans)
 //   seldon_ref mltt  <row|sym|array> <d|f> A x out                 (SeldonTrans)
+//   seldon_ref coo2csr <d|f> rows cols vals index m_min n_min out
+//                    (ConvertMatrix_from_Coordinates into a RowSparse matrix
+//                     first sized m_min x n_min; writes it as a binary CSR)
 //   seldon_ref solve <cg|bicgstab|gmres> <row|sym|array> <id|jacobi>
 //                    A b x0|- tol maxit restart out
 // The Jacobi preconditioner does not exist in the reference; it is defined
@@ -227,6 +230,25 @@ static int run_solve(char** a)
   return 0;
 }
 
+template <class T>
+static int run_coo2csr(char** a)
+{
+  // The conversion Clear()s the index vectors, and Seldon's Vector::Clear
+  // runs the destructor explicitly (vector/Vector.cxx:336): at -O2 the
+  // vectors' own destructors would free the same memory again, so they are
+  // heap objects that are not deleted.
+  Vector<int>* row = new Vector<int>;
+  Vector<int>* col = new Vector<int>;
+  Vector<T>* val = new Vector<T>;
+  row->Read(a[0]);
+  col->Read(a[1]);
+  val->Read(a[2]);
+  Matrix<T, General, RowSparse> A(atoi(a[4]), atoi(a[5]));
+  ConvertMatrix_from_Coordinates(*row, *col, *val, A, atoi(a[3]));
+  A.Write(a[6]);
+  return 0;
+}
+
 int main(int argc, char** argv)
 {
   try
@@ -243,6 +265,8 @@ int main(int argc, char** argv)
       if (argc >= 7 && string(argv[1]) == "mltt")
         return string(argv[3]) == "f" ? run_spmv<float>(false, argv[2], argv + 4, true)
                                       : run_spmv<double>(false, argv[2], argv + 4, true);
+      if (argc >= 10 && string(argv[1]) == "coo2csr")
+        return string(argv[2]) == "f" ? run_coo2csr<float>(argv + 3) : run_coo2csr<double>(argv + 3);
       if (argc >= 12 && string(argv[1]) == "solve")
         return run_solve(argv + 2);
     }

@@ -641,6 +641,8 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
     std::vector<int> hptr(m + 1), hind(nnz);
     std::vector<char> hval(vs * (size_t)nnz);
     if (src_on_device) {
+        // the arrays may come from work queued on the context's stream
+        SHK_HIP(hipStreamSynchronize(ctx->stream));
         SHK_HIP(hipMemcpy(hptr.data(), ptr, sizeof(int) * (m + 1), hipMemcpyDeviceToHost));
         if (nnz) {
             SHK_HIP(hipMemcpy(hind.data(), ind, sizeof(int) * nnz, hipMemcpyDeviceToHost));

@@ -77,6 +77,17 @@ def _ptr(t: torch.Tensor):
     return ctypes.c_void_p(t.data_ptr())
 
 
+@dataclass
+class DeviceCSR:
+    """CSR arrays already on the device (torch tensors), e.g. from
+    ConvertMatrix_from_Coordinates; Matrix(DeviceCSR(...)) takes them."""
+    m: int
+    n: int
+    ptr: torch.Tensor
+    ind: torch.Tensor
+    val: torch.Tensor
+
+
 class Matrix:
     """A sparse matrix resident on the GPU.
 
@@ -93,16 +104,20 @@ class Matrix:
         self.storage = storage
         self.dtype = dtype
         self.m, self.n = int(csr.m), int(csr.n)
-        ptr = np.ascontiguousarray(csr.ptr, np.int32)
-        ind = np.ascontiguousarray(csr.ind, np.int32)
-        val = np.ascontiguousarray(np.asarray(csr.val), _NP[dtype])
-        self.nnz = int(ptr[-1])
+        if isinstance(csr.ptr, torch.Tensor):  # CSR arrays already on the device
+            assert csr.ptr.is_cuda and csr.val.dtype == dtype
+            ptr, ind, val = (t.contiguous() for t in (csr.ptr, csr.ind, csr.val))
+            self.nnz = int(ind.numel())
+            args, on_dev = (_ptr(ptr), _ptr(ind) if self.nnz else None, _ptr(val) if self.nnz else None), 1
+        else:
+            ptr = np.ascontiguousarray(csr.ptr, np.int32)
+            ind = np.ascontiguousarray(csr.ind, np.int32)
+            val = np.ascontiguousarray(np.asarray(csr.val), _NP[dtype])
+            self.nnz = int(ptr[-1])
+            args, on_dev = tuple(a.ctypes.data_as(ctypes.c_void_p) for a in (ptr, ind, val)), 0
         h = ctypes.c_void_p()
         L.check(L.load().shk_csr_create_layout(self.ctx.h, _STORAGE[storage], _DT[dtype], self.m, self.n,
-                                               self.nnz, ptr.ctypes.data_as(ctypes.c_void_p),
-                                               ind.ctypes.data_as(ctypes.c_void_p),
-                                               val.ctypes.data_as(ctypes.c_void_p), 0, self.LAYOUTS[layout],
-                                               ctypes.byref(h)))
+                                               self.nnz, *args, on_dev, self.LAYOUTS[layout], ctypes.byref(h)))
         self.h = h
 
     @property
@@ -218,6 +233,34 @@ def MltDot(A: Matrix, X, Y) -> float:
     r = ctypes.c_double()
     L.check(L.load().shk_mlt_dot(A.ctx.h, A.h, _ptr(X), _ptr(Y), ctypes.byref(r)))
     return r.value
+
+
+def ConvertMatrix_from_Coordinates(IndRow: torch.Tensor, IndCol: torch.Tensor, Val: torch.Tensor, index: int = 0,
+                                   m: int = 0, n: int = 0):
+    """CSR arrays (device tensors ptr, ind, val) and (m, n) from coordinates,
+    as Seldon's ConvertMatrix_from_Coordinates builds a RowSparse matrix
+    (Matrix_Conversions.cxx:1288-1335): entries ordered by (row, column),
+    duplicates kept in input order; m, n at least the given ones (the
+    matrix's size before the call).  Matrix(DeviceCSR(m, n, ptr, ind, val))
+    uploads nothing: it plans from the device arrays."""
+    dt = _DT[Val.dtype]
+    row = IndRow.to(torch.int32).contiguous()
+    col = IndCol.to(torch.int32).contiguous()
+    val = Val.contiguous()
+    nnz = row.numel()
+    ctx = Context.default()
+    mm, nn = ctypes.c_int(), ctypes.c_int()
+    null = ctypes.c_void_p(0)
+    L.check(L.load().shk_coo_to_csr(ctx.h, dt, nnz, _ptr(row) if nnz else null, _ptr(col) if nnz else null,
+                                    _ptr(val) if nnz else null, int(index), int(m), int(n), 0,
+                                    ctypes.byref(mm), ctypes.byref(nn), null, null, null))
+    ptr = torch.empty(mm.value + 1, dtype=torch.int32, device=row.device)
+    ind = torch.empty(max(nnz, 1), dtype=torch.int32, device=row.device)
+    out = torch.empty(max(nnz, 1), dtype=val.dtype, device=row.device)
+    L.check(L.load().shk_coo_to_csr(ctx.h, dt, nnz, _ptr(row) if nnz else null, _ptr(col) if nnz else null,
+                                    _ptr(val) if nnz else null, int(index), int(m), int(n), mm.value,
+                                    ctypes.byref(mm), ctypes.byref(nn), _ptr(ptr), _ptr(ind), _ptr(out)))
+    return mm.value, nn.value, ptr, ind[:nnz], out[:nnz]
 
 
 def DotProd(X, Y) -> float:

@@ -218,12 +218,59 @@ def make_spmvt():
     print("spmvt fixtures:", len(out))
 
 
+COO_CASES = {
+    # name: (seed, nnz, rows, cols, duplicates, index, m_min, n_min)
+    "dup": (31, 4000, 300, 200, 400, 1, 0, 250),
+    "gaps": (32, 2500, 900, 700, 50, 0, 1200, 0),
+    "none": (33, 0, 1, 1, 0, 0, 7, 3),
+}
+
+
+def coo_case(name):
+    """Seeded coordinates with duplicates (repeated (row, col) pairs with other
+    values) and, for "gaps", only every third row used."""
+    seed, nnz, rows, cols, dup, index, m_min, n_min = COO_CASES[name]
+    rng = np.random.default_rng(seed)
+    row = rng.integers(0, rows, nnz)
+    if name == "gaps":
+        row = 3 * (row // 3)
+    col = rng.integers(0, cols, nnz)
+    if dup:
+        pick = rng.integers(0, nnz, dup)
+        where = rng.integers(0, nnz, dup)
+        row[where] = row[pick]
+        col[where] = col[pick]
+    val = rng.standard_normal(nnz)
+    return (row + index).astype(np.int32), (col + index).astype(np.int32), val, index, m_min, n_min
+
+
+def make_coo():
+    """ConvertMatrix_from_Coordinates (RowSparse) from the reference build."""
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in COO_CASES:
+            row, col, val, index, m_min, n_min = coo_case(name)
+            for dt, c in ((np.float64, "d"), (np.float32, "f")):
+                orc.write_seldon_vec(f"{tmp}/r.bin", row)
+                orc.write_seldon_vec(f"{tmp}/c.bin", col)
+                orc.write_seldon_vec(f"{tmp}/v.bin", val.astype(dt))
+                orc.run_ref(["coo2csr", c, f"{tmp}/r.bin", f"{tmp}/c.bin", f"{tmp}/v.bin", index, m_min, n_min,
+                             f"{tmp}/o.bin"])
+                m, n, ptr, ind, v = orc.read_seldon_csr(f"{tmp}/o.bin", dt)
+                out[f"{name}_{c}_mn"] = np.array([m, n])
+                out[f"{name}_{c}_ptr"], out[f"{name}_{c}_ind"], out[f"{name}_{c}_val"] = ptr, ind, v
+    np.savez_compressed(os.path.join(OUT, "coo_to_csr.npz"), spec=json.dumps(COO_CASES), **out)
+    print("coo fixtures:", len(out))
+
+
 if __name__ == "__main__":
     if not orc.ref_available("generic") or not orc.ref_available("blas"):
         orc.build()
         import subprocess
         subprocess.run(["make", "-s", "-C", orc.HERE, "ref"], check=True)
-    which = sys.argv[1:] or ["spmv", "spmvt", "solves"]
+    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "solves"]
+    if "coo" in which:
+        make_coo()
     if "spmv" in which:
         make_spmv()
     if "spmvt" in which:

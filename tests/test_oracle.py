@@ -4,7 +4,9 @@ Pins, each chosen so that a plausible slip (a dropped term, a wrong sign or
 index, a transposed operand, a wrong stopping count) fails one of them:
   * the golden fixtures written by the real Seldon build (tests/golden):
     SpMV outputs, plain and with SeldonTrans (on a rectangular matrix), on
-    three storages x two precisions x six (alpha, beta), bitwise; Cg / BiCgStab residual histories, iteration counts, error codes
+    three storages x two precisions x six (alpha, beta), bitwise; the CSR
+    that ConvertMatrix_from_Coordinates assembles (duplicates, 1-based
+    input, empty rows, no entries), bitwise; Cg / BiCgStab residual histories, iteration counts, error codes
     and solutions, bitwise against the generic-template build; Gmres against
     the MKL-linked build (the only one that compiles) within its recorded
     spread, and bitwise against its own recorded history;
@@ -208,6 +210,56 @@ def test_oracle_trans_bitwise_vs_live_seldon():
                 orc.run_ref(["mltt", st, "d", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/o.bin"])
                 ref = orc.read_seldon_vec(f"{tmp}/o.bin")
                 assert np.array_equal(ref, orc.mlt_trans(M, x, storage=st))
+
+
+@pytest.mark.parametrize("name", ["dup", "gaps", "none"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_oracle_coo_to_csr_matches_seldon_fixture(name, c):
+    """ConvertMatrix_from_Coordinates (RowSparse) of the real Seldon build:
+    duplicates kept in input order, 1-based input, empty rows, m / n from
+    the prior size, no entries at all (m = 1 then, from GetNormInf)."""
+    from tests.golden.make_golden import coo_case
+    fx = load_fixture("coo_to_csr.npz")
+    dt = np.float64 if c == "d" else np.float32
+    row, col, val, index, m_min, n_min = coo_case(name)
+    m, n, ptr, ind, out = orc.coo_to_csr(row, col, val.astype(dt), index, m_min, n_min, dtype=dt)
+    assert [m, n] == list(fx[f"{name}_{c}_mn"])
+    assert np.array_equal(ptr, fx[f"{name}_{c}_ptr"])
+    assert np.array_equal(ind, fx[f"{name}_{c}_ind"])
+    assert np.array_equal(out, fx[f"{name}_{c}_val"])
+
+
+def test_oracle_coo_to_csr_is_a_stable_sort():
+    """The restated MergeSort (two passes) equals numpy's stable lexsort by
+    (row, column); an unstable order of the duplicates must not pass."""
+    from tests.golden.make_golden import coo_case
+    row, col, val, index, _, _ = coo_case("dup")
+    m, n, ptr, ind, out = orc.coo_to_csr(row, col, val, index)
+    o = np.lexsort((col, row))
+    assert np.array_equal(ind, col[o] - index) and np.array_equal(out, val[o])
+    assert np.array_equal(ptr, np.concatenate([[0], np.cumsum(np.bincount(row - index, minlength=m))]))
+    rev = np.lexsort((-np.arange(len(row)), col, row))  # duplicates reversed
+    assert not np.array_equal(out, val[rev])
+
+
+@pytest.mark.skipif(not orc.ref_available("generic"), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_coo_to_csr_vs_live_seldon():
+    rng = np.random.default_rng(9)
+    with tempfile.TemporaryDirectory() as tmp:
+        for k in range(3):
+            nnz = 20000 + 5000 * k
+            row = rng.integers(0, 3000, nnz).astype(np.int32)
+            col = rng.integers(0, 2000 + 500 * k, nnz).astype(np.int32)
+            row[: nnz // 10] = row[nnz // 10: 2 * (nnz // 10)]
+            col[: nnz // 10] = col[nnz // 10: 2 * (nnz // 10)]
+            val = rng.standard_normal(nnz)
+            orc.write_seldon_vec(f"{tmp}/r.bin", row)
+            orc.write_seldon_vec(f"{tmp}/c.bin", col)
+            orc.write_seldon_vec(f"{tmp}/v.bin", val)
+            orc.run_ref(["coo2csr", "d", f"{tmp}/r.bin", f"{tmp}/c.bin", f"{tmp}/v.bin", 0, 0, 0, f"{tmp}/o.bin"])
+            ref = orc.read_seldon_csr(f"{tmp}/o.bin")
+            got = orc.coo_to_csr(row, col, val)
+            assert ref[:2] == got[:2] and all(np.array_equal(a, b) for a, b in zip(ref[2:], got[2:]))
 
 
 def test_laplacian_eigenpairs():
