@@ -108,6 +108,15 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
                           int64_t nnz, const int *ptr, const int *ind, const void *val,
                           int src_on_device, int layout, shk_csr *out);
 int shk_csr_layout(shk_csr A, int *layout);
+/* Seldon's binary CSR file (int m, n, nz; ptr[m+1]; ind[nz]; values[nz]):
+ * Matrix_Sparse::Write / Read (matrix_sparse/Matrix_Sparse.cxx:1532-1556,
+ * 1649-1682), the same layout for RowSymSparse's upper part
+ * (Matrix_SymSparse.cxx).  shk_csr_write writes the stored arrays back in
+ * the caller's order (panel layout: rows by increasing column, which that
+ * layout requires); shk_csr_read creates a matrix of the given storage,
+ * value type (the file's) and layout. */
+int shk_csr_write(shk_ctx ctx, shk_csr A, const char *path);
+int shk_csr_read(shk_ctx ctx, const char *path, shk_storage storage, shk_dtype dtype, int layout, shk_csr *out);
 int shk_csr_destroy(shk_csr A);
 /* New values on the same pattern, in the caller's stored order (RowSymSparse:
  * upper part).  Used when A depends on parameters (the differentiable solve). */

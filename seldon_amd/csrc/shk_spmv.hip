@@ -1,8 +1,10 @@
 // shk_spmv.hip -- sparse matrix upload, row-block planning and the SpMV
 // entry points of the C ABI.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "shk_kernels.hpp"
@@ -598,10 +600,88 @@ static int build_trans(shk_csr A)
     return SHK_OK;
 }
 
+// The caller's stored arrays (RowSymSparse: the upper part) read back from
+// the device.  Row layout: RowSparse / ArrayRowSparse entries are stored in
+// the caller's order; RowSymSparse rows are [stored part ; transposed part]
+// with mid[r] the split.  Panel layout (rows with sorted columns): the
+// coordinates of the device entries ordered by (row, column), the transposed
+// copies of a symmetric matrix left out.
+template <typename T>
+static int stored_download(shk_csr A, std::vector<int> &ptr, std::vector<int> &ind, std::vector<T> &val)
+{
+    shk_ctx ctx = A->ctx;
+    const bool sym = A->storage == SHK_ROWSYMSPARSE;
+    if (A->layout == SHK_LAYOUT_PANELS) {
+        std::vector<int> row, col;
+        std::vector<T> v;
+        SHK_TRY(coo_download<T>(A, row, col, v));
+        if (sym) {
+            size_t q = 0;
+            for (size_t k = 0; k < row.size(); k++)
+                if (col[k] >= row[k]) {
+                    row[q] = row[k];
+                    col[q] = col[k];
+                    v[q] = v[k];
+                    q++;
+                }
+            row.resize(q);
+            col.resize(q);
+            v.resize(q);
+        }
+        coo_to_csr<T>(A->m, A->n, row, col, v, ptr, ind, val);
+        return SHK_OK;
+    }
+    const int m = A->m;
+    const int64_t nd = A->nnz_dev;
+    std::vector<int> dptr((size_t)m + 1), dmid(sym ? (size_t)m : 0);
+    std::vector<uint32_t> dind((size_t)nd);
+    std::vector<T> dval((size_t)nd);
+    SHK_HIP(hipMemcpyAsync(dptr.data(), A->ptr, sizeof(int) * ((size_t)m + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (sym && m) SHK_HIP(hipMemcpyAsync(dmid.data(), A->mid, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost, ctx->stream));
+    if (nd) {
+        SHK_HIP(hipMemcpyAsync(dind.data(), A->ind, sizeof(uint32_t) * (size_t)nd, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        SHK_HIP(hipMemcpyAsync(dval.data(), A->val, sizeof(T) * (size_t)nd, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    SHK_HIP(hipStreamSynchronize(ctx->stream));
+    ptr.assign((size_t)m + 1, 0);
+    ind.clear();
+    val.clear();
+    for (int i = 0; i < m; i++) {
+        const int e = sym ? dmid[i] : dptr[(size_t)i + 1];
+        for (int j = dptr[i]; j < e; j++) {
+            ind.push_back((int)(dind[j] & ~kLowerFlag));
+            val.push_back(dval[j]);
+        }
+        ptr[(size_t)i + 1] = (int)ind.size();
+    }
+    return SHK_OK;
+}
+
 static int ensure_trans(shk_csr A)
 {
     if (A->trans) return SHK_OK;
     return A->dtype == SHK_F64 ? build_trans<double>(A) : build_trans<float>(A);
+}
+
+template <typename T>
+static int csr_write_typed(shk_csr A, const char *path)
+{
+    std::vector<int> ptr, ind;
+    std::vector<T> val;
+    SHK_TRY(stored_download<T>(A, ptr, ind, val));
+    FILE *f = std::fopen(path, "wb");
+    SHK_CHECK(f, SHK_ERR_ARG, std::string("shk_csr_write: cannot open ") + path);
+    const int head[3] = {A->m, A->n, (int)ind.size()};
+    bool ok = std::fwrite(head, sizeof(int), 3, f) == 3;
+    ok = ok && std::fwrite(ptr.data(), sizeof(int), ptr.size(), f) == ptr.size();
+    if (!ind.empty()) {
+        ok = ok && std::fwrite(ind.data(), sizeof(int), ind.size(), f) == ind.size();
+        ok = ok && std::fwrite(val.data(), sizeof(T), val.size(), f) == val.size();
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    SHK_CHECK(ok, SHK_ERR_ARG, std::string("shk_csr_write: write failed: ") + path);
+    return SHK_OK;
 }
 
 }  // namespace shk
@@ -755,6 +835,35 @@ int shk_csr_layout(shk_csr A, int *layout)
     SHK_CHECK(A && layout, SHK_ERR_ARG, "null argument");
     *layout = A->layout;
     return SHK_OK;
+}
+
+int shk_csr_write(shk_ctx ctx, shk_csr A, const char *path)
+{
+    SHK_TRY(check_csr(ctx, A));
+    SHK_CHECK(path, SHK_ERR_ARG, "shk_csr_write: null path");
+    return A->dtype == SHK_F64 ? csr_write_typed<double>(A, path) : csr_write_typed<float>(A, path);
+}
+
+int shk_csr_read(shk_ctx ctx, const char *path, shk_storage storage, shk_dtype dtype, int layout, shk_csr *out)
+{
+    SHK_CHECK(ctx && path && out, SHK_ERR_ARG, "shk_csr_read: null argument");
+    SHK_CHECK(dtype == SHK_F64 || dtype == SHK_F32, SHK_ERR_ARG, "shk_csr_read: bad dtype");
+    FILE *f = std::fopen(path, "rb");
+    SHK_CHECK(f, SHK_ERR_ARG, std::string("shk_csr_read: cannot open ") + path);
+    int head[3] = {0, 0, 0};
+    bool ok = std::fread(head, sizeof(int), 3, f) == 3 && head[0] >= 0 && head[1] >= 0 && head[2] >= 0;
+    const size_t vs = dsize(dtype);
+    std::vector<int> ptr(ok ? (size_t)head[0] + 1 : 0), ind(ok ? (size_t)head[2] : 0);
+    std::vector<char> val(ok ? vs * (size_t)head[2] : 0);
+    ok = ok && std::fread(ptr.data(), sizeof(int), ptr.size(), f) == ptr.size();
+    ok = ok && std::fread(ind.data(), sizeof(int), ind.size(), f) == ind.size();
+    ok = ok && std::fread(val.data(), vs, (size_t)head[2], f) == (size_t)head[2];
+    std::fclose(f);
+    SHK_CHECK(ok, SHK_ERR_ARG, std::string("shk_csr_read: not a Seldon binary CSR file of this value type: ") + path);
+    SHK_CHECK(ptr[0] == 0 && ptr[(size_t)head[0]] == head[2], SHK_ERR_ARG, "shk_csr_read: inconsistent row pointer");
+    return shk_csr_create_layout(ctx, storage, dtype, head[0], head[1], head[2], ptr.data(),
+                                 ind.empty() ? nullptr : ind.data(), val.empty() ? nullptr : val.data(), 0, layout,
+                                 out);
 }
 
 int shk_csr_info(shk_csr A, int *m, int *n, int64_t *nnz_stored, int64_t *nnz_dev, int *dtype,

@@ -120,6 +120,28 @@ class Matrix:
                                                self.nnz, *args, on_dev, self.LAYOUTS[layout], ctypes.byref(h)))
         self.h = h
 
+    def Write(self, path: str):
+        """Seldon's binary CSR file of the stored arrays (Matrix_Sparse.cxx:1532-1556)."""
+        L.check(L.load().shk_csr_write(self.ctx.h, self.h, str(path).encode()))
+
+    @classmethod
+    def Read(cls, path: str, storage: str = "row", dtype=torch.float64, layout: str = "auto",
+             ctx: Context | None = None) -> "Matrix":
+        """A matrix from Seldon's binary CSR file (Matrix_Sparse.cxx:1649-1682)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or Context.default()
+        self.storage, self.dtype = storage, dtype
+        h = ctypes.c_void_p()
+        L.check(L.load().shk_csr_read(self.ctx.h, str(path).encode(), _STORAGE[storage], _DT[dtype],
+                                      cls.LAYOUTS[layout], ctypes.byref(h)))
+        self.h = h
+        m, n, a, b, d, st = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(),
+                             ctypes.c_int(), ctypes.c_int())
+        L.check(L.load().shk_csr_info(h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(a), ctypes.byref(b),
+                                      ctypes.byref(d), ctypes.byref(st)))
+        self.m, self.n, self.nnz = m.value, n.value, int(a.value)
+        return self
+
     @property
     def layout(self) -> str:
         """The device layout chosen at creation: "rows" or "panels"."""

@@ -85,3 +85,49 @@ def test_coo_to_csr_errors(gpu):
     st = L.load().shk_coo_to_csr(K.Context.default().h, 0, 3, K._ptr(row), K._ptr(col), K._ptr(val), 0, 0, 0, 1,
                                  ctypes.byref(mm), ctypes.byref(nn), K._ptr(ptr), K._ptr(ind), K._ptr(out))
     assert st != 0 and (mm.value, nn.value) == (3, 6)  # the row pointer is too short: m = 3
+
+
+@pytest.mark.parametrize("layout", ["rows", "auto"])
+@pytest.mark.parametrize("st", ["row", "sym", "array"])
+def test_csr_binary_file_roundtrip(gpu, tmp_path, st, layout):
+    """Seldon's binary CSR file (Matrix_Sparse.cxx:1532-1556, 1649-1682):
+    the device matrix writes the caller's arrays byte for byte as Seldon's
+    layout has them, and a matrix read back multiplies bitwise the same."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    A = wl.powerlaw(30000, 8, kmin=2, kmax=200, half_width=4000)
+    M = wl.upper(A) if st == "sym" else A
+    Ag = K.Matrix(M, storage=st, layout=layout)
+    Ag.Write(tmp_path / "dev.bin")
+    orc.write_seldon_csr(tmp_path / "ref.bin", M)
+    assert (tmp_path / "dev.bin").read_bytes() == (tmp_path / "ref.bin").read_bytes()
+    Bg = K.Matrix.Read(tmp_path / "ref.bin", storage=st, layout=layout)
+    x = torch.tensor(wl.uniform(A.n, 4), device="cuda")
+    y1 = torch.empty(A.m, dtype=torch.float64, device="cuda")
+    y2 = torch.empty_like(y1)
+    K.Mlt(Ag, x, y1)
+    K.Mlt(Bg, x, y2)
+    assert torch.equal(y1, y2)
+    assert np.array_equal(y1.cpu().numpy(), orc.mlt(M, x.cpu().numpy(), storage=st))
+
+
+@pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["x"]).ref_available("generic"),
+                    reason="oracle/_ref not built (no reference tree)")
+def test_csr_read_file_written_by_seldon(gpu, tmp_path):
+    """A CSR file written by the real Seldon (its ConvertMatrix_from_Coordinates
+    then Matrix::Write) reads into the same matrix the device assembles."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    row, col, val, index, m_min, n_min = coo_case("dup")
+    orc.write_seldon_vec(tmp_path / "r.bin", row)
+    orc.write_seldon_vec(tmp_path / "c.bin", col)
+    orc.write_seldon_vec(tmp_path / "v.bin", val)
+    orc.run_ref(["coo2csr", "d", tmp_path / "r.bin", tmp_path / "c.bin", tmp_path / "v.bin", index, m_min, n_min,
+                 tmp_path / "A.bin"])
+    Ag = K.Matrix.Read(tmp_path / "A.bin")
+    m, n, ptr, ind, out = _run(row, col, val, index, m_min, n_min, torch.float64)
+    assert (Ag.m, Ag.n) == (m, n)
+    Bg = K.Matrix(K.DeviceCSR(m, n, ptr, ind, out))
+    Ag.Write(tmp_path / "a2.bin")
+    Bg.Write(tmp_path / "b2.bin")
+    assert (tmp_path / "a2.bin").read_bytes() == (tmp_path / "b2.bin").read_bytes() == (tmp_path / "A.bin").read_bytes()
