@@ -159,6 +159,7 @@ def main():
         del A64, A32, x64, y64, x32, y32
         torch.cuda.empty_cache()
         result["assembly"] = assembly(K, A, np, torch, cpu=not args.no_cpu)
+        result["cg_laplace256"] = cg_config1(K, wl, np, torch, cpu=not args.no_cpu)
         torch.cuda.empty_cache()
         result["convdiff128"] = convdiff_solves(K, wl, np, torch)
         torch.cuda.empty_cache()
@@ -219,6 +220,39 @@ def _true_relres(K, A, x, b, bn):
     K.Copy(b, r)
     K.MltAdd(-1.0, A, x, 1.0, r)
     return K.Norm2(r) / bn
+
+
+def cg_config1(K, wl, np, torch, cpu=True):
+    """Plain FP64 Cg, identity preconditioner, on the 256 x 256 5-point
+    Laplacian (65,536 rows), b = A x_true, to a 1e-10 relative residual: the
+    reference's own CPU-runnable case.  Time to solution (second of two runs),
+    iterations, and the oracle's Cg (Seldon's generic template restated in C,
+    one core) on the same system."""
+    A, b, xt = wl.cg_laplace(256)
+    Ad = K.Matrix(A, dtype=torch.float64)
+    bg = torch.tensor(b, device="cuda")
+    out = {"rows": A.m, "nnz": A.nnz}
+    for _ in range(2):
+        x = torch.zeros_like(bg)
+        it = K.Iteration(10000, 1e-10)
+        it.HideMessages()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        code = K.Cg(Ad, x, bg, K.Preconditioner_Base(), it)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    err = float(np.abs(x.cpu().numpy() - xt).max())
+    out["gpu"] = {"seconds": round(dt, 4), "iterations": it.nb_iter, "code": code,
+                  "us_per_iteration": round(dt / max(it.nb_iter, 1) * 1e6, 2), "max_abs_err_vs_xtrue": err}
+    log(f"config-1 Cg: {dt * 1e3:.1f} ms, {it.nb_iter} its ({dt / max(it.nb_iter, 1) * 1e6:.1f} us/it), code {code}")
+    if cpu:
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        r = orc.solve("cg", A, b, tol=1e-10, max_iter=10000)
+        ct = time.perf_counter() - t0
+        out["cpu_oracle"] = {"seconds": round(ct, 4), "iterations": r.nb_iter, "cores": 1, "kind": "port"}
+        log(f"config-1 Cg on the oracle (1 core): {ct * 1e3:.1f} ms, {r.nb_iter} its")
+    return out
 
 
 def convdiff_solves(K, wl, np, torch):

@@ -185,12 +185,13 @@ __device__ __forceinline__ void block_partial(T (&v)[NV], T *partials, int nbloc
     }
 }
 
-template <typename T, typename Fin>
-__global__ __launch_bounds__(kBlock) void k_grid_finish(const T *partials, int nblocks, const int *skip, Fin fin)
+// The fixed-order total of a split reduction's partials, valid in thread 0
+// (a kBlock-thread work-group): the finish kernel below uses it, and so do
+// kernels that fold the previous reduction's finish into their own start.
+template <typename T>
+__device__ __forceinline__ void grid_total(const T *partials, int nblocks, T (&out)[2])
 {
-    if (skip && *skip) return;
     __shared__ T lds[2 * 16];
-    T out[2];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         T s = 0;
@@ -198,6 +199,14 @@ __global__ __launch_bounds__(kBlock) void k_grid_finish(const T *partials, int n
         out[k] = s;
     }
     block_sum<T, 2>(out, lds);
+}
+
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_grid_finish(const T *partials, int nblocks, const int *skip, Fin fin)
+{
+    if (skip && *skip) return;
+    T out[2];
+    grid_total<T>(partials, nblocks, out);
     if (threadIdx.x == 0) fin(out);
 }
 

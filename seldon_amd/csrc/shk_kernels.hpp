@@ -21,6 +21,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "shk_internal.hpp"
 
@@ -63,7 +64,8 @@ struct SpmvArgs {
     const T *diag;     // Jacobi epilogue: out = y / diag (null = none)
     int dot0, dot1;    // DotKind for the two partial sums
     const T *u0, *u1;  // dot vectors
-    T *partials;       // [2][grid] block sums of the two dots
+    T *partials;       // [2][grid] block sums of the two dots (null: the context's buffer)
+    int grid_used;     // set by launch_spmv: work-groups of the launch (partials per dot)
     const int *skip;   // device flag: nonzero = return at once (solver done)
     // panel layout
     int npanel;
@@ -260,6 +262,13 @@ __device__ __forceinline__ T seq_sum(T acc, const T *prod, int q0, int q1)
 }
 
 struct FinNone {
+    template <typename T>
+    __device__ void operator()(const T (&)[2]) const {}
+};
+
+// The SpMV's dot partials are left for the next kernel, which folds the
+// finish into its own start (no finish launch).
+struct FinDefer {
     template <typename T>
     __device__ void operator()(const T (&)[2]) const {}
 };
@@ -719,10 +728,11 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     a.pchunk = A->pchunk;
     a.slot_row = A->slot_row;
     const bool dots = a.dot0 != kDotNone || a.dot1 != kDotNone;
-    if (dots) {
+    if (dots && !a.partials) {
         SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * (size_t)A->nblk_launch));
         a.partials = (T *)ctx->red.partials;
     }
+    constexpr bool defer = std::is_same<Fin, FinDefer>::value;
     if (A->m == 0) return SHK_OK;
     if (A->layout == SHK_LAYOUT_PANELS) {
         constexpr int PB = panel_budget<T>();
@@ -734,7 +744,8 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
             hipLaunchKernelGGL((k_panel<T, kSym, PB>), grid, block, 0, ctx->stream, a, chunk);
         else
             hipLaunchKernelGGL((k_panel<T, kCsr, PB>), grid, block, 0, ctx->stream, a, chunk);
-        if (dots) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
+        a.grid_used = (int)grid.x;
+        if (dots && !defer) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
         SHK_HIP(hipGetLastError());
         return SHK_OK;
     }
@@ -753,7 +764,8 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
         hipLaunchKernelGGL((k_spmv<T, kArray, B>), grid, block, 0, ctx->stream, a, a.blk_info);
         break;
     }
-    if (dots) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
+    a.grid_used = (int)grid.x;
+    if (dots && !defer) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
     SHK_HIP(hipGetLastError());
     return SHK_OK;
 }

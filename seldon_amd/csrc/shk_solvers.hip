@@ -71,13 +71,29 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *__restrict_
     }
 }
 
+// delta = (p, q) from the SpMV's partials and alpha = rho / delta (the
+// finish folded in: every work-group forms the same total, the first one
+// records it, a zero delta ends the solve, Cg.cxx:98-104); then
 // x += alpha p; r += (-alpha) q; sum r^2   (Cg.cxx:107-108, then Finished)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *__restrict__ p, const T *__restrict__ q, T *__restrict__ x, T *__restrict__ r,
-                                                  KState<T> *st, T *partials)
+                                                  KState<T> *st, const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
-    const T alpha = st->alpha, malpha = -st->alpha;
+    __shared__ T alpha_sh;
+    __shared__ int fail_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            fail_sh = tot[0] == (T)0;
+            alpha_sh = fail_sh ? (T)0 : st->rho / tot[0];
+            if (blockIdx.x == 0) FinCgDelta<T>{st}(tot);
+        }
+        __syncthreads();
+    }
+    if (fail_sh) return;
+    const T alpha = alpha_sh, malpha = -alpha_sh;
     const bool ax = alpha != (T)0, ar = malpha != (T)0;
     T c[2] = {(T)0, (T)0};
     GRID_STRIDE(i, n) {
@@ -141,12 +157,28 @@ __global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const T *__restrict_
 }
 
 // s = r; s += (-alpha) v (:111-112); shat = M^{-1} s (:124); sum s^2
+// (sigma = (rtilde, v) from the SpMV's partials, breakdown #3 and
+// alpha = rho_1 / sigma folded in, :104-110)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *__restrict__ r, const T *__restrict__ v, T *__restrict__ s, const T *__restrict__ d,
-                                                 T *__restrict__ shat, KState<T> *st, T *partials)
+                                                 T *__restrict__ shat, KState<T> *st, const T *prev, int prev_nb,
+                                                 T *partials)
 {
     if (st->skip) return;
-    const T malpha = -st->alpha;
+    __shared__ T alpha_sh;
+    __shared__ int fail_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            fail_sh = tot[0] == (T)0;
+            alpha_sh = fail_sh ? (T)0 : st->rho_1 / tot[0];
+            if (blockIdx.x == 0) FinBiSigma<T>{st}(tot);
+        }
+        __syncthreads();
+    }
+    if (fail_sh) return;
+    const T malpha = -alpha_sh;
     const bool aa = malpha != (T)0;
     T c[2] = {(T)0, (T)0};
     GRID_STRIDE(i, n) {
@@ -175,10 +207,21 @@ __global__ __launch_bounds__(kBlock) void k_bi_fix(int64_t n, const T *__restric
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *__restrict__ phat, const T *__restrict__ shat, const T *__restrict__ s,
                                                   const T *__restrict__ t, const T *__restrict__ rt, T *__restrict__ x, T *__restrict__ r, KState<T> *st,
-                                                  T *partials)
+                                                  const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
-    const T alpha = st->alpha, omega = st->omega, momega = -st->omega;
+    // omega = (t, s) / (t, t) from the SpMV's partials (:129), folded in
+    __shared__ T omega_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            omega_sh = tot[0] / tot[1];
+            if (blockIdx.x == 0) FinBiOmega<T>{st}(tot);
+        }
+        __syncthreads();
+    }
+    const T alpha = st->alpha, omega = omega_sh, momega = -omega_sh;
     const bool aa = alpha != (T)0, ao = omega != (T)0, am = momega != (T)0;
     T c[2] = {(T)0, (T)0};
     GRID_STRIDE(i, n) {
@@ -221,13 +264,32 @@ __global__ __launch_bounds__(kBlock) void k_gm_v0(int64_t n, const T *__restrict
     GRID_STRIDE(i, n) v0[i] = r[i] * sc;
 }
 
-// MGS step k >= 1 of Arnoldi step i: w += (-H(k-1,i)) V_{k-1}; H(k,i) = (V_k, w)
+// H(k, i) from the previous kernel's partials of (V_k, w) (the finish folded
+// into this kernel: every work-group forms the same fixed-order total, the
+// first one stores it for the rotations and the least squares).
+template <typename T>
+__device__ __forceinline__ T gm_fold_h(const T *prev, int prev_nb, int hpos, KState<T> *st)
+{
+    __shared__ double h_sh;
+    T tot[2];
+    grid_total<T>(prev, prev_nb, tot);
+    if (threadIdx.x == 0) {
+        const double h = (double)tot[0];
+        h_sh = h;
+        if (blockIdx.x == 0) st->H[hpos] = h;
+    }
+    __syncthreads();
+    return (T)(-h_sh);
+}
+
+// MGS step k >= 1 of Arnoldi step i: H(k-1,i) from the previous partials;
+// w += (-H(k-1,i)) V_{k-1}; partials of (V_k, w) for H(k,i)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *__restrict__ vprev, const T *__restrict__ vk, T *__restrict__ w,
-                                                   int hpos, KState<T> *st, T *partials)
+                                                   int hpos, KState<T> *st, const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
-    const T a = (T)(-st->H[hpos]);
+    const T a = gm_fold_h<T>(prev, prev_nb, hpos, st);
     const bool aa = a != (T)0;
     T c[2] = {(T)0, (T)0};
     GRID_STRIDE(j, n) {
@@ -241,13 +303,14 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *__restric
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
-// last MGS update w += (-H(i,i)) V_i, then hi_ip1 = |w| (:131-135)
+// H(i,i) from the previous partials; last MGS update w += (-H(i,i)) V_i,
+// then the partials of |w|^2 for hi_ip1 (:131-135)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gm_last(int64_t n, const T *__restrict__ vi, T *__restrict__ w, int hpos, KState<T> *st,
-                                                    T *partials)
+                                                    const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
-    const T a = (T)(-st->H[hpos]);
+    const T a = gm_fold_h<T>(prev, prev_nb, hpos, st);
     const bool aa = a != (T)0;
     T c[2] = {(T)0, (T)0};
     GRID_STRIDE(j, n) {
@@ -321,6 +384,7 @@ struct Solve {
     double *dhist = nullptr;
     int *dhist_it = nullptr;
     T *partials;
+    T *part[2];             // alternating partial buffers (Gmres's folded finishes)
     char *ws;
     size_t off = 0;
 
@@ -341,10 +405,13 @@ struct Solve {
         size_t hist_cap = it->hist && it->hist_cap > 0 ? (size_t)it->hist_cap : 0;
         size_t need = vec_count * vb + extra_bytes + 256 + hist_cap * (sizeof(double) + sizeof(int)) + 512;
         SHK_TRY(ensure_ws(ctx, need));
-        SHK_TRY(ensure_red(ctx, sizeof(T) * 2 * (size_t)std::max(nb, a->nblk_launch)));
+        const size_t maxb = (size_t)std::max(nb, a->nblk_launch);
+        SHK_TRY(ensure_red(ctx, sizeof(T) * 4 * maxb));  // two buffers of [2][maxb]
         ws = (char *)ctx->ws;
         off = 0;
         partials = (T *)ctx->red.partials;
+        part[0] = partials;
+        part[1] = partials + 2 * maxb;
         dst = (KState<T> *)ctx->state;
         hst = (KState<T> *)ctx->state_host;
         if (hist_cap) {
@@ -409,6 +476,22 @@ struct Solve {
         return launch_spmv<T, Fin>(A, a, fin);
     }
 
+    // The iteration loop: body(stream) enqueues one chunk of iterations on
+    // the context's stream and the state is read back after each chunk,
+    // until the solve stops or `limit` iterations have been enqueued.
+    // (Recording the chunk once into a hipGraph and replaying it was
+    // measured: no gain, the per-kernel cost is on the device side.)
+    template <class Body>
+    int run_chunks(int64_t limit, int chunk, Body body)
+    {
+        for (int64_t done_its = 0; done_its <= limit; done_its += chunk) {
+            SHK_TRY(body(ctx->stream));
+            SHK_TRY(read_state());
+            if (hst->done) break;
+        }
+        return SHK_OK;
+    }
+
     // Second half of the split reduction of the kernel just enqueued (which
     // skipped itself, and wrote no partials, when the solve had ended).
     template <typename Fin>
@@ -448,24 +531,26 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
     SHK_HIP(hipGetLastError());
     const T *dg = (const T *)A->diag;
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
-    for (int64_t done_its = 0; done_its <= limit; done_its += kChunk) {
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
         for (int k = 0; k < kChunk; k++) {
             if (jac) {
-                hipLaunchKernelGGL(k_cg_z<T>, g, blk, 0, st, n, r, dg, z, S.dst, S.partials);
+                hipLaunchKernelGGL(k_cg_z<T>, g, blk, 0, cs, n, r, dg, z, S.dst, S.partials);
                 S.fin(FinCgRho<T>{S.dst});
             }
-            hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, st, n, z, p, S.dst);
+            hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, cs, n, z, p, S.dst);
             SpmvArgs<T> a{};
             a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = p; a.skip = &S.dst->skip;
-            SHK_TRY(S.spmv(a, FinCgDelta<T>{S.dst}));
-            hipLaunchKernelGGL(k_cg_xr<T>, g, blk, 0, st, n, p, q, x, r, S.dst, S.partials);
-            S.fin(FinCgTop<T>{S.dst, 1, jac ? 0 : 1});
+            a.partials = S.part[0];
+            SHK_TRY(S.spmv(a, FinDefer{}));  // (p, q): finished inside k_cg_xr
+            hipLaunchKernelGGL(k_cg_xr<T>, g, blk, 0, cs, n, p, q, x, r, S.dst, (const T *)S.part[0],
+                               a.grid_used, S.part[1]);
+            grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip,
+                           FinCgTop<T>{S.dst, 1, jac ? 0 : 1});
         }
         SHK_HIP(hipGetLastError());
-        SHK_TRY(S.read_state());
-        if (S.hst->done) break;
-    }
+        return SHK_OK;
+    }));
     SHK_CHECK(S.hst->done, SHK_ERR_STATE, "cg: solver did not reach a stopping state");
     return S.finish(it, code);
 }
@@ -495,29 +580,33 @@ int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iterat
     S.fin(FinBiTop<T>{S.dst, 0});
     SHK_HIP(hipGetLastError());
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
-    int slot = 0;
-    for (int64_t loops = 0; loops <= limit; loops += kChunk) {
-        for (int k = 0; k < kChunk; k++, slot++) {
-            hipLaunchKernelGGL(k_bi_p<T>, g, blk, 0, st, n, r, v, p, dg, phat, S.dst);
+    // a loop's slot (its place in the chunk) names the Finished(s) that
+    // stopped the solve; the chunk is replayed, so slots repeat across
+    // chunks, but a stopped solve replays no further chunk
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int slot = 0; slot < kChunk; slot++) {
+            hipLaunchKernelGGL(k_bi_p<T>, g, blk, 0, cs, n, r, v, p, dg, phat, S.dst);
             SpmvArgs<T> a{};
             a.x = phat; a.y_out = v; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = rt; a.skip = &S.dst->skip;
-            SHK_TRY(S.spmv(a, FinBiSigma<T>{S.dst}));
-            hipLaunchKernelGGL(k_bi_s<T>, g, blk, 0, st, n, r, v, s, dg, shat, S.dst, S.partials);
-            S.fin(FinBiS<T>{S.dst, slot});
-            hipLaunchKernelGGL(k_bi_fix<T>, g, blk, 0, st, n, phat, x, S.dst, slot);
+            a.partials = S.part[0];
+            SHK_TRY(S.spmv(a, FinDefer{}));  // sigma: finished inside k_bi_s
+            hipLaunchKernelGGL(k_bi_s<T>, g, blk, 0, cs, n, r, v, s, dg, shat, S.dst, (const T *)S.part[0],
+                               a.grid_used, S.part[1]);
+            grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip, FinBiS<T>{S.dst, slot});
+            hipLaunchKernelGGL(k_bi_fix<T>, g, blk, 0, cs, n, phat, x, S.dst, slot);
             SpmvArgs<T> b2{};
             b2.x = shat; b2.y_out = t; b2.alpha = (T)1; b2.mlt = 1;
             b2.dot0 = kDotVec; b2.u0 = s; b2.dot1 = kDotSelf; b2.skip = &S.dst->skip;
-            SHK_TRY(S.spmv(b2, FinBiOmega<T>{S.dst}));
-            hipLaunchKernelGGL(k_bi_xr<T>, g, blk, 0, st, n, phat, shat, s, t, rt, x, r, S.dst,
-                               S.partials);
-            S.fin(FinBiTop<T>{S.dst, 1});
+            b2.partials = S.part[0];
+            SHK_TRY(S.spmv(b2, FinDefer{}));  // omega: finished inside k_bi_xr
+            hipLaunchKernelGGL(k_bi_xr<T>, g, blk, 0, cs, n, phat, shat, s, t, rt, x, r, S.dst,
+                               (const T *)S.part[0], b2.grid_used, S.part[1]);
+            grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip, FinBiTop<T>{S.dst, 1});
         }
         SHK_HIP(hipGetLastError());
-        SHK_TRY(S.read_state());
-        if (S.hst->done) break;
-    }
+        return SHK_OK;
+    }));
     SHK_CHECK(S.hst->done, SHK_ERR_STATE, "bicgstab: solver did not reach a stopping state");
     return S.finish(it, code);
 }
@@ -573,15 +662,22 @@ int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
                 SpmvArgs<T> a{};
                 a.x = Vk(i); a.y_out = w; a.alpha = (T)1; a.mlt = 1; a.diag = dg;
                 a.dot0 = kDotVec; a.u0 = Vk(0); a.skip = &S.dst->skip;
-                SHK_TRY(S.spmv(a, FinGmH<T>{S.dst, 0, i}));
+                // each dot's finish is folded into the next kernel; the two
+                // partial buffers alternate so a kernel never overwrites the
+                // partials it is still reading
+                a.partials = S.part[0];
+                SHK_TRY(S.spmv(a, FinDefer{}));
+                int prev_nb = a.grid_used;
                 for (int k = 1; k <= i; k++) {
-                    hipLaunchKernelGGL(k_gm_mgs<T>, g, blk, 0, st, n, (const T *)Vk(k - 1),
-                                       (const T *)Vk(k), w, i * (m + 1) + (k - 1), S.dst, S.partials);
-                    S.fin(FinGmH<T>{S.dst, k, i});
+                    hipLaunchKernelGGL(k_gm_mgs<T>, g, blk, 0, st, n, (const T *)Vk(k - 1), (const T *)Vk(k), w,
+                                       i * (m + 1) + (k - 1), S.dst, (const T *)S.part[(k - 1) & 1], prev_nb,
+                                       S.part[k & 1]);
+                    prev_nb = S.nb;
                 }
-                hipLaunchKernelGGL(k_gm_last<T>, g, blk, 0, st, n, (const T *)Vk(i), w,
-                                   i * (m + 1) + i, S.dst, S.partials);
-                S.fin(FinGmStep<T>{S.dst, i});
+                hipLaunchKernelGGL(k_gm_last<T>, g, blk, 0, st, n, (const T *)Vk(i), w, i * (m + 1) + i, S.dst,
+                                   (const T *)S.part[i & 1], prev_nb, S.part[(i + 1) & 1]);
+                grid_finish<T>(st, (const T *)S.part[(i + 1) & 1], S.nb, (const int *)&S.dst->skip,
+                               FinGmStep<T>{S.dst, i});
                 hipLaunchKernelGGL(k_gm_vnext<T>, g, blk, 0, st, n, (const T *)w, Vk(i + 1), S.dst);
             }
             SHK_HIP(hipGetLastError());
