@@ -259,7 +259,8 @@ def convdiff_solves(K, wl, np, torch):
     """FP64 BiCgStab and Gmres(30) with the Jacobi preconditioner on the
     128^3 7-point upwind convection-diffusion matrix (2,097,152 rows), to a
     1e-10 relative residual: time to solution and iterations/s, plus the
-    SpMV bandwidth on this matrix."""
+    SpMV bandwidth on this matrix and the time of one symmetric SOR sweep
+    (with the oracle's sequential sweep on one core beside it)."""
     import ctypes
 
     from seldon_amd import _lib as L
@@ -274,6 +275,21 @@ def convdiff_solves(K, wl, np, torch):
     sb = Ad.spmv_bytes(False)
     out["spmv"] = {"kernel_ms": round(avg.value, 5), "GBs": round(sb / (avg.value * 1e-3) / 1e9, 2),
                    "frac": round(sb / (avg.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "bytes": sb}
+    # one symmetric SOR sweep (forward then backward, omega 1), level-scheduled
+    x = torch.zeros_like(bg)
+    K.SOR(Ad, x, bg, 1.0, 1, 0)  # builds the level plan
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        K.SOR(Ad, x, bg, 1.0, 1, 0)
+    torch.cuda.synchronize()
+    st = (time.perf_counter() - t0) / 5
+    from oracle import oracle as orc
+    t0 = time.perf_counter()
+    orc.sor(A, np.zeros(A.m), b, 1.0, 1, 0)
+    ct = time.perf_counter() - t0
+    out["ssor_sweep"] = {"seconds": round(st, 5), "cpu_oracle_seconds": round(ct, 4), "cores": 1}
+    log(f"convdiff128 SSOR sweep: {st * 1e3:.2f} ms (oracle, 1 core: {ct * 1e3:.1f} ms)")
     for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30)):
         x = torch.zeros_like(bg)
         it = K.Iteration(20000, 1e-10)

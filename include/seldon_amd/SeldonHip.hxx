@@ -408,6 +408,55 @@ namespace Seldon
              "Mlt(SeldonTrans)");
   }
 
+  // SOR(A, X, B, omega, iter, type_ssor) (matrix_sparse/Relaxation_MatVect.cxx:
+  // 41-95 RowSparse, :106-167 ArrayRowSparse), level-scheduled on the GPU and
+  // bitwise Seldon's sweeps; with it Seldon's own SorPreconditioner
+  // (Precond_Ssor.cxx), which calls Seldon::SOR, runs on the GPU storages.
+  template <class T0, class Prop0, class Storage0, class Allocator0, class T1, class Allocator1, class T2,
+            class Allocator2, class T3>
+  typename std::enable_if<IsHipSparse<Storage0>::value>::type
+  SOR(const Matrix<T0, Prop0, Storage0, Allocator0>& A, Vector<T2, VectHip, Allocator2>& X,
+      const Vector<T1, VectHip, Allocator1>& B, const T3& omega, int iter, int type_ssor = 2)
+  {
+    if (X.GetM() != A.GetM() || B.GetM() != A.GetM())
+      throw WrongDim("SOR", "Matrix and vector dimensions are incompatible.");
+    HipCheck(shk_sor(HipContext::Get().Handle(), A.GetHandle(), X.GetData(), B.GetData(), double(omega), iter,
+                     type_ssor),
+             "SOR");
+  }
+
+#ifdef SELDON_FILE_PRECOND_SSOR_CXX
+  // Seldon's SorPreconditioner calls Seldon::SOR qualified
+  // (Precond_Ssor.cxx:67-90), which binds to the overloads declared before it;
+  // its Solve / TransSolve are therefore specialised here for the GPU
+  // storages, with the same steps: z.Zero() when init_guess_null, then SOR
+  // with type 0 (symmetric) or 2 / 3 (forward / backward).
+#define SELDON_HIP_SOR_PRECOND(T, S)                                                                     \
+  template <>                                                                                           \
+  template <>                                                                                           \
+  inline void SorPreconditioner<T>::Solve(const Matrix<T, General, S, HipAlloc<T> >& A,                 \
+                                          const Vector<T, VectHip, HipAlloc<T> >& r,                    \
+                                          Vector<T, VectHip, HipAlloc<T> >& z, bool init_guess_null)    \
+  {                                                                                                     \
+    if (init_guess_null) z.Zero();                                                                      \
+    SOR(A, z, r, omega, nb_iter, symmetric_precond ? 0 : 2);                                           \
+  }                                                                                                     \
+  template <>                                                                                           \
+  template <>                                                                                           \
+  inline void SorPreconditioner<T>::TransSolve(const Matrix<T, General, S, HipAlloc<T> >& A,            \
+                                               const Vector<T, VectHip, HipAlloc<T> >& r,               \
+                                               Vector<T, VectHip, HipAlloc<T> >& z, bool init_guess_null) \
+  {                                                                                                     \
+    if (init_guess_null) z.Zero();                                                                      \
+    SOR(A, z, r, omega, nb_iter, symmetric_precond ? 0 : 3);                                           \
+  }
+  SELDON_HIP_SOR_PRECOND(double, RowSparseHip)
+  SELDON_HIP_SOR_PRECOND(double, ArrayRowSparseHip)
+  SELDON_HIP_SOR_PRECOND(float, RowSparseHip)
+  SELDON_HIP_SOR_PRECOND(float, ArrayRowSparseHip)
+#undef SELDON_HIP_SOR_PRECOND
+#endif
+
   // Swap(X, Y): exchanges the two vectors' storage (Functions_Vector.cxx:295-304)
   template <class T, class Allocator>
   void Swap(Vector<T, VectHip, Allocator>& X, Vector<T, VectHip, Allocator>& Y)

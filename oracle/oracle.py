@@ -67,6 +67,7 @@ def lib():
             getattr(L, f"orc_{sfx}_mlt").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
             getattr(L, f"orc_{sfx}_mltadd_trans").argtypes = [T, ctypes.POINTER(OrcMat), vp, T, vp]
             getattr(L, f"orc_{sfx}_mlt_trans").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
+            getattr(L, f"orc_{sfx}_sor").argtypes = [ctypes.POINTER(OrcMat), vp, vp, T, ctypes.c_int, ctypes.c_int]
             getattr(L, f"orc_{sfx}_coo_to_csr").argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int,
                                                             ctypes.c_int, vp, vp, vp]
             getattr(L, f"orc_{sfx}_dot").argtypes = [ctypes.c_int, vp, vp]
@@ -150,6 +151,15 @@ def mlt_trans(A, x, storage=None, dtype=np.float64):
     y = np.empty(A.m if st == ROWSYMSPARSE else A.n, dtype)
     getattr(lib(), f"orc_{_sfx(dtype)}_mlt_trans")(ctypes.byref(M.s), _p(x), _p(y))
     return y
+
+
+def sor(A, x, b, omega, iter, type_ssor=2, storage=None, dtype=np.float64):
+    """SOR(A, X, B, omega, iter, type_ssor) on a copy of x; returns it."""
+    M = _Mat(A, storage_of(A, storage), dtype)
+    x = np.array(x, dtype=dtype, copy=True)
+    b = np.ascontiguousarray(b, dtype)
+    getattr(lib(), f"orc_{_sfx(dtype)}_sor")(ctypes.byref(M.s), _p(x), _p(b), omega, iter, type_ssor)
+    return x
 
 
 def coo_to_csr(row, col, val, index=0, m_min=0, n_min=0, dtype=np.float64):

@@ -199,6 +199,38 @@ void FN(mlt_trans)(const orc_mat *A, const T *x, T *y)
     free(er); free(ec); free(ev); free(cnt); free(t1); free(t2); free(cptr);
 }
 
+/* SOR(A, X, B, omega, iter, type_ssor): matrix_sparse/Relaxation_MatVect.cxx:
+ * 41-95 (RowSparse: ajj = A(j, j), the stored entry or 0) and :106-167
+ * (ArrayRowSparse: ajj = 0 plus the row's diagonal entries).  type_ssor % 2
+ * == 0: iter forward sweeps; then type_ssor % 3 == 0: iter backward sweeps. */
+void FN(sor)(const orc_mat *A, T *x, const T *b, T omega, int iter, int type_ssor)
+{
+    const int ma = A->m;
+    const int *ptr = A->ptr, *ind = A->ind;
+    const T *data = (const T *)A->val;
+    for (int d = 0; d < 2; d++) {
+        if (d == 0 && type_ssor % 2 != 0) continue;
+        if (d == 1 && type_ssor % 3 != 0) continue;
+        for (int it = 0; it < iter; it++)
+            for (int q = 0; q < ma; q++) {
+                const int j = d == 0 ? q : ma - 1 - q;
+                T temp = 0, ajj = 0;
+                if (A->storage == ORC_ARRAYROWSPARSE) {
+                    for (int k = ptr[j]; k < ptr[j + 1]; k++) {
+                        temp += data[k] * x[ind[k]];
+                        if (ind[k] == j) ajj += data[k];
+                    }
+                } else {
+                    for (int k = ptr[j]; k < ptr[j + 1]; k++)
+                        if (ind[k] == j) { ajj = data[k]; break; }
+                    for (int k = ptr[j]; k < ptr[j + 1]; k++) temp += data[k] * x[ind[k]];
+                }
+                temp = b[j] - temp + ajj * x[j];
+                x[j] = ((T)1 - omega) * x[j] + omega * temp / ajj;
+            }
+    }
+}
+
 /* MergeSort(m, n, tab1, tab2, tab3) of vector/Functions_Arrays.cxx:412-480:
  * bottom-up, runs of inc, the right run's element taken only when strictly
  * smaller (so the sort is stable); tab1 the keys. */

@@ -57,6 +57,11 @@ struct RedSlot {
 
 }  // namespace shk
 
+namespace shk {
+struct SorPlan;  // level-scheduled SOR sweeps (shk_sor.hip)
+void sor_free(SorPlan *P);
+}  // namespace shk
+
 struct shk_ctx_s {
     int device = 0;
     hipStream_t own = nullptr;
@@ -117,6 +122,9 @@ struct shk_csr_s {
     // first use as an ArrayRowSparse matrix whose rows list, for each column
     // c of A, the entries (i, c) in increasing i
     shk_csr_s *trans = nullptr;
+    // SOR / SSOR sweeps: the caller's CSR and the row levels of both sweep
+    // directions, built on first use
+    shk::SorPlan *sor = nullptr;
 };
 
 namespace shk {
@@ -229,6 +237,10 @@ inline int red_blocks(int64_t n, int items_per_thread = 8)
 // ----------------------------------------------------- host helpers ----
 int ensure_red(shk_ctx ctx, size_t bytes);
 int ensure_ws(shk_ctx ctx, size_t bytes);
+// The caller's stored arrays (RowSymSparse: upper part) read back from the
+// device, in the caller's order (shk_spmv.hip; T = double, float).
+template <typename T>
+int stored_arrays(shk_csr A, std::vector<int> &ptr, std::vector<int> &ind, std::vector<T> &val);
 int ensure_ws2(shk_ctx ctx, size_t bytes);
 int check_device_ptr(const void *p, const char *what);
 

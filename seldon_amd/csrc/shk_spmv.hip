@@ -658,6 +658,14 @@ static int stored_download(shk_csr A, std::vector<int> &ptr, std::vector<int> &i
     return SHK_OK;
 }
 
+template <typename T>
+int stored_arrays(shk_csr A, std::vector<int> &ptr, std::vector<int> &ind, std::vector<T> &val)
+{
+    return stored_download<T>(A, ptr, ind, val);
+}
+template int stored_arrays<double>(shk_csr, std::vector<int> &, std::vector<int> &, std::vector<double> &);
+template int stored_arrays<float>(shk_csr, std::vector<int> &, std::vector<int> &, std::vector<float> &);
+
 static int ensure_trans(shk_csr A)
 {
     if (A->trans) return SHK_OK;
@@ -788,6 +796,7 @@ int shk_csr_destroy(shk_csr A)
     if (!A) return SHK_OK;
     if (A->trans) shk_csr_destroy(A->trans);
     if (A->ctx) hipStreamSynchronize(A->ctx->stream);
+    sor_free(A->sor);
     hipFree(A->ptr);
     hipFree(A->mid);
     hipFree(A->ind);
@@ -812,6 +821,11 @@ int shk_csr_set_values(shk_ctx ctx, shk_csr A, const void *val, int src_on_devic
     if (A->trans) {  // rebuilt from the new values when next needed
         shk_csr_destroy(A->trans);
         A->trans = nullptr;
+    }
+    if (A->sor) {
+        hipStreamSynchronize(ctx->stream);
+        sor_free(A->sor);
+        A->sor = nullptr;
     }
     if (A->nnz_stored == 0) return SHK_OK;
     const void *dsrc = val;

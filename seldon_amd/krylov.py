@@ -285,6 +285,13 @@ def ConvertMatrix_from_Coordinates(IndRow: torch.Tensor, IndCol: torch.Tensor, V
     return mm.value, nn.value, ptr, ind[:nnz], out[:nnz]
 
 
+def SOR(A: Matrix, X: torch.Tensor, B: torch.Tensor, omega, iter: int, type_ssor: int = 2):
+    """SOR(A, X, B, omega, iter, type_ssor): 2 forward sweeps, 3 backward, 0
+    both (Relaxation_MatVect.cxx:41-95, :106-167); X updated in place,
+    bitwise as Seldon's sequential sweeps (RowSparse, ArrayRowSparse)."""
+    L.check(L.load().shk_sor(A.ctx.h, A.h, _ptr(X), _ptr(B), float(omega), int(iter), int(type_ssor)))
+
+
 def DotProd(X, Y) -> float:
     r = ctypes.c_double()
     L.check(L.load().shk_dot(Context.default().h, _DT[X.dtype], X.numel(), _ptr(X), _ptr(Y), ctypes.byref(r)))

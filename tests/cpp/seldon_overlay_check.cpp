@@ -11,9 +11,10 @@
 // then, on a 2-D convection-diffusion system (nonsymmetric), the solvers that
 // use the transposed product -- BiCg, Qmr, Cgne, Lsqr -- and Cgs, QCgs,
 // TfQmr, Gcr, BiCgStabl, and on the Laplacian the symmetric ones -- MinRes,
-// Symmlq, CoCg, BiCgcr, QmrSym -- with Seldon's own templates, once on the
-// CPU storages and once on the GPU storages (Mlt(SeldonTrans, ...), Swap and
-// the rest through the overload set),
+// Symmlq, CoCg, BiCgcr, QmrSym, and Cg with Seldon's SorPreconditioner
+// (SSOR) -- with Seldon's own templates, once on the CPU storages and once on
+// the GPU storages (Mlt(SeldonTrans, ...), SOR, Swap and the rest through the
+// overload set),
 // and prints each residual history (Iteration::Finished lines) after a
 // "== <label>" marker, then "ERRCODE e NBITER n".
 #include "Seldon.hxx"
@@ -86,6 +87,24 @@ static void run_trans_solver(const string& which, const string& label, Mat& A, c
   else if (which == "cocg") code = CoCg(A, x, b, M, it);
   else if (which == "bicgcr") code = BiCgcr(A, x, b, M, it);
   else code = QmrSym(A, x, b, M, it);
+  cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+}
+
+// Cg / BiCgStab with Seldon's own SorPreconditioner (SSOR, one sweep each
+// way, omega 1.2), whose Solve calls Seldon::SOR: on the GPU storages that is
+// the level-scheduled device sweep
+template <class Mat, class Vec>
+static void run_ssor_solver(const string& which, const string& label, Mat& A, const Vec& b, int maxit, double tol)
+{
+  cout << "== " << which << "_" << label << endl;
+  Vec x(b);
+  x.Fill(0.0);
+  Iteration<double> it(maxit, tol);
+  it.ShowFullHistory();
+  SorPreconditioner<double> M;
+  M.SetParameterRelaxation(1.2);
+  M.SetNumberIterations(1);
+  int code = which == "ssor_cg" ? Cg(A, x, b, M, it) : BiCgStab(A, x, b, M, it);
   cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
 }
 
@@ -175,6 +194,8 @@ int main(int argc, char** argv)
         run_trans_solver(solvers[k], "cpu", A, bh, 2000, 1e-8);
         run_trans_solver(solvers[k], "gpu", Ag, bg, 2000, 1e-8);
       }
+    run_ssor_solver("ssor_cg", "cpu", A, bh, 2000, 1e-10);
+    run_ssor_solver("ssor_cg", "gpu", Ag, bg, 2000, 1e-10);
   }
   return 0;
 }

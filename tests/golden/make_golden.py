@@ -218,6 +218,38 @@ def make_spmvt():
     print("spmvt fixtures:", len(out))
 
 
+SOR_CASES = [(1.0, 1, 2), (1.3, 2, 2), (1.3, 2, 3), (0.8, 1, 0), (1.3, 2, 0)]  # (omega, iter, type_ssor)
+
+
+def sor_case():
+    """A 512-row 3-D convection-diffusion matrix (nonsymmetric) and a 400-row
+    power-law matrix given a dominant diagonal, with seeded x0 and b."""
+    C = wl.convdiff3d(8)
+    P = wl.powerlaw(400, 17, kmin=2, kmax=30, half_width=60)
+    return {"convdiff8": C, "powerlaw400": P}
+
+
+def make_sor():
+    """SOR(A, X, B, omega, iter, type_ssor) from the reference build."""
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, A in sor_case().items():
+            x = wl.uniform(A.m, 51)
+            b = wl.uniform(A.m, 52)
+            for dt, c in ((np.float64, "d"), (np.float32, "f")):
+                orc.write_seldon_csr(f"{tmp}/A.bin", A, dt)
+                orc.write_seldon_vec(f"{tmp}/x.bin", x.astype(dt))
+                orc.write_seldon_vec(f"{tmp}/b.bin", b.astype(dt))
+                for st in ("row", "array"):
+                    for k, (om, it, ty) in enumerate(SOR_CASES):
+                        orc.run_ref(["sor", st, c, f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/b.bin", repr(om), it, ty,
+                                     f"{tmp}/o.bin"])
+                        out[f"{name}_{st}_{c}_{k}"] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
+    np.savez_compressed(os.path.join(OUT, "sor.npz"), spec=json.dumps({"cases": SOR_CASES, "xseed": 51, "bseed": 52}),
+                        **out)
+    print("sor fixtures:", len(out))
+
+
 COO_CASES = {
     # name: (seed, nnz, rows, cols, duplicates, index, m_min, n_min)
     "dup": (31, 4000, 300, 200, 400, 1, 0, 250),
@@ -268,9 +300,11 @@ if __name__ == "__main__":
         orc.build()
         import subprocess
         subprocess.run(["make", "-s", "-C", orc.HERE, "ref"], check=True)
-    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "solves"]
+    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "sor", "solves"]
     if "coo" in which:
         make_coo()
+    if "sor" in which:
+        make_sor()
     if "spmv" in which:
         make_spmv()
     if "spmvt" in which:

@@ -266,3 +266,46 @@ def test_spmv_trans_follows_new_values(gpu):
     y2 = torch.empty(A.m, dtype=torch.float64, device="cuda")
     K.Mlt(K.SeldonNoTrans, Ag, _dev(wl.uniform(A.n, 42), np.float64), y2)
     assert _same(y2.cpu().numpy(), orc.mlt(B, wl.uniform(A.n, 42)))
+
+
+# ---------------------------------------------------------------- SOR --
+
+@pytest.mark.parametrize("layout", ["rows", "auto"])
+@pytest.mark.parametrize("st", ["row", "array"])
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_sor_golden_fixture(gpu, st, c, layout):
+    """SOR / SSOR sweeps (level-scheduled) bitwise against the real Seldon."""
+    from seldon_amd import krylov as K
+    from tests.golden.make_golden import SOR_CASES, sor_case
+    fx = load_fixture("sor.npz")
+    tdt, ndt = DT[c]
+    for name, A in sor_case().items():
+        Ag = K.Matrix(A, storage=st, dtype=tdt, layout=layout)
+        x = wl.uniform(A.m, fx["spec"]["xseed"])
+        bg = _dev(wl.uniform(A.m, fx["spec"]["bseed"]), ndt)
+        for k, (om, it, ty) in enumerate(SOR_CASES):
+            xg = _dev(x, ndt)
+            K.SOR(Ag, xg, bg, om, it, ty)
+            assert _same(xg.cpu().numpy(), fx[f"{name}_{st}_{c}_{k}"]), (name, st, c, om, it, ty)
+
+
+@pytest.mark.parametrize("st", ["row", "array"])
+def test_sor_bitwise_vs_oracle(gpu, st):
+    """Larger matrices (a stencil with few long levels, a banded random one
+    with many short ones), and new values dropping the level plan."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    for name, A in (("laplace96", wl.laplace2d(96)), ("convdiff24", wl.convdiff3d(24)),
+                    ("powerlaw6k", wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500))):
+        Ag = K.Matrix(A, storage=st)
+        x = wl.uniform(A.m, 61)
+        b = wl.uniform(A.m, 62)
+        print(f"[sor] {name} {st}", flush=True)
+        xg, bg = _dev(x, np.float64), _dev(b, np.float64)
+        K.SOR(Ag, xg, bg, 1.25, 2, 0)
+        assert _same(xg.cpu().numpy(), orc.sor(A, x, b, 1.25, 2, 0, storage=st)), (name, st)
+        B = wl.CSR(A.m, A.n, A.ptr, A.ind, A.val * 1.5)
+        Ag.set_values(B.val)
+        xg = _dev(x, np.float64)
+        K.SOR(Ag, xg, bg, 0.9, 1, 2)
+        assert _same(xg.cpu().numpy(), orc.sor(B, x, b, 0.9, 1, 2, storage=st)), (name, st, "new values")

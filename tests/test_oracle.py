@@ -262,6 +262,57 @@ def test_oracle_coo_to_csr_vs_live_seldon():
             assert ref[:2] == got[:2] and all(np.array_equal(a, b) for a, b in zip(ref[2:], got[2:]))
 
 
+@pytest.mark.parametrize("c", ["d", "f"])
+@pytest.mark.parametrize("st", ["row", "array"])
+def test_oracle_sor_matches_seldon_fixture(st, c):
+    """SOR / SSOR sweeps of the real Seldon build (forward, backward, both;
+    one and two sweeps; several omega), bitwise."""
+    from tests.golden.make_golden import SOR_CASES, sor_case
+    fx = load_fixture("sor.npz")
+    dt = np.float64 if c == "d" else np.float32
+    for name, A in sor_case().items():
+        x = wl.uniform(A.m, fx["spec"]["xseed"]).astype(dt)
+        b = wl.uniform(A.m, fx["spec"]["bseed"]).astype(dt)
+        for k, (om, it, ty) in enumerate(SOR_CASES):
+            got = orc.sor(A, x, b, dt(om), it, ty, storage=st, dtype=dt)
+            assert np.array_equal(got, fx[f"{name}_{st}_{c}_{k}"]), (name, st, c, om, it, ty)
+
+
+def test_oracle_sor_closed_forms():
+    """With omega = 1 one forward sweep on a lower-triangular matrix is the
+    forward substitution (numpy's triangular solve to rounding), and
+    symmetric sweeps converge to A^-1 b on an SPD matrix."""
+    import scipy.linalg as sla
+    A = wl.laplace2d(12)
+    D = A.to_scipy().toarray()
+    Lo = np.tril(D)
+    rows, cols = np.nonzero(Lo)
+    T = wl.CSR(A.m, A.n, np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=A.m))]).astype(np.int32),
+               cols.astype(np.int32), Lo[rows, cols])
+    b = wl.uniform(A.m, 3)
+    x = orc.sor(T, np.zeros(A.m), b, 1.0, 1, 2)
+    ref = sla.solve_triangular(Lo, b, lower=True)
+    assert np.abs(x - ref).max() <= 1e-12 * np.abs(ref).max()
+    x = orc.sor(A, np.zeros(A.m), b, 1.5, 400, 0)
+    assert np.abs(x - np.linalg.solve(D, b)).max() < 1e-8
+
+
+@pytest.mark.skipif(not orc.ref_available("generic"), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_sor_vs_live_seldon():
+    with tempfile.TemporaryDirectory() as tmp:
+        for seed in range(2):
+            A = wl.powerlaw(3000 + 1000 * seed, 70 + seed, kmin=2, kmax=40, half_width=300)
+            x = wl.uniform(A.m, 80 + seed)
+            b = wl.uniform(A.m, 90 + seed)
+            orc.write_seldon_csr(f"{tmp}/A.bin", A)
+            orc.write_seldon_vec(f"{tmp}/x.bin", x)
+            orc.write_seldon_vec(f"{tmp}/b.bin", b)
+            for st in ("row", "array"):
+                orc.run_ref(["sor", st, "d", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/b.bin", "1.2", 3, 0,
+                             f"{tmp}/o.bin"])
+                assert np.array_equal(orc.read_seldon_vec(f"{tmp}/o.bin"), orc.sor(A, x, b, 1.2, 3, 0, storage=st))
+
+
 def test_laplacian_eigenpairs():
     g = 40
     A = wl.laplace2d(g)
