@@ -409,7 +409,8 @@ namespace Seldon
   }
 
   // SOR(A, X, B, omega, iter, type_ssor) (matrix_sparse/Relaxation_MatVect.cxx:
-  // 41-95 RowSparse, :106-167 ArrayRowSparse), level-scheduled on the GPU and
+  // 41-95 RowSparse, :106-167 ArrayRowSparse, :179-300 RowSymSparse),
+  // level-scheduled on the GPU and
   // bitwise Seldon's sweeps; with it Seldon's own SorPreconditioner
   // (Precond_Ssor.cxx), which calls Seldon::SOR, runs on the GPU storages.
   template <class T0, class Prop0, class Storage0, class Allocator0, class T1, class Allocator1, class T2,
@@ -420,8 +421,10 @@ namespace Seldon
   {
     if (X.GetM() != A.GetM() || B.GetM() != A.GetM())
       throw WrongDim("SOR", "Matrix and vector dimensions are incompatible.");
-    HipCheck(shk_sor(HipContext::Get().Handle(), A.GetHandle(), X.GetData(), B.GetData(), double(omega), iter,
-                     type_ssor),
+    // the relaxation step runs in the type C++ gives (value, omega)
+    typedef typename std::common_type<T2, T3>::type W;
+    HipCheck(shk_sor(HipContext::Get().Handle(), A.GetHandle(), X.GetData(), B.GetData(), double(omega),
+                     std::is_same<W, float>::value ? SHK_F32 : SHK_F64, iter, type_ssor),
              "SOR");
   }
 
@@ -431,10 +434,10 @@ namespace Seldon
   // its Solve / TransSolve are therefore specialised here for the GPU
   // storages, with the same steps: z.Zero() when init_guess_null, then SOR
   // with type 0 (symmetric) or 2 / 3 (forward / backward).
-#define SELDON_HIP_SOR_PRECOND(T, S)                                                                     \
+#define SELDON_HIP_SOR_PRECOND(T, P, S)                                                                  \
   template <>                                                                                           \
   template <>                                                                                           \
-  inline void SorPreconditioner<T>::Solve(const Matrix<T, General, S, HipAlloc<T> >& A,                 \
+  inline void SorPreconditioner<T>::Solve(const Matrix<T, P, S, HipAlloc<T> >& A,                       \
                                           const Vector<T, VectHip, HipAlloc<T> >& r,                    \
                                           Vector<T, VectHip, HipAlloc<T> >& z, bool init_guess_null)    \
   {                                                                                                     \
@@ -443,17 +446,19 @@ namespace Seldon
   }                                                                                                     \
   template <>                                                                                           \
   template <>                                                                                           \
-  inline void SorPreconditioner<T>::TransSolve(const Matrix<T, General, S, HipAlloc<T> >& A,            \
+  inline void SorPreconditioner<T>::TransSolve(const Matrix<T, P, S, HipAlloc<T> >& A,                  \
                                                const Vector<T, VectHip, HipAlloc<T> >& r,               \
                                                Vector<T, VectHip, HipAlloc<T> >& z, bool init_guess_null) \
   {                                                                                                     \
     if (init_guess_null) z.Zero();                                                                      \
     SOR(A, z, r, omega, nb_iter, symmetric_precond ? 0 : 3);                                           \
   }
-  SELDON_HIP_SOR_PRECOND(double, RowSparseHip)
-  SELDON_HIP_SOR_PRECOND(double, ArrayRowSparseHip)
-  SELDON_HIP_SOR_PRECOND(float, RowSparseHip)
-  SELDON_HIP_SOR_PRECOND(float, ArrayRowSparseHip)
+  SELDON_HIP_SOR_PRECOND(double, General, RowSparseHip)
+  SELDON_HIP_SOR_PRECOND(double, General, ArrayRowSparseHip)
+  SELDON_HIP_SOR_PRECOND(double, Symmetric, RowSymSparseHip)
+  SELDON_HIP_SOR_PRECOND(float, General, RowSparseHip)
+  SELDON_HIP_SOR_PRECOND(float, General, ArrayRowSparseHip)
+  SELDON_HIP_SOR_PRECOND(float, Symmetric, RowSymSparseHip)
 #undef SELDON_HIP_SOR_PRECOND
 #endif
 

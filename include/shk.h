@@ -162,14 +162,17 @@ int shk_mlt_dot(shk_ctx ctx, shk_csr A, const void *dx, void *dy, double *result
 
 /* ---- relaxation --------------------------------------------------------- */
 /* SOR(A, X, B, omega, iter, type_ssor): type_ssor 2 forward sweeps, 3
- * backward, 0 both (all forward sweeps, then all backward ones), each row
- *   X(j) = (1 - omega) X(j) + omega (B(j) - sum_k a_jk X(k) + a_jj X(j)) / a_jj
- * (matrix_sparse/Relaxation_MatVect.cxx:41-95 RowSparse, :106-167
- * ArrayRowSparse), bitwise: rows run level by level (no row of a level reads
- * another's X).  The levels are built on the first call and kept with A.
- * Seldon's SorPreconditioner (computation/solver/preconditioner/
- * Precond_Ssor.cxx:24-90) calls this with type 0 (symmetric) or 2 / 3. */
-int shk_sor(shk_ctx ctx, shk_csr A, void *dx, const void *db, double omega, int iter, int type_ssor);
+ * backward, 0 both (all forward sweeps, then all backward ones), bitwise
+ * Seldon's (matrix_sparse/Relaxation_MatVect.cxx:41-95 RowSparse, :106-167
+ * ArrayRowSparse, :179-300 RowSymSparse).  Rows run level by level (no row
+ * of a level reads another's X); the levels are built on the first call and
+ * kept with A.  omega_type is the C++ type of Seldon's omega argument:
+ * SHK_F64 with float values makes each relaxation step double, as C++
+ * promotion does; SHK_F32 keeps it in float.  Seldon's SorPreconditioner
+ * (computation/solver/preconditioner/Precond_Ssor.cxx:24-90) calls SOR with
+ * type 0 (symmetric) or 2 / 3 and omega in its value type. */
+int shk_sor(shk_ctx ctx, shk_csr A, void *dx, const void *db, double omega, shk_dtype omega_type, int iter,
+            int type_ssor);
 
 /* ---- assembly ----------------------------------------------------------- */
 /* ConvertMatrix_from_Coordinates(IndRow, IndCol, Val, A, index) for RowSparse

@@ -67,7 +67,8 @@ def lib():
             getattr(L, f"orc_{sfx}_mlt").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
             getattr(L, f"orc_{sfx}_mltadd_trans").argtypes = [T, ctypes.POINTER(OrcMat), vp, T, vp]
             getattr(L, f"orc_{sfx}_mlt_trans").argtypes = [ctypes.POINTER(OrcMat), vp, vp]
-            getattr(L, f"orc_{sfx}_sor").argtypes = [ctypes.POINTER(OrcMat), vp, vp, T, ctypes.c_int, ctypes.c_int]
+            getattr(L, f"orc_{sfx}_sor").argtypes = [ctypes.POINTER(OrcMat), vp, vp, ctypes.c_double,
+                                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
             getattr(L, f"orc_{sfx}_coo_to_csr").argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int,
                                                             ctypes.c_int, vp, vp, vp]
             getattr(L, f"orc_{sfx}_dot").argtypes = [ctypes.c_int, vp, vp]
@@ -153,12 +154,15 @@ def mlt_trans(A, x, storage=None, dtype=np.float64):
     return y
 
 
-def sor(A, x, b, omega, iter, type_ssor=2, storage=None, dtype=np.float64):
-    """SOR(A, X, B, omega, iter, type_ssor) on a copy of x; returns it."""
+def sor(A, x, b, omega, iter, type_ssor=2, storage=None, dtype=np.float64, omega_dtype=None):
+    """SOR(A, X, B, omega, iter, type_ssor) on a copy of x; returns it.
+    omega_dtype: the C++ type of omega (default: the value type); float64
+    with float32 values makes the relaxation step double, as in C++."""
     M = _Mat(A, storage_of(A, storage), dtype)
     x = np.array(x, dtype=dtype, copy=True)
     b = np.ascontiguousarray(b, dtype)
-    getattr(lib(), f"orc_{_sfx(dtype)}_sor")(ctypes.byref(M.s), _p(x), _p(b), omega, iter, type_ssor)
+    wide = int(np.dtype(dtype) == np.float32 and omega_dtype is not None and np.dtype(omega_dtype) == np.float64)
+    getattr(lib(), f"orc_{_sfx(dtype)}_sor")(ctypes.byref(M.s), _p(x), _p(b), float(omega), wide, iter, type_ssor)
     return x
 
 

@@ -69,8 +69,8 @@ void orc_coo_dims(int nnz, const int *row, const int *col, int index, int m_min,
     void orc_##SFX##_mltadd_trans(T alpha, const orc_mat *A, const T *x,       \
                                   T beta, T *y);                               \
     void orc_##SFX##_mlt_trans(const orc_mat *A, const T *x, T *y);            \
-    void orc_##SFX##_sor(const orc_mat *A, T *x, const T *b, T omega,          \
-                         int iter, int type_ssor);                             \
+    void orc_##SFX##_sor(const orc_mat *A, T *x, const T *b, double omega,     \
+                         int wide, int iter, int type_ssor);                   \
     void orc_##SFX##_coo_to_csr(int nnz, const int *row, const int *col,       \
                                 const T *val, int index, int m, int *ptr,      \
                                 int *ind, T *out);                             \

@@ -199,12 +199,101 @@ void FN(mlt_trans)(const orc_mat *A, const T *x, T *y)
     free(er); free(ec); free(ev); free(cnt); free(t1); free(t2); free(cptr);
 }
 
-/* SOR(A, X, B, omega, iter, type_ssor): matrix_sparse/Relaxation_MatVect.cxx:
- * 41-95 (RowSparse: ajj = A(j, j), the stored entry or 0) and :106-167
- * (ArrayRowSparse: ajj = 0 plus the row's diagonal entries).  type_ssor % 2
- * == 0: iter forward sweeps; then type_ssor % 3 == 0: iter backward sweeps. */
-void FN(sor)(const orc_mat *A, T *x, const T *b, T omega, int iter, int type_ssor)
+/* Seldon's A(j, j) on RowSparse (matrix_sparse/Matrix_Sparse.cxx:1054-1084):
+ * the scan moves on while the column is below j and stops at the row's last
+ * entry; the entry it stops on is returned if its column is j, else 0.  On a
+ * row with unsorted columns that can miss a diagonal entry stored after a
+ * larger column. */
+static T FN(rowsparse_diag)(const int *ptr, const int *ind, const T *data, int j)
 {
+    const int first = ptr[j], last = ptr[j + 1] - 1;
+    if (last < first) return 0;
+    int k = first;
+    while (k < last && ind[k] < j) k++;
+    return ind[k] == j ? data[k] : (T)0;
+}
+
+/* (1 - omega) x_j + omega temp / a_jj, in the type of (value, omega):
+ * omega's own when it is double and the values float (C++ promotion in
+ * Relaxation_MatVect.cxx:80 with T3 = double), the value type otherwise. */
+static T FN(sor_relax)(T xj, T temp, T ajj, double omega, int wide)
+{
+    if (wide) return (T)((1.0 - omega) * (double)xj + omega * (double)temp / (double)ajj);
+    const T om = (T)omega;
+    return ((T)1 - om) * xj + om * temp / ajj;
+}
+
+/* RowSymSparse (matrix_sparse/Relaxation_MatVect.cxx:179-300), upper triangle
+ * stored.  Forward: x_j = (1-w)/w a_jj x_j + (b_j - sum_{p>j} a_jp x_p) for
+ * every j with the old x, then the lower solve by columns (x_j *= w / a_jj,
+ * then x_p -= a_jp x_j for the row's other entries).  Backward: y_p +=
+ * a_jp x_j (old x) and x_j = (1-w)/w a_jj x_j + b_j - y_j row by row, then the
+ * upper solve in decreasing j: x_j = (x_j - sum_p a_jp x_p) w / a_jj.  a_jj is
+ * 0 plus the row's diagonal entries. */
+static void FN(sor_sym)(const orc_mat *A, T *x, const T *b, double omega, int wide, int iter, int type_ssor)
+{
+    const int ma = A->m;
+    const int *ptr = A->ptr, *ind = A->ind;
+    const T *data = (const T *)A->val;
+    const T omt = (T)omega;
+    if (type_ssor % 2 == 0)
+        for (int it = 0; it < iter; it++) {
+            for (int j = 0; j < ma; j++) {
+                T temp = 0, ajj = 0;
+                for (int k = ptr[j]; k < ptr[j + 1]; k++) {
+                    if (ind[k] == j) ajj += data[k];
+                    else temp += data[k] * x[ind[k]];
+                }
+                temp = b[j] - temp;
+                x[j] = wide ? (T)((1.0 - omega) / omega * (double)ajj * (double)x[j] + (double)temp)
+                            : ((T)1 - omt) / omt * ajj * x[j] + temp;
+            }
+            for (int j = 0; j < ma; j++) {
+                T ajj = 0;
+                for (int k = ptr[j]; k < ptr[j + 1]; k++)
+                    if (ind[k] == j) ajj += data[k];
+                x[j] = wide ? (T)((double)x[j] * (omega / (double)ajj)) : x[j] * (omt / ajj);
+                for (int k = ptr[j]; k < ptr[j + 1]; k++)
+                    if (ind[k] != j) x[ind[k]] -= data[k] * x[j];
+            }
+        }
+    if (type_ssor % 3 == 0) {
+        T *y = (T *)malloc(sizeof(T) * (size_t)(ma > 0 ? ma : 1));
+        for (int it = 0; it < iter; it++) {
+            for (int j = 0; j < ma; j++) y[j] = 0;
+            for (int j = 0; j < ma; j++) {
+                T ajj = 0;
+                for (int k = ptr[j]; k < ptr[j + 1]; k++) {
+                    if (ind[k] == j) ajj += data[k];
+                    else y[ind[k]] += data[k] * x[j];
+                }
+                x[j] = wide ? (T)((1.0 - omega) / omega * (double)ajj * (double)x[j] + (double)b[j] - (double)y[j])
+                            : ((T)1 - omt) / omt * ajj * x[j] + b[j] - y[j];
+            }
+            for (int j = ma - 1; j >= 0; j--) {
+                T temp = 0, ajj = 0;
+                for (int k = ptr[j]; k < ptr[j + 1]; k++) {
+                    if (ind[k] == j) ajj += data[k];
+                    else temp += data[k] * x[ind[k]];
+                }
+                x[j] = wide ? (T)((double)(x[j] - temp) * omega / (double)ajj) : (x[j] - temp) * omt / ajj;
+            }
+        }
+        free(y);
+    }
+}
+
+/* SOR(A, X, B, omega, iter, type_ssor): matrix_sparse/Relaxation_MatVect.cxx:
+ * 41-95 (RowSparse: a_jj = A(j, j)), :106-167 (ArrayRowSparse: a_jj = 0 plus
+ * the row's diagonal entries), :179-300 (RowSymSparse, above).  type_ssor % 2
+ * == 0: iter forward sweeps; then type_ssor % 3 == 0: iter backward sweeps.
+ * wide: omega is a double applied to float values. */
+void FN(sor)(const orc_mat *A, T *x, const T *b, double omega, int wide, int iter, int type_ssor)
+{
+    if (A->storage == ORC_ROWSYMSPARSE) {
+        FN(sor_sym)(A, x, b, omega, wide, iter, type_ssor);
+        return;
+    }
     const int ma = A->m;
     const int *ptr = A->ptr, *ind = A->ind;
     const T *data = (const T *)A->val;
@@ -221,12 +310,11 @@ void FN(sor)(const orc_mat *A, T *x, const T *b, T omega, int iter, int type_sso
                         if (ind[k] == j) ajj += data[k];
                     }
                 } else {
-                    for (int k = ptr[j]; k < ptr[j + 1]; k++)
-                        if (ind[k] == j) { ajj = data[k]; break; }
+                    ajj = FN(rowsparse_diag)(ptr, ind, data, j);
                     for (int k = ptr[j]; k < ptr[j + 1]; k++) temp += data[k] * x[ind[k]];
                 }
                 temp = b[j] - temp + ajj * x[j];
-                x[j] = ((T)1 - omega) * x[j] + omega * temp / ajj;
+                x[j] = FN(sor_relax)(x[j], temp, ajj, omega, wide);
             }
     }
 }

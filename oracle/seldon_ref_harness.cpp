@@ -16,7 +16,7 @@
 //   seldon_ref mlt   <row|sym|array> <d|f> A x out
 //   seldon_ref spmvt <row|sym|array> <d|f> A x y alpha beta out   (SeldonTrans)
 //   seldon_ref mltt  <row|sym|array> <d|f> A x out                 (SeldonTrans)
-//   seldon_ref sor <row|array> <d|f> A x b omega iter type_ssor out
+//   seldon_ref sor <row|array|sym> <d|f> A x b omega iter type_ssor out [wide]
 //   seldon_ref coo2csr <d|f> rows cols vals index m_min n_min out
 //                    (ConvertMatrix_from_Coordinates into a RowSparse matrix
 //                     first sized m_min x n_min; writes it as a binary CSR)
@@ -231,23 +231,32 @@ static int run_solve(char** a)
   return 0;
 }
 
-template <class T>
+template <class T, class W>
 static int run_sor(const string& st, char** a)
 {
-  Matrix<T, General, RowSparse> A;
-  A.Read(a[0]);
   Vector<T> x, b;
   x.Read(a[1]);
   b.Read(a[2]);
-  const T omega = T(atof(a[3]));
-  if (st == "array")
+  const W omega = W(atof(a[3]));
+  if (st == "sym")
     {
-      Matrix<T, General, ArrayRowSparse> B;
-      to_array(A, B);
-      SOR(B, x, b, omega, atoi(a[4]), atoi(a[5]));
+      Matrix<T, Symmetric, RowSymSparse> S;
+      S.Read(a[0]);
+      SOR(S, x, b, omega, atoi(a[4]), atoi(a[5]));
     }
   else
-    SOR(A, x, b, omega, atoi(a[4]), atoi(a[5]));
+    {
+      Matrix<T, General, RowSparse> A;
+      A.Read(a[0]);
+      if (st == "array")
+        {
+          Matrix<T, General, ArrayRowSparse> B;
+          to_array(A, B);
+          SOR(B, x, b, omega, atoi(a[4]), atoi(a[5]));
+        }
+      else
+        SOR(A, x, b, omega, atoi(a[4]), atoi(a[5]));
+    }
   x.Write(a[6]);
   return 0;
 }
@@ -288,7 +297,14 @@ int main(int argc, char** argv)
         return string(argv[3]) == "f" ? run_spmv<float>(false, argv[2], argv + 4, true)
                                       : run_spmv<double>(false, argv[2], argv + 4, true);
       if (argc >= 11 && string(argv[1]) == "sor")
-        return string(argv[3]) == "f" ? run_sor<float>(argv[2], argv + 4) : run_sor<double>(argv[2], argv + 4);
+        {
+          // a last argument "wide": omega is a double (float values only)
+          if (string(argv[3]) != "f")
+            return run_sor<double, double>(argv[2], argv + 4);
+          if (argc >= 12 && string(argv[11]) == "wide")
+            return run_sor<float, double>(argv[2], argv + 4);
+          return run_sor<float, float>(argv[2], argv + 4);
+        }
       if (argc >= 10 && string(argv[1]) == "coo2csr")
         return string(argv[2]) == "f" ? run_coo2csr<float>(argv + 3) : run_coo2csr<double>(argv + 3);
       if (argc >= 12 && string(argv[1]) == "solve")

@@ -1,19 +1,33 @@
 // shk_sor.hip -- Seldon's SOR / SSOR sweeps on the device, level-scheduled.
 //
-// SOR(A, X, B, omega, iter, type_ssor) (matrix_sparse/Relaxation_MatVect.cxx:
-// 41-95 RowSparse, :106-167 ArrayRowSparse) updates X row after row:
+// SOR(A, X, B, omega, iter, type_ssor) runs iter forward sweeps when
+// type_ssor % 2 == 0, then iter backward sweeps when type_ssor % 3 == 0.
+//
+// General storages (matrix_sparse/Relaxation_MatVect.cxx:41-95 RowSparse,
+// :106-167 ArrayRowSparse) update X row after row:
 //   temp = sum_k a_jk X(ind_k)          (the whole row, stored order)
 //   temp = B(j) - temp + ajj * X(j)
 //   X(j) = (1 - omega) * X(j) + omega * temp / ajj
-// with the rows in increasing order (forward sweep, type_ssor 2), decreasing
-// (backward, 3), or all forward sweeps then all backward ones (0).  A row
-// reads the new X of the rows swept before it and the old X of the others.
-// Rows are grouped in levels such that a row comes after every earlier-swept
-// row whose X it reads and before every later-swept row that reads its X
-// (both directions of each stored entry count); rows of one level then
-// neither read nor write each other's X, so a level runs as one launch, one
-// thread per row, and every row sees exactly the operands of the sequential
-// sweep: the results are bitwise Seldon's.
+// with the rows in increasing order (forward) or decreasing (backward).  A
+// row reads the new X of the rows swept before it and the old X of the
+// others.  Rows are grouped in levels such that a row comes after every
+// earlier-swept row whose X it reads and before every later-swept row that
+// reads its X (both directions of each stored entry count); rows of one level
+// then neither read nor write each other's X, so a level runs as one launch,
+// one thread per row, and every row sees exactly the operands of the
+// sequential sweep: the results are bitwise Seldon's.
+//
+// RowSymSparse (:179-300, upper triangle stored) splits each sweep in two
+// passes.  Forward: Z(j) = (1-w)/w ajj X(j) + (B(j) - sum_{p>j} a_jp X(p))
+// from the old X (one launch over all rows), then the lower solve
+// X(p) = (Z(p) - sum_{j<p} a_jp X(j)) * (w / ajj), the sum over column p of
+// the stored triangle in Seldon's scatter order (rows increasing), by levels.
+// Backward: Y(j) = sum_{i<j} a_ij X(i) (old X, same order) and
+// Z(j) = (1-w)/w ajj X(j) + B(j) - Y(j) in one launch, then the upper solve
+// X(j) = (Z(j) - sum_{p>j} a_jp X(p)) w / ajj in decreasing levels.
+//
+// omega is applied in the type C++ gives (value, omega): a double omega on
+// float values makes each relaxation step double, rounded once on the store.
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -24,17 +38,20 @@ namespace shk {
 
 struct SorPlan {
     int m = 0, storage = 0, dtype = 0;
-    int *ptr = nullptr, *ind = nullptr;
+    int *ptr = nullptr, *ind = nullptr;  // the stored rows
     void *val = nullptr, *diag = nullptr;
-    int *rows[2] = {nullptr, nullptr};  // forward, backward: rows grouped by level
-    std::vector<int> off[2];            // level offsets into rows[d] (host)
+    int *tptr = nullptr, *tind = nullptr;  // RowSymSparse: strict upper entries by column
+    void *tval = nullptr, *z = nullptr;    // (rows increasing), and the half-step vector
+    int *rows[2] = {nullptr, nullptr};     // forward, backward: rows grouped by level
+    std::vector<int> off[2];               // level offsets into rows[d] (host)
 };
 
-template <typename T>
+// ---- general storages ----------------------------------------------------
+template <typename T, typename W>
 __global__ __launch_bounds__(kBlock) void k_sor_rows(const int *__restrict__ rows, int count, const int *__restrict__ ptr,
                                                      const int *__restrict__ ind, const T *__restrict__ val,
                                                      const T *__restrict__ diag, const T *__restrict__ B, T *X,
-                                                     T omega)
+                                                     W omega)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= count) return;
@@ -44,9 +61,72 @@ __global__ __launch_bounds__(kBlock) void k_sor_rows(const int *__restrict__ row
     const T ajj = diag[j];
     const T xj = X[j];
     temp = B[j] - temp + ajj * xj;
-    X[j] = ((T)1 - omega) * xj + omega * temp / ajj;
+    X[j] = (T)(((W)1 - omega) * (W)xj + omega * (W)temp / (W)ajj);
 }
 
+// ---- RowSymSparse --------------------------------------------------------
+// forward, first pass: Z(j) = (1-w)/w ajj X(j) + (B(j) - sum_{p != j} a_jp X(p))
+template <typename T, typename W>
+__global__ __launch_bounds__(kBlock) void k_sor_sym_f1(int m, const int *__restrict__ ptr, const int *__restrict__ ind,
+                                                       const T *__restrict__ val, const T *__restrict__ diag,
+                                                       const T *__restrict__ B, const T *__restrict__ X,
+                                                       T *__restrict__ Z, W omega)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= m) return;
+    T temp = (T)0;
+    for (int k = ptr[j]; k < ptr[j + 1]; k++)
+        if (ind[k] != j) temp += val[k] * X[ind[k]];
+    temp = B[j] - temp;
+    Z[j] = (T)(((W)1 - omega) / omega * (W)diag[j] * (W)X[j] + (W)temp);
+}
+
+// forward, second pass (one level): X(p) = (Z(p) - sum_j a_jp X(j)) * (w / ajj)
+template <typename T, typename W>
+__global__ __launch_bounds__(kBlock) void k_sor_sym_f2(const int *__restrict__ rows, int count,
+                                                       const int *__restrict__ tptr, const int *__restrict__ tind,
+                                                       const T *__restrict__ tval, const T *__restrict__ diag,
+                                                       const T *__restrict__ Z, T *X, W omega)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int p = rows[t];
+    T x = Z[p];
+    for (int k = tptr[p]; k < tptr[p + 1]; k++) x -= tval[k] * X[tind[k]];
+    X[p] = (T)((W)x * (omega / (W)diag[p]));
+}
+
+// backward, first pass: Z(j) = (1-w)/w ajj X(j) + B(j) - sum_{i<j} a_ij X(i)
+template <typename T, typename W>
+__global__ __launch_bounds__(kBlock) void k_sor_sym_b1(int m, const int *__restrict__ tptr,
+                                                       const int *__restrict__ tind, const T *__restrict__ tval,
+                                                       const T *__restrict__ diag, const T *__restrict__ B,
+                                                       const T *__restrict__ X, T *__restrict__ Z, W omega)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= m) return;
+    T y = (T)0;
+    for (int k = tptr[j]; k < tptr[j + 1]; k++) y += tval[k] * X[tind[k]];
+    Z[j] = (T)(((W)1 - omega) / omega * (W)diag[j] * (W)X[j] + (W)B[j] - (W)y);
+}
+
+// backward, second pass (one level): X(j) = (Z(j) - sum_{p != j} a_jp X(p)) w / ajj
+template <typename T, typename W>
+__global__ __launch_bounds__(kBlock) void k_sor_sym_b2(const int *__restrict__ rows, int count,
+                                                       const int *__restrict__ ptr, const int *__restrict__ ind,
+                                                       const T *__restrict__ val, const T *__restrict__ diag,
+                                                       const T *__restrict__ Z, T *X, W omega)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int j = rows[t];
+    T temp = (T)0;
+    for (int k = ptr[j]; k < ptr[j + 1]; k++)
+        if (ind[k] != j) temp += val[k] * X[ind[k]];
+    X[j] = (T)((W)(Z[j] - temp) * omega / (W)diag[j]);
+}
+
+// ---- plan ----------------------------------------------------------------
 // Levels of one sweep direction (dir 0: rows 0..m-1, dir 1: m-1..0).
 static void sor_levels(int m, const std::vector<int> &ptr, const std::vector<int> &ind, int dir,
                        std::vector<int> &rows, std::vector<int> &off)
@@ -77,6 +157,28 @@ static void sor_levels(int m, const std::vector<int> &ptr, const std::vector<int
     for (int j = 0; j < m; j++) rows[next[lev[j]]++] = j;
 }
 
+// Seldon's A(j, j) on RowSparse (matrix_sparse/Matrix_Sparse.cxx:1054-1084):
+// the scan moves on while the column is below j and stops at the row's last
+// entry; the entry it stops on counts if its column is j, else a_jj = 0.
+template <typename T>
+static T rowsparse_diag(const std::vector<int> &ptr, const std::vector<int> &ind, const std::vector<T> &val, int j)
+{
+    const int first = ptr[j], last = ptr[(size_t)j + 1] - 1;
+    if (last < first) return (T)0;
+    int k = first;
+    while (k < last && ind[k] < j) k++;
+    return ind[k] == j ? val[k] : (T)0;
+}
+
+template <typename V>
+static hipError_t put(V **dst, const std::vector<V> &src, hipStream_t s)
+{
+    hipError_t e = hipMalloc((void **)dst, sizeof(V) * std::max<size_t>(src.size(), 1));
+    if (e == hipSuccess && !src.empty())
+        e = hipMemcpyAsync(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice, s);
+    return e;
+}
+
 template <typename T>
 static int sor_build(shk_csr A, SorPlan **out)
 {
@@ -84,38 +186,66 @@ static int sor_build(shk_csr A, SorPlan **out)
     std::vector<T> val;
     SHK_TRY(stored_arrays<T>(A, ptr, ind, val));
     const int m = A->m;
-    // ajj: RowSparse reads A(j, j) (the stored entry, 0 when absent,
-    // Matrix_Sparse operator()); ArrayRowSparse adds the row's diagonal
-    // entries onto 0 (Relaxation_MatVect.cxx:136-140)
+    const bool sym = A->storage == SHK_ROWSYMSPARSE;
+    // ajj: RowSparse reads A(j, j); ArrayRowSparse and RowSymSparse add the
+    // row's diagonal entries onto 0 (Relaxation_MatVect.cxx:136-140, :217-225)
     std::vector<T> diag(m, (T)0);
     for (int j = 0; j < m; j++) {
-        bool found = false;
-        for (int k = ptr[j]; k < ptr[(size_t)j + 1]; k++)
-            if (ind[k] == j) {
-                if (A->storage == SHK_ARRAYROWSPARSE) diag[j] += val[k];
-                else if (!found) diag[j] = val[k];
-                found = true;
-            }
+        if (A->storage == SHK_ROWSPARSE) {
+            diag[j] = rowsparse_diag(ptr, ind, val, j);
+            continue;
+        }
+        for (int k = ptr[j]; k < ptr[(size_t)j + 1]; k++) {
+            SHK_CHECK(!sym || ind[k] >= j, SHK_ERR_ARG, "sor: a RowSymSparse entry below the diagonal");
+            if (ind[k] == j) diag[j] += val[k];
+        }
+    }
+    std::vector<int> rows[2], tptr, tind;
+    std::vector<T> tval;
+    std::vector<int> off[2];
+    if (sym) {
+        // strict upper entries gathered by column, rows increasing (a stable
+        // counting sort of the row-major entries)
+        tptr.assign((size_t)m + 1, 0);
+        for (int j = 0; j < m; j++)
+            for (int k = ptr[j]; k < ptr[(size_t)j + 1]; k++)
+                if (ind[k] != j) tptr[(size_t)ind[k] + 1]++;
+        for (int p = 0; p < m; p++) tptr[(size_t)p + 1] += tptr[p];
+        tind.resize(tptr[m]);
+        tval.resize(tptr[m]);
+        std::vector<int> next(tptr.begin(), tptr.end() - 1);
+        for (int j = 0; j < m; j++)
+            for (int k = ptr[j]; k < ptr[(size_t)j + 1]; k++)
+                if (ind[k] != j) {
+                    const int q = next[ind[k]]++;
+                    tind[q] = j;
+                    tval[q] = val[k];
+                }
+        // the forward solve's row p reads the X(j) of its column entries, the
+        // backward solve's row j the X(p) of its row entries
+        sor_levels(m, tptr, tind, 0, rows[0], off[0]);
+        sor_levels(m, ptr, ind, 1, rows[1], off[1]);
+    } else {
+        for (int d = 0; d < 2; d++) sor_levels(m, ptr, ind, d, rows[d], off[d]);
     }
     SorPlan *P = new SorPlan;
     P->m = m;
     P->storage = A->storage;
     P->dtype = A->dtype;
-    std::vector<int> rows[2];
-    for (int d = 0; d < 2; d++) sor_levels(m, ptr, ind, d, rows[d], P->off[d]);
-    const size_t nz = ind.size();
-    hipError_t e = hipMalloc(&P->ptr, sizeof(int) * ((size_t)m + 1));
-    if (e == hipSuccess) e = hipMalloc(&P->ind, sizeof(int) * std::max<size_t>(nz, 1));
-    if (e == hipSuccess) e = hipMalloc(&P->val, sizeof(T) * std::max<size_t>(nz, 1));
-    if (e == hipSuccess) e = hipMalloc(&P->diag, sizeof(T) * std::max(m, 1));
-    for (int d = 0; d < 2 && e == hipSuccess; d++) e = hipMalloc(&P->rows[d], sizeof(int) * std::max(m, 1));
+    P->off[0] = std::move(off[0]);
+    P->off[1] = std::move(off[1]);
     hipStream_t s = A->ctx->stream;
-    if (e == hipSuccess) e = hipMemcpyAsync(P->ptr, ptr.data(), sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && nz) e = hipMemcpyAsync(P->ind, ind.data(), sizeof(int) * nz, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && nz) e = hipMemcpyAsync(P->val, val.data(), sizeof(T) * nz, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && m) e = hipMemcpyAsync(P->diag, diag.data(), sizeof(T) * m, hipMemcpyHostToDevice, s);
-    for (int d = 0; d < 2 && e == hipSuccess && m; d++)
-        e = hipMemcpyAsync(P->rows[d], rows[d].data(), sizeof(int) * m, hipMemcpyHostToDevice, s);
+    hipError_t e = put(&P->ptr, ptr, s);
+    if (e == hipSuccess) e = put(&P->ind, ind, s);
+    if (e == hipSuccess) e = put((T **)&P->val, val, s);
+    if (e == hipSuccess) e = put((T **)&P->diag, diag, s);
+    for (int d = 0; d < 2 && e == hipSuccess; d++) e = put(&P->rows[d], rows[d], s);
+    if (sym) {
+        if (e == hipSuccess) e = put(&P->tptr, tptr, s);
+        if (e == hipSuccess) e = put(&P->tind, tind, s);
+        if (e == hipSuccess) e = put((T **)&P->tval, tval, s);
+        if (e == hipSuccess) e = hipMalloc(&P->z, sizeof(T) * std::max(m, 1));
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         sor_free(P);
@@ -132,22 +262,48 @@ void sor_free(SorPlan *P)
     hipFree(P->ind);
     hipFree(P->val);
     hipFree(P->diag);
+    hipFree(P->tptr);
+    hipFree(P->tind);
+    hipFree(P->tval);
+    hipFree(P->z);
     hipFree(P->rows[0]);
     hipFree(P->rows[1]);
     delete P;
 }
 
-template <typename T>
-static int sor_run(shk_ctx ctx, const SorPlan *P, T *X, const T *B, T omega, int iter, int type_ssor)
+static inline dim3 grid_for(int count) { return dim3((count + kBlock - 1) / kBlock); }
+
+template <typename T, typename W>
+static int sor_run(shk_ctx ctx, const SorPlan *P, T *X, const T *B, W omega, int iter, int type_ssor)
 {
     hipStream_t s = ctx->stream;
-    auto sweep = [&](int d) {
+    const T *val = (const T *)P->val, *diag = (const T *)P->diag, *tval = (const T *)P->tval;
+    T *Z = (T *)P->z;
+    // one launch per level of direction d
+    auto levels = [&](int d, auto launch) {
         const std::vector<int> &off = P->off[d];
-        for (size_t l = 0; l + 1 < off.size(); l++) {
-            const int count = off[l + 1] - off[l];
-            hipLaunchKernelGGL(k_sor_rows<T>, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                               P->rows[d] + off[l], count, P->ptr, P->ind, (const T *)P->val, (const T *)P->diag, B,
-                               X, omega);
+        for (size_t l = 0; l + 1 < off.size(); l++) launch(P->rows[d] + off[l], off[l + 1] - off[l]);
+    };
+    auto sweep = [&](int d) {
+        if (P->storage != SHK_ROWSYMSPARSE) {
+            levels(d, [&](const int *rows, int count) {
+                hipLaunchKernelGGL((k_sor_rows<T, W>), grid_for(count), dim3(kBlock), 0, s, rows, count, P->ptr,
+                                   P->ind, val, diag, B, X, omega);
+            });
+        } else if (d == 0) {
+            hipLaunchKernelGGL((k_sor_sym_f1<T, W>), grid_for(P->m), dim3(kBlock), 0, s, P->m, P->ptr, P->ind, val,
+                               diag, B, (const T *)X, Z, omega);
+            levels(0, [&](const int *rows, int count) {
+                hipLaunchKernelGGL((k_sor_sym_f2<T, W>), grid_for(count), dim3(kBlock), 0, s, rows, count, P->tptr,
+                                   P->tind, tval, diag, (const T *)Z, X, omega);
+            });
+        } else {
+            hipLaunchKernelGGL((k_sor_sym_b1<T, W>), grid_for(P->m), dim3(kBlock), 0, s, P->m, P->tptr, P->tind,
+                               tval, diag, B, (const T *)X, Z, omega);
+            levels(1, [&](const int *rows, int count) {
+                hipLaunchKernelGGL((k_sor_sym_b2<T, W>), grid_for(count), dim3(kBlock), 0, s, rows, count, P->ptr,
+                                   P->ind, val, diag, (const T *)Z, X, omega);
+            });
         }
     };
     if (type_ssor % 2 == 0)
@@ -164,14 +320,16 @@ using namespace shk;
 
 extern "C" {
 
-int shk_sor(shk_ctx ctx, shk_csr A, void *dx, const void *db, double omega, int iter, int type_ssor)
+int shk_sor(shk_ctx ctx, shk_csr A, void *dx, const void *db, double omega, shk_dtype omega_type, int iter,
+            int type_ssor)
 {
     SHK_CHECK(ctx && A && A->ctx == ctx, SHK_ERR_ARG, "sor: null or foreign argument");
-    SHK_CHECK(A->storage == SHK_ROWSPARSE || A->storage == SHK_ARRAYROWSPARSE, SHK_ERR_ARG,
-              "sor: RowSparse and ArrayRowSparse only");
+    SHK_CHECK(A->storage == SHK_ROWSPARSE || A->storage == SHK_ARRAYROWSPARSE || A->storage == SHK_ROWSYMSPARSE,
+              SHK_ERR_ARG, "sor: unknown storage");
+    SHK_CHECK(omega_type == SHK_F64 || omega_type == SHK_F32, SHK_ERR_ARG, "sor: bad omega type");
     SHK_CHECK(A->m == A->n, SHK_ERR_DIM, "sor: the matrix must be square");
     SHK_CHECK(iter >= 0, SHK_ERR_ARG, "sor: negative iteration count");
-    if (A->m == 0 || iter == 0) return SHK_OK;
+    if (A->m == 0 || iter == 0 || (type_ssor % 2 != 0 && type_ssor % 3 != 0)) return SHK_OK;
     SHK_TRY(check_device_ptr(dx, "x"));
     SHK_TRY(check_device_ptr(db, "b"));
     if (!A->sor) {
@@ -181,8 +339,10 @@ int shk_sor(shk_ctx ctx, shk_csr A, void *dx, const void *db, double omega, int 
     }
     const SorPlan *P = A->sor;
     if (A->dtype == SHK_F64)
-        return sor_run<double>(ctx, P, (double *)dx, (const double *)db, omega, iter, type_ssor);
-    return sor_run<float>(ctx, P, (float *)dx, (const float *)db, (float)omega, iter, type_ssor);
+        return sor_run<double, double>(ctx, P, (double *)dx, (const double *)db, omega, iter, type_ssor);
+    if (omega_type == SHK_F64)
+        return sor_run<float, double>(ctx, P, (float *)dx, (const float *)db, omega, iter, type_ssor);
+    return sor_run<float, float>(ctx, P, (float *)dx, (const float *)db, (float)omega, iter, type_ssor);
 }
 
 }  // extern "C"

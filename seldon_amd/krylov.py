@@ -285,11 +285,16 @@ def ConvertMatrix_from_Coordinates(IndRow: torch.Tensor, IndCol: torch.Tensor, V
     return mm.value, nn.value, ptr, ind[:nnz], out[:nnz]
 
 
-def SOR(A: Matrix, X: torch.Tensor, B: torch.Tensor, omega, iter: int, type_ssor: int = 2):
+def SOR(A: Matrix, X: torch.Tensor, B: torch.Tensor, omega, iter: int, type_ssor: int = 2, omega_dtype=None):
     """SOR(A, X, B, omega, iter, type_ssor): 2 forward sweeps, 3 backward, 0
-    both (Relaxation_MatVect.cxx:41-95, :106-167); X updated in place,
-    bitwise as Seldon's sequential sweeps (RowSparse, ArrayRowSparse)."""
-    L.check(L.load().shk_sor(A.ctx.h, A.h, _ptr(X), _ptr(B), float(omega), int(iter), int(type_ssor)))
+    both (Relaxation_MatVect.cxx:41-95, :106-167, :179-300); X updated in
+    place, bitwise as Seldon's sequential sweeps on all three storages.
+    omega_dtype is the C++ type of omega: by default the value type (as
+    SorPreconditioner passes it); torch.float64 on float32 values makes the
+    relaxation step double, as C++ does for a double omega."""
+    wide = omega_dtype is not None and torch.empty(0, dtype=omega_dtype).element_size() == 8
+    L.check(L.load().shk_sor(A.ctx.h, A.h, _ptr(X), _ptr(B), float(omega),
+                             L.SHK_F64 if (wide or X.dtype == torch.float64) else L.SHK_F32, int(iter), int(type_ssor)))
 
 
 def DotProd(X, Y) -> float:

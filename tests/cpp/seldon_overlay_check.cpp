@@ -104,8 +104,31 @@ static void run_ssor_solver(const string& which, const string& label, Mat& A, co
   SorPreconditioner<double> M;
   M.SetParameterRelaxation(1.2);
   M.SetNumberIterations(1);
-  int code = which == "ssor_cg" ? Cg(A, x, b, M, it) : BiCgStab(A, x, b, M, it);
+  int code = which.compare(0, 7, "ssor_cg") == 0 ? Cg(A, x, b, M, it) : BiCgStab(A, x, b, M, it);
   cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+}
+
+// The upper triangle of A as a RowSymSparse matrix.
+static void upper_of(const Matrix<double, General, RowSparse>& A, Matrix<double, Symmetric, RowSymSparse>& S)
+{
+  const int n = A.GetM();
+  const int* p = A.GetPtr();
+  const int* c = A.GetInd();
+  const double* v = A.GetData();
+  std::vector<int> vi;
+  std::vector<double> vv;
+  Vector<int> ptr(n + 1);
+  ptr(0) = 0;
+  for (int r = 0; r < n; r++)
+    {
+      for (int k = p[r]; k < p[r + 1]; k++)
+        if (c[k] >= r) { vi.push_back(c[k]); vv.push_back(v[k]); }
+      ptr(r + 1) = (int)vi.size();
+    }
+  Vector<double> values(vv.size());
+  Vector<int> ind(vi.size());
+  for (size_t k = 0; k < vv.size(); k++) { values(k) = vv[k]; ind(k) = vi[k]; }
+  S.SetData(n, n, values, ptr, ind);
 }
 
 int main(int argc, char** argv)
@@ -196,6 +219,12 @@ int main(int argc, char** argv)
       }
     run_ssor_solver("ssor_cg", "cpu", A, bh, 2000, 1e-10);
     run_ssor_solver("ssor_cg", "gpu", Ag, bg, 2000, 1e-10);
+    // the same on the symmetric storage (upper triangle)
+    Matrix<double, Symmetric, RowSymSparse> S;
+    upper_of(A, S);
+    Matrix<double, Symmetric, RowSymSparseHip, HipAlloc<double> > Sg(S);
+    run_ssor_solver("ssor_cg_sym", "cpu", S, bh, 2000, 1e-10);
+    run_ssor_solver("ssor_cg_sym", "gpu", Sg, bg, 2000, 1e-10);
   }
   return 0;
 }

@@ -219,34 +219,69 @@ def make_spmvt():
 
 
 SOR_CASES = [(1.0, 1, 2), (1.3, 2, 2), (1.3, 2, 3), (0.8, 1, 0), (1.3, 2, 0)]  # (omega, iter, type_ssor)
+SOR_BACKWARD_ONCE = [(1.3, 1, 3), (1.0, 1, 3)]
+
+
+def scramble_rows(A, seed, hide_first_diagonal):
+    """Each row reordered: its columns below the diagonal in random order,
+    the diagonal, then its larger columns in random order, so Seldon's
+    A(j, j) (a scan that stops at the first column not below j) still finds
+    the diagonal.  With hide_first_diagonal, row 0 starts with a larger
+    column and that scan reads A(0, 0) as 0."""
+    rng = np.random.default_rng(seed)
+    ind, val = A.ind.copy(), A.val.copy()
+    for j in range(A.m):
+        a, b = A.ptr[j], A.ptr[j + 1]
+        c = ind[a:b]
+        lo, dg, hi = np.flatnonzero(c < j), np.flatnonzero(c == j), np.flatnonzero(c > j)
+        order = np.concatenate([rng.permutation(lo), dg, rng.permutation(hi)])
+        if j == 0 and hide_first_diagonal and len(hi):
+            order = np.concatenate([hi[:1], dg, hi[1:]])
+        ind[a:b], val[a:b] = c[order], A.val[a:b][order]
+    return wl.CSR(A.m, A.n, A.ptr.copy(), ind, val)
 
 
 def sor_case():
-    """A 512-row 3-D convection-diffusion matrix (nonsymmetric) and a 400-row
-    power-law matrix given a dominant diagonal, with seeded x0 and b."""
+    """name -> (matrix, storages, cases): a 512-row 3-D convection-diffusion
+    matrix (nonsymmetric), a 400-row power-law matrix given a dominant
+    diagonal, the upper triangle of a 16^2 Laplacian (RowSymSparse), and the
+    convection-diffusion matrix with scrambled rows whose row 0 hides its
+    diagonal (swept backward once, so only x(0) is not finite)."""
     C = wl.convdiff3d(8)
     P = wl.powerlaw(400, 17, kmin=2, kmax=30, half_width=60)
-    return {"convdiff8": C, "powerlaw400": P}
+    return {"convdiff8": (C, ("row", "array"), SOR_CASES),
+            "powerlaw400": (P, ("row", "array"), SOR_CASES),
+            "laplace16": (wl.upper(wl.laplace2d(16)), ("sym",), SOR_CASES),
+            "convdiff8scrambled": (scramble_rows(C, 53, True), ("row",), SOR_BACKWARD_ONCE)}
+
+
+SOR_TYPES = (("d", np.float64, False), ("f", np.float32, False), ("fw", np.float32, True))  # fw: double omega
+
+
+def sor_fixture_cases():
+    """(key, A, storage, dtype, wide, omega, iter, type_ssor) of every fixture."""
+    for name, (A, storages, cases) in sor_case().items():
+        for c, dt, wide in SOR_TYPES:
+            for st in storages:
+                for k, (om, it, ty) in enumerate(cases):
+                    yield f"{name}_{st}_{c}_{k}", A, st, dt, wide, om, it, ty
 
 
 def make_sor():
     """SOR(A, X, B, omega, iter, type_ssor) from the reference build."""
     out = {}
     with tempfile.TemporaryDirectory() as tmp:
-        for name, A in sor_case().items():
-            x = wl.uniform(A.m, 51)
-            b = wl.uniform(A.m, 52)
-            for dt, c in ((np.float64, "d"), (np.float32, "f")):
-                orc.write_seldon_csr(f"{tmp}/A.bin", A, dt)
-                orc.write_seldon_vec(f"{tmp}/x.bin", x.astype(dt))
-                orc.write_seldon_vec(f"{tmp}/b.bin", b.astype(dt))
-                for st in ("row", "array"):
-                    for k, (om, it, ty) in enumerate(SOR_CASES):
-                        orc.run_ref(["sor", st, c, f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/b.bin", repr(om), it, ty,
-                                     f"{tmp}/o.bin"])
-                        out[f"{name}_{st}_{c}_{k}"] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
-    np.savez_compressed(os.path.join(OUT, "sor.npz"), spec=json.dumps({"cases": SOR_CASES, "xseed": 51, "bseed": 52}),
-                        **out)
+        for key, A, st, dt, wide, om, it, ty in sor_fixture_cases():
+            orc.write_seldon_csr(f"{tmp}/A.bin", A, dt)
+            orc.write_seldon_vec(f"{tmp}/x.bin", wl.uniform(A.m, 51).astype(dt))
+            orc.write_seldon_vec(f"{tmp}/b.bin", wl.uniform(A.m, 52).astype(dt))
+            args = ["sor", st, "d" if dt == np.float64 else "f", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/b.bin",
+                    repr(om), it, ty, f"{tmp}/o.bin"] + (["wide"] if wide else [])
+            orc.run_ref(args)
+            out[key] = orc.read_seldon_vec(f"{tmp}/o.bin", dt)
+    np.savez_compressed(os.path.join(OUT, "sor.npz"),
+                        spec=json.dumps({"cases": SOR_CASES, "backward_once": SOR_BACKWARD_ONCE, "xseed": 51,
+                                         "bseed": 52, "scramble_seed": 53}), **out)
     print("sor fixtures:", len(out))
 
 

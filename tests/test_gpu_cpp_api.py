@@ -112,7 +112,8 @@ def test_overlay_in_real_seldon(gpu):
     # convection-diffusion system the transpose-using BiCg, Qmr, Cgne, Lsqr
     # and Cgs, QCgs, TfQmr, Gcr, BiCgStabl; on the 64 x 64 Laplacian MinRes,
     # Symmlq, CoCg, BiCgcr, QmrSym, and Cg with Seldon's SorPreconditioner
-    # (SSOR sweeps level-scheduled on the GPU, bitwise; to 1e-10); the others
+    # on RowSparse and on RowSymSparse (SSOR sweeps level-scheduled on the
+    # GPU, bitwise; to 1e-10); the others
     # to 1e-8.  Where Seldon's own builds
     # agree bitwise (generic, FMA, MKL-linked: spread 0) the GPU run must take
     # the same number of iterations with histories within 1e-6.  The others
@@ -124,7 +125,7 @@ def test_overlay_in_real_seldon(gpu):
     # differently from the first dot product on).
     agree = {"cgne": 100, "lsqr": 100, "cgs": 40, "qcgs": 40, "tfqmr": 70, "bicgstabl": 2}
     for solver in ("bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr", "bicgstabl",
-                   "minres", "symmlq", "cocg", "bicgcr", "qmrsym", "ssor_cg"):
+                   "minres", "symmlq", "cocg", "bicgcr", "qmrsym", "ssor_cg", "ssor_cg_sym"):
         h_c, it_c, c_c, n_c = s[f"{solver}_cpu"]
         h_g, it_g, c_g, n_g = s[f"{solver}_gpu"]
         k = min(len(h_c), len(h_g))

@@ -271,32 +271,37 @@ def test_spmv_trans_follows_new_values(gpu):
 # ---------------------------------------------------------------- SOR --
 
 @pytest.mark.parametrize("layout", ["rows", "auto"])
-@pytest.mark.parametrize("st", ["row", "array"])
-@pytest.mark.parametrize("c", ["d", "f"])
-def test_sor_golden_fixture(gpu, st, c, layout):
-    """SOR / SSOR sweeps (level-scheduled) bitwise against the real Seldon."""
+def test_sor_golden_fixture(gpu, layout):
+    """SOR / SSOR sweeps (level-scheduled) bitwise against the real Seldon:
+    RowSparse, ArrayRowSparse, RowSymSparse; float values with a float and a
+    double omega; scrambled rows, one hiding its diagonal from A(j, j)."""
     from seldon_amd import krylov as K
-    from tests.golden.make_golden import SOR_CASES, sor_case
+    from tests.golden.make_golden import sor_fixture_cases
     fx = load_fixture("sor.npz")
-    tdt, ndt = DT[c]
-    for name, A in sor_case().items():
-        Ag = K.Matrix(A, storage=st, dtype=tdt, layout=layout)
-        x = wl.uniform(A.m, fx["spec"]["xseed"])
+    mats = {}
+    for key, A, st, ndt, wide, om, it, ty in sor_fixture_cases():
+        tdt = torch.float64 if ndt == np.float64 else torch.float32
+        mk = (id(A), st, tdt)
+        if mk not in mats:
+            mats[mk] = K.Matrix(A, storage=st, dtype=tdt, layout=layout)
+        xg = _dev(wl.uniform(A.m, fx["spec"]["xseed"]), ndt)
         bg = _dev(wl.uniform(A.m, fx["spec"]["bseed"]), ndt)
-        for k, (om, it, ty) in enumerate(SOR_CASES):
-            xg = _dev(x, ndt)
-            K.SOR(Ag, xg, bg, om, it, ty)
-            assert _same(xg.cpu().numpy(), fx[f"{name}_{st}_{c}_{k}"]), (name, st, c, om, it, ty)
+        K.SOR(mats[mk], xg, bg, om, it, ty, omega_dtype=torch.float64 if wide else None)
+        assert _same(xg.cpu().numpy(), fx[key]), key
 
 
-@pytest.mark.parametrize("st", ["row", "array"])
+@pytest.mark.parametrize("st", ["row", "array", "sym"])
 def test_sor_bitwise_vs_oracle(gpu, st):
     """Larger matrices (a stencil with few long levels, a banded random one
     with many short ones), and new values dropping the level plan."""
     from oracle import oracle as orc
     from seldon_amd import krylov as K
-    for name, A in (("laplace96", wl.laplace2d(96)), ("convdiff24", wl.convdiff3d(24)),
-                    ("powerlaw6k", wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500))):
+    mats = (("laplace96", wl.laplace2d(96)), ("convdiff24", wl.convdiff3d(24)),
+            ("powerlaw6k", wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500)))
+    if st == "sym":
+        mats = (("laplace96", wl.upper(wl.laplace2d(96))), ("aniso64", wl.upper(wl.aniso2d(64, 48, 0.01))),
+                ("powerlaw6k", wl.upper(wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500))))
+    for name, A in mats:
         Ag = K.Matrix(A, storage=st)
         x = wl.uniform(A.m, 61)
         b = wl.uniform(A.m, 62)
@@ -304,7 +309,7 @@ def test_sor_bitwise_vs_oracle(gpu, st):
         xg, bg = _dev(x, np.float64), _dev(b, np.float64)
         K.SOR(Ag, xg, bg, 1.25, 2, 0)
         assert _same(xg.cpu().numpy(), orc.sor(A, x, b, 1.25, 2, 0, storage=st)), (name, st)
-        B = wl.CSR(A.m, A.n, A.ptr, A.ind, A.val * 1.5)
+        B = wl.CSR(A.m, A.n, A.ptr, A.ind, A.val * 1.5, A.sym)
         Ag.set_values(B.val)
         xg = _dev(x, np.float64)
         K.SOR(Ag, xg, bg, 0.9, 1, 2)

@@ -262,26 +262,76 @@ def test_oracle_coo_to_csr_vs_live_seldon():
             assert ref[:2] == got[:2] and all(np.array_equal(a, b) for a, b in zip(ref[2:], got[2:]))
 
 
-@pytest.mark.parametrize("c", ["d", "f"])
-@pytest.mark.parametrize("st", ["row", "array"])
-def test_oracle_sor_matches_seldon_fixture(st, c):
-    """SOR / SSOR sweeps of the real Seldon build (forward, backward, both;
-    one and two sweeps; several omega), bitwise."""
-    from tests.golden.make_golden import SOR_CASES, sor_case
+def _sor_inputs(fx, A, dt):
+    return wl.uniform(A.m, fx["spec"]["xseed"]).astype(dt), wl.uniform(A.m, fx["spec"]["bseed"]).astype(dt)
+
+
+def _bytes_equal(a, b):
+    return a.dtype == b.dtype and a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def test_oracle_sor_matches_seldon_fixture():
+    """SOR / SSOR sweeps of the real Seldon build, bitwise: RowSparse,
+    ArrayRowSparse and RowSymSparse; forward, backward, both; one and two
+    sweeps; several omega; float values with a float and with a double
+    omega; rows with scrambled columns, one of which hides its diagonal
+    from Seldon's A(j, j)."""
+    from tests.golden.make_golden import sor_fixture_cases
     fx = load_fixture("sor.npz")
-    dt = np.float64 if c == "d" else np.float32
-    for name, A in sor_case().items():
-        x = wl.uniform(A.m, fx["spec"]["xseed"]).astype(dt)
-        b = wl.uniform(A.m, fx["spec"]["bseed"]).astype(dt)
-        for k, (om, it, ty) in enumerate(SOR_CASES):
-            got = orc.sor(A, x, b, dt(om), it, ty, storage=st, dtype=dt)
-            assert np.array_equal(got, fx[f"{name}_{st}_{c}_{k}"]), (name, st, c, om, it, ty)
+    n = 0
+    for key, A, st, dt, wide, om, it, ty in sor_fixture_cases():
+        x, b = _sor_inputs(fx, A, dt)
+        got = orc.sor(A, x, b, om, it, ty, storage=st, dtype=dt, omega_dtype=np.float64 if wide else None)
+        assert _bytes_equal(got, fx[key]), key
+        n += 1
+    assert n == len(fx) - 1
+
+
+def test_oracle_sor_fixture_has_teeth():
+    """The fixture tells apart the rules a restatement could plausibly get
+    wrong: taking the first diagonal entry of a row as a_jj (the scrambled
+    matrix with its diagonal moved to the front of row 0 gives a finite
+    x(0), Seldon's gives inf), omega in the value type where C++ promotes to
+    double, and the symmetric storage's sweeps read as the general ones on
+    the full matrix (equal to rounding, not bitwise)."""
+    from tests.golden.make_golden import sor_case
+    fx = load_fixture("sor.npz")
+    C, _, cases = sor_case()["convdiff8scrambled"]
+    front = C.ind.copy(), C.val.copy()
+    a, b = C.ptr[0], C.ptr[1]
+    o = np.argsort(C.ind[a:b] != 0, kind="stable")
+    front[0][a:b], front[1][a:b] = C.ind[a:b][o], C.val[a:b][o]
+    F = wl.CSR(C.m, C.n, C.ptr, *front)
+    x, bb = _sor_inputs(fx, C, np.float64)
+    om, it, ty = cases[0]
+    got = orc.sor(F, x, bb, om, it, ty)
+    assert np.isfinite(got[0]) and not np.isfinite(fx["convdiff8scrambled_row_d_0"][0])
+    # omega's type: at least one float case differs between float and double omega
+    A, _, cases = sor_case()["convdiff8"]
+    x, bb = _sor_inputs(fx, A, np.float32)
+    diffs = [not _bytes_equal(orc.sor(A, x, bb, om, it, ty, dtype=np.float32), fx[f"convdiff8_row_fw_{k}"])
+             for k, (om, it, ty) in enumerate(cases)]
+    assert any(diffs)
+    # symmetric storage vs the general sweep on the full matrix
+    U, _, cases = sor_case()["laplace16"]
+    full = U.to_scipy().tocsr()
+    full.sort_indices()
+    G = wl.CSR(U.m, U.n, full.indptr.astype(np.int32), full.indices.astype(np.int32), full.data.astype(np.float64))
+    x, bb = _sor_inputs(fx, U, np.float64)
+    same = []
+    for k, (om, it, ty) in enumerate(cases):
+        g = orc.sor(G, x, bb, om, it, ty)
+        ref = fx[f"laplace16_sym_d_{k}"]
+        assert np.abs(g - ref).max() <= 1e-12 * np.abs(ref).max()
+        same.append(_bytes_equal(g, ref))
+    assert not all(same)
 
 
 def test_oracle_sor_closed_forms():
     """With omega = 1 one forward sweep on a lower-triangular matrix is the
     forward substitution (numpy's triangular solve to rounding), and
-    symmetric sweeps converge to A^-1 b on an SPD matrix."""
+    symmetric sweeps converge to A^-1 b on an SPD matrix, on the general and
+    on the symmetric storage."""
     import scipy.linalg as sla
     A = wl.laplace2d(12)
     D = A.to_scipy().toarray()
@@ -293,24 +343,35 @@ def test_oracle_sor_closed_forms():
     x = orc.sor(T, np.zeros(A.m), b, 1.0, 1, 2)
     ref = sla.solve_triangular(Lo, b, lower=True)
     assert np.abs(x - ref).max() <= 1e-12 * np.abs(ref).max()
-    x = orc.sor(A, np.zeros(A.m), b, 1.5, 400, 0)
-    assert np.abs(x - np.linalg.solve(D, b)).max() < 1e-8
+    sol = np.linalg.solve(D, b)
+    assert np.abs(orc.sor(A, np.zeros(A.m), b, 1.5, 400, 0) - sol).max() < 1e-8
+    assert np.abs(orc.sor(wl.upper(A), np.zeros(A.m), b, 1.5, 400, 0) - sol).max() < 1e-8
 
 
 @pytest.mark.skipif(not orc.ref_available("generic"), reason="oracle/_ref not built (no reference tree)")
 def test_oracle_sor_vs_live_seldon():
+    """Fresh inputs through the reference build: general storages on
+    power-law matrices, the symmetric storage on an upper Laplacian, float
+    values with a double omega."""
     with tempfile.TemporaryDirectory() as tmp:
+        runs = []
         for seed in range(2):
             A = wl.powerlaw(3000 + 1000 * seed, 70 + seed, kmin=2, kmax=40, half_width=300)
-            x = wl.uniform(A.m, 80 + seed)
-            b = wl.uniform(A.m, 90 + seed)
-            orc.write_seldon_csr(f"{tmp}/A.bin", A)
+            runs += [(A, st, np.float64, False) for st in ("row", "array")]
+        runs += [(wl.upper(wl.laplace2d(40, 30)), "sym", np.float64, False),
+                 (wl.upper(wl.laplace2d(40, 30)), "sym", np.float32, True),
+                 (wl.powerlaw(2000, 72, kmin=2, kmax=40, half_width=300), "row", np.float32, True)]
+        for i, (A, st, dt, wide) in enumerate(runs):
+            x = wl.uniform(A.m, 80 + i).astype(dt)
+            b = wl.uniform(A.m, 90 + i).astype(dt)
+            orc.write_seldon_csr(f"{tmp}/A.bin", A, dt)
             orc.write_seldon_vec(f"{tmp}/x.bin", x)
             orc.write_seldon_vec(f"{tmp}/b.bin", b)
-            for st in ("row", "array"):
-                orc.run_ref(["sor", st, "d", f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/b.bin", "1.2", 3, 0,
-                             f"{tmp}/o.bin"])
-                assert np.array_equal(orc.read_seldon_vec(f"{tmp}/o.bin"), orc.sor(A, x, b, 1.2, 3, 0, storage=st))
+            c = "d" if dt == np.float64 else "f"
+            orc.run_ref(["sor", st, c, f"{tmp}/A.bin", f"{tmp}/x.bin", f"{tmp}/b.bin", "1.2", 3, 0, f"{tmp}/o.bin"]
+                        + (["wide"] if wide else []))
+            got = orc.sor(A, x, b, 1.2, 3, 0, storage=st, dtype=dt, omega_dtype=np.float64 if wide else None)
+            assert _bytes_equal(orc.read_seldon_vec(f"{tmp}/o.bin", dt), got), (i, st)
 
 
 def test_laplacian_eigenpairs():
