@@ -13,7 +13,7 @@
 // others.  Rows are grouped in levels such that a row comes after every
 // earlier-swept row whose X it reads and before every later-swept row that
 // reads its X (both directions of each stored entry count); rows of one level
-// then neither read nor write each other's X, so a level runs as one launch,
+// then neither read nor write each other's X, so a level runs in parallel,
 // one thread per row, and every row sees exactly the operands of the
 // sequential sweep: the results are bitwise Seldon's.
 //
@@ -36,6 +36,18 @@
 
 namespace shk {
 
+// Consecutive levels of at most kRunRows rows are run by one work-group of
+// kRunThreads threads, level after level with a barrier between them (the
+// barrier orders each level's X stores before the next level's loads within
+// the work-group); larger levels get a launch each.
+constexpr int kRunThreads = 1024;
+constexpr int kRunRows = 1024;
+
+struct LevelSeg {
+    int l0, l1;  // levels [l0, l1)
+    bool run;    // one work-group steps through them; else one level, one launch
+};
+
 struct SorPlan {
     int m = 0, storage = 0, dtype = 0;
     int *ptr = nullptr, *ind = nullptr;  // the stored rows
@@ -43,29 +55,83 @@ struct SorPlan {
     int *tptr = nullptr, *tind = nullptr;  // RowSymSparse: strict upper entries by column
     void *tval = nullptr, *z = nullptr;    // (rows increasing), and the half-step vector
     int *rows[2] = {nullptr, nullptr};     // forward, backward: rows grouped by level
-    std::vector<int> off[2];               // level offsets into rows[d] (host)
+    int *doff[2] = {nullptr, nullptr};     // level offsets into rows[d] (device)
+    std::vector<int> off[2];               // the same (host)
+    std::vector<LevelSeg> segs[2];         // launches of direction d
 };
 
-// ---- general storages ----------------------------------------------------
+// ---- row operations ------------------------------------------------------
+// general storages: one row of the sweep
 template <typename T, typename W>
-__global__ __launch_bounds__(kBlock) void k_sor_rows(const int *__restrict__ rows, int count, const int *__restrict__ ptr,
-                                                     const int *__restrict__ ind, const T *__restrict__ val,
-                                                     const T *__restrict__ diag, const T *__restrict__ B, T *X,
-                                                     W omega)
+struct SorRow {
+    const int *ptr, *ind;
+    const T *val, *diag, *B;
+    T *X;
+    W omega;
+    __device__ __forceinline__ void operator()(int j) const
+    {
+        T temp = (T)0;
+        for (int k = ptr[j]; k < ptr[j + 1]; k++) temp += val[k] * X[ind[k]];
+        const T ajj = diag[j];
+        const T xj = X[j];
+        temp = B[j] - temp + ajj * xj;
+        X[j] = (T)(((W)1 - omega) * (W)xj + omega * (W)temp / (W)ajj);
+    }
+};
+
+// RowSymSparse forward solve: X(p) = (Z(p) - sum_j a_jp X(j)) * (w / ajj)
+template <typename T, typename W>
+struct SymForward {
+    const int *tptr, *tind;
+    const T *tval, *diag, *Z;
+    T *X;
+    W omega;
+    __device__ __forceinline__ void operator()(int p) const
+    {
+        T x = Z[p];
+        for (int k = tptr[p]; k < tptr[p + 1]; k++) x -= tval[k] * X[tind[k]];
+        X[p] = (T)((W)x * (omega / (W)diag[p]));
+    }
+};
+
+// RowSymSparse backward solve: X(j) = (Z(j) - sum_{p != j} a_jp X(p)) w / ajj
+template <typename T, typename W>
+struct SymBackward {
+    const int *ptr, *ind;
+    const T *val, *diag, *Z;
+    T *X;
+    W omega;
+    __device__ __forceinline__ void operator()(int j) const
+    {
+        T temp = (T)0;
+        for (int k = ptr[j]; k < ptr[j + 1]; k++)
+            if (ind[k] != j) temp += val[k] * X[ind[k]];
+        X[j] = (T)((W)(Z[j] - temp) * omega / (W)diag[j]);
+    }
+};
+
+// one level, one thread per row
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_level(const int *__restrict__ rows, int count, Op op)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= count) return;
-    const int j = rows[t];
-    T temp = (T)0;
-    for (int k = ptr[j]; k < ptr[j + 1]; k++) temp += val[k] * X[ind[k]];
-    const T ajj = diag[j];
-    const T xj = X[j];
-    temp = B[j] - temp + ajj * xj;
-    X[j] = (T)(((W)1 - omega) * (W)xj + omega * (W)temp / (W)ajj);
+    if (t < count) op(rows[t]);
 }
 
-// ---- RowSymSparse --------------------------------------------------------
-// forward, first pass: Z(j) = (1-w)/w ajj X(j) + (B(j) - sum_{p != j} a_jp X(p))
+// levels [l0, l1) in one work-group
+template <class Op>
+__global__ __launch_bounds__(kRunThreads) void k_level_run(const int *__restrict__ rows, const int *__restrict__ off,
+                                                           int l0, int l1, Op op)
+{
+    for (int l = l0; l < l1; l++) {
+        const int end = off[l + 1];
+        for (int t = off[l] + (int)threadIdx.x; t < end; t += kRunThreads) op(rows[t]);
+        __syncthreads();
+    }
+}
+
+// ---- RowSymSparse explicit halves ----------------------------------------
+// forward: Z(j) = (1-w)/w ajj X(j) + (B(j) - sum_{p != j} a_jp X(p))
 template <typename T, typename W>
 __global__ __launch_bounds__(kBlock) void k_sor_sym_f1(int m, const int *__restrict__ ptr, const int *__restrict__ ind,
                                                        const T *__restrict__ val, const T *__restrict__ diag,
@@ -81,22 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_sym_f1(int m, const int *__restr
     Z[j] = (T)(((W)1 - omega) / omega * (W)diag[j] * (W)X[j] + (W)temp);
 }
 
-// forward, second pass (one level): X(p) = (Z(p) - sum_j a_jp X(j)) * (w / ajj)
-template <typename T, typename W>
-__global__ __launch_bounds__(kBlock) void k_sor_sym_f2(const int *__restrict__ rows, int count,
-                                                       const int *__restrict__ tptr, const int *__restrict__ tind,
-                                                       const T *__restrict__ tval, const T *__restrict__ diag,
-                                                       const T *__restrict__ Z, T *X, W omega)
-{
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= count) return;
-    const int p = rows[t];
-    T x = Z[p];
-    for (int k = tptr[p]; k < tptr[p + 1]; k++) x -= tval[k] * X[tind[k]];
-    X[p] = (T)((W)x * (omega / (W)diag[p]));
-}
-
-// backward, first pass: Z(j) = (1-w)/w ajj X(j) + B(j) - sum_{i<j} a_ij X(i)
+// backward: Z(j) = (1-w)/w ajj X(j) + B(j) - sum_{i<j} a_ij X(i)
 template <typename T, typename W>
 __global__ __launch_bounds__(kBlock) void k_sor_sym_b1(int m, const int *__restrict__ tptr,
                                                        const int *__restrict__ tind, const T *__restrict__ tval,
@@ -108,22 +159,6 @@ __global__ __launch_bounds__(kBlock) void k_sor_sym_b1(int m, const int *__restr
     T y = (T)0;
     for (int k = tptr[j]; k < tptr[j + 1]; k++) y += tval[k] * X[tind[k]];
     Z[j] = (T)(((W)1 - omega) / omega * (W)diag[j] * (W)X[j] + (W)B[j] - (W)y);
-}
-
-// backward, second pass (one level): X(j) = (Z(j) - sum_{p != j} a_jp X(p)) w / ajj
-template <typename T, typename W>
-__global__ __launch_bounds__(kBlock) void k_sor_sym_b2(const int *__restrict__ rows, int count,
-                                                       const int *__restrict__ ptr, const int *__restrict__ ind,
-                                                       const T *__restrict__ val, const T *__restrict__ diag,
-                                                       const T *__restrict__ Z, T *X, W omega)
-{
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= count) return;
-    const int j = rows[t];
-    T temp = (T)0;
-    for (int k = ptr[j]; k < ptr[j + 1]; k++)
-        if (ind[k] != j) temp += val[k] * X[ind[k]];
-    X[j] = (T)((W)(Z[j] - temp) * omega / (W)diag[j]);
 }
 
 // ---- plan ----------------------------------------------------------------
@@ -155,6 +190,25 @@ static void sor_levels(int m, const std::vector<int> &ptr, const std::vector<int
     std::vector<int> next(off.begin(), off.end() - 1);
     rows.assign(m, 0);
     for (int j = 0; j < m; j++) rows[next[lev[j]]++] = j;
+}
+
+// Launches of one direction: runs of consecutive small levels, and the
+// larger levels one by one.
+static void level_segments(const std::vector<int> &off, std::vector<LevelSeg> &segs)
+{
+    segs.clear();
+    const int nlev = (int)off.size() - 1;
+    for (int l = 0; l < nlev;) {
+        int e = l;
+        while (e < nlev && off[(size_t)e + 1] - off[e] <= kRunRows) e++;
+        if (e - l >= 2) {
+            segs.push_back({l, e, true});
+            l = e;
+        } else {
+            segs.push_back({l, l + 1, false});
+            l++;
+        }
+    }
 }
 
 // Seldon's A(j, j) on RowSparse (matrix_sparse/Matrix_Sparse.cxx:1054-1084):
@@ -234,12 +288,14 @@ static int sor_build(shk_csr A, SorPlan **out)
     P->dtype = A->dtype;
     P->off[0] = std::move(off[0]);
     P->off[1] = std::move(off[1]);
+    for (int d = 0; d < 2; d++) level_segments(P->off[d], P->segs[d]);
     hipStream_t s = A->ctx->stream;
     hipError_t e = put(&P->ptr, ptr, s);
     if (e == hipSuccess) e = put(&P->ind, ind, s);
     if (e == hipSuccess) e = put((T **)&P->val, val, s);
     if (e == hipSuccess) e = put((T **)&P->diag, diag, s);
     for (int d = 0; d < 2 && e == hipSuccess; d++) e = put(&P->rows[d], rows[d], s);
+    for (int d = 0; d < 2 && e == hipSuccess; d++) e = put(&P->doff[d], P->off[d], s);
     if (sym) {
         if (e == hipSuccess) e = put(&P->tptr, tptr, s);
         if (e == hipSuccess) e = put(&P->tind, tind, s);
@@ -268,10 +324,27 @@ void sor_free(SorPlan *P)
     hipFree(P->z);
     hipFree(P->rows[0]);
     hipFree(P->rows[1]);
+    hipFree(P->doff[0]);
+    hipFree(P->doff[1]);
     delete P;
 }
 
 static inline dim3 grid_for(int count) { return dim3((count + kBlock - 1) / kBlock); }
+
+template <class Op>
+static void run_levels(const SorPlan *P, int d, const Op &op, hipStream_t s)
+{
+    for (const LevelSeg &g : P->segs[d]) {
+        if (g.run) {
+            hipLaunchKernelGGL(k_level_run<Op>, dim3(1), dim3(kRunThreads), 0, s, (const int *)P->rows[d],
+                               (const int *)P->doff[d], g.l0, g.l1, op);
+        } else {
+            const int first = P->off[d][g.l0], count = P->off[d][g.l1] - first;
+            hipLaunchKernelGGL(k_level<Op>, grid_for(count), dim3(kBlock), 0, s, (const int *)P->rows[d] + first,
+                               count, op);
+        }
+    }
+}
 
 template <typename T, typename W>
 static int sor_run(shk_ctx ctx, const SorPlan *P, T *X, const T *B, W omega, int iter, int type_ssor)
@@ -279,31 +352,17 @@ static int sor_run(shk_ctx ctx, const SorPlan *P, T *X, const T *B, W omega, int
     hipStream_t s = ctx->stream;
     const T *val = (const T *)P->val, *diag = (const T *)P->diag, *tval = (const T *)P->tval;
     T *Z = (T *)P->z;
-    // one launch per level of direction d
-    auto levels = [&](int d, auto launch) {
-        const std::vector<int> &off = P->off[d];
-        for (size_t l = 0; l + 1 < off.size(); l++) launch(P->rows[d] + off[l], off[l + 1] - off[l]);
-    };
     auto sweep = [&](int d) {
         if (P->storage != SHK_ROWSYMSPARSE) {
-            levels(d, [&](const int *rows, int count) {
-                hipLaunchKernelGGL((k_sor_rows<T, W>), grid_for(count), dim3(kBlock), 0, s, rows, count, P->ptr,
-                                   P->ind, val, diag, B, X, omega);
-            });
+            run_levels(P, d, SorRow<T, W>{P->ptr, P->ind, val, diag, B, X, omega}, s);
         } else if (d == 0) {
             hipLaunchKernelGGL((k_sor_sym_f1<T, W>), grid_for(P->m), dim3(kBlock), 0, s, P->m, P->ptr, P->ind, val,
                                diag, B, (const T *)X, Z, omega);
-            levels(0, [&](const int *rows, int count) {
-                hipLaunchKernelGGL((k_sor_sym_f2<T, W>), grid_for(count), dim3(kBlock), 0, s, rows, count, P->tptr,
-                                   P->tind, tval, diag, (const T *)Z, X, omega);
-            });
+            run_levels(P, 0, SymForward<T, W>{P->tptr, P->tind, tval, diag, Z, X, omega}, s);
         } else {
             hipLaunchKernelGGL((k_sor_sym_b1<T, W>), grid_for(P->m), dim3(kBlock), 0, s, P->m, P->tptr, P->tind,
                                tval, diag, B, (const T *)X, Z, omega);
-            levels(1, [&](const int *rows, int count) {
-                hipLaunchKernelGGL((k_sor_sym_b2<T, W>), grid_for(count), dim3(kBlock), 0, s, rows, count, P->ptr,
-                                   P->ind, val, diag, (const T *)Z, X, omega);
-            });
+            run_levels(P, 1, SymBackward<T, W>{P->ptr, P->ind, val, diag, Z, X, omega}, s);
         }
     };
     if (type_ssor % 2 == 0)

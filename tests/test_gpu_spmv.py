@@ -296,11 +296,15 @@ def test_sor_bitwise_vs_oracle(gpu, st):
     with many short ones), and new values dropping the level plan."""
     from oracle import oracle as orc
     from seldon_amd import krylov as K
+    # levels of at most 1024 rows run in one work-group, larger ones get a
+    # launch each: the 3-D matrices (middle levels of ~1700-3000 rows) mix both
     mats = (("laplace96", wl.laplace2d(96)), ("convdiff24", wl.convdiff3d(24)),
-            ("powerlaw6k", wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500)))
+            ("powerlaw6k", wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500)),
+            ("convdiff64", wl.convdiff3d(64)))
     if st == "sym":
         mats = (("laplace96", wl.upper(wl.laplace2d(96))), ("aniso64", wl.upper(wl.aniso2d(64, 48, 0.01))),
-                ("powerlaw6k", wl.upper(wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500))))
+                ("powerlaw6k", wl.upper(wl.powerlaw(6000, 23, kmin=2, kmax=60, half_width=500))),
+                ("convdiff48", wl.upper(wl.convdiff3d(48))))
     for name, A in mats:
         Ag = K.Matrix(A, storage=st)
         x = wl.uniform(A.m, 61)
