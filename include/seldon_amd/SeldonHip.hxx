@@ -295,6 +295,15 @@ namespace Seldon
       SetData(A.GetM(), A.GetN(), (int)ind.size(), &ptr[0], ind.empty() ? 0 : &ind[0],
               val.empty() ? 0 : &val[0]);
     }
+    // takes over a matrix the library created (read from a file)
+    void Adopt(shk_csr h)
+    {
+      release();
+      h_ = h;
+      int64_t nz = 0;
+      HipCheck(shk_csr_info(h, &m_, &n_, &nz, 0, 0, 0), "Matrix<Hip>::Adopt");
+      nz_ = (int)nz;
+    }
     void SetValues(const T* values)
     {
       HipCheck(shk_csr_set_values(HipContext::Get().Handle(), h_, values, 0), "Matrix<Hip>::SetValues");
@@ -332,6 +341,33 @@ namespace Seldon
     template <class HostArray>
     explicit Matrix(const HostArray& A) { this->CopyArray(A); }
   };
+
+  // WriteHarwellBoeing(A, file_name) (matrix_sparse/IOMatrixMarket.cxx:397-510):
+  // the file Seldon writes for the same storage and arrays, byte for byte.
+  // One overload per device storage, more specialised than Seldon's template.
+#define SELDON_HIP_WRITE_HB(S)                                                                           \
+  template <class T, class Prop, class Allocator>                                                        \
+  void WriteHarwellBoeing(const Matrix<T, Prop, S, Allocator>& A, const std::string& file_name)          \
+  {                                                                                                      \
+    HipCheck(shk_csr_write_hb(HipContext::Get().Handle(), A.GetHandle(), file_name.c_str()),             \
+             "WriteHarwellBoeing");                                                                      \
+  }
+  SELDON_HIP_WRITE_HB(RowSparseHip)
+  SELDON_HIP_WRITE_HB(RowSymSparseHip)
+  SELDON_HIP_WRITE_HB(ArrayRowSparseHip)
+#undef SELDON_HIP_WRITE_HB
+
+  // ReadHarwellBoeing(filename, A) (:80-360) into a RowSparse device matrix:
+  // the matrix a Seldon program gets by reading into ColSparse and converting
+  // through coordinates to RowSparse, its rows assembled on the device.
+  template <class T, class Prop, class Allocator>
+  void ReadHarwellBoeing(std::string filename, Matrix<T, Prop, RowSparseHip, Allocator>& A)
+  {
+    shk_csr h = 0;
+    HipCheck(shk_csr_read_hb(HipContext::Get().Handle(), filename.c_str(), HipType<T>::value, 0, &h),
+             "ReadHarwellBoeing");
+    A.Adopt(h);
+  }
 
   template <class S> struct IsHipSparse : std::false_type {};
   template <> struct IsHipSparse<RowSparseHip> : std::true_type {};

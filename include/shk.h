@@ -117,6 +117,18 @@ int shk_csr_layout(shk_csr A, int *layout);
  * value type (the file's) and layout. */
 int shk_csr_write(shk_ctx ctx, shk_csr A, const char *path);
 int shk_csr_read(shk_ctx ctx, const char *path, shk_storage storage, shk_dtype dtype, int layout, shk_csr *out);
+/* Harwell-Boeing files (matrix_sparse/IOMatrixMarket.cxx).  shk_csr_write_hb
+ * is WriteHarwellBoeing(A, path) (:397-510): the file Seldon writes for the
+ * same storage and arrays, byte for byte (compressed columns in ConvertToCSC's
+ * order, values at 15 digits; the symmetric storage is written whole).
+ * shk_csr_read_hb is ReadHarwellBoeing(path, A) (:80-360) into a ColSparse
+ * matrix followed by ConvertMatrix_to_Coordinates / _from_Coordinates into a
+ * RowSparse matrix of the same size: it creates that RowSparse matrix, its
+ * rows assembled on the device.  Real and pattern matrices, as Seldon reads;
+ * Seldon's reader cannot parse a D-format value line such as the "(3D23.15)"
+ * its own writer puts, and neither can this one. */
+int shk_csr_write_hb(shk_ctx ctx, shk_csr A, const char *path);
+int shk_csr_read_hb(shk_ctx ctx, const char *path, shk_dtype dtype, int layout, shk_csr *out);
 int shk_csr_destroy(shk_csr A);
 /* New values on the same pattern, in the caller's stored order (RowSymSparse:
  * upper part).  Used when A depends on parameters (the differentiable solve). */

@@ -367,3 +367,144 @@ def diffusion_grad_reference(n, theta, values, pattern, b, x_obs, tol=1e-13, max
         dk[bnd] += gd[bnd]  # boundary face: k_f = k_c
     return dict(x=x, lam=lam, grad_theta=k * dk, grad_b=lam, loss=float(np.sum((x - x_obs) ** 2)),
                 its=(fx.nb_iter, ad.nb_iter))
+
+
+# ---- Harwell-Boeing files (matrix_sparse/IOMatrixMarket.cxx) --------------
+def _libc():
+    c = ctypes.CDLL(None)
+    c.strtof.restype = ctypes.c_float
+    c.strtof.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    c.strtod.restype = ctypes.c_double
+    c.strtod.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    return c
+
+
+def hb_write_text(A, storage, dtype=np.float64) -> bytes:
+    """WriteHarwellBoeing(A, name) (IOMatrixMarket.cxx:397-510) for the
+    stored arrays of A: ConvertToCSC's order (row-major coordinates, the
+    symmetric storage mirrored after each off-diagonal entry,
+    Matrix_Conversions.cxx:113-160, then MergeSort by column -- stable;
+    Functions_Arrays.cxx:412-480), then Seldon's left-aligned layout."""
+    import math
+    rows, cols, vals = [], [], []
+    val = np.asarray(A.val, dtype)
+    for i in range(A.m):
+        for k in range(A.ptr[i], A.ptr[i + 1]):
+            c = int(A.ind[k])
+            rows.append(i); cols.append(c); vals.append(val[k])
+            if storage == "sym" and c != i:
+                rows.append(c); cols.append(i); vals.append(val[k])
+    nnz, m, n = len(rows), A.m, A.n
+    key = (lambda e: (cols[e], rows[e])) if storage == "sym" else (lambda e: cols[e])
+    order = sorted(range(nnz), key=key)  # Python's sort is stable
+    cptr = np.zeros(n + 1, np.int64)
+    np.add.at(cptr, np.asarray(cols, np.int64) + 1, 1)
+    cptr = np.cumsum(cptr)
+    ptr_len = int(math.ceil(math.log10(0.1 + nnz + 1))) + 1
+    ptr_np = min(80 // ptr_len, n + 1)
+    ind_len = int(math.ceil(math.log10(0.1 + n + 1))) + 1
+    ind_np = min(80 // ind_len, nnz)
+    ptrcrd, indcrd, valcrd = n // ptr_np + 1, (nnz - 1) // ind_np + 1, (nnz - 1) // 3 + 1
+    out = [f"{'Seldon':<72}{'S0000008':<8}\n",
+           "".join(f"{x:<14d}" for x in (ptrcrd + indcrd + valcrd, ptrcrd, indcrd, valcrd, 0)) + "\n",
+           ("RUA" if m == n else "RRA") + " " * 11 + "".join(f"{x:<14d}" for x in (m, n, nnz, 0)) + "\n",
+           f"{'(%dI%d)' % (ptr_np, ptr_len):<16}{'(%dI%d)' % (ind_np, ind_len):<16}{'(3D23.15)':<20}"
+           f"{'(3D23.15)':<20}\n"]
+    for i in range(0, n + 1, ptr_np):
+        out.append("".join(f"{int(cptr[j]) + 1:<{ptr_len}d}" for j in range(i, min(i + ptr_np, n + 1))) + "\n")
+    for i in range(0, nnz, ind_np):
+        out.append("".join(f"{rows[order[j]] + 1:<{ind_len}d}" for j in range(i, min(i + ind_np, nnz))) + "\n")
+    for i in range(0, nnz, 3):
+        out.append("".join(f"{float(vals[order[j]]):<23.15e}" for j in range(i, min(i + 3, nnz))) + "\n")
+    return "".join(out).encode()
+
+
+def _istream_int(s):
+    t = s.lstrip(" \t\n\r\v\f")
+    j = 1 if t[:1] in "+-" and t[:1] != "" else 0
+    k = j
+    while k < len(t) and t[k].isdigit():
+        k += 1
+    if k == j:
+        raise ValueError(f"no integer in {s!r}")
+    return int(t[:k])
+
+
+def _istream_value(s, dtype):
+    import re
+    t = s.lstrip(" \t\n\r\v\f")
+    mt = re.match(r"[+-]?(\d*)(\.\d*)?", t)
+    if not mt or not (mt.group(1) or (mt.group(2) or "")[1:]):
+        raise ValueError(f"no value in {s!r}")
+    tok = mt.group(0)
+    me = re.match(r"[eE][+-]?\d+", t[len(tok):])
+    if t[len(tok):len(tok) + 1] in ("e", "E") and not me:
+        raise ValueError(f"exponent without digits in {s!r}")
+    if me:
+        tok += me.group(0)
+    c = _libc()
+    return dtype((c.strtod if dtype == np.float64 else c.strtof)(tok.encode(), None))
+
+
+def hb_read(text: bytes, dtype=np.float64):
+    """ReadHarwellBoeing(name, A) (IOMatrixMarket.cxx:80-360) into a
+    ColSparse matrix, then ConvertMatrix_to_Coordinates and
+    ConvertMatrix_from_Coordinates into a RowSparse matrix of the same size:
+    returns (m, n, ptr, ind, val) of the latter."""
+    lines = text.decode().split("\n")
+    if len(lines[0]) < 72:
+        raise ValueError("title line shorter than 72 characters")
+    pos = 1
+    toks = []
+    while len(toks) < 10:  # formatted extraction over lines 2 and 3
+        toks += lines[pos].split()
+        pos += 1
+    nl_ptr, nl_ind, nl_val, nl_rhs = (int(x) for x in toks[1:5])
+    typ, nrow, ncol, nnz, nelt = toks[5], int(toks[6]), int(toks[7]), int(toks[8]), int(toks[9])
+    if len(typ) != 3 or typ[0] not in "RCP" or typ[0] == "C" or typ[1] in "HZS" or nelt != 0 or typ[2] == "E":
+        raise ValueError(f"unsupported matrix type {typ}")
+    fmt = lines[pos]
+    pos += 1
+    import re
+    lw_ptr, ew_ptr = (int(x) for x in re.match(r"\((\d+)I(\d+)\)", fmt[0:16]).groups())
+    lw_ind, ew_ind = (int(x) for x in re.match(r"\((\d+)I(\d+)\)", fmt[16:32]).groups())
+    if typ[0] != "P":
+        element = fmt[32:48]
+        vect = []
+        for tok in re.split(r"[ (ABEFGILOPZ*.)]+", element):
+            if tok:
+                try:
+                    vect.append(_istream_int(tok))
+                except ValueError:
+                    vect.append(0)
+        if len(vect) < 3:
+            raise ValueError(f"value format {element!r} could not be parsed")
+        lw_val, ew_val = vect[-3], vect[-2]
+    if nl_rhs != 0:
+        pos += 1
+
+    def fields(count, per_line, width, parse):
+        nonlocal pos
+        out = []
+        while len(out) < count:
+            line = lines[pos]
+            pos += 1
+            k = 0
+            for _ in range(per_line):
+                if len(out) == count:
+                    break
+                if k > len(line):
+                    raise ValueError("missing field")
+                out.append(parse(line[k:k + width]))
+                k += width
+        return out
+
+    cptr = np.array(fields(ncol + 1, lw_ptr, ew_ptr, _istream_int), np.int64) - 1
+    rind = np.array(fields(nnz, lw_ind, ew_ind, _istream_int), np.int64) - 1
+    if typ[0] == "P":
+        val = np.ones(nnz, dtype)
+    else:
+        val = np.array(fields(nnz, lw_val, ew_val, lambda s: _istream_value(s, dtype)), dtype)
+    ccol = np.repeat(np.arange(ncol), np.diff(cptr))
+    m, n, ptr, ind, vv = coo_to_csr(rind.astype(np.int32), ccol.astype(np.int32), val, 0, nrow, ncol, dtype)
+    return m, n, ptr, ind, vv

@@ -20,6 +20,11 @@
 //   seldon_ref coo2csr <d|f> rows cols vals index m_min n_min out
 //                    (ConvertMatrix_from_Coordinates into a RowSparse matrix
 //                     first sized m_min x n_min; writes it as a binary CSR)
+//   seldon_ref hbwrite <row|sym|array> <d|f> A out   (WriteHarwellBoeing)
+//   seldon_ref hbread <d|f> file out
+//                    (ReadHarwellBoeing into a ColSparse matrix, then
+//                     ConvertMatrix_to_Coordinates / _from_Coordinates into a
+//                     RowSparse matrix of the same size; a binary CSR)
 //   seldon_ref solve <cg|bicgstab|gmres> <row|sym|array> <id|jacobi>
 //                    A b x0|- tol maxit restart out
 // The Jacobi preconditioner does not exist in the reference; it is defined
@@ -27,6 +32,7 @@
 // division form of SOR's /a_jj, matrix_sparse/Relaxation_MatVect.cxx:79).
 #include "Seldon.hxx"
 #include "SeldonSolver.hxx"
+#include "matrix_sparse/IOMatrixMarket.cxx"
 
 #include <cstdlib>
 #include <cstring>
@@ -280,6 +286,46 @@ static int run_coo2csr(char** a)
   return 0;
 }
 
+template <class T>
+static int run_hbwrite(const string& st, char** a)
+{
+  if (st == "sym")
+    {
+      Matrix<T, Symmetric, RowSymSparse> S;
+      S.Read(a[0]);
+      WriteHarwellBoeing(S, a[1]);
+      return 0;
+    }
+  Matrix<T, General, RowSparse> A;
+  A.Read(a[0]);
+  if (st == "array")
+    {
+      Matrix<T, General, ArrayRowSparse> B;
+      to_array(A, B);
+      WriteHarwellBoeing(B, a[1]);
+    }
+  else
+    WriteHarwellBoeing(A, a[1]);
+  return 0;
+}
+
+template <class T>
+static int run_hbread(char** a)
+{
+  Matrix<T, General, ColSparse> C;
+  ReadHarwellBoeing(a[0], C);
+  // the conversion Clear()s the index vectors (see run_coo2csr): heap
+  // objects that are not deleted
+  Vector<int>* row = new Vector<int>;
+  Vector<int>* col = new Vector<int>;
+  Vector<T>* val = new Vector<T>;
+  ConvertMatrix_to_Coordinates(C, *row, *col, *val, 0, false);
+  Matrix<T, General, RowSparse> A(C.GetM(), C.GetN());
+  ConvertMatrix_from_Coordinates(*row, *col, *val, A, 0);
+  A.Write(a[1]);
+  return 0;
+}
+
 int main(int argc, char** argv)
 {
   try
@@ -305,6 +351,11 @@ int main(int argc, char** argv)
             return run_sor<float, double>(argv[2], argv + 4);
           return run_sor<float, float>(argv[2], argv + 4);
         }
+      if (argc >= 6 && string(argv[1]) == "hbwrite")
+        return string(argv[3]) == "f" ? run_hbwrite<float>(argv[2], argv + 4)
+                                      : run_hbwrite<double>(argv[2], argv + 4);
+      if (argc >= 5 && string(argv[1]) == "hbread")
+        return string(argv[2]) == "f" ? run_hbread<float>(argv + 3) : run_hbread<double>(argv + 3);
       if (argc >= 10 && string(argv[1]) == "coo2csr")
         return string(argv[2]) == "f" ? run_coo2csr<float>(argv + 3) : run_coo2csr<double>(argv + 3);
       if (argc >= 12 && string(argv[1]) == "solve")

@@ -125,22 +125,25 @@ class Matrix:
         L.check(L.load().shk_csr_write(self.ctx.h, self.h, str(path).encode()))
 
     @classmethod
-    def Read(cls, path: str, storage: str = "row", dtype=torch.float64, layout: str = "auto",
-             ctx: Context | None = None) -> "Matrix":
-        """A matrix from Seldon's binary CSR file (Matrix_Sparse.cxx:1649-1682)."""
+    def _wrap(cls, h, ctx, storage, dtype) -> "Matrix":
         self = cls.__new__(cls)
-        self.ctx = ctx or Context.default()
-        self.storage, self.dtype = storage, dtype
-        h = ctypes.c_void_p()
-        L.check(L.load().shk_csr_read(self.ctx.h, str(path).encode(), _STORAGE[storage], _DT[dtype],
-                                      cls.LAYOUTS[layout], ctypes.byref(h)))
-        self.h = h
+        self.ctx, self.storage, self.dtype, self.h = ctx, storage, dtype, h
         m, n, a, b, d, st = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(),
                              ctypes.c_int(), ctypes.c_int())
         L.check(L.load().shk_csr_info(h, ctypes.byref(m), ctypes.byref(n), ctypes.byref(a), ctypes.byref(b),
                                       ctypes.byref(d), ctypes.byref(st)))
         self.m, self.n, self.nnz = m.value, n.value, int(a.value)
         return self
+
+    @classmethod
+    def Read(cls, path: str, storage: str = "row", dtype=torch.float64, layout: str = "auto",
+             ctx: Context | None = None) -> "Matrix":
+        """A matrix from Seldon's binary CSR file (Matrix_Sparse.cxx:1649-1682)."""
+        ctx = ctx or Context.default()
+        h = ctypes.c_void_p()
+        L.check(L.load().shk_csr_read(ctx.h, str(path).encode(), _STORAGE[storage], _DT[dtype],
+                                      cls.LAYOUTS[layout], ctypes.byref(h)))
+        return cls._wrap(h, ctx, storage, dtype)
 
     @property
     def layout(self) -> str:
@@ -283,6 +286,23 @@ def ConvertMatrix_from_Coordinates(IndRow: torch.Tensor, IndCol: torch.Tensor, V
                                     _ptr(val) if nnz else null, int(index), int(m), int(n), mm.value,
                                     ctypes.byref(mm), ctypes.byref(nn), _ptr(ptr), _ptr(ind), _ptr(out)))
     return mm.value, nn.value, ptr, ind[:nnz], out[:nnz]
+
+
+def WriteHarwellBoeing(A: Matrix, path: str):
+    """WriteHarwellBoeing(A, path) (IOMatrixMarket.cxx:397-510): the file
+    Seldon writes for the same storage and arrays, byte for byte."""
+    L.check(L.load().shk_csr_write_hb(A.ctx.h, A.h, str(path).encode()))
+
+
+def ReadHarwellBoeing(path: str, dtype=torch.float64, layout: str = "auto", ctx: Context | None = None) -> Matrix:
+    """ReadHarwellBoeing(path, A) (IOMatrixMarket.cxx:80-360) into a ColSparse
+    matrix, then to RowSparse through coordinates, as a Seldon program does:
+    the RowSparse matrix, assembled on the device."""
+    ctx = ctx or Context.default()
+    h = ctypes.c_void_p()
+    L.check(L.load().shk_csr_read_hb(ctx.h, str(path).encode(), _DT[dtype], Matrix.LAYOUTS[layout],
+                                     ctypes.byref(h)))
+    return Matrix._wrap(h, ctx, "row", dtype)
 
 
 def SOR(A: Matrix, X: torch.Tensor, B: torch.Tensor, omega, iter: int, type_ssor: int = 2, omega_dtype=None):

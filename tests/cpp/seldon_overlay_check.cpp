@@ -19,6 +19,7 @@
 // "== <label>" marker, then "ERRCODE e NBITER n".
 #include "Seldon.hxx"
 #include "SeldonSolver.hxx"
+#include "matrix_sparse/IOMatrixMarket.cxx"
 #include "seldon_amd/SeldonHip.hxx"
 
 #include <cstdlib>
@@ -225,6 +226,13 @@ int main(int argc, char** argv)
     Matrix<double, Symmetric, RowSymSparseHip, HipAlloc<double> > Sg(S);
     run_ssor_solver("ssor_cg_sym", "cpu", S, bh, 2000, 1e-10);
     run_ssor_solver("ssor_cg_sym", "gpu", Sg, bg, 2000, 1e-10);
+    // Harwell-Boeing: the device matrices write Seldon's files
+    const string dir = argc > 3 ? argv[3] : ".";
+    WriteHarwellBoeing(A, dir + "/cpu.rua");
+    WriteHarwellBoeing(Ag, dir + "/gpu.rua");
+    WriteHarwellBoeing(S, dir + "/cpu_sym.rua");
+    WriteHarwellBoeing(Sg, dir + "/gpu_sym.rua");
+    cout << "== hb" << endl;
   }
   return 0;
 }

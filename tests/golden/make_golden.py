@@ -330,12 +330,94 @@ def make_coo():
     print("coo fixtures:", len(out))
 
 
+def hb_cases():
+    """Writer: name -> (matrix, storage).  Reader: name -> HB text in the
+    formats Seldon's reader parses (E values; an RHS descriptor line; a
+    pattern matrix; a rectangular one; rows unsorted within columns with a
+    duplicated entry)."""
+    P = wl.powerlaw(150, 41, kmin=2, kmax=12, half_width=40)
+    writers = {"powerlaw150_row": (P, "row"), "powerlaw150_array": (P, "array"),
+               "scrambled150_row": (scramble_rows(P, 42, False), "row"),
+               "rect120x90_row": (wl.first_columns(wl.powerlaw(120, 43, kmin=2, kmax=12, half_width=40), 90), "row"),
+               "laplace12_sym": (wl.upper(wl.laplace2d(12)), "sym")}
+    rng = np.random.default_rng(44)
+    readers = {}
+    Q = wl.powerlaw(200, 45, kmin=2, kmax=10, half_width=50)
+    S = Q.to_scipy().tocsc()
+    readers["e4x20"] = hb_text_e(S.shape[0], S.shape[1], S.indptr, S.indices, S.data, "R", (8, 10), (8, 10),
+                                 (4, 20, 12))
+    R = wl.first_columns(wl.powerlaw(160, 46, kmin=2, kmax=10, half_width=50), 130).to_scipy().tocsc()
+    readers["rect_rhs"] = hb_text_e(R.shape[0], R.shape[1], R.indptr, R.indices, R.data, "R", (10, 7), (12, 6),
+                                    (3, 25, 16), rhs=True)
+    readers["pattern"] = hb_text_e(S.shape[0], S.shape[1], S.indptr, S.indices, None, "P", (16, 5), (16, 5), None)
+    # rows unsorted inside each column, and one entry given twice
+    ptr, ind, dat = S.indptr.copy(), S.indices.copy(), S.data.copy()
+    for j in range(S.shape[1]):
+        a, b = ptr[j], ptr[j + 1]
+        p = rng.permutation(b - a)
+        ind[a:b], dat[a:b] = ind[a:b][p], dat[a:b][p]
+    ind[ptr[3] + 1] = ind[ptr[3]]
+    readers["unsorted_dup"] = hb_text_e(S.shape[0], S.shape[1], ptr, ind, dat, "R", (5, 16), (5, 16), (2, 40, 30))
+    return writers, readers
+
+
+def hb_text_e(m, n, cptr, rind, vals, kind, pf, inf, vf, rhs=False):
+    """A Harwell-Boeing file with right-aligned integer fields and E-format
+    values (Fortran widths pf = (per line, width) etc.), 1-based."""
+    nnz = int(cptr[-1])
+    cell = lambda k, w: (k + w - 1) // w
+    nl_ptr, nl_ind = cell(n + 1, pf[0]), cell(nnz, inf[0])
+    nl_val = 0 if kind == "P" else cell(nnz, vf[0])
+    lines = [f"{'generated test matrix':<72}{'TEST0001':<8}",
+             "".join(f"{x:14d}" for x in (nl_ptr + nl_ind + nl_val, nl_ptr, nl_ind, nl_val, 1 if rhs else 0)),
+             f"{kind + ('UA' if m == n else 'RA'):<14}" + "".join(f"{x:14d}" for x in (m, n, nnz, 0)),
+             f"{'(%dI%d)' % pf:<16}{'(%dI%d)' % inf:<16}" + (f"{'(%dE%d.%d)' % vf:<20}{'':<20}" if vf else "")]
+    if rhs:
+        lines.append(f"{'F':<14}{1:14d}{0:14d}")
+
+    def block(items, per, w, fmt):
+        for i in range(0, len(items), per):
+            lines.append("".join(fmt(x).rjust(w) for x in items[i:i + per]))
+    block([int(x) + 1 for x in cptr], pf[0], pf[1], str)
+    block([int(x) + 1 for x in rind], inf[0], inf[1], str)
+    if kind != "P":
+        block(list(vals), vf[0], vf[1], lambda x: f"{float(x):.{vf[2]}E}")
+    return ("\n".join(lines) + "\n").encode()
+
+
+def make_hb():
+    """Seldon's WriteHarwellBoeing output (the text) and what its
+    ReadHarwellBoeing + coordinate conversion gives (the arrays)."""
+    writers, readers = hb_cases()
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, (A, st) in writers.items():
+            for dt, c in ((np.float64, "d"), (np.float32, "f")):
+                orc.write_seldon_csr(f"{tmp}/A.bin", A, dt)
+                orc.run_ref(["hbwrite", st, c, f"{tmp}/A.bin", f"{tmp}/o.rua"])
+                out[f"w_{name}_{c}"] = np.frombuffer(open(f"{tmp}/o.rua", "rb").read(), np.uint8)
+        for name, text in readers.items():
+            out[f"r_{name}_text"] = np.frombuffer(text, np.uint8)
+            with open(f"{tmp}/i.rua", "wb") as f:
+                f.write(text)
+            for dt, c in ((np.float64, "d"), (np.float32, "f")):
+                orc.run_ref(["hbread", c, f"{tmp}/i.rua", f"{tmp}/o.bin"])
+                m, n, ptr, ind, val = orc.read_seldon_csr(f"{tmp}/o.bin", dt)
+                out[f"r_{name}_{c}_dims"] = np.array([m, n])
+                out[f"r_{name}_{c}_ptr"], out[f"r_{name}_{c}_ind"], out[f"r_{name}_{c}_val"] = ptr, ind, val
+    np.savez_compressed(os.path.join(OUT, "harwell_boeing.npz"), spec=json.dumps({"writers": sorted(writers)}),
+                        **out)
+    print("hb fixtures:", len(out))
+
+
 if __name__ == "__main__":
     if not orc.ref_available("generic") or not orc.ref_available("blas"):
         orc.build()
         import subprocess
         subprocess.run(["make", "-s", "-C", orc.HERE, "ref"], check=True)
-    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "sor", "solves"]
+    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "sor", "hb", "solves"]
+    if "hb" in which:
+        make_hb()
     if "coo" in which:
         make_coo()
     if "sor" in which:

@@ -462,3 +462,73 @@ def test_sddmm_matches_dense_outer_product():
     dense = -np.outer(lam, x)
     rows = np.repeat(np.arange(A.m), np.diff(A.ptr))
     assert np.array_equal(g, dense[rows, A.ind])
+
+
+# ------------------------------------------------------- Harwell-Boeing --
+
+def test_oracle_hb_writer_matches_seldon():
+    """WriteHarwellBoeing's text, byte for byte: RowSparse (sorted and
+    scrambled rows, square and rectangular), ArrayRowSparse, RowSymSparse
+    (written whole), FP64 and FP32."""
+    from tests.golden.make_golden import hb_cases
+    fx = load_fixture("harwell_boeing.npz")
+    writers, _ = hb_cases()
+    for name, (A, st) in writers.items():
+        for dt, c in ((np.float64, "d"), (np.float32, "f")):
+            assert orc.hb_write_text(A, st, dt) == fx[f"w_{name}_{c}"].tobytes(), (name, c)
+
+
+def test_oracle_hb_reader_matches_seldon():
+    """ReadHarwellBoeing + the coordinate conversion to RowSparse: E values,
+    an RHS line, a pattern matrix, a rectangular one, rows unsorted within
+    columns with a duplicate (kept, in file order)."""
+    from tests.golden.make_golden import hb_cases
+    fx = load_fixture("harwell_boeing.npz")
+    _, readers = hb_cases()
+    for name, text in readers.items():
+        assert fx[f"r_{name}_text"].tobytes() == text
+        for dt, c in ((np.float64, "d"), (np.float32, "f")):
+            m, n, ptr, ind, val = orc.hb_read(text, dt)
+            assert [m, n] == list(fx[f"r_{name}_{c}_dims"]), name
+            assert np.array_equal(ptr, fx[f"r_{name}_{c}_ptr"]) and np.array_equal(ind, fx[f"r_{name}_{c}_ind"])
+            assert fx[f"r_{name}_{c}_val"].tobytes() == val.astype(dt).tobytes(), (name, c)
+
+
+def test_oracle_hb_teeth():
+    """What the fixtures rule out: the symmetric storage written as its upper
+    triangle only, and a column sort that is not stable (the duplicated
+    entry's two copies differ, so their order shows); and Seldon's reader
+    refuses Seldon's own D-format value line."""
+    from tests.golden.make_golden import hb_cases
+    fx = load_fixture("harwell_boeing.npz")
+    writers, readers = hb_cases()
+    U, _ = writers["laplace12_sym"]
+    assert orc.hb_write_text(U, "row") != fx["w_laplace12_sym_d"].tobytes()
+    m, n, ptr, ind, val = orc.hb_read(readers["unsorted_dup"])
+    r = next(i for i in range(m) if len(set(ind[ptr[i]:ptr[i + 1]])) < ptr[i + 1] - ptr[i])
+    a, b = ptr[r], ptr[r + 1]
+    dup = [k for k in range(a, b) if list(ind[a:b]).count(ind[k]) > 1]
+    assert val[dup[0]] != val[dup[1]]  # the two copies differ, so their order is pinned
+    swapped = val.copy()
+    swapped[dup[0]], swapped[dup[1]] = val[dup[1]], val[dup[0]]
+    assert swapped.tobytes() != fx["r_unsorted_dup_d_val"].tobytes()
+    with pytest.raises(ValueError, match="could not be parsed"):
+        orc.hb_read(fx["w_powerlaw150_row_d"].tobytes())
+
+
+@pytest.mark.skipif(not orc.ref_available("generic"), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_hb_vs_live_seldon():
+    from tests.golden.make_golden import hb_text_e
+    with tempfile.TemporaryDirectory() as tmp:
+        A = wl.powerlaw(3000, 91, kmin=2, kmax=30, half_width=400)
+        orc.write_seldon_csr(f"{tmp}/A.bin", A)
+        orc.run_ref(["hbwrite", "row", "d", f"{tmp}/A.bin", f"{tmp}/o.rua"])
+        assert open(f"{tmp}/o.rua", "rb").read() == orc.hb_write_text(A, "row")
+        S = A.to_scipy().tocsc()
+        text = hb_text_e(S.shape[0], S.shape[1], S.indptr, S.indices, S.data, "R", (8, 10), (8, 10), (4, 24, 16))
+        with open(f"{tmp}/i.rua", "wb") as f:
+            f.write(text)
+        orc.run_ref(["hbread", "d", f"{tmp}/i.rua", f"{tmp}/o.bin"])
+        ref = orc.read_seldon_csr(f"{tmp}/o.bin")
+        got = orc.hb_read(text)
+        assert ref[:2] == got[:2] and all(np.array_equal(x, y) for x, y in zip(ref[2:], got[2:]))
