@@ -24,6 +24,7 @@ struct KState {
     int *hist_it;
     // Cg / BiCgStab scalars, typed like Seldon's `Complexe` (= value type)
     T rho, rho_1, rho_2, alpha, beta, delta, omega, sigma;
+    T rho_pp[2];       // Cg: rho after the loop head of iteration k, in slot k % 2
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -102,11 +103,13 @@ struct FinInitNorm {  // Init(b): 1/|b|
     __device__ void operator()(const T (&s)[2]) const { it_init(st, s[0]); }
 };
 
-// Loop head of Cg: [rho_1 = rho; ++iter;] Finished(r); identity: rho = |r|^2.
+// Loop head of Cg: [rho_1 = rho; ++iter;] Finished(r); identity: rho = |r|^2,
+// also kept in rho_pp[slot] for a later kernel that must read it while this
+// head runs again (the head folded into the p update).
 template <typename T>
 struct FinCgTop {
     KState<T> *st;
-    int inc, identity;
+    int inc, identity, slot;
     __device__ void operator()(const T (&s)[2]) const
     {
         if (inc) {
@@ -117,7 +120,10 @@ struct FinCgTop {
             it_stop(st);
             return;
         }
-        if (identity) cg_after_rho(st, s[0]);
+        if (identity) {
+            cg_after_rho(st, s[0]);
+            st->rho_pp[slot] = st->rho;
+        }
     }
 };
 

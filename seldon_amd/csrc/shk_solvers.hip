@@ -71,6 +71,37 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *__restrict_
     }
 }
 
+// Identity Cg: the loop head folded into p = beta p + z.  Every work-group
+// forms the same |r|^2 total from k_cg_xr's partials (the finish kernel's own
+// arithmetic); the first runs the loop head (rho_1 = rho, ++iter,
+// Finished(r), rho = |r|^2, beta) and records rho in rho_pp[slot]; each forms
+// beta = |r|^2 / rho_old itself, rho_old read from rho_pp[slot ^ 1], which an
+// earlier kernel wrote (the first work-group rewrites st->rho meanwhile).
+// Same division as the head's, so beta is bitwise the same.  When the head
+// ends the solve, the later kernels skip; p is then not used again.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_p_top(int64_t n, const T *__restrict__ z, T *__restrict__ p,
+                                                     KState<T> *st, const T *prev, int prev_nb, int slot)
+{
+    if (st->skip) return;
+    __shared__ T beta_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            beta_sh = tot[0] / st->rho_pp[slot ^ 1];
+            if (blockIdx.x == 0) FinCgTop<T>{st, 1, 1, slot}(tot);
+        }
+        __syncthreads();
+    }
+    const T beta = beta_sh;
+    GRID_STRIDE(i, n) {
+        T v = p[i] * beta;
+        v += (T)1 * z[i];
+        p[i] = v;
+    }
+}
+
 // delta = (p, q) from the SpMV's partials and alpha = rho / delta (the
 // finish folded in: every work-group forms the same total, the first one
 // records it, a zero delta ends the solve, Cg.cxx:98-104); then
@@ -518,7 +549,7 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
                        (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
     S.fin(FinInitNorm<T>{S.dst});
     // r = b - A x or x = 0, r = b; then the first Finished(r)
-    FinCgTop<T> top0{S.dst, 0, jac ? 0 : 1};
+    FinCgTop<T> top0{S.dst, 0, jac ? 0 : 1, 0};
     if (it->init_guess_null) {
         hipLaunchKernelGGL(k_cg_init0<T>, g, blk, 0, st, n, b, x, r, S.dst, S.partials);
         S.fin(top0);
@@ -531,13 +562,20 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
     SHK_HIP(hipGetLastError());
     const T *dg = (const T *)A->diag;
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
+    // identity: the loop head of iterations 1 .. kChunk - 1 of a chunk is
+    // folded into k_cg_p_top; the head before a chunk's first iteration runs
+    // as its own finish (slot 0), so each chunk ends with the state read back
     SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
         for (int k = 0; k < kChunk; k++) {
             if (jac) {
                 hipLaunchKernelGGL(k_cg_z<T>, g, blk, 0, cs, n, r, dg, z, S.dst, S.partials);
                 S.fin(FinCgRho<T>{S.dst});
             }
-            hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, cs, n, z, p, S.dst);
+            if (jac || k == 0)
+                hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, cs, n, z, p, S.dst);
+            else
+                hipLaunchKernelGGL(k_cg_p_top<T>, g, blk, 0, cs, n, z, p, S.dst, (const T *)S.part[1], S.nb,
+                                   k & 1);
             SpmvArgs<T> a{};
             a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = p; a.skip = &S.dst->skip;
@@ -545,8 +583,9 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
             SHK_TRY(S.spmv(a, FinDefer{}));  // (p, q): finished inside k_cg_xr
             hipLaunchKernelGGL(k_cg_xr<T>, g, blk, 0, cs, n, p, q, x, r, S.dst, (const T *)S.part[0],
                                a.grid_used, S.part[1]);
-            grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip,
-                           FinCgTop<T>{S.dst, 1, jac ? 0 : 1});
+            if (jac || k == kChunk - 1)
+                grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip,
+                               FinCgTop<T>{S.dst, 1, jac ? 0 : 1, 0});
         }
         SHK_HIP(hipGetLastError());
         return SHK_OK;
