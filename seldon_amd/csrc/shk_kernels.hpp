@@ -457,6 +457,13 @@ constexpr int panel_win() { return sizeof(T) == 8 ? 3072 : 6144; }  // 24 KiB of
 #ifndef SHK_PANEL_MINW
 #define SHK_PANEL_MINW 8
 #endif
+// products read per step of a row's sum in the panel kernel's phase 2: a
+// wave steps as long as its longest segment, so a larger step reads more
+// products that its shorter rows discard (measured on the power-law matrix:
+// one per step beats four by ~1% FP64 and ~4% FP32, six is slower)
+#ifndef SHK_PANEL_P2_UNROLL
+#define SHK_PANEL_P2_UNROLL 1
+#endif
 template <typename T>
 constexpr int panel_sets() { return sizeof(T) == 8 ? SHK_PANEL_SETS_F64 : SHK_PANEL_SETS_F32; }
 // x windows in flight (register sets of the window loads)
@@ -595,20 +602,21 @@ __device__ __forceinline__ void panel_step(const SpmvArgs<T> &a, PanelSmem<T, BU
         }
     }
     // each thread adds its rows' segments in order, slot by slot (a slot's
-    // loop runs as long as its longest segment in the wave), four products
-    // per step read before they are added
+    // loop runs as long as its longest segment in the wave), U products per
+    // step read before they are added
+    constexpr int U = SHK_PANEL_P2_UNROLL;
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         const int rr = t + i * NT;
         const unsigned sw = sm.seg[rr];
         const int s0 = (int)(sw & 0xffffu);
         const int n0 = (int)(sw >> 16) - s0;
-        for (int j = 0; j < n0; j += 4) {
-            T pv[4];
+        for (int j = 0; j < n0; j += U) {
+            T pv[U];
 #pragma unroll
-            for (int h = 0; h < 4; h++) pv[h] = sm.prod[(j + h < n0) ? s0 + j + h : 0];
+            for (int h = 0; h < U; h++) pv[h] = sm.prod[(j + h < n0) ? s0 + j + h : 0];
 #pragma unroll
-            for (int h = 0; h < 4; h++) {
+            for (int h = 0; h < U; h++) {
                 const T sum = acc[i] + pv[h];
                 acc[i] = (j + h < n0) ? sum : acc[i];
             }
