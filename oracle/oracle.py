@@ -2,19 +2,26 @@
 
 ctypes front-end over ``oracle/liboracle.so`` (built from ``seldon_oracle.c``
 by ``oracle/Makefile``), a plain-C restatement of Seldon's generic templates:
-SpMV on RowSparse / RowSymSparse / ArrayRowSparse, DotProd / Norm2 / Add /
-Mlt / Copy, Iteration, Cg, BiCgStab and Gmres (file:line citations are in
-``seldon_oracle_body.h``).  Only ``tests/``, ``__graft_entry__.smoke()`` and
-the ``cpu_baseline`` leg of ``bench.py`` may use it, and only as the checker.
+SpMV on RowSparse / RowSymSparse / ArrayRowSparse and its transposed forms,
+DotProd / Norm2 / Add / Mlt / Copy, Iteration, Cg, BiCgStab and Gmres, SOR,
+and ConvertMatrix_from_Coordinates (file:line citations are in
+``seldon_oracle_body.h``); plus, in Python below, the Harwell-Boeing writer
+and reader.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` legs of ``bench.py`` may use it, and only as the checker.
 
 Parity pins (see ``tests/test_oracle.py``):
   * bitwise equality with drivers compiled from the real Seldon headers
-    (``oracle/_ref``) on SpMV, Cg and BiCgStab, and on committed fixtures
-    (``tests/golden``) made by them;
+    (``oracle/_ref``) on SpMV (plain and transposed), Cg, BiCgStab, SOR on
+    the three storages, the coordinate conversion and the Harwell-Boeing
+    files, and on committed fixtures (``tests/golden``) made by them;
   * Gmres: iteration counts and residual histories against the MKL-linked
     real Seldon build (tolerance, MKL's reductions differ in order);
-  * closed forms (5-point Laplacian eigenpairs), dense numpy solves,
-    scipy/numpy triangular solves and the textbook Givens rotation.
+  * closed forms (5-point Laplacian eigenpairs, SOR as forward substitution
+    and its convergence to A^-1 b), dense numpy solves, scipy/numpy
+    triangular solves, numpy's stable lexsort and the textbook Givens
+    rotation;
+  * parity unpinned: the FP32 inner Gmres (``orc_s_gmres``) has no Seldon
+    counterpart (Seldon's float Gmres needs Iteration<float>).
 """
 from __future__ import annotations
 

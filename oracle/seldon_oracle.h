@@ -2,9 +2,11 @@
  * seldon_oracle.h -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C CPU restatement of Seldon's sparse Krylov path (SpMV on the
- * RowSparse / RowSymSparse / ArrayRowSparse storages, the BLAS-1 steps, the
- * Iteration stopping logic, Cg, BiCgStab and Gmres), used as the checker for
- * the HIP library.  Only the tests, the smoke check and the benchmark's CPU
+ * RowSparse / RowSymSparse / ArrayRowSparse storages and its transposed
+ * forms, the BLAS-1 steps, the Iteration stopping logic, Cg, BiCgStab and
+ * Gmres, SOR / SSOR, ConvertMatrix_from_Coordinates), used as the checker
+ * for the HIP library.  orc_s_gmres (FP32 vectors, FP64 Hessenberg) has no
+ * Seldon counterpart: parity unpinned.  Only the tests, the smoke check and the benchmark's CPU
  * baseline leg may load it; the product never does.
  *
  * Every function follows the reference file:line named in its comment, in
