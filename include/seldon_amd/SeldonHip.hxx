@@ -364,7 +364,8 @@ namespace Seldon
   void ReadHarwellBoeing(std::string filename, Matrix<T, Prop, RowSparseHip, Allocator>& A)
   {
     shk_csr h = 0;
-    HipCheck(shk_csr_read_hb(HipContext::Get().Handle(), filename.c_str(), HipType<T>::value, 0, &h),
+    HipCheck(shk_csr_read_hb(HipContext::Get().Handle(), filename.c_str(), HipType<T>::value, SHK_LAYOUT_AUTO,
+                             &h),
              "ReadHarwellBoeing");
     A.Adopt(h);
   }
