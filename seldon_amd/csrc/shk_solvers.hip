@@ -16,6 +16,14 @@ namespace shk {
 // The vector arguments of these kernels are distinct buffers (__restrict__),
 // so an unrolled loop keeps several elements' loads in flight; each thread
 // still accumulates its elements in increasing index order.
+// A kernel that folds the previous kernel's finish (every work-group forms
+// the total of its partials before it can go on) first loads its first kPre
+// grid-stride elements of each vector into registers, so those loads are in
+// flight while the partials are fetched and summed; the rest follow the
+// fold.  Each thread still visits its elements in GRID_STRIDE's order, so its
+// sums are the same.
+constexpr int kPre = 8;
+
 #define GRID_STRIDE(i, n) \
     _Pragma("unroll 4") for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (n); \
                              i += (int64_t)gridDim.x * kBlock)
@@ -84,6 +92,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_p_top(int64_t n, const T *__restr
                                                      KState<T> *st, const T *prev, int prev_nb, int slot)
 {
     if (st->skip) return;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T pv[kPre], zv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            pv[u] = p[i];
+            zv[u] = z[i];
+        }
+    }
     __shared__ T beta_sh;
     {
         T tot[2];
@@ -95,7 +113,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_p_top(int64_t n, const T *__restr
         __syncthreads();
     }
     const T beta = beta_sh;
-    GRID_STRIDE(i, n) {
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            T v = pv[u] * beta;
+            v += (T)1 * zv[u];
+            p[i] = v;
+        }
+    }
+    for (int64_t i = i0 + kPre * stride; i < n; i += stride) {
         T v = p[i] * beta;
         v += (T)1 * z[i];
         p[i] = v;
@@ -111,6 +138,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *__restrict
                                                   KState<T> *st, const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T pv[kPre], qv[kPre], xv[kPre], rv0[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            pv[u] = p[i];
+            qv[u] = q[i];
+            xv[u] = x[i];
+            rv0[u] = r[i];
+        }
+    }
     __shared__ T alpha_sh;
     __shared__ int fail_sh;
     {
@@ -127,7 +166,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(int64_t n, const T *__restrict
     const T alpha = alpha_sh, malpha = -alpha_sh;
     const bool ax = alpha != (T)0, ar = malpha != (T)0;
     T c[2] = {(T)0, (T)0};
-    GRID_STRIDE(i, n) {
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            if (ax) x[i] = xv[u] + alpha * pv[u];
+            T rv = rv0[u];
+            if (ar) {
+                rv += malpha * qv[u];
+                r[i] = rv;
+            }
+            c[0] += rv * rv;
+        }
+    }
+    for (int64_t i = i0 + kPre * stride; i < n; i += stride) {
         if (ax) x[i] += alpha * p[i];
         T rv = r[i];
         if (ar) {
@@ -320,10 +372,33 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *__restric
                                                    int hpos, KState<T> *st, const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
+    const int64_t j0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T wp[kPre], vp[kPre], kp[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t j = j0 + u * stride;
+        if (j < n) {
+            wp[u] = w[j];
+            vp[u] = vprev[j];
+            kp[u] = vk[j];
+        }
+    }
     const T a = gm_fold_h<T>(prev, prev_nb, hpos, st);
     const bool aa = a != (T)0;
     T c[2] = {(T)0, (T)0};
-    GRID_STRIDE(j, n) {
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t j = j0 + u * stride;
+        if (j < n) {
+            T wv = wp[u];
+            if (aa) {
+                wv += a * vp[u];
+                w[j] = wv;
+            }
+            c[0] += kp[u] * wv;
+        }
+    }
+    for (int64_t j = j0 + kPre * stride; j < n; j += stride) {
         T wv = w[j];
         if (aa) {
             wv += a * vprev[j];
@@ -341,10 +416,32 @@ __global__ __launch_bounds__(kBlock) void k_gm_last(int64_t n, const T *__restri
                                                     const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
+    const int64_t j0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T wp[kPre], vp[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t j = j0 + u * stride;
+        if (j < n) {
+            wp[u] = w[j];
+            vp[u] = vi[j];
+        }
+    }
     const T a = gm_fold_h<T>(prev, prev_nb, hpos, st);
     const bool aa = a != (T)0;
     T c[2] = {(T)0, (T)0};
-    GRID_STRIDE(j, n) {
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t j = j0 + u * stride;
+        if (j < n) {
+            T wv = wp[u];
+            if (aa) {
+                wv += a * vp[u];
+                w[j] = wv;
+            }
+            c[0] += wv * wv;
+        }
+    }
+    for (int64_t j = j0 + kPre * stride; j < n; j += stride) {
         T wv = w[j];
         if (aa) {
             wv += a * vi[j];
