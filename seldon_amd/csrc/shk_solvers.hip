@@ -248,6 +248,17 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *__restrict_
                                                  T *partials)
 {
     if (st->skip) return;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T rp[kPre], vp[kPre], dp[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            rp[u] = r[i];
+            vp[u] = v[i];
+            if (d) dp[u] = d[i];
+        }
+    }
     __shared__ T alpha_sh;
     __shared__ int fail_sh;
     {
@@ -264,7 +275,18 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const T *__restrict_
     const T malpha = -alpha_sh;
     const bool aa = malpha != (T)0;
     T c[2] = {(T)0, (T)0};
-    GRID_STRIDE(i, n) {
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            T sv = rp[u];
+            if (aa) sv += malpha * vp[u];
+            s[i] = sv;
+            if (d) shat[i] = sv / dp[u];
+            c[0] += sv * sv;
+        }
+    }
+    for (int64_t i = i0 + kPre * stride; i < n; i += stride) {
         T sv = r[i];
         if (aa) sv += malpha * v[i];
         s[i] = sv;
@@ -293,6 +315,22 @@ __global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *__restrict
                                                   const T *prev, int prev_nb, T *partials)
 {
     if (st->skip) return;
+    // six vectors: half the usual number of elements ahead of the fold
+    constexpr int P = kPre / 2;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T xp[P], php[P], shp[P], sp[P], tp[P], rtp[P];
+#pragma unroll
+    for (int u = 0; u < P; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            xp[u] = x[i];
+            php[u] = phat[i];
+            shp[u] = shat[i];
+            sp[u] = s[i];
+            tp[u] = t[i];
+            rtp[u] = rt[i];
+        }
+    }
     // omega = (t, s) / (t, t) from the SpMV's partials (:129), folded in
     __shared__ T omega_sh;
     {
@@ -307,7 +345,22 @@ __global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *__restrict
     const T alpha = st->alpha, omega = omega_sh, momega = -omega_sh;
     const bool aa = alpha != (T)0, ao = omega != (T)0, am = momega != (T)0;
     T c[2] = {(T)0, (T)0};
-    GRID_STRIDE(i, n) {
+#pragma unroll
+    for (int u = 0; u < P; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            T xv = xp[u];
+            if (aa) xv += alpha * php[u];
+            if (ao) xv += omega * shp[u];
+            x[i] = xv;
+            T rv = sp[u];
+            if (am) rv += momega * tp[u];
+            r[i] = rv;
+            c[0] += rv * rv;
+            c[1] += rtp[u] * rv;
+        }
+    }
+    for (int64_t i = i0 + P * stride; i < n; i += stride) {
         T xv = x[i];
         if (aa) xv += alpha * phat[i];
         if (ao) xv += omega * shat[i];
