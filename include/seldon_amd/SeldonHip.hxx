@@ -229,6 +229,28 @@ namespace Seldon
     // element read by value, one device-to-host transfer (CoCg.cxx:53 reads
     // b(0) to form a zero of the value type)
     T operator()(int i) const { return GetElement(i); }
+    // Seldon's binary vector file (vector/Vector.cxx:846-874 Write,
+    // :1060-1091 Read): the length as an int when with_size, then the
+    // elements; Read without the length fills the vector's current length
+    void Write(std::string FileName, bool with_size = true) const
+    {
+      HipCheck(shk_vec_write(HipContext::Get().Handle(), HipType<T>::value, m_, data_, FileName.c_str(),
+                             with_size ? 1 : 0),
+               "Vector<VectHip>::Write");
+    }
+    void Read(std::string FileName, bool with_size = true)
+    {
+      int64_t n = m_;
+      if (with_size)
+        {
+          HipCheck(shk_vec_read(HipContext::Get().Handle(), HipType<T>::value, FileName.c_str(), 1, 0, 0, &n),
+                   "Vector<VectHip>::Read");
+          Reallocate((int)n);
+        }
+      HipCheck(shk_vec_read(HipContext::Get().Handle(), HipType<T>::value, FileName.c_str(), with_size ? 1 : 0,
+                            m_, data_, &n),
+               "Vector<VectHip>::Read");
+    }
     // element read for inspection (one transfer)
     T GetElement(int i) const
     {

@@ -127,6 +127,14 @@ int shk_csr_read(shk_ctx ctx, const char *path, shk_storage storage, shk_dtype d
  * rows assembled on the device.  Real and pattern matrices, as Seldon reads;
  * Seldon's reader cannot parse a D-format value line such as the "(3D23.15)"
  * its own writer puts, and neither can this one. */
+/* Seldon's binary vector file (vector/Vector.cxx:846-874 Write, :1060-1091
+ * Read): the length as an int when with_size, then the n elements of the
+ * device vector dx.  shk_vec_read puts the file's length in *n (dx NULL: the
+ * length only) and refuses more than n_cap elements; without the length it
+ * reads n_cap elements, as Seldon's Read(stream, false) reads the vector's
+ * current length. */
+int shk_vec_write(shk_ctx ctx, shk_dtype dt, int64_t n, const void *dx, const char *path, int with_size);
+int shk_vec_read(shk_ctx ctx, shk_dtype dt, const char *path, int with_size, int64_t n_cap, void *dx, int64_t *n);
 int shk_csr_write_hb(shk_ctx ctx, shk_csr A, const char *path);
 int shk_csr_read_hb(shk_ctx ctx, const char *path, shk_dtype dtype, int layout, shk_csr *out);
 int shk_csr_destroy(shk_csr A);

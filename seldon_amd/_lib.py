@@ -69,6 +69,8 @@ SIGNATURES = [
     ("shk_csr_write", _I, [_P, _P, ctypes.c_char_p]),
     ("shk_sor", _I, [_P, _P, _P, _P, _D, _I, _I, _I]),
     ("shk_csr_read", _I, [_P, ctypes.c_char_p, _I, _I, _I, _PP]),
+    ("shk_vec_write", _I, [_P, _I, ctypes.c_int64, _P, ctypes.c_char_p, _I]),
+    ("shk_vec_read", _I, [_P, _I, ctypes.c_char_p, _I, ctypes.c_int64, _P, ctypes.POINTER(ctypes.c_int64)]),
     ("shk_csr_write_hb", _I, [_P, _P, ctypes.c_char_p]),
     ("shk_csr_read_hb", _I, [_P, ctypes.c_char_p, _I, _I, _PP]),
     ("shk_coo_to_csr", _I, [_P, _I, ctypes.c_int64, _P, _P, _P, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_int),

@@ -288,6 +288,29 @@ def ConvertMatrix_from_Coordinates(IndRow: torch.Tensor, IndCol: torch.Tensor, V
     return mm.value, nn.value, ptr, ind[:nnz], out[:nnz]
 
 
+def WriteVector(X: torch.Tensor, path: str, with_size: bool = True):
+    """Vector::Write(path, with_size) (Vector.cxx:846-874) of a device vector:
+    the length as an int, then the elements."""
+    L.check(L.load().shk_vec_write(Context.default().h, _DT[X.dtype], X.numel(), _ptr(X), str(path).encode(),
+                                   int(with_size)))
+
+
+def ReadVector(path: str, dtype=torch.float64, with_size: bool = True, n: int | None = None) -> torch.Tensor:
+    """Vector::Read(path, with_size) (Vector.cxx:1060-1091) into a new device
+    vector: the file's length, or n elements when the file has none."""
+    ctx = Context.default()
+    got = ctypes.c_int64()
+    if with_size:
+        L.check(L.load().shk_vec_read(ctx.h, _DT[dtype], str(path).encode(), 1, 0, None, ctypes.byref(got)))
+        n = got.value
+    elif n is None:
+        raise ValueError("ReadVector without the length needs n")
+    X = torch.empty(n, dtype=dtype, device="cuda")
+    L.check(L.load().shk_vec_read(ctx.h, _DT[dtype], str(path).encode(), int(with_size), n,
+                                  _ptr(X) if n else None, ctypes.byref(got)))
+    return X
+
+
 def WriteHarwellBoeing(A: Matrix, path: str):
     """WriteHarwellBoeing(A, path) (IOMatrixMarket.cxx:397-510): the file
     Seldon writes for the same storage and arrays, byte for byte."""

@@ -232,6 +232,12 @@ int main(int argc, char** argv)
     WriteHarwellBoeing(Ag, dir + "/gpu.rua");
     WriteHarwellBoeing(S, dir + "/cpu_sym.rua");
     WriteHarwellBoeing(Sg, dir + "/gpu_sym.rua");
+    // Seldon's binary vector file from both storages, and read back
+    bh.Write(dir + "/b_cpu.bin");
+    bg.Write(dir + "/b_gpu.bin");
+    Vector<double, VectHip, HipAlloc<double> > rb;
+    rb.Read(dir + "/b_cpu.bin");
+    rb.Write(dir + "/b_back.bin");
     cout << "== hb" << endl;
   }
   return 0;

@@ -131,3 +131,48 @@ def test_csr_read_file_written_by_seldon(gpu, tmp_path):
     Ag.Write(tmp_path / "a2.bin")
     Bg.Write(tmp_path / "b2.bin")
     assert (tmp_path / "a2.bin").read_bytes() == (tmp_path / "b2.bin").read_bytes() == (tmp_path / "A.bin").read_bytes()
+
+
+@pytest.mark.parametrize("c", ["d", "f"])
+def test_vector_file_roundtrip(gpu, c, tmp_path):
+    """Seldon's binary vector file (Vector.cxx:846-874, :1060-1091) from a
+    device vector: the same bytes as the oracle's writer (whose layout the
+    real-Seldon fixtures pin), read back exactly, with and without the
+    length; a vector the real Seldon wrote reads back the same."""
+    import ctypes
+
+    from oracle import oracle as orc
+    from seldon_amd import _lib as L
+    from seldon_amd import krylov as K
+    tdt, ndt = (torch.float64, np.float64) if c == "d" else (torch.float32, np.float32)
+    v = wl.uniform(12345, 17).astype(ndt)
+    X = torch.tensor(v, device="cuda")
+    K.WriteVector(X, tmp_path / "v.bin")
+    orc.write_seldon_vec(tmp_path / "o.bin", v)
+    assert (tmp_path / "v.bin").read_bytes() == (tmp_path / "o.bin").read_bytes()
+    Y = K.ReadVector(tmp_path / "v.bin", dtype=tdt)
+    assert Y.cpu().numpy().tobytes() == v.tobytes()
+    K.WriteVector(X, tmp_path / "n.bin", with_size=False)
+    assert (tmp_path / "n.bin").read_bytes() == v.tobytes()
+    Z = K.ReadVector(tmp_path / "n.bin", dtype=tdt, with_size=False, n=v.size)
+    assert Z.cpu().numpy().tobytes() == v.tobytes()
+    K.WriteVector(torch.empty(0, dtype=tdt, device="cuda"), tmp_path / "e.bin")
+    assert (tmp_path / "e.bin").read_bytes() == np.int32(0).tobytes()
+    assert K.ReadVector(tmp_path / "e.bin", dtype=tdt).numel() == 0
+    small = torch.empty(10, dtype=tdt, device="cuda")
+    got = ctypes.c_int64()
+    with pytest.raises(RuntimeError, match="longer than the destination"):
+        L.check(L.load().shk_vec_read(K.Context.default().h, K._DT[tdt], str(tmp_path / "v.bin").encode(), 1, 10,
+                                      K._ptr(small), ctypes.byref(got)))
+    if orc.ref_available("generic"):
+        # the real Seldon's Vector::Write, through its SOR driver (one
+        # forward sweep of a diagonal matrix leaves x = b / a_jj)
+        A = wl.CSR(v.size, v.size, np.arange(v.size + 1, dtype=np.int32), np.arange(v.size, dtype=np.int32),
+                   np.full(v.size, 2.0))
+        orc.write_seldon_csr(tmp_path / "A.bin", A, ndt)
+        orc.write_seldon_vec(tmp_path / "x.bin", np.zeros(v.size, ndt))
+        orc.write_seldon_vec(tmp_path / "b.bin", v)
+        orc.run_ref(["sor", "row", c, str(tmp_path / "A.bin"), str(tmp_path / "x.bin"), str(tmp_path / "b.bin"),
+                     "1.0", 1, 2, str(tmp_path / "s.bin")])
+        S = K.ReadVector(tmp_path / "s.bin", dtype=tdt)
+        assert S.cpu().numpy().tobytes() == orc.read_seldon_vec(tmp_path / "s.bin", ndt).tobytes()

@@ -100,7 +100,8 @@ def test_overlay_in_real_seldon(gpu):
     r = subprocess.run([OVERLAY, "64", "24", tmp], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     # Seldon's WriteHarwellBoeing on the device storages: the CPU matrix's file
-    for a, b in (("cpu.rua", "gpu.rua"), ("cpu_sym.rua", "gpu_sym.rua")):
+    for a, b in (("cpu.rua", "gpu.rua"), ("cpu_sym.rua", "gpu_sym.rua"), ("b_cpu.bin", "b_gpu.bin"),
+                 ("b_cpu.bin", "b_back.bin")):
         assert open(os.path.join(tmp, a), "rb").read() == open(os.path.join(tmp, b), "rb").read(), b
     s = _sections(r.stdout)
     h_cpu, it_cpu, c_cpu, n_cpu = s["cpu"]
