@@ -24,6 +24,14 @@ namespace shk {
 // sums are the same.
 constexpr int kPre = 8;
 
+// elements per thread of the solvers' grid-stride kernels, and the fewest
+// work-groups a system gets while it has two elements per thread to give:
+// together they fix the grid, and so the shape of every reduction, per n
+#ifndef SHK_SOLVE_ITEMS
+#define SHK_SOLVE_ITEMS 8
+#endif
+constexpr int kSolveMinBlocks = 256;
+
 #define GRID_STRIDE(i, n) \
     _Pragma("unroll 4") for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < (n); \
                              i += (int64_t)gridDim.x * kBlock)
@@ -581,7 +589,11 @@ struct Solve {
         ctx = c;
         A = a;
         n = a->m;
-        nb = red_blocks(n);
+        nb = red_blocks(n, SHK_SOLVE_ITEMS);
+        // a small system gets more work-groups (up to one per CU, down to two
+        // elements per thread): its kernels are latency-bound, not streaming
+        if (nb < kSolveMinBlocks)
+            nb = (int)std::min<int64_t>(kSolveMinBlocks, std::max<int64_t>(nb, (n + 2 * kBlock - 1) / (2 * kBlock)));
         size_t vb = (((size_t)n * sizeof(T) + 255) & ~(size_t)255);
         size_t hist_cap = it->hist && it->hist_cap > 0 ? (size_t)it->hist_cap : 0;
         size_t need = vec_count * vb + extra_bytes + 256 + hist_cap * (sizeof(double) + sizeof(int)) + 512;
