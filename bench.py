@@ -17,7 +17,7 @@ on the same matrix as `cpu_baseline`.
 Also: the 128^3 convection-diffusion BiCgStab / Gmres(30) solves and the
 512-system batched Cg (FP64 and FP32+IR), each timed to its tolerance.
 
-Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 10] [--quick] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 10] [--quick] [--no-cpu] [--only convdiff128]
                        [--no-solve] [--no-extra] [--probe]
 """
 import argparse
@@ -60,6 +60,8 @@ def main():
                     help="device layout of the benchmark matrix (auto: panels where they apply)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 128^3 convection-diffusion solves and the batched Cg")
+    ap.add_argument("--only", choices=["convdiff128"],
+                    help="run one extra section alone and print its JSON (for profiling its kernels)")
     ap.add_argument("--probe", action="store_true",
                     help="profiling run: one 1 GiB copy kernel (counter calibration) and 10 FP64 + 10 FP32 SpMVs")
     args = ap.parse_args()
@@ -73,6 +75,11 @@ def main():
 
     from seldon_amd import krylov as K
     from seldon_amd import workloads as wl
+
+    if args.only == "convdiff128":
+        torch.cuda.set_device(0)
+        print(json.dumps({"convdiff128": convdiff_solves(K, wl, np, torch)}), flush=True)
+        return
 
     n_rows = (1 << 20) if args.quick else (1 << 22)
     t0 = time.time()
