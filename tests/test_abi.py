@@ -78,3 +78,15 @@ def test_diffusion_is_symmetric_positive():
     A = wl.diffusion3d(6, th).to_scipy()
     assert abs(A - A.T).max() == 0
     assert np.linalg.eigvalsh(A.toarray()).min() > 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and a message,
+    before touching a device."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2, r.stderr
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
