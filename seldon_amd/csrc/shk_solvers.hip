@@ -69,6 +69,89 @@ __global__ __launch_bounds__(kBlock) void k_cg_z(int64_t n, const T *__restrict_
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// Jacobi Cg: the loop head of the previous iteration (rho_1 = rho, ++iter,
+// Finished(r)) folded into z = M^-1 r.  Nothing here needs its result, so
+// only the first work-group forms the |r|^2 total and runs it; when it ends
+// the solve, the later kernels skip.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_z_top(int64_t n, const T *__restrict__ r, const T *__restrict__ d,
+                                                     T *__restrict__ z, KState<T> *st, const T *prev, int prev_nb,
+                                                     T *partials)
+{
+    if (st->skip) return;
+    if (blockIdx.x == 0) {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) FinCgTop<T>{st, 1, 0, 0}(tot);
+    }
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T rv = r[i];
+        T zv = rv / d[i];
+        z[i] = zv;
+        c[0] += rv * zv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// Jacobi Cg: rho = (r, z) folded into the p update.  Every work-group forms
+// the same total and beta = rho / rho_1 itself (rho_1 was set by an earlier
+// kernel; the first work-group, which records rho and beta, changes neither
+// rho_1 nor the iteration count), so beta is bitwise the state's.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_p_rho(int64_t n, const T *__restrict__ z, T *__restrict__ p,
+                                                     KState<T> *st, const T *prev, int prev_nb)
+{
+    if (st->skip) return;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T pv[kPre], zv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            pv[u] = p[i];
+            zv[u] = z[i];
+        }
+    }
+    __shared__ T beta_sh;
+    __shared__ int fail_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            fail_sh = tot[0] == (T)0;
+            beta_sh = fail_sh ? (T)0 : tot[0] / st->rho_1;
+            if (blockIdx.x == 0) FinCgRho<T>{st}(tot);
+        }
+        __syncthreads();
+    }
+    if (fail_sh) return;
+    const bool first = st->nb_iter == 0;
+    const T beta = beta_sh;
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            if (first) {
+                p[i] = zv[u];
+            } else {
+                T v = pv[u] * beta;
+                v += (T)1 * zv[u];
+                p[i] = v;
+            }
+        }
+    }
+    for (int64_t i = i0 + kPre * stride; i < n; i += stride) {
+        if (first) {
+            p[i] = z[i];
+        } else {
+            T v = p[i] * beta;
+            v += (T)1 * z[i];
+            p[i] = v;
+        }
+    }
+}
+
 // First ? p = z : { p *= beta; p += 1 * z }   (Cg.cxx:86-94)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_p(int64_t n, const T *__restrict__ z, T *__restrict__ p, const KState<T> *st)
@@ -724,20 +807,25 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
     SHK_HIP(hipGetLastError());
     const T *dg = (const T *)A->diag;
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
-    // identity: the loop head of iterations 1 .. kChunk - 1 of a chunk is
-    // folded into k_cg_p_top; the head before a chunk's first iteration runs
+    // the loop head of iterations 1 .. kChunk - 1 of a chunk is folded into
+    // the next kernel (identity: k_cg_p_top; Jacobi: k_cg_z_top, and rho =
+    // (r, z) into k_cg_p_rho); the head before a chunk's first iteration runs
     // as its own finish (slot 0), so each chunk ends with the state read back
     SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
         for (int k = 0; k < kChunk; k++) {
             if (jac) {
-                hipLaunchKernelGGL(k_cg_z<T>, g, blk, 0, cs, n, r, dg, z, S.dst, S.partials);
-                S.fin(FinCgRho<T>{S.dst});
-            }
-            if (jac || k == 0)
+                if (k == 0)
+                    hipLaunchKernelGGL(k_cg_z<T>, g, blk, 0, cs, n, r, dg, z, S.dst, S.part[0]);
+                else
+                    hipLaunchKernelGGL(k_cg_z_top<T>, g, blk, 0, cs, n, r, dg, z, S.dst, (const T *)S.part[1], S.nb,
+                                       S.part[0]);
+                hipLaunchKernelGGL(k_cg_p_rho<T>, g, blk, 0, cs, n, z, p, S.dst, (const T *)S.part[0], S.nb);
+            } else if (k == 0) {
                 hipLaunchKernelGGL(k_cg_p<T>, g, blk, 0, cs, n, z, p, S.dst);
-            else
+            } else {
                 hipLaunchKernelGGL(k_cg_p_top<T>, g, blk, 0, cs, n, z, p, S.dst, (const T *)S.part[1], S.nb,
                                    k & 1);
+            }
             SpmvArgs<T> a{};
             a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = p; a.skip = &S.dst->skip;
@@ -745,7 +833,7 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
             SHK_TRY(S.spmv(a, FinDefer{}));  // (p, q): finished inside k_cg_xr
             hipLaunchKernelGGL(k_cg_xr<T>, g, blk, 0, cs, n, p, q, x, r, S.dst, (const T *)S.part[0],
                                a.grid_used, S.part[1]);
-            if (jac || k == kChunk - 1)
+            if (k == kChunk - 1)
                 grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip,
                                FinCgTop<T>{S.dst, 1, jac ? 0 : 1, 0});
         }
