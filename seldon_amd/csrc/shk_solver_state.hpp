@@ -24,7 +24,7 @@ struct KState {
     int *hist_it;
     // Cg / BiCgStab scalars, typed like Seldon's `Complexe` (= value type)
     T rho, rho_1, rho_2, alpha, beta, delta, omega, sigma;
-    T rho_pp[2];       // Cg: rho after the loop head of iteration k, in slot k % 2
+    T rho_pp[2];       // Cg rho / BiCgStab rho_1 after the loop head of iteration k, in slot k % 2
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -153,7 +153,7 @@ struct FinCgDelta {  // delta = (p, q) (Cg.cxx:98-104)
 template <typename T>
 struct FinBiTop {
     KState<T> *st;
-    int inc;
+    int inc, slot;  // rho_1 also kept in rho_pp[slot] (the head folded into the p update)
     __device__ void operator()(const T (&s)[2]) const
     {
         if (inc) {
@@ -165,6 +165,7 @@ struct FinBiTop {
             return;
         }
         st->rho_1 = s[1];
+        st->rho_pp[slot] = s[1];
         if (s[1] == (T)0) {
             it_fail(st, 1);
             return;

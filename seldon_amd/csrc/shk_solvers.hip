@@ -330,6 +330,65 @@ __global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const T *__restrict_
     }
 }
 
+// BiCgStab: the loop head (rho_2 = rho_1, ++iter, Finished(r), rho_1 =
+// (rtilde, r), breakdowns #1 and #2, beta) folded into the p update of loop
+// slots after a chunk's first.  Every work-group forms the same totals and
+// beta = (rho_1 / rho_2) * (alpha / omega) itself, rho_2 read from
+// rho_pp[slot ^ 1] (the first work-group, which runs the head, rewrites
+// rho_1 meanwhile; alpha and omega come from earlier kernels), the head's
+// own expression, so beta is bitwise the state's.  When the head ends the
+// solve, the later kernels skip; p is then not used again.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bi_p_top(int64_t n, const T *__restrict__ r, const T *__restrict__ v,
+                                                     T *__restrict__ p, const T *__restrict__ d, T *__restrict__ phat,
+                                                     KState<T> *st, const T *prev, int prev_nb, int slot)
+{
+    if (st->skip) return;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    T pv0[kPre], vv[kPre], rv[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            pv0[u] = p[i];
+            vv[u] = v[i];
+            rv[u] = r[i];
+        }
+    }
+    __shared__ T beta_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            beta_sh = (tot[1] / st->rho_pp[slot ^ 1]) * (st->alpha / st->omega);
+            if (blockIdx.x == 0) FinBiTop<T>{st, 1, slot}(tot);
+        }
+        __syncthreads();
+    }
+    const T momega = -st->omega, beta = beta_sh;
+    const bool aw = momega != (T)0;
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+        const int64_t i = i0 + u * stride;
+        if (i < n) {
+            T pv = pv0[u];
+            if (aw) pv += momega * vv[u];
+            pv *= beta;
+            pv += (T)1 * rv[u];
+            p[i] = pv;
+            if (d) phat[i] = pv / d[i];
+        }
+    }
+    for (int64_t i = i0 + kPre * stride; i < n; i += stride) {
+        T pv = p[i];
+        if (aw) pv += momega * v[i];
+        pv *= beta;
+        pv += (T)1 * r[i];
+        p[i] = pv;
+        if (d) phat[i] = pv / d[i];
+    }
+}
+
 // s = r; s += (-alpha) v (:111-112); shat = M^{-1} s (:124); sum s^2
 // (sigma = (rtilde, v) from the SpMV's partials, breakdown #3 and
 // alpha = rho_1 / sigma folded in, :104-110)
@@ -866,7 +925,7 @@ int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iterat
     }
     hipLaunchKernelGGL(k_bi_init<T>, g, blk, 0, st, n, b, x, r, rt, it->init_guess_null ? 1 : 0,
                        S.dst, S.partials);
-    S.fin(FinBiTop<T>{S.dst, 0});
+    S.fin(FinBiTop<T>{S.dst, 0, 0});
     SHK_HIP(hipGetLastError());
     const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
     // a loop's slot (its place in the chunk) names the Finished(s) that
@@ -874,7 +933,13 @@ int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iterat
     // chunks, but a stopped solve replays no further chunk
     SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
         for (int slot = 0; slot < kChunk; slot++) {
-            hipLaunchKernelGGL(k_bi_p<T>, g, blk, 0, cs, n, r, v, p, dg, phat, S.dst);
+            // the loop head of slots 1 .. kChunk - 1 is folded into k_bi_p_top;
+            // the head before a chunk's first slot runs as its own finish
+            if (slot == 0)
+                hipLaunchKernelGGL(k_bi_p<T>, g, blk, 0, cs, n, r, v, p, dg, phat, S.dst);
+            else
+                hipLaunchKernelGGL(k_bi_p_top<T>, g, blk, 0, cs, n, r, v, p, dg, phat, S.dst,
+                                   (const T *)S.part[1], S.nb, slot & 1);
             SpmvArgs<T> a{};
             a.x = phat; a.y_out = v; a.alpha = (T)1; a.mlt = 1;
             a.dot0 = kDotVec; a.u0 = rt; a.skip = &S.dst->skip;
@@ -891,7 +956,9 @@ int bicgstab_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iterat
             SHK_TRY(S.spmv(b2, FinDefer{}));  // omega: finished inside k_bi_xr
             hipLaunchKernelGGL(k_bi_xr<T>, g, blk, 0, cs, n, phat, shat, s, t, rt, x, r, S.dst,
                                (const T *)S.part[0], b2.grid_used, S.part[1]);
-            grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip, FinBiTop<T>{S.dst, 1});
+            if (slot == kChunk - 1)
+                grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip,
+                               FinBiTop<T>{S.dst, 1, 0});
         }
         SHK_HIP(hipGetLastError());
         return SHK_OK;
