@@ -20,8 +20,17 @@ Parity pins (see ``tests/test_oracle.py``):
     and its convergence to A^-1 b), dense numpy solves, scipy/numpy
     triangular solves, numpy's stable lexsort and the textbook Givens
     rotation;
-  * parity unpinned: the FP32 inner Gmres (``orc_s_gmres``) has no Seldon
-    counterpart (Seldon's float Gmres needs Iteration<float>).
+  * the FP32 family: ``orc_s_cg`` and ``orc_s_bicgstab`` bitwise against
+    Seldon's own float build (Iteration<double>, generic templates) on the
+    three storages (``tests/golden/solve_f32_*``); ``orc_s_gmres`` keeps its
+    Hessenberg, rotations and triangular solve in double (the FP32 inner
+    solver of the mixed-precision path), so it is pinned by the iteration
+    counts and error codes of Seldon's float Gmres (Iteration<float>, float
+    Hessenberg), by its residual history within FP32 rounding of that build,
+    by A = I (one step, happy breakdown), by diagonal A with k distinct
+    values (at most k steps), by the Arnoldi relation A M^-1 V_k = V_{k+1}
+    Hbar_k with V orthonormal to FP32 rounding, and by its solution within
+    kappa * eps_32 of the FP64 oracle's.
 """
 from __future__ import annotations
 
@@ -52,7 +61,9 @@ class OrcIter(ctypes.Structure):
                 ("print_level", ctypes.c_int), ("init_guess_null", ctypes.c_int),
                 ("parameter_restart", ctypes.c_int),
                 ("hist", ctypes.c_void_p), ("hist_it", ctypes.c_void_p),
-                ("hist_len", ctypes.c_int), ("hist_cap", ctypes.c_int)]
+                ("hist_len", ctypes.c_int), ("hist_cap", ctypes.c_int),
+                ("arnoldi_h", ctypes.c_void_p), ("arnoldi_v", ctypes.c_void_p),
+                ("arnoldi_steps", ctypes.c_int)]
 
 
 _lib = None
@@ -212,10 +223,11 @@ class SolveResult:
     nb_iter: int
     hist: np.ndarray      # every relative residual Finished() compared
     hist_it: np.ndarray   # iteration number at each of those calls
+    arnoldi: tuple | None = None  # Gmres, first cycle: (V (k+1) x n, Hbar (k+1) x k)
 
 
 def solve(method, A, b, precond="id", tol=1e-10, max_iter=10000, restart=10,
-          x0=None, storage=None, dtype=np.float64, hist_cap=None) -> SolveResult:
+          x0=None, storage=None, dtype=np.float64, hist_cap=None, arnoldi=False) -> SolveResult:
     st = storage_of(A, storage)
     M = _Mat(A, st, dtype)
     b = np.ascontiguousarray(b, dtype)
@@ -229,10 +241,64 @@ def solve(method, A, b, precond="id", tol=1e-10, max_iter=10000, restart=10,
     it.parameter_restart = restart
     it.init_guess_null = 1 if x0 is None else 0
     it.hist, it.hist_it, it.hist_cap = _p(hist), _p(hist_it), cap
+    if arnoldi:
+        mm = restart
+        ah = np.zeros((mm, mm + 1), np.float64)          # column i = Hbar(:, i)
+        av = np.zeros((mm + 1, A.m), dtype)
+        it.arnoldi_h, it.arnoldi_v = _p(ah), _p(av)
     code = getattr(lib(), f"orc_{_sfx(dtype)}_{method}")(
         ctypes.byref(M.s), _p(x), _p(b), PREC[precond], _p(diag), ctypes.byref(it))
     n = min(it.hist_len, cap)
-    return SolveResult(x, code, it.nb_iter, hist[:n].copy(), hist_it[:n].copy())
+    arn = None
+    if arnoldi:
+        k = it.arnoldi_steps
+        arn = (av[:k + 1].copy(), ah[:k, :k + 1].T.copy())
+    return SolveResult(x, code, it.nb_iter, hist[:n].copy(), hist_it[:n].copy(), arn)
+
+
+@dataclass
+class IrResult:
+    x: np.ndarray
+    code: int
+    refinements: int
+    inner_iters: list      # inner iterations of each refinement step
+    relres_hist: np.ndarray
+
+
+def solve_ir(A, b, precond="jacobi", tol=1e-10, inner_method="gmres", inner_restart=50, inner_tol=1e-6,
+             inner_max_iter=2000, max_refinements=50) -> IrResult:
+    """FP32 inner solve + FP64 iterative refinement, the definition the
+    library's shk_solve_ir implements (no Seldon counterpart): x0 = 0;
+    while |b - A x| / |b| >= tol: r = b - A x (FP64 MltAdd), d = the FP32
+    solver (``solve(..., dtype=float32)``, a null initial guess) on A d =
+    float(r * (1/|r|)), x += |r| * double(d); stop with code 1 when the true
+    residual does not decrease, -2 after max_refinements."""
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(A.m, np.float64)
+    bn = float(norm2(b))
+    if bn == 0.0:
+        return IrResult(x, 0, 0, [], np.zeros(1))
+    r = mltadd(-1.0, A, x, 1.0, b)
+    rn = float(norm2(r))
+    rel = rn / bn
+    hist, inner, code, k = [rel], [], 0, 0
+    while rel >= tol and k < max_refinements:
+        r32 = (r * (1.0 / rn)).astype(np.float32)
+        res = solve(inner_method, A, r32, precond=precond, tol=inner_tol, max_iter=inner_max_iter,
+                    restart=inner_restart, dtype=np.float32)
+        inner.append(res.nb_iter)
+        x += rn * res.x.astype(np.float64)
+        r = mltadd(-1.0, A, x, 1.0, b)
+        rn = float(norm2(r))
+        prev, rel = rel, rn / bn
+        k += 1
+        hist.append(rel)
+        if not rel < prev:
+            code = 1
+            break
+    if code == 0 and rel >= tol:
+        code = -2
+    return IrResult(x, code, k, inner, np.array(hist))
 
 
 def sddmm(A, lam, x):

@@ -58,6 +58,14 @@ typedef struct {
     int *hist_it;
     int hist_len;
     int hist_cap;
+    /* optional (Gmres only): the first restart cycle's Arnoldi basis and
+     * Hessenberg matrix before any rotation, for the invariant
+     * A M^{-1}... V_k = V_{k+1} Hbar_k.  arnoldi_v holds (m+1) vectors of n
+     * values of the family's type, arnoldi_h (m+1) x m doubles, column
+     * major; arnoldi_steps = the k the cycle reached. */
+    double *arnoldi_h;
+    void *arnoldi_v;
+    int arnoldi_steps;
 } orc_iter;
 
 void orc_iter_init(orc_iter *it, int max_iter, double tol);

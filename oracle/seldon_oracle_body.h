@@ -556,6 +556,9 @@ int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
         /* inner iteration copies outer (Iterative.cxx:113-124) with
          * nb = outer nb and max = outer nb + m (Gmres.cxx:114-116) */
         int inner_nb = outer->nb_iter, inner_max = outer->nb_iter + m;
+        /* test-only record of the first cycle (the Arnoldi invariant) */
+        const int record = outer->arnoldi_h != NULL && outer->arnoldi_v != NULL && outer->nb_iter == 0;
+        if (record) FN(copy)(N, V[0], (T *)outer->arnoldi_v);
         do {
             FN(mlt)(A, V[i], u);
             FN(precond)(precond, N, diag, u, w);
@@ -567,6 +570,12 @@ int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
             double hi_ip1 = (double)FN(norm2)(N, w);
             FN(copy)(N, w, V[i + 1]);
             if (hi_ip1 != 0.0) FN(scal)(N, (T)(1.0 / hi_ip1), V[i + 1]);
+            if (record) {
+                for (k = 0; k <= i; k++) outer->arnoldi_h[(size_t)i * (m + 1) + k] = H[i * ld + k];
+                outer->arnoldi_h[(size_t)i * (m + 1) + i + 1] = hi_ip1;
+                FN(copy)(N, V[i + 1], (T *)outer->arnoldi_v + (size_t)(i + 1) * N);
+                outer->arnoldi_steps = i + 1;
+            }
             for (k = 0; k < i; k++)
                 orc_applyrot(&H[i * ld + k], &H[i * ld + k + 1], rc[k], rs[k]);
             if (hi_ip1 != 0.0) {

@@ -27,6 +27,9 @@
 //                     RowSparse matrix of the same size; a binary CSR)
 //   seldon_ref solve <cg|bicgstab|gmres> <row|sym|array> <id|jacobi>
 //                    A b x0|- tol maxit restart out
+//   seldon_ref solvef ...  (the same arguments; float matrix and vectors:
+//                    Iteration<double> for Cg / BiCgStab, Iteration<float>
+//                    for Gmres)
 // The Jacobi preconditioner does not exist in the reference; it is defined
 // here as the Preconditioner_Base-shaped class z_i = r_i / a_ii (the
 // division form of SOR's /a_jj, matrix_sparse/Relaxation_MatVect.cxx:79).
@@ -160,30 +163,49 @@ static int run_spmv(bool add, const string& st, char** a, bool trans = false)
   return 0;
 }
 
-template <class Mat, class Prec>
-static int solve_with(const string& method, Mat& A, Vector<double>& x,
-                      const Vector<double>& b, Prec& M, Iteration<double>& it)
+// Seldon's Gmres compiles only where the vector type and Iteration's type
+// agree (its rotations mix them, Gmres.cxx:75-76, :145-157), and only with
+// SELDON_WITH_BLAS (Solve(ColUpTriang), Blas_2.cxx:3225-3238).
+template <class Mat, class Prec, class T, class Titer>
+static int run_gmres(Mat& A, Vector<T>& x, const Vector<T>& b, Prec& M, Iteration<Titer>& it, T*, Titer*)
+{
+  cerr << "Gmres does not compile for these types" << endl;
+  exit(3);
+}
+
+template <class Mat, class Prec, class T>
+static int run_gmres(Mat& A, Vector<T>& x, const Vector<T>& b, Prec& M, Iteration<T>& it, T*, T*)
+{
+#ifdef SELDON_WITH_BLAS
+  return Gmres(A, x, b, M, it);
+#else
+  cerr << "method not available in this build: gmres" << endl;
+  exit(3);
+#endif
+}
+
+template <class Mat, class Prec, class T, class Titer>
+static int solve_with(const string& method, Mat& A, Vector<T>& x,
+                      const Vector<T>& b, Prec& M, Iteration<Titer>& it)
 {
   if (method == "cg")
     return Cg(A, x, b, M, it);
   if (method == "bicgstab")
     return BiCgStab(A, x, b, M, it);
-#ifdef SELDON_WITH_BLAS
   if (method == "gmres")
-    return Gmres(A, x, b, M, it);
-#endif
+    return run_gmres(A, x, b, M, it, (T*)0, (Titer*)0);
   cerr << "method not available in this build: " << method << endl;
   exit(3);
 }
 
-template <class Mat>
+template <class Mat, class T, class Titer>
 static int solve_prec(const string& method, const string& prec, Mat& A,
-                      const Vector<double>& d, Vector<double>& x,
-                      const Vector<double>& b, Iteration<double>& it)
+                      const Vector<T>& d, Vector<T>& x,
+                      const Vector<T>& b, Iteration<Titer>& it)
 {
   if (prec == "jacobi")
     {
-      JacobiPreconditioner<double> M;
+      JacobiPreconditioner<T> M;
       M.diag = d;
       return solve_with(method, A, x, b, M, it);
     }
@@ -191,12 +213,17 @@ static int solve_prec(const string& method, const string& prec, Mat& A,
   return solve_with(method, A, x, b, M, it);
 }
 
+// T: the vector and matrix value type; Titer: Iteration's type.  Seldon's
+// Cg and BiCgStab on float vectors run with Iteration<double>; its Gmres
+// needs Iteration<float> for float vectors (rotations_cos is a
+// Vector<Titer> that GenRot takes beside float values, Gmres.cxx:75-76).
+template <class T, class Titer>
 static int run_solve(char** a)
 {
   string method = a[0], st = a[1], prec = a[2];
-  Vector<double> b, x, d;
+  Vector<T> b, x, d;
   b.Read(a[4]);
-  Iteration<double> it(atoi(a[7]), atof(a[6]));
+  Iteration<Titer> it(atoi(a[7]), Titer(atof(a[6])));
   it.SetRestart(atoi(a[8]));
   it.ShowFullHistory();
   if (string(a[5]) != "-")
@@ -207,25 +234,25 @@ static int run_solve(char** a)
   else
     {
       x.Reallocate(b.GetM());
-      x.Fill(0.0);
+      x.Fill(T(0));
     }
   cout.precision(17);
   int code;
   if (st == "sym")
     {
-      Matrix<double, Symmetric, RowSymSparse> A;
+      Matrix<T, Symmetric, RowSymSparse> A;
       A.Read(a[3]);
       sym_diag(A, d);
       code = solve_prec(method, prec, A, d, x, b, it);
     }
   else
     {
-      Matrix<double, General, RowSparse> A;
+      Matrix<T, General, RowSparse> A;
       A.Read(a[3]);
       csr_diag(A, d);
       if (st == "array")
         {
-          Matrix<double, General, ArrayRowSparse> B;
+          Matrix<T, General, ArrayRowSparse> B;
           to_array(A, B);
           code = solve_prec(method, prec, B, d, x, b, it);
         }
@@ -359,7 +386,14 @@ int main(int argc, char** argv)
       if (argc >= 10 && string(argv[1]) == "coo2csr")
         return string(argv[2]) == "f" ? run_coo2csr<float>(argv + 3) : run_coo2csr<double>(argv + 3);
       if (argc >= 12 && string(argv[1]) == "solve")
-        return run_solve(argv + 2);
+        return run_solve<double, double>(argv + 2);
+      if (argc >= 12 && string(argv[1]) == "solvef")
+        {
+          // float vectors and matrix; Iteration<float> for Gmres only
+          if (string(argv[2]) == "gmres")
+            return run_solve<float, float>(argv + 2);
+          return run_solve<float, double>(argv + 2);
+        }
     }
   catch (Error& e)
     {

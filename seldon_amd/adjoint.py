@@ -24,9 +24,15 @@ from . import workloads as wl
 
 
 class SparseSolve(torch.autograd.Function):
+    """x = A(values)^{-1} b.  A and AT are device handles whose values are
+    overwritten by every solve on them, so backward sets them again from the
+    saved ``values`` (and ``perm_T``, AT's values as a permutation of A's)
+    before the adjoint solve: several solves on one operator before one
+    backward each get their own matrix."""
+
     @staticmethod
     def forward(ctx, values, b, A: "K.Matrix", AT: "K.Matrix | None", spd: bool, tol: float, max_iter: int,
-                restart: int):
+                restart: int, perm_T: "torch.Tensor | None" = None):
         A.set_values(values.detach().contiguous())
         x = torch.zeros_like(b)
         it = K.Iteration(max_iter, tol)
@@ -40,6 +46,7 @@ class SparseSolve(torch.autograd.Function):
             raise RuntimeError(f"forward solve did not converge (code {code}, {it.nb_iter} iterations)")
         ctx.save_for_backward(values, x)
         ctx.A, ctx.AT, ctx.spd, ctx.tol, ctx.max_iter, ctx.restart = A, AT, spd, tol, max_iter, restart
+        ctx.perm_T = perm_T
         ctx.forward_iters = it.nb_iter
         return x
 
@@ -56,6 +63,7 @@ class SparseSolve(torch.autograd.Function):
             code = K.Cg(A, lam, gx, K.JacobiPreconditioner(), it)
         else:
             AT = ctx.AT
+            AT.set_values(values.detach()[ctx.perm_T].contiguous())
             it.SetRestart(ctx.restart)
             code = K.Gmres(AT, lam, gx, K.JacobiPreconditioner(), it)
         if code != 0:
@@ -63,7 +71,7 @@ class SparseSolve(torch.autograd.Function):
         gvals = torch.empty_like(values)
         from . import _lib as L
         L.check(L.load().shk_sddmm(A.ctx.h, A.h, K._ptr(lam), K._ptr(x), K._ptr(gvals)))
-        return gvals, lam, None, None, None, None, None, None
+        return gvals, lam, None, None, None, None, None, None, None
 
 
 class SparseOperator:
@@ -88,9 +96,7 @@ class SparseOperator:
             self.AT = K.Matrix(PT, storage="row", dtype=torch.float64, ctx=ctx)
 
     def solve(self, values, b, tol=1e-12, max_iter=20000, restart=50):
-        if self.AT is not None:
-            self.AT.set_values(values.detach()[self.perm_T].contiguous())
-        return SparseSolve.apply(values, b, self.A, self.AT, self.spd, tol, max_iter, restart)
+        return SparseSolve.apply(values, b, self.A, self.AT, self.spd, tol, max_iter, restart, self.perm_T)
 
 
 class DiffusionOperator(SparseOperator):

@@ -132,9 +132,18 @@ static void upper_of(const Matrix<double, General, RowSparse>& A, Matrix<double,
   S.SetData(n, n, values, ptr, ind);
 }
 
+// Usage: seldon_overlay_check [g] [gc] [dir] [cpu|gpu|both]
+//   cpu   only Seldon's CPU storages (what tests/golden/make_golden.py
+//         records as a fixture, in this container);
+//   gpu   only the GPU storages, reading dir/b_cpu.bin (Seldon's vector
+//         file of the Laplacian right-hand side) from the fixture;
+//   both  (default) the two, interleaved.
 int main(int argc, char** argv)
 {
   const int g = argc > 1 ? atoi(argv[1]) : 64, n = g * g;
+  const string dir = argc > 3 ? argv[3] : ".";
+  const string mode = argc > 4 ? argv[4] : "both";
+  const bool cpu = mode != "gpu", gpu = mode != "cpu";
   // 5-point Laplacian, natural order, sorted columns
   Vector<double> values, xt(n), bh(n);
   Vector<int> ptr(n + 1), ind;
@@ -161,38 +170,44 @@ int main(int argc, char** argv)
   Mlt(A, xt, bh);
   cout.precision(17);
 
-  {
-    cout << "== cpu" << endl;
-    Vector<double> x(n);
-    x.Fill(0.0);
-    Iteration<double> it(2000, 1e-10);
-    it.ShowFullHistory();
-    Preconditioner_Base M;
-    int code = Cg(A, x, bh, M, it);
-    cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
-  }
-  Matrix<double, General, RowSparseHip, HipAlloc<double> > Ag(A);
-  Vector<double, VectHip, HipAlloc<double> > bg(bh);
-  {
-    cout << "== gpu_fused" << endl;
-    Vector<double, VectHip, HipAlloc<double> > x(n);
-    x.Fill(0.0);
-    Iteration<double> it(2000, 1e-10);
-    it.ShowFullHistory();
-    Preconditioner_Base M;
-    int code = Cg(Ag, x, bg, M, it);
-    cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
-  }
-  {
-    cout << "== gpu_generic" << endl;
-    Vector<double, VectHip, HipAlloc<double> > x(n);
-    x.Fill(0.0);
-    Iteration<double> it(2000, 1e-10);
-    it.ShowFullHistory();
-    UserIdentity M;
-    int code = Cg(Ag, x, bg, M, it);
-    cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
-  }
+  if (cpu)
+    {
+      cout << "== cpu" << endl;
+      Vector<double> x(n);
+      x.Fill(0.0);
+      Iteration<double> it(2000, 1e-10);
+      it.ShowFullHistory();
+      Preconditioner_Base M;
+      int code = Cg(A, x, bh, M, it);
+      cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+    }
+  Matrix<double, General, RowSparseHip, HipAlloc<double> > Ag;
+  Vector<double, VectHip, HipAlloc<double> > bg;
+  if (gpu)
+    {
+      Ag.Copy(A);
+      bg.Upload(bh);
+      {
+        cout << "== gpu_fused" << endl;
+        Vector<double, VectHip, HipAlloc<double> > x(n);
+        x.Fill(0.0);
+        Iteration<double> it(2000, 1e-10);
+        it.ShowFullHistory();
+        Preconditioner_Base M;
+        int code = Cg(Ag, x, bg, M, it);
+        cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+      }
+      {
+        cout << "== gpu_generic" << endl;
+        Vector<double, VectHip, HipAlloc<double> > x(n);
+        x.Fill(0.0);
+        Iteration<double> it(2000, 1e-10);
+        it.ShowFullHistory();
+        UserIdentity M;
+        int code = Cg(Ag, x, bg, M, it);
+        cout << "ERRCODE " << code << " NBITER " << it.GetNumberIteration() << endl;
+      }
+    }
   {
     const int gc = argc > 2 ? atoi(argv[2]) : 24, nc = gc * gc;
     Matrix<double, General, RowSparse> C;
@@ -200,14 +215,19 @@ int main(int argc, char** argv)
     Vector<double> xc(nc), bc(nc);
     for (int i = 0; i < nc; i++) xc(i) = std::cos(0.21 * i) - 0.3 * std::sin(0.7 * i);
     Mlt(C, xc, bc);
-    Matrix<double, General, RowSparseHip, HipAlloc<double> > Cg_(C);
-    Vector<double, VectHip, HipAlloc<double> > bcg(bc);
+    Matrix<double, General, RowSparseHip, HipAlloc<double> > Cg_;
+    Vector<double, VectHip, HipAlloc<double> > bcg;
+    if (gpu)
+      {
+        Cg_.Copy(C);
+        bcg.Upload(bc);
+      }
     const char* solvers[9] = {"bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr", "bicgstabl"};
     for (int k = 0; k < 9; k++)
       {
         const int maxit = (k == 2 || k == 3) ? 4000 : 2000;
-        run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
-        run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
+        if (cpu) run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
+        if (gpu) run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
       }
   }
   {
@@ -215,29 +235,36 @@ int main(int argc, char** argv)
     const char* solvers[5] = {"minres", "symmlq", "cocg", "bicgcr", "qmrsym"};
     for (int k = 0; k < 5; k++)
       {
-        run_trans_solver(solvers[k], "cpu", A, bh, 2000, 1e-8);
-        run_trans_solver(solvers[k], "gpu", Ag, bg, 2000, 1e-8);
+        if (cpu) run_trans_solver(solvers[k], "cpu", A, bh, 2000, 1e-8);
+        if (gpu) run_trans_solver(solvers[k], "gpu", Ag, bg, 2000, 1e-8);
       }
-    run_ssor_solver("ssor_cg", "cpu", A, bh, 2000, 1e-10);
-    run_ssor_solver("ssor_cg", "gpu", Ag, bg, 2000, 1e-10);
+    if (cpu) run_ssor_solver("ssor_cg", "cpu", A, bh, 2000, 1e-10);
+    if (gpu) run_ssor_solver("ssor_cg", "gpu", Ag, bg, 2000, 1e-10);
     // the same on the symmetric storage (upper triangle)
     Matrix<double, Symmetric, RowSymSparse> S;
     upper_of(A, S);
-    Matrix<double, Symmetric, RowSymSparseHip, HipAlloc<double> > Sg(S);
-    run_ssor_solver("ssor_cg_sym", "cpu", S, bh, 2000, 1e-10);
-    run_ssor_solver("ssor_cg_sym", "gpu", Sg, bg, 2000, 1e-10);
+    Matrix<double, Symmetric, RowSymSparseHip, HipAlloc<double> > Sg;
+    if (gpu) Sg.Copy(S);
+    if (cpu) run_ssor_solver("ssor_cg_sym", "cpu", S, bh, 2000, 1e-10);
+    if (gpu) run_ssor_solver("ssor_cg_sym", "gpu", Sg, bg, 2000, 1e-10);
     // Harwell-Boeing: the device matrices write Seldon's files
-    const string dir = argc > 3 ? argv[3] : ".";
-    WriteHarwellBoeing(A, dir + "/cpu.rua");
-    WriteHarwellBoeing(Ag, dir + "/gpu.rua");
-    WriteHarwellBoeing(S, dir + "/cpu_sym.rua");
-    WriteHarwellBoeing(Sg, dir + "/gpu_sym.rua");
-    // Seldon's binary vector file from both storages, and read back
-    bh.Write(dir + "/b_cpu.bin");
-    bg.Write(dir + "/b_gpu.bin");
-    Vector<double, VectHip, HipAlloc<double> > rb;
-    rb.Read(dir + "/b_cpu.bin");
-    rb.Write(dir + "/b_back.bin");
+    if (cpu)
+      {
+        WriteHarwellBoeing(A, dir + "/cpu.rua");
+        WriteHarwellBoeing(S, dir + "/cpu_sym.rua");
+        bh.Write(dir + "/b_cpu.bin");  // Seldon's binary vector file
+      }
+    if (gpu)
+      {
+        WriteHarwellBoeing(Ag, dir + "/gpu.rua");
+        WriteHarwellBoeing(Sg, dir + "/gpu_sym.rua");
+        // the same vector file from the device vector, and Seldon's file
+        // read back into one
+        bg.Write(dir + "/b_gpu.bin");
+        Vector<double, VectHip, HipAlloc<double> > rb;
+        rb.Read(dir + "/b_cpu.bin");
+        rb.Write(dir + "/b_back.bin");
+      }
     cout << "== hb" << endl;
   }
   return 0;

@@ -120,6 +120,84 @@ SOLVES = {
 }
 
 
+def run_solve_f32(tmp, spec, kind):
+    """Seldon's solver on float matrix and vectors (the harness's `solvef`:
+    Iteration<double> for Cg / BiCgStab; Iteration<float> for Gmres, the only
+    float Gmres that compiles)."""
+    A, b = build_case(spec)
+    orc.write_seldon_csr(f"{tmp}/A.bin", A, np.float32)
+    orc.write_seldon_vec(f"{tmp}/b.bin", b.astype(np.float32))
+    out = orc.run_ref(["solvef", spec["method"], spec.get("storage", "row"), spec["precond"],
+                       f"{tmp}/A.bin", f"{tmp}/b.bin", "-", spec["tol"], spec["max_iter"],
+                       spec.get("restart", 10), f"{tmp}/x.bin"], kind=kind)
+    hist, its, code, nb = orc.parse_ref_history(out)
+    x = orc.read_seldon_vec(f"{tmp}/x.bin", np.float32)
+    return A, hist, its, code, nb, x
+
+
+# FP32 solves: every vector, matrix value and recurrence scalar in float.
+SOLVES_F32 = {
+    "cg_laplace64": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-6, max_iter=2000),
+    "cg_laplace64_sym": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-6,
+                             max_iter=2000, storage="sym"),
+    "cg_laplace64_array": dict(kind="laplace2d", grid=64, method="cg", precond="id", tol=1e-6,
+                               max_iter=2000, storage="array"),
+    "cg_diffusion16_jacobi": dict(kind="diffusion3d", grid=16, seed=31, method="cg", precond="jacobi",
+                                  tol=1e-6, max_iter=2000),
+    "cg_laplace128": dict(kind="laplace2d", grid=128, method="cg", precond="id", tol=1e-5, max_iter=4000),
+    "bicgstab_convdiff16_jacobi": dict(kind="convdiff3d", grid=16, method="bicgstab", precond="jacobi",
+                                       tol=1e-6, max_iter=2000, xseed=wl.SEED_XTRUE + 1),
+    "bicgstab_laplace64_id": dict(kind="laplace2d", grid=64, method="bicgstab", precond="id", tol=1e-5,
+                                  max_iter=2000),
+    # Gmres tolerances stay a decade above FP32's attainable accuracy (about
+    # 1e-6 relative on these systems), where the residual estimate stagnates
+    "gmres30_convdiff16_jacobi": dict(kind="convdiff3d", grid=16, method="gmres", precond="jacobi",
+                                      tol=1e-5, max_iter=2000, restart=30, xseed=wl.SEED_XTRUE + 1),
+    "gmres50_powerlaw20k_jacobi": dict(kind="powerlaw", n=20000, seed=2024, half_width=2000,
+                                       method="gmres", precond="jacobi", tol=1e-5, max_iter=2000,
+                                       restart=50, xseed=wl.SEED_XTRUE + 2),
+    "gmres10_laplace32_id": dict(kind="laplace2d", grid=32, method="gmres", precond="id", tol=1e-5,
+                                 max_iter=3000, restart=10),
+}
+
+
+def make_solves_f32():
+    """solve_f32_*.npz.  Cg / BiCgStab: `hist` etc. from Seldon's generic
+    float build, which the oracle's FP32 Cg / BiCgStab equal bit for bit;
+    `other_*` from the MKL build.  Gmres: `hist` from Seldon's float Gmres
+    (Iteration<float>: float Hessenberg and rotations, MKL's srotg / strsv);
+    `other_*` from the oracle's FP32 Gmres, which keeps the Hessenberg,
+    rotations and triangular solve in double (the FP32 inner solver of the
+    mixed-precision path)."""
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, spec in SOLVES_F32.items():
+            spec = dict(spec, dtype="float32")
+            A, b = build_case(spec)
+            if spec["method"] == "gmres":
+                _, hist, its, code, nb, x = run_solve_f32(tmp, spec, "blas")
+                g = orc.solve("gmres", A, b, precond=spec["precond"], tol=spec["tol"], max_iter=spec["max_iter"],
+                              restart=spec.get("restart", 10), storage=spec.get("storage"), dtype=np.float32)
+                other = (g.hist, g.hist_it, g.code, g.nb_iter, g.x)
+            else:
+                _, hist, its, code, nb, x = run_solve_f32(tmp, spec, "generic")
+                _, oh, oi, oc, onb, ox = run_solve_f32(tmp, spec, "blas")
+                other = (oh, oi, oc, onb, ox)
+            n = min(len(other[0]), len(hist))
+            spread = np.zeros(len(hist))
+            if n:
+                spread[:n] = np.abs(other[0][:n] - hist[:n]) / np.maximum(np.abs(hist[:n]), 1e-300)
+            np.savez_compressed(
+                os.path.join(OUT, f"solve_f32_{name}.npz"),
+                spec=json.dumps(spec), digest=csr_digest(A),
+                hist=hist, hist_it=its, code=code, nb_iter=nb, x=x,
+                x_sum=np.sum(x.astype(np.float64)), x_norm=np.linalg.norm(x.astype(np.float64)),
+                build="blas-float-iteration" if spec["method"] == "gmres" else "generic",
+                other_hist=other[0], other_code=other[2], other_nb_iter=other[3],
+                spread=spread)
+            print(f"f32 {name}: code {code} its {nb} hist {len(hist)} "
+                  f"(other: code {other[2]} its {other[3]}, max spread {spread.max() if spread.size else 0:.2e})")
+
+
 def make_solves():
     with tempfile.TemporaryDirectory() as tmp:
         for name, spec in SOLVES.items():
@@ -410,12 +488,89 @@ def make_hb():
     print("hb fixtures:", len(out))
 
 
+OVERLAY_ARGS = ["64", "24"]
+
+
+def parse_sections(text):
+    """'== label' sections of the overlay program's output, each parsed as the
+    harness's history lines: {label: (hist, its, code, nb_iter)}."""
+    sec, cur = {}, None
+    for line in text.splitlines():
+        if line.startswith("== "):
+            cur = line[3:].strip()
+            sec[cur] = []
+        elif cur:
+            sec[cur].append(line)
+    return {k: orc.parse_ref_history("\n".join(v)) for k, v in sec.items()}
+
+
+def make_overlay():
+    """overlay_cpu.npz: the CPU legs of tests/cpp/seldon_overlay_check (Seldon's
+    own Cg and fourteen more Krylov templates on its CPU storages, Seldon's
+    WriteHarwellBoeing and Vector::Write), run here from the program built
+    against the reference headers; the GPU test runs only its GPU legs and
+    compares them with these.  The Harwell-Boeing files are kept as SHA-256
+    digests (600 KB each); Seldon's vector file whole (the GPU legs read it)."""
+    import hashlib
+    import subprocess
+    exe = os.path.join(orc.REF_DIR, "seldon_overlay_check")
+    with tempfile.TemporaryDirectory() as tmp:
+        out = subprocess.run([exe] + OVERLAY_ARGS + [tmp, "cpu"], capture_output=True, text=True, check=True,
+                             timeout=600).stdout
+        files = {k: open(os.path.join(tmp, k), "rb").read() for k in ("cpu.rua", "cpu_sym.rua", "b_cpu.bin")}
+    secs = parse_sections(out)
+    arrays = {}
+    for label, (hist, its, code, nb) in secs.items():
+        if label == "hb":
+            continue
+        arrays[f"{label}_hist"] = hist
+        arrays[f"{label}_its"] = its
+        arrays[f"{label}_code"] = code
+        arrays[f"{label}_nb"] = nb
+    np.savez_compressed(os.path.join(OUT, "overlay_cpu.npz"), args=json.dumps(OVERLAY_ARGS),
+                        labels=json.dumps(sorted(k for k in secs if k != "hb")),
+                        hb_sha256=hashlib.sha256(files["cpu.rua"]).hexdigest(),
+                        hb_sym_sha256=hashlib.sha256(files["cpu_sym.rua"]).hexdigest(),
+                        hb_size=len(files["cpu.rua"]), hb_sym_size=len(files["cpu_sym.rua"]),
+                        b_cpu=np.frombuffer(files["b_cpu.bin"], np.uint8), **arrays)
+    print("overlay fixture:", sorted(k for k in secs if k != "hb"))
+
+
+def make_files():
+    """seldon_files.npz: files written by the real Seldon, as bytes: the CSR
+    file of ConvertMatrix_from_Coordinates + Matrix::Write on the "dup"
+    coordinates, and Vector::Write of a double and of a float vector (from
+    its SOR driver: one forward sweep of 2 I leaves x = b / 2)."""
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        row, col, val, index, m_min, n_min = coo_case("dup")
+        orc.write_seldon_vec(f"{tmp}/r.bin", row)
+        orc.write_seldon_vec(f"{tmp}/c.bin", col)
+        orc.write_seldon_vec(f"{tmp}/v.bin", val)
+        orc.run_ref(["coo2csr", "d", f"{tmp}/r.bin", f"{tmp}/c.bin", f"{tmp}/v.bin", index, m_min, n_min,
+                     f"{tmp}/A.bin"])
+        out["csr_dup_d"] = np.frombuffer(open(f"{tmp}/A.bin", "rb").read(), np.uint8)
+        for c, dt in (("d", np.float64), ("f", np.float32)):
+            v = wl.uniform(1000, 17).astype(dt)
+            n = v.size
+            A = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.full(n, 2.0))
+            orc.write_seldon_csr(f"{tmp}/D.bin", A, dt)
+            orc.write_seldon_vec(f"{tmp}/x.bin", np.zeros(n, dt))
+            orc.write_seldon_vec(f"{tmp}/b.bin", v)
+            orc.run_ref(["sor", "row", c, f"{tmp}/D.bin", f"{tmp}/x.bin", f"{tmp}/b.bin", "1.0", 1, 2,
+                         f"{tmp}/s.bin"])
+            out[f"vec_{c}"] = np.frombuffer(open(f"{tmp}/s.bin", "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(OUT, "seldon_files.npz"),
+                        spec=json.dumps(dict(coo="dup", vec_n=1000, vec_seed=17)), **out)
+    print("seldon files:", {k: v.size for k, v in out.items()})
+
+
 if __name__ == "__main__":
     if not orc.ref_available("generic") or not orc.ref_available("blas"):
         orc.build()
         import subprocess
         subprocess.run(["make", "-s", "-C", orc.HERE, "ref"], check=True)
-    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "sor", "hb", "solves"]
+    which = sys.argv[1:] or ["spmv", "spmvt", "coo", "sor", "hb", "solves", "solves_f32", "overlay", "files"]
     if "hb" in which:
         make_hb()
     if "coo" in which:
@@ -428,3 +583,9 @@ if __name__ == "__main__":
         make_spmvt()
     if "solves" in which:
         make_solves()
+    if "solves_f32" in which:
+        make_solves_f32()
+    if "overlay" in which:
+        make_overlay()
+    if "files" in which:
+        make_files()

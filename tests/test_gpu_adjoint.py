@@ -118,3 +118,33 @@ def test_adjoint_nonsymmetric_values_gradient(gpu):
     want = -(lam[rows] * xd[A.ind])
     got = v.grad.cpu().numpy()
     assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)
+
+
+def test_adjoint_nonsymmetric_two_solves_one_backward(gpu):
+    """Two solves on one nonsymmetric operator at different values, one
+    backward through both: each adjoint Gmres must run on A^T of its own
+    forward's values (the handle holds the second solve's values when the
+    first backward runs).  Checked against the dense formula per solve."""
+    from seldon_amd.adjoint import SparseOperator
+    A = wl.convdiff3d(8)
+    b = wl.uniform(A.m, 3)
+    x_obs = wl.uniform(A.m, 4)
+    op = SparseOperator(A, spd=False)
+    v1 = torch.tensor(A.val, device="cuda", requires_grad=True)
+    scale = np.where(np.arange(A.nnz) % 3 == 0, 1.3, 0.8)  # a different nonsymmetric matrix
+    v2 = torch.tensor(A.val * scale, device="cuda", requires_grad=True)
+    bt = torch.tensor(b, device="cuda")
+    xo = torch.tensor(x_obs, device="cuda")
+    x1 = op.solve(v1, bt, tol=1e-13)
+    x2 = op.solve(v2, bt, tol=1e-13)
+    loss = torch.sum((x1 - xo) ** 2) + 0.5 * torch.sum((x2 - xo) ** 2)
+    loss.backward()
+    rows = np.repeat(np.arange(A.m), np.diff(A.ptr))
+    for v, vals, wgt in ((v1, A.val, 2.0), (v2, A.val * scale, 1.0)):
+        M = wl.CSR(A.m, A.n, A.ptr, A.ind, vals)
+        D = M.to_scipy().toarray()
+        xd = np.linalg.solve(D, b)
+        lam = np.linalg.solve(D.T, wgt * (xd - x_obs))
+        want = -(lam[rows] * xd[A.ind])
+        got = v.grad.cpu().numpy()
+        assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)

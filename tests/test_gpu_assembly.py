@@ -111,19 +111,14 @@ def test_csr_binary_file_roundtrip(gpu, tmp_path, st, layout):
     assert np.array_equal(y1.cpu().numpy(), orc.mlt(M, x.cpu().numpy(), storage=st))
 
 
-@pytest.mark.skipif(not __import__("oracle.oracle", fromlist=["x"]).ref_available("generic"),
-                    reason="oracle/_ref not built (no reference tree)")
 def test_csr_read_file_written_by_seldon(gpu, tmp_path):
     """A CSR file written by the real Seldon (its ConvertMatrix_from_Coordinates
-    then Matrix::Write) reads into the same matrix the device assembles."""
-    from oracle import oracle as orc
+    then Matrix::Write, recorded in tests/golden/seldon_files.npz) reads into
+    the same matrix the device assembles."""
     from seldon_amd import krylov as K
+    fx = load_fixture("seldon_files.npz")
     row, col, val, index, m_min, n_min = coo_case("dup")
-    orc.write_seldon_vec(tmp_path / "r.bin", row)
-    orc.write_seldon_vec(tmp_path / "c.bin", col)
-    orc.write_seldon_vec(tmp_path / "v.bin", val)
-    orc.run_ref(["coo2csr", "d", tmp_path / "r.bin", tmp_path / "c.bin", tmp_path / "v.bin", index, m_min, n_min,
-                 tmp_path / "A.bin"])
+    (tmp_path / "A.bin").write_bytes(fx["csr_dup_d"].tobytes())
     Ag = K.Matrix.Read(tmp_path / "A.bin")
     m, n, ptr, ind, out = _run(row, col, val, index, m_min, n_min, torch.float64)
     assert (Ag.m, Ag.n) == (m, n)
@@ -164,15 +159,16 @@ def test_vector_file_roundtrip(gpu, c, tmp_path):
     with pytest.raises(RuntimeError, match="longer than the destination"):
         L.check(L.load().shk_vec_read(K.Context.default().h, K._DT[tdt], str(tmp_path / "v.bin").encode(), 1, 10,
                                       K._ptr(small), ctypes.byref(got)))
-    if orc.ref_available("generic"):
-        # the real Seldon's Vector::Write, through its SOR driver (one
-        # forward sweep of a diagonal matrix leaves x = b / a_jj)
-        A = wl.CSR(v.size, v.size, np.arange(v.size + 1, dtype=np.int32), np.arange(v.size, dtype=np.int32),
-                   np.full(v.size, 2.0))
-        orc.write_seldon_csr(tmp_path / "A.bin", A, ndt)
-        orc.write_seldon_vec(tmp_path / "x.bin", np.zeros(v.size, ndt))
-        orc.write_seldon_vec(tmp_path / "b.bin", v)
-        orc.run_ref(["sor", "row", c, str(tmp_path / "A.bin"), str(tmp_path / "x.bin"), str(tmp_path / "b.bin"),
-                     "1.0", 1, 2, str(tmp_path / "s.bin")])
-        S = K.ReadVector(tmp_path / "s.bin", dtype=tdt)
-        assert S.cpu().numpy().tobytes() == orc.read_seldon_vec(tmp_path / "s.bin", ndt).tobytes()
+    # a file the real Seldon's Vector::Write wrote (tests/golden/seldon_files.npz:
+    # uniform(1000, 17) / 2, through its SOR driver)
+    fx = load_fixture("seldon_files.npz")
+    raw = fx[f"vec_{c}"].tobytes()
+    (tmp_path / "s.bin").write_bytes(raw)
+    S = K.ReadVector(tmp_path / "s.bin", dtype=tdt)
+    assert np.int32(S.numel()).tobytes() + S.cpu().numpy().tobytes() == raw
+    n1 = 1000
+    D = wl.CSR(n1, n1, np.arange(n1 + 1, dtype=np.int32), np.arange(n1, dtype=np.int32), np.full(n1, 2.0))
+    want = orc.sor(D, np.zeros(n1, ndt), wl.uniform(n1, 17).astype(ndt), 1.0, 1, 2, dtype=ndt)
+    assert np.array_equal(S.cpu().numpy(), want)
+    K.WriteVector(S, tmp_path / "s2.bin")
+    assert (tmp_path / "s2.bin").read_bytes() == raw

@@ -4,13 +4,15 @@
   harness's command line and prints Iteration::Finished lines; its solves are
   compared with the real-Seldon fixtures, its SpMV bitwise.
 * oracle/_ref/seldon_overlay_check (compiled against the real Seldon
-  headers where that tree exists) runs Seldon's CPU Cg, the fused GPU Cg and
-  Seldon's generic Cg template on GPU vectors; all three must agree.  It
+  headers where that tree exists; the binary embeds those headers) runs the
+  fused GPU Cg and Seldon's generic Cg template on GPU vectors; both must
+  agree with Seldon's CPU Cg, recorded in tests/golden/overlay_cpu.npz.  It
   also runs fourteen more of Seldon's Krylov templates (BiCg, Qmr, Cgne,
   Lsqr with Mlt(SeldonTrans, ...) inside; Cgs, QCgs, TfQmr, Gcr, BiCgStabl,
   MinRes, Symmlq, CoCg, BiCgcr, QmrSym) on the CPU and on the GPU storages;
   they must agree within the spread of Seldon's own builds.
 """
+import json
 import os
 import subprocess
 import tempfile
@@ -93,18 +95,34 @@ def _sections(out):
     return {k: orc.parse_ref_history("\n".join(v)) for k, v in sec.items()}
 
 
-@pytest.mark.skipif(not os.path.exists(OVERLAY), reason="overlay check not built (needs the reference headers)")
 def test_overlay_in_real_seldon(gpu):
+    """The GPU legs of the overlay program (built against the real Seldon
+    headers into oracle/_ref, which embeds those headers) against its CPU
+    legs recorded in this repository (tests/golden/overlay_cpu.npz, written
+    by tests/golden/make_golden.py where the reference tree exists): the
+    reference itself does not run here."""
+    import hashlib
     import tempfile
+    assert os.path.exists(OVERLAY), "oracle/_ref/seldon_overlay_check missing: build it where the reference exists"
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "overlay_cpu.npz"), allow_pickle=False)
     tmp = tempfile.mkdtemp()
-    r = subprocess.run([OVERLAY, "64", "24", tmp], capture_output=True, text=True, timeout=300)
+    with open(os.path.join(tmp, "b_cpu.bin"), "wb") as f:  # Seldon's Vector::Write of the right-hand side
+        f.write(fx["b_cpu"].tobytes())
+    r = subprocess.run([OVERLAY] + json.loads(str(fx["args"])) + [tmp, "gpu"], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     # Seldon's WriteHarwellBoeing on the device storages: the CPU matrix's file
-    for a, b in (("cpu.rua", "gpu.rua"), ("cpu_sym.rua", "gpu_sym.rua"), ("b_cpu.bin", "b_gpu.bin"),
-                 ("b_cpu.bin", "b_back.bin")):
-        assert open(os.path.join(tmp, a), "rb").read() == open(os.path.join(tmp, b), "rb").read(), b
+    for name, key in (("gpu.rua", "hb_sha256"), ("gpu_sym.rua", "hb_sym_sha256")):
+        assert hashlib.sha256(open(os.path.join(tmp, name), "rb").read()).hexdigest() == str(fx[key]), name
+    # Vector::Write from the device vector, and Seldon's file read back
+    for name in ("b_gpu.bin", "b_back.bin"):
+        assert open(os.path.join(tmp, name), "rb").read() == fx["b_cpu"].tobytes(), name
     s = _sections(r.stdout)
-    h_cpu, it_cpu, c_cpu, n_cpu = s["cpu"]
+
+    def cpu(label):
+        return (fx[f"{label}_hist"], fx[f"{label}_its"], int(fx[f"{label}_code"]), int(fx[f"{label}_nb"]))
+
+    h_cpu, it_cpu, c_cpu, n_cpu = cpu("cpu")
     for label in ("gpu_fused", "gpu_generic"):
         h, it, c, n = s[label]
         assert c == c_cpu == 0 and n == n_cpu, (label, c, n, n_cpu)
@@ -132,7 +150,7 @@ def test_overlay_in_real_seldon(gpu):
     agree = {"cgne": 100, "lsqr": 100, "cgs": 40, "qcgs": 40, "tfqmr": 70, "bicgstabl": 2}
     for solver in ("bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr", "bicgstabl",
                    "minres", "symmlq", "cocg", "bicgcr", "qmrsym", "ssor_cg", "ssor_cg_sym"):
-        h_c, it_c, c_c, n_c = s[f"{solver}_cpu"]
+        h_c, it_c, c_c, n_c = cpu(f"{solver}_cpu")
         h_g, it_g, c_g, n_g = s[f"{solver}_gpu"]
         k = min(len(h_c), len(h_g))
         rel = np.abs(h_g[:k] - h_c[:k]) / np.abs(h_c[:k])

@@ -1,0 +1,221 @@
+"""FP32 solvers and the mixed-precision solve (FP32 inner Krylov + FP64
+iterative refinement) on the GPU, and the solvers' breakdown exits.
+
+FP32 Cg / BiCgStab / Gmres against the real-Seldon float fixtures
+(tests/golden/solve_f32_*): Cg and BiCgStab follow Seldon's float build
+(Iteration<double>); the FP32 Gmres keeps its Hessenberg, rotations and
+triangular solve in double, and is compared with the oracle's FP32 Gmres,
+the same definition (Seldon's own float Gmres, with a float Hessenberg, runs
+the same iteration counts on these inputs).  Bars as for FP64: same error
+code and iteration count, residual history within max(2e-5, 10 x the
+reference's own build-to-build spread) -- 2e-5 is about 170 float ulps, the
+FP32 counterpart of FP64's 1e-12; BiCgStab by its count and early history.
+
+shk_solve_ir against the oracle's restatement of the same refinement
+(oracle.solve_ir) and the oracle's FP64 Gmres solution.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests.conftest import GOLDEN, load_fixture
+from tests.golden.make_golden import build_case
+
+pytestmark = pytest.mark.gpu
+
+F32_FIXTURES = sorted(f for f in os.listdir(GOLDEN) if f.startswith("solve_f32_"))
+
+
+def _gpu_solve(A, b, method, precond, tol, max_iter, restart=10, storage="row", dtype=torch.float64, x0=3.0):
+    from seldon_amd import krylov as K
+    Ag = K.Matrix(A, storage=storage, dtype=dtype)
+    bg = torch.tensor(b, dtype=dtype, device="cuda")
+    it = K.Iteration(max_iter, tol)
+    it.SetRestart(restart)
+    it.HideMessages()
+    xg = torch.full((A.m,), x0, dtype=dtype, device="cuda")  # overwritten (x.Zero())
+    M = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
+    fn = {"cg": K.Cg, "bicgstab": K.BiCgStab, "gmres": K.Gmres}[method]
+    code = fn(Ag, xg, bg, M, it)
+    return code, it, xg.cpu().numpy()
+
+
+@pytest.mark.parametrize("fname", F32_FIXTURES)
+def test_f32_solver_matches_seldon(gpu, fname):
+    from oracle import oracle as orc
+    fx = load_fixture(fname)
+    spec = fx["spec"]
+    A, b = build_case(spec)
+    st = spec.get("storage", "row")
+    code, it, x = _gpu_solve(A, b.astype(np.float32), spec["method"], spec["precond"], spec["tol"],
+                             spec["max_iter"], spec.get("restart", 10), st, torch.float32)
+    if spec["method"] == "gmres":
+        # the FP32 inner Gmres's definition is the oracle's (double
+        # Hessenberg); Seldon's float Gmres gives the same counts here
+        ref_h, ref_code, ref_its = fx["other_hist"], int(fx["other_code"]), int(fx["other_nb_iter"])
+        assert ref_its == int(fx["nb_iter"])
+    else:
+        ref_h, ref_code, ref_its = fx["hist"], int(fx["code"]), int(fx["nb_iter"])
+    got_h = it.history
+    n = min(len(ref_h), len(got_h))
+    rel = np.abs(got_h[:n] - ref_h[:n]) / np.maximum(np.abs(ref_h[:n]), 1e-300)
+    spread = np.maximum.accumulate(fx["spread"][:n])
+    tol = np.maximum(2e-5, 10 * spread)
+    assert code == ref_code
+    if spec["method"] == "bicgstab":
+        lo = min(ref_its, int(fx["other_nb_iter"]))
+        hi = max(ref_its, int(fx["other_nb_iter"]))
+        slack = max(hi - lo, 2)
+        assert lo - slack <= it.nb_iter <= hi + slack, (it.nb_iter, lo, hi)
+        agree = np.nonzero(spread > 1e-5)[0]
+        k = int(agree[0]) if agree.size else n
+        assert k >= 5
+        assert np.all(rel[:k] <= tol[:k]), (int(np.argmax(rel[:k] > tol[:k])), k)
+    else:
+        assert it.nb_iter == ref_its
+        assert len(got_h) == len(ref_h)
+        assert np.all(rel <= tol), (int(np.argmax(rel > tol)), rel.max())
+    # the solution: a small true residual in FP64 and Seldon's accuracy
+    r = b - orc.mlt(A, x.astype(np.float64), storage=st)  # A: the upper triangle for "sym"
+    assert np.linalg.norm(r) <= 20 * spec["tol"] * np.linalg.norm(b) * (20 if spec["precond"] == "jacobi" else 1)
+    xs = fx["x"].astype(np.float64)
+    assert np.linalg.norm(x.astype(np.float64) - xs) <= 20 * spec["tol"] * np.linalg.norm(xs) * 100
+
+
+def _ir_case(name):
+    from seldon_amd import workloads as wl
+    if name == "powerlaw20k":
+        A = wl.powerlaw(20000, 2024, half_width=2000)
+        b = wl.apply(A, wl.uniform(A.m, wl.SEED_XTRUE + 2))
+    elif name == "powerlaw300k":
+        A = wl.powerlaw(300000, 2024, half_width=20000)
+        b = wl.apply(A, wl.uniform(A.m, 7))
+    elif name == "convdiff24":
+        A, b, _ = wl.convdiff(24)
+    elif name == "diffusion16":
+        A = wl.diffusion3d(16, wl.normal(16 ** 3, 31, 0.0, 0.5))
+        b = wl.apply(A, wl.uniform(A.m, 5))
+    else:
+        raise ValueError(name)
+    return A, b
+
+
+@pytest.mark.parametrize("name,method,inner_tol", [
+    ("powerlaw20k", "gmres", 1e-4), ("powerlaw20k", "gmres", 1e-6), ("powerlaw300k", "gmres", 1e-3),
+    ("convdiff24", "gmres", 1e-4), ("diffusion16", "cg", 1e-4)])
+def test_solve_ir_matches_oracle(gpu, name, method, inner_tol):
+    """shk_solve_ir (FP32 GMRES(50) or Cg inner, Jacobi, FP64 refinement) to
+    a 1e-10 relative true residual: the FP64 residual decreases at every
+    refinement and ends below 1e-10; the refinement count equals the
+    oracle's and the inner iterations match the oracle's FP32 solver (within
+    two per refinement: the residual's scaling comes from the GPU's own
+    norm); the solution agrees with the oracle's refined solution and with
+    the oracle's FP64 Gmres solution within 1e-9 relative."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    A, b = _ir_case(name)
+    A64 = K.Matrix(A, dtype=torch.float64)
+    A32 = K.Matrix(A, dtype=torch.float32)
+    bg = torch.tensor(b, device="cuda")
+    x = torch.zeros_like(bg)
+    info = K.solve_ir(A64, A32, x, bg, K.JacobiPreconditioner(), tol=1e-10, inner_restart=50,
+                      inner_tol=inner_tol, inner_method=method)
+    ref = orc.solve_ir(A, b, precond="jacobi", tol=1e-10, inner_method=method, inner_restart=50,
+                       inner_tol=inner_tol)
+    h = info["relres_hist"]
+    assert info["code"] == 0 == ref.code
+    assert np.all(np.diff(h) < 0), h
+    assert h[-1] < 1e-10 and info["relres"] < 1e-10
+    assert info["refinements"] == ref.refinements, (info["refinements"], ref.refinements)
+    assert abs(info["inner_iters"] - sum(ref.inner_iters)) <= 2 * ref.refinements, (info["inner_iters"], ref.inner_iters)
+    # each refinement contracts the residual like the oracle's
+    assert np.all(np.abs(np.log10(h[1:]) - np.log10(ref.relres_hist[1:])) <= 1.0), (h, ref.relres_hist)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - ref.x) <= 1e-9 * np.linalg.norm(ref.x)
+    x64 = orc.solve("gmres", A, b, precond="jacobi", tol=1e-13, max_iter=5000, restart=50).x
+    assert np.linalg.norm(xg - x64) <= 1e-9 * np.linalg.norm(x64)
+    # the true residual recomputed by the oracle
+    r = b - orc.mlt(A, xg)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b) * 1.01
+
+
+def test_solve_ir_zero_rhs_and_limit(gpu):
+    """b = 0 gives x = 0 at once; one refinement allowed on a 1e-10 target
+    stops with -2 (the max_iter code) after one contraction."""
+    from seldon_amd import krylov as K
+    A, b = _ir_case("powerlaw20k")
+    A64 = K.Matrix(A, dtype=torch.float64)
+    A32 = K.Matrix(A, dtype=torch.float32)
+    x = torch.full((A.m,), 5.0, dtype=torch.float64, device="cuda")
+    info = K.solve_ir(A64, A32, x, torch.zeros_like(x), K.JacobiPreconditioner(), tol=1e-10)
+    assert info["code"] == 0 and info["refinements"] == 0 and float(x.abs().max()) == 0.0
+    x = torch.zeros_like(x)
+    info = K.solve_ir(A64, A32, x, torch.tensor(b, device="cuda"), K.JacobiPreconditioner(), tol=1e-10,
+                      inner_tol=1e-3, max_refinements=1)
+    assert info["code"] == -2 and info["refinements"] == 1 and 1e-10 < info["relres"] < 1e-2
+
+
+# ------------------------------------------------------ breakdown exits --
+
+def _embedded(block, nbig=200000):
+    """block (a small dense matrix) on the first rows, the identity on the
+    other nbig rows: the solvers' vectors span many work-groups while the
+    breakdown comes from the block."""
+    from seldon_amd import workloads as wl
+    k = block.shape[0]
+    ptr, ind, val = [0], [], []
+    for i in range(k):
+        for j in range(k):
+            if block[i, j] != 0:
+                ind.append(j); val.append(block[i, j])
+        ptr.append(len(ind))
+    n = k + nbig
+    ind = np.concatenate([np.array(ind, np.int32), np.arange(k, n, dtype=np.int32)])
+    val = np.concatenate([np.array(val), np.ones(nbig)])
+    ptr = np.concatenate([np.array(ptr, np.int32), ptr[-1] + np.arange(1, nbig + 1, dtype=np.int32)])
+    return wl.CSR(n, n, ptr, ind, val)
+
+
+def _rhs(head, n):
+    b = np.zeros(n)
+    b[:len(head)] = head
+    return b
+
+
+# (name, method, precond, block, rhs head, expected code, expected nb_iter)
+SKEW = np.array([[0.0, 1.0], [-1.0, 0.0]])
+BREAKDOWNS = [
+    # Cg: rho = (r, z) = 0 with Jacobi on diag(1, -1), b = (1, 1)  (Cg.cxx:79-83)
+    ("cg_rho", "cg", "jacobi", np.diag([1.0, -1.0]), [1.0, 1.0], 1, 0),
+    # Cg: delta = (p, A p) = 0, identity on diag(1, -1)  (Cg.cxx:98-102)
+    ("cg_delta", "cg", "id", np.diag([1.0, -1.0]), [1.0, 1.0], 2, 0),
+    # BiCgStab: sigma = (rtilde, A p) = 0 on a skew block  (BiCgStab.cxx:104-108)
+    ("bicgstab_sigma", "bicgstab", "id", SKEW, [1.0, 2.0], 3, 0),
+    # BiCgStab: omega = (t, s) / (t, t) = 0 exactly (s lands on the
+    # skew-plus-indefinite null direction), then rho_1 = (rtilde, r) = 0 at
+    # the next loop head  (BiCgStab.cxx:74-78)
+    ("bicgstab_rho", "bicgstab", "id",
+     np.block([[np.diag([1.0, -1.0]), np.zeros((2, 2))], [np.zeros((2, 2)), SKEW]]), [2.0, 1.0, 2.0, 0.0], 1, 2),
+]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("case", BREAKDOWNS, ids=[c[0] for c in BREAKDOWNS])
+def test_breakdown_codes_match_oracle(gpu, case, dtype):
+    """Each breakdown exit of Cg and BiCgStab (Iteration::Fail codes 1-3)
+    on a system whose first rows force it and whose other 200,000 rows keep
+    every work-group of the fused kernels busy: the code, the iteration
+    count and x equal the oracle's (x bitwise: the exits happen before any
+    reduction-order difference reaches x)."""
+    from oracle import oracle as orc
+    name, method, precond, block, head, want_code, want_its = case
+    A = _embedded(block)
+    b = _rhs(head, A.m)
+    npdt = np.float64 if dtype == torch.float64 else np.float32
+    ref = orc.solve(method, A, b.astype(npdt), precond=precond, tol=1e-10, max_iter=50, dtype=npdt)
+    assert (ref.code, ref.nb_iter) == (want_code, want_its)
+    code, it, x = _gpu_solve(A, b.astype(npdt), method, precond, 1e-10, 50, dtype=dtype)
+    assert code == want_code and it.nb_iter == want_its, (code, it.nb_iter)
+    assert np.array_equal(x, ref.x)

@@ -23,7 +23,8 @@ from tests.golden.make_golden import build_case
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "solve_*.npz")))
+FIXTURES = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "solve_*.npz"))
+                  if not os.path.basename(p).startswith("solve_f32_"))  # FP32: tests/test_gpu_ir.py
 # where the measured histories are written for inspection, when set
 OUT = os.environ.get("SHK_PARITY_DIR")
 

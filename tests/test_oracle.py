@@ -35,7 +35,7 @@ def _solve(spec):
     x0 = None if spec.get("x0seed") is None else wl.uniform(A.m, spec["x0seed"])
     return A, orc.solve(spec["method"], A, b, precond=spec["precond"], tol=spec["tol"],
                         max_iter=spec["max_iter"], restart=spec.get("restart", 10), x0=x0,
-                        storage=spec.get("storage"))
+                        storage=spec.get("storage"), dtype=np.dtype(spec.get("dtype", "float64")))
 
 
 def test_fixture_specs_match_generator():
@@ -53,12 +53,29 @@ def test_oracle_solver_matches_seldon_fixture(fname):
     assert r.code == int(fx["code"])
     assert r.nb_iter == int(fx["nb_iter"])
     if str(fx["build"]) == "generic":
-        # Seldon's generic templates: the restatement is bitwise equal
+        # Seldon's generic templates (float vectors with Iteration<double>
+        # for the FP32 family): the restatement is bitwise equal
         assert np.array_equal(r.hist, fx["hist"])
         assert np.array_equal(r.hist_it, fx["hist_it"])
-        if not spec.get("zero_rhs"):
+        if "x" in fx:
+            assert np.array_equal(r.x, fx["x"])
+        elif not spec.get("zero_rhs"):
             assert np.sum(r.x) == float(fx["x_sum"])
             assert np.array_equal(r.x[:16], fx["x_head"])
+    elif str(fx["build"]) == "blas-float-iteration":
+        # FP32 Gmres: Seldon's float Gmres (Iteration<float>: float
+        # Hessenberg and rotations) against the oracle's, which keeps them in
+        # double.  Same iteration count and Finished() sequence; histories
+        # within FP32 rounding (a few hundred float ulps relative to the
+        # tolerance, since both end near it); the oracle's own history is
+        # pinned bitwise
+        assert np.array_equal(r.hist_it, fx["hist_it"])
+        assert np.array_equal(r.hist, fx["other_hist"])
+        rel = np.abs(r.hist - fx["hist"]) / np.abs(fx["hist"])
+        assert np.all(rel <= 2e-2), rel.max()
+        assert np.all(rel[:5] <= 1e-5), rel[:5]
+        xs = np.linalg.norm(fx["x"].astype(np.float64))
+        assert np.linalg.norm(r.x.astype(np.float64) - fx["x"]) <= 10 * spec["tol"] * xs * 100
     else:
         # Gmres exists only in the MKL-linked build: same iteration count and
         # Finished() sequence, history within the recorded spread, and the
@@ -532,3 +549,117 @@ def test_oracle_hb_vs_live_seldon():
         ref = orc.read_seldon_csr(f"{tmp}/o.bin")
         got = orc.hb_read(text)
         assert ref[:2] == got[:2] and all(np.array_equal(x, y) for x, y in zip(ref[2:], got[2:]))
+
+
+# ---------------------------------------------- FP32 family: invariants --
+
+def test_fp32_identity_and_diagonal_cases():
+    """A = I: Cg in one step, Gmres in one step through the happy breakdown
+    (h_{1,0} = 0); diagonal A with k distinct values: both in at most k
+    steps (the Krylov space of b holds the solution after k)."""
+    n = 256
+    b = wl.uniform(n, 3).astype(np.float32)
+    I = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n))
+    for method in ("cg", "gmres", "bicgstab"):
+        r = orc.solve(method, I, b, tol=1e-6, max_iter=10, restart=5, dtype=np.float32)
+        assert r.code == 0 and r.nb_iter == 1, (method, r.code, r.nb_iter)
+        assert np.allclose(r.x, b, rtol=1e-6)
+    vals = np.array([1.0, 2.0, 3.5, 7.0])
+    d = vals[np.arange(n) % vals.size]
+    D = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    for method in ("cg", "gmres"):
+        r = orc.solve(method, D, b, tol=1e-5, max_iter=50, restart=10, dtype=np.float32)
+        assert r.code == 0 and r.nb_iter <= vals.size, (method, r.nb_iter)
+        assert np.allclose(r.x, b / d.astype(np.float32), rtol=1e-5, atol=1e-6)
+
+
+def _arnoldi_residuals(A, V, H, diag):
+    """|| M^{-1} A V_k - V_{k+1} Hbar_k ||_F / ||Hbar||_F and V^T V - I, in
+    double from the recorded basis (Gmres.cxx:120-135 with the left
+    preconditioner M = diag(A))."""
+    k = H.shape[1]
+    S = A.to_scipy()
+    V64 = V.astype(np.float64)
+    AV = np.stack([(S @ V64[i]) / diag for i in range(k)])      # rows = vectors
+    rel = np.linalg.norm(AV - H.T @ V64) / np.linalg.norm(H)
+    return rel, V64 @ V64.T - np.eye(k + 1)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gmres_arnoldi_relation(dtype):
+    """The first restart cycle of the oracle's Gmres satisfies M^-1 A V_k =
+    V_{k+1} Hbar_k to the rounding of its vector type (FP32: the inner
+    solver of the mixed-precision path); its basis vectors have unit norm
+    and the first ones are orthogonal to rounding; modified Gram-Schmidt
+    then loses orthogonality in inverse proportion to the residual it has
+    reached (Paige, Rozloznik and Strakos, SIAM J. Matrix Anal. Appl. 28,
+    2006), so the whole basis is held to eps / (relative residual)."""
+    A = wl.convdiff3d(10)
+    b = wl.apply(A, wl.uniform(A.m, 17))
+    r = orc.solve("gmres", A, b, precond="jacobi", tol=1e-12, max_iter=40, restart=25, dtype=dtype, arnoldi=True)
+    V, H = r.arnoldi
+    assert H.shape == (26, 25) and V.shape == (26, A.m)
+    eps = float(np.finfo(dtype).eps)
+    diag = orc.jacobi_diag(A).astype(np.float64)
+    rel, G = _arnoldi_residuals(A, V, H, diag)
+    assert rel <= 100 * eps, rel
+    assert np.abs(np.diag(G)).max() <= 20 * eps
+    # after j steps the cycle's residual estimate is hist[j]: the leading
+    # (j+1) x (j+1) block of V^T V - I stays below 20 eps / hist[j]
+    for j in range(1, 26):
+        assert np.abs(G[:j + 1, :j + 1]).max() <= 20 * eps / r.hist[j], (j, np.abs(G[:j + 1, :j + 1]).max())
+    # and it is not exact: FP32 loses far more than FP64 (the basis really
+    # is built in the vector type)
+    assert np.abs(G).max() >= 10 * eps
+
+
+def test_gmres_arnoldi_check_has_teeth():
+    """Slips a restatement could make, applied to the recorded cycle, each
+    break the relation by orders of magnitude more than FP32 rounding: a
+    sign flip of an MGS coefficient, a dropped sub-diagonal norm, the
+    Hessenberg transposed in its upper part, the preconditioner left out."""
+    A = wl.convdiff3d(10)
+    b = wl.apply(A, wl.uniform(A.m, 17))
+    r = orc.solve("gmres", A, b, precond="jacobi", tol=1e-12, max_iter=40, restart=25, dtype=np.float32,
+                  arnoldi=True)
+    V, H = r.arnoldi
+    diag = orc.jacobi_diag(A).astype(np.float64)
+    k = H.shape[1]
+    bad = []
+    H1 = H.copy(); H1[2, 5] = -H1[2, 5]; bad.append((H1, diag))
+    H2 = H.copy(); H2[4, 3] = 0.0; bad.append((H2, diag))
+    H3 = H.copy(); H3[:k, :k] = np.triu(H[:k, :k]).T + np.tril(H[:k, :k], -1); bad.append((H3, diag))
+    bad.append((H, np.ones_like(diag)))
+    for Hb, db in bad:
+        rel, _ = _arnoldi_residuals(A, V, Hb, db)
+        assert rel > 10 * 100 * float(np.finfo(np.float32).eps), rel  # 10x the relation's bar
+
+
+@pytest.mark.parametrize("method,precond", [("cg", "id"), ("bicgstab", "jacobi"), ("gmres", "jacobi")])
+def test_fp32_solution_within_kappa_eps(method, precond):
+    """FP32 solves land within kappa(A) * eps_32 (times a modest factor) of the
+    FP64 oracle's solution, the forward-error bound of a backward-stable
+    solve; kappa from numpy's dense 2-norm condition number."""
+    A = wl.laplace2d(20) if method == "cg" else wl.convdiff3d(8)
+    b = wl.apply(A, wl.uniform(A.m, 41))
+    r64 = orc.solve(method, A, b, precond=precond, tol=1e-13, max_iter=5000, restart=40)
+    r32 = orc.solve(method, A, b, precond=precond, tol=1e-6, max_iter=5000, restart=40, dtype=np.float32)
+    assert r64.code == 0 and r32.code == 0
+    kappa = np.linalg.cond(A.to_scipy().toarray())
+    err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert err <= 10 * kappa * max(eps32, 1e-6), (err, kappa)
+    assert err >= eps32 / 10  # it is an FP32 solve, not an FP64 one
+
+
+def test_fp32_cg_and_bicgstab_use_float_arithmetic():
+    """The FP32 family's arithmetic is float, not a rounded FP64 run: its
+    history departs from the FP64 oracle's after the first steps by far more
+    than FP64 rounding, while staying on the same convergence curve."""
+    A = wl.laplace2d(32)
+    b = wl.apply(A, wl.uniform(A.m, 5))
+    h64 = orc.solve("cg", A, b, tol=1e-6, max_iter=500).hist
+    h32 = orc.solve("cg", A, b, tol=1e-6, max_iter=500, dtype=np.float32).hist
+    n = min(h64.size, h32.size)
+    rel = np.abs(h32[:n] - h64[:n]) / h64[:n]
+    assert rel[1:].max() > 1e-9 and rel[:10].max() < 1e-3
