@@ -267,6 +267,17 @@ typedef struct {
 /* Cg (computation/solver/iterative/Cg.cxx:45-116). */
 int shk_cg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
            shk_iteration *iter, int *code);
+/* shk_cg without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory during the solve (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels, arguments and results as shk_cg, but the work vectors, the
+ * device residual history and the reduction partials live in a fixed 32 MiB
+ * array of the library (one per device, held until the process ends) instead
+ * of the context's grow-only buffers: about 1M FP64 or 2M FP32 rows with the
+ * Jacobi preconditioner; a larger system returns SHK_ERR_ARG.  Calls that
+ * overlap on one device race on that array. */
+int shk_cg_noalloc(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+                   shk_iteration *iter, int *code);
 /* BiCgStab (computation/solver/iterative/BiCgStab.cxx:44-145). */
 int shk_bicgstab(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
                  shk_iteration *iter, int *code);

@@ -88,6 +88,7 @@ SIGNATURES = [
     ("shk_convert", _I, [_P, _I64, _I, _P, _I, _P]),
     ("shk_jacobi", _I, [_P, _P, _P, _P]),
     ("shk_cg", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_cg_noalloc", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_bicgstab", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_gmres", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_solve_ir", _I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ShkIrParams), _PI]),

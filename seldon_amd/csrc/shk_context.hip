@@ -100,6 +100,8 @@ int shk_ctx_create(int device, shk_ctx *out)
     c->stream = c->own;
     int st = ensure_red(c, 1u << 20);
     if (st != SHK_OK) { delete c; return st; }
+    st = noalloc_scratch_init(device);
+    if (st != SHK_OK) { delete c; return st; }
     SHK_HIP(hipMalloc(&c->scalars, 4096));
     SHK_HIP(hipMemset(c->scalars, 0, 4096));
     SHK_HIP(hipMalloc(&c->red.ticket, 64 * sizeof(unsigned)));

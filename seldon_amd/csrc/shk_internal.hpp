@@ -242,6 +242,9 @@ int ensure_ws(shk_ctx ctx, size_t bytes);
 template <typename T>
 int stored_arrays(shk_csr A, std::vector<int> &ptr, std::vector<int> &ind, std::vector<T> &val);
 int ensure_ws2(shk_ctx ctx, size_t bytes);
+// device address of shk_cg_noalloc's fixed scratch array, looked up once per
+// device with that device current (shk_solvers.hip)
+int noalloc_scratch_init(int device);
 int check_device_ptr(const void *p, const char *what);
 
 }  // namespace shk

@@ -8,10 +8,45 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "shk_solver_state.hpp"
 
 namespace shk {
+
+// ------------------------------------------------- allocation-free Cg ----
+// shk_cg_noalloc keeps the solver's work vectors, its device residual
+// history and its reduction partials in this array instead of the context's
+// grow-only buffers, so the call makes no device allocation.  Every device
+// the library runs on holds its own copy until the process ends; calls that
+// overlap on one device race on it (one at a time per device).
+constexpr size_t kNoAllocBytes = (size_t)32 << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+// device address of `scratch` per device (hipGetSymbolAddress with that
+// device current); host code uses these, never the array's name
+static void *g_scratch[kNoAllocDevices];
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kNoAllocDevices)
+        return fail(SHK_ERR_ARG, "no-alloc scratch: device ordinal out of range");
+    if (g_scratch[device]) return SHK_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SHK_ERR_HIP, std::string("no-alloc scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch[device] = p;
+    return SHK_OK;
+}
+
+static void *noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kNoAllocDevices) ? g_scratch[device] : nullptr;
+}
 
 // The vector arguments of these kernels are distinct buffers (__restrict__),
 // so an unrolled loop keeps several elements' loads in flight; each thread
@@ -718,6 +753,10 @@ struct Solve {
     T *part[2];             // alternating partial buffers (Gmres's folded finishes)
     char *ws;
     size_t off = 0;
+    // caller-provided work memory (shk_cg_noalloc): vectors, history and
+    // partials all come from [ext, ext + ext_cap), nothing is allocated
+    char *ext = nullptr;
+    size_t ext_cap = 0;
 
     T *take(int64_t count)
     {
@@ -739,12 +778,23 @@ struct Solve {
         size_t vb = (((size_t)n * sizeof(T) + 255) & ~(size_t)255);
         size_t hist_cap = it->hist && it->hist_cap > 0 ? (size_t)it->hist_cap : 0;
         size_t need = vec_count * vb + extra_bytes + 256 + hist_cap * (sizeof(double) + sizeof(int)) + 512;
-        SHK_TRY(ensure_ws(ctx, need));
         const size_t maxb = (size_t)std::max(nb, a->nblk_launch);
-        SHK_TRY(ensure_red(ctx, sizeof(T) * 4 * maxb));  // two buffers of [2][maxb]
-        ws = (char *)ctx->ws;
+        const size_t red = sizeof(T) * 4 * maxb;  // two buffers of [2][maxb]
+        if (ext) {
+            need = (need + 255) & ~(size_t)255;
+            SHK_CHECK(need + red <= ext_cap, SHK_ERR_ARG,
+                      "shk_cg_noalloc: the system needs " + std::to_string(need + red) +
+                          " bytes of work memory, more than the " + std::to_string(ext_cap) +
+                          " bytes of the library's fixed scratch (at most 32 MiB: about 1M FP64 or 2M FP32 rows)");
+            ws = ext;
+            partials = (T *)(ext + need);
+        } else {
+            SHK_TRY(ensure_ws(ctx, need));
+            SHK_TRY(ensure_red(ctx, red));
+            ws = (char *)ctx->ws;
+            partials = (T *)ctx->red.partials;
+        }
         off = 0;
-        partials = (T *)ctx->red.partials;
         part[0] = partials;
         part[1] = partials + 2 * maxb;
         dst = (KState<T> *)ctx->state;
@@ -839,9 +889,12 @@ struct Solve {
 constexpr int kChunk = 16;
 
 template <typename T>
-int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code,
+            char *ext = nullptr, size_t ext_cap = 0)
 {
     Solve<T> S;
+    S.ext = ext;
+    S.ext_cap = ext_cap;
     const bool jac = prec == SHK_PREC_JACOBI;
     SHK_TRY(S.setup(ctx, A, jac ? 4 : 3, 0, it));
     const int64_t n = S.n;
@@ -861,6 +914,7 @@ int cg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *i
         SpmvArgs<T> a{};
         a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1;
         a.dot0 = kDotSelf; a.skip = &S.dst->skip;
+        a.partials = S.partials;
         SHK_TRY(S.spmv(a, top0));
     }
     SHK_HIP(hipGetLastError());
@@ -1064,7 +1118,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
 }
 
 static int solve_entry(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
-                       shk_iteration *it, int *code)
+                       shk_iteration *it, int *code, bool noalloc = false)
 {
     SHK_CHECK(ctx && A && it && code, SHK_ERR_ARG, "solver: null argument");
     SHK_CHECK(A->ctx == ctx, SHK_ERR_ARG, "solver: matrix belongs to another context");
@@ -1077,6 +1131,13 @@ static int solve_entry(int which, shk_ctx ctx, shk_csr A, void *dx, const void *
     SHK_TRY(check_device_ptr(db, "b"));
     SHK_CHECK(!(it->hist_cap > 0 && !it->hist), SHK_ERR_ARG, "solver: hist_cap > 0 needs hist");
     it->hist_len = 0;
+    if (noalloc) {
+        char *ext = (char *)noalloc_scratch(ctx->device);
+        SHK_CHECK(ext, SHK_ERR_STATE, "shk_cg_noalloc: the device's scratch address was not set up");
+        if (A->dtype == SHK_F64)
+            return cg_impl<double>(ctx, A, (double *)dx, (const double *)db, prec, it, code, ext, kNoAllocBytes);
+        return cg_impl<float>(ctx, A, (float *)dx, (const float *)db, prec, it, code, ext, kNoAllocBytes);
+    }
     if (A->dtype == SHK_F64) return solve_dispatch<double>(which, ctx, A, dx, db, prec, it, code);
     return solve_dispatch<float>(which, ctx, A, dx, db, prec, it, code);
 }
@@ -1195,6 +1256,12 @@ extern "C" {
 int shk_cg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(0, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_cg_noalloc(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it,
+                   int *code)
+{
+    return solve_entry(0, ctx, A, dx, db, prec, it, code, true);
 }
 
 int shk_bicgstab(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it,
