@@ -67,7 +67,11 @@ typedef enum { SHK_PREC_IDENTITY = 0, SHK_PREC_JACOBI = 1 } shk_precond;
  *             and FEM matrices in natural order; RowSymSparse upper parts
  *             and their transposes are walked one after the other);
  *   AUTO   -- PANELS where it applies, else ROWS. */
-typedef enum { SHK_LAYOUT_AUTO = 0, SHK_LAYOUT_ROWS = 1, SHK_LAYOUT_PANELS = 2 } shk_layout;
+/* Device layouts of a matrix: ROWS (row blocks, the caller's order), PANELS
+ * (column-window chunks of 2048-row panels), SLICED (RowSparse only: the row
+ * layout plus a copy of the entries in 8192-row panels whose lanes own rows,
+ * for the SpMV).  AUTO picks one from the matrix's shape. */
+typedef enum { SHK_LAYOUT_AUTO = 0, SHK_LAYOUT_ROWS = 1, SHK_LAYOUT_PANELS = 2, SHK_LAYOUT_SLICED = 3 } shk_layout;
 
 typedef struct shk_ctx_s *shk_ctx;
 typedef struct shk_csr_s *shk_csr;
@@ -102,8 +106,8 @@ int shk_memcpy_d2d(shk_ctx ctx, void *dst, const void *src, size_t bytes);
 int shk_csr_create(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n,
                    int64_t nnz, const int *ptr, const int *ind, const void *val,
                    int src_on_device, shk_csr *out);
-/* Same, with an explicit device layout (SHK_LAYOUT_PANELS fails with
- * SHK_ERR_ARG where it does not apply). */
+/* Same, with an explicit device layout (SHK_LAYOUT_PANELS and
+ * SHK_LAYOUT_SLICED fail with SHK_ERR_ARG where they do not apply). */
 int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int m, int n,
                           int64_t nnz, const int *ptr, const int *ind, const void *val,
                           int src_on_device, int layout, shk_csr *out);
@@ -318,17 +322,18 @@ int shk_sddmm(shk_ctx ctx, shk_csr A, const void *dlam, const void *dx, void *dg
  * [row_off[s], row_off[s+1]).  One persistent kernel runs every CG (identity
  * preconditioner, Seldon's Cg recurrences per system) to tol relative to
  * ||b_s||; per-system iterations and codes come back in its[], codes[].
- * group = threads per system work-group (64..1024, power of two; 0 = the
- * tuned default). */
+ * group = threads per system work-group (64..1024, power of two; 0 = 512);
+ * rows_per_pass = rows a thread keeps in flight per pass (1, 2, 4 or 8;
+ * 0 = 4). */
 int shk_cg_batched(shk_ctx ctx, shk_csr A, int count, const int *row_off_host,
                    void *dx, const void *db, double tol, int max_iter, int group,
-                   int use_graph, int *its, int *codes);
+                   int rows_per_pass, int use_graph, int *its, int *codes);
 /* Mixed variant: FP32 inner CG per system with FP64 residual refinement
  * in the same persistent kernel; A64/A32 share the pattern. */
 int shk_cg_batched_ir(shk_ctx ctx, shk_csr A64, shk_csr A32, int count,
                       const int *row_off_host, void *dx, const void *db, double tol,
-                      int max_iter, double inner_tol, int group, int use_graph,
-                      int *its, int *codes);
+                      int max_iter, double inner_tol, int group, int rows_per_pass,
+                      int use_graph, int *its, int *codes);
 
 /* ---- timing of the last launch of a kernel family (HIP events on the
  *      stream the kernel ran on) ---------------------------------------- */

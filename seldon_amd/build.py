@@ -31,7 +31,7 @@ HIP_FLAGS = [
 
 HIP_SOURCES = ["shk_context.hip", "shk_spmv.hip", "shk_blas1.hip", "shk_solvers.hip",
                "shk_extras.hip", "shk_batched.hip", "shk_assembly.hip", "shk_sor.hip", "shk_hb.hip",
-               "shk_vector_io.hip"]
+               "shk_vector_io.hip", "shk_sliced.hip"]
 
 
 def _run(cmd, cwd=None):

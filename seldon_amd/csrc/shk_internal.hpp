@@ -118,6 +118,20 @@ struct shk_csr_s {
     int *pchunk = nullptr;    // [npanel+1] first chunk of each panel
     int *chunk = nullptr;     // [4*nchunk] {c0, c1, e0, e1} per chunk
     unsigned short *slot_row = nullptr;  // [npanel*kPanelRows] panel-local row of each row slot
+    // Sliced layout (RowSparse): the row layout above serves every other
+    // use; the SpMV reads this copy of the entries (shk_sliced.hpp)
+    int sl_npanel = 0, sl_nchunk = 0, sl_launch = 0, sl_q = 0;
+    int64_t sl_ns = 0;                 // stream length
+    int sl_stream_len = 0;             // with its padding
+    int sl_bits_bytes = 0;
+    int *sl_pchunk = nullptr;          // [sl_npanel + 1]
+    void *sl_desc = nullptr;           // [sl_nchunk * 8] SlDesc
+    void *sl_wstart = nullptr;         // [sl_npanel * 8] int2
+    void *sl_bits = nullptr;           // [blocks * 64] activity bits
+    unsigned short *sl_rowmap = nullptr;
+    unsigned short *sl_col = nullptr;  // [sl_ns] window-local columns
+    void *sl_val = nullptr;            // [sl_ns] values
+    int *sl_src = nullptr;             // [sl_ns] caller's entry index of each stream position
     // A^T for the transposed products (RowSparse, ArrayRowSparse), built on
     // first use as an ArrayRowSparse matrix whose rows list, for each column
     // c of A, the entries (i, c) in increasing i
@@ -245,6 +259,16 @@ int ensure_ws2(shk_ctx ctx, size_t bytes);
 // device address of shk_cg_noalloc's fixed scratch array, looked up once per
 // device with that device current (shk_solvers.hip)
 int noalloc_scratch_init(int device);
+// sliced layout (shk_sliced.hip): plan and upload the SpMV copy of a
+// RowSparse matrix already uploaded in the row layout (*applies = false when
+// the layout does not fit it); refresh its values from the row layout's;
+// free it
+int sliced_build(shk_csr A, const int *ptr, const int *ind, bool *applies);
+int sliced_set_values(shk_csr A);
+void sliced_free(shk_csr A);
+// AUTO picks the sliced layout for RowSparse matrices with at least this
+// many rows (256 panels of 8192 rows)
+constexpr int kSlAutoMinRows = INT32_MAX;
 int check_device_ptr(const void *p, const char *what);
 
 }  // namespace shk

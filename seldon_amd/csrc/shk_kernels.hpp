@@ -717,6 +717,12 @@ __global__ __launch_bounds__(NT, MINW) void k_panel(SpmvArgs<T> a, const int4 *_
     if (a.dot0 != kDotNone || a.dot1 != kDotNone) block_partial<T, 2>(c, a.partials, gridDim.x, blockIdx.x);
 }
 
+}  // namespace shk
+
+#include "shk_sliced.hpp"
+
+namespace shk {
+
 // ------------------------------------------------------------ launch ----
 template <typename T, typename Fin>
 inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
@@ -742,6 +748,31 @@ inline int launch_spmv(shk_csr A, SpmvArgs<T> &a, Fin fin)
     }
     constexpr bool defer = std::is_same<Fin, FinDefer>::value;
     if (A->m == 0) return SHK_OK;
+    if (A->layout == SHK_LAYOUT_SLICED) {
+        SHK_CHECK(A->sl_pchunk && A->sl_desc && A->sl_wstart && A->sl_bits && A->sl_rowmap && A->sl_col &&
+                      A->sl_val,
+                  SHK_ERR_STATE, "spmv: sliced layout arrays missing");
+        const SlicedView v{A->sl_npanel,
+                           A->n,
+                           A->sl_pchunk,
+                           (const SlDesc *)A->sl_desc,
+                           (const int2 *)A->sl_wstart,
+                           (const unsigned short *)A->sl_bits,
+                           A->sl_bits_bytes,
+                           A->sl_rowmap,
+                           A->sl_col,
+                           A->sl_stream_len};
+        a.val = (const T *)A->sl_val;
+        a.xcd_q = A->sl_q;
+        dim3 grid(A->sl_launch), block(kSlThreads);
+        hipLaunchKernelGGL((k_sliced<T>), grid, block, 0, ctx->stream, a, v);
+        a.grid_used = (int)grid.x;
+        if (dots && !defer) grid_finish<T>(ctx->stream, a.partials, (int)grid.x, a.skip, fin);
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }
+    SHK_CHECK(A->layout == SHK_LAYOUT_PANELS ? (A->chunk && A->pchunk && A->slot_row) : (A->blk_info != nullptr),
+              SHK_ERR_STATE, "spmv: layout arrays missing");
     if (A->layout == SHK_LAYOUT_PANELS) {
         constexpr int PB = panel_budget<T>();
         dim3 grid(A->nblk_launch), block(kPanelThreads);

@@ -434,6 +434,7 @@ static int set_values_dev(shk_csr A, const T *dval)
         hipLaunchKernelGGL(k_gather_diag<T>, dim3(red_blocks(A->m)), dim3(kBlock), 0, s, A->m,
                            (const int64_t *)A->map_diag, (const T *)A->val, (T *)A->diag);
     SHK_HIP(hipGetLastError());
+    if (A->layout == SHK_LAYOUT_SLICED) SHK_TRY(sliced_set_values(A));
     return SHK_OK;
 }
 
@@ -717,8 +718,11 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
                           shk_csr *out)
 {
     SHK_CHECK(ctx && out, SHK_ERR_ARG, "shk_csr_create: null argument");
-    SHK_CHECK(want == SHK_LAYOUT_AUTO || want == SHK_LAYOUT_ROWS || want == SHK_LAYOUT_PANELS, SHK_ERR_ARG,
-              "shk_csr_create: bad layout");
+    SHK_CHECK(want == SHK_LAYOUT_AUTO || want == SHK_LAYOUT_ROWS || want == SHK_LAYOUT_PANELS ||
+                  want == SHK_LAYOUT_SLICED,
+              SHK_ERR_ARG, "shk_csr_create: bad layout");
+    SHK_CHECK(want != SHK_LAYOUT_SLICED || storage == SHK_ROWSPARSE, SHK_ERR_ARG,
+              "shk_csr_create: the sliced layout is for RowSparse matrices");
     *out = nullptr;
     SHK_CHECK(m >= 0 && n >= 0 && nnz >= 0, SHK_ERR_ARG, "shk_csr_create: negative size");
     SHK_CHECK(storage >= 0 && storage <= 2, SHK_ERR_ARG, "shk_csr_create: bad storage");
@@ -749,7 +753,12 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
     SHK_TRY(build_plan(storage, m, n, nnz, hptr.data(), hind.data(), budget, align, P));
     PanelPlan Q;
     int layout = SHK_LAYOUT_ROWS;
-    if (want != SHK_LAYOUT_ROWS) {
+    // the sliced layout: asked for, or AUTO on a RowSparse matrix with enough
+    // rows to give every CU two of its 8192-row panels' work-groups
+    const bool try_sliced = want == SHK_LAYOUT_SLICED ||
+                            (want == SHK_LAYOUT_AUTO && storage == SHK_ROWSPARSE && m >= kSlAutoMinRows);
+    if (try_sliced) layout = SHK_LAYOUT_SLICED;  // confirmed after the plan below, else reverted
+    else if (want != SHK_LAYOUT_ROWS) {
         const int win = dtype == SHK_F64 ? panel_win<double>() : panel_win<float>();
         const int pbudget = dtype == SHK_F64 ? panel_budget<double>() : panel_budget<float>();
         if (plan_panels(m, n, P, pbudget, win, align, vs, Q)) layout = SHK_LAYOUT_PANELS;
@@ -783,6 +792,21 @@ int shk_csr_create_layout(shk_ctx ctx, shk_storage storage, shk_dtype dtype, int
         st = dtype == SHK_F64 ? upload<double>(A, P, (const double *)hval.data())
                               : upload<float>(A, P, (const float *)hval.data());
     }
+    if (st == SHK_OK && layout == SHK_LAYOUT_SLICED) {
+        bool applies = false;
+        st = sliced_build(A, hptr.data(), hind.data(), &applies);
+        if (st == SHK_OK && applies) {
+            st = sliced_set_values(A);
+            if (st == SHK_OK) st = hipStreamSynchronize(ctx->stream) == hipSuccess ? SHK_OK : SHK_ERR_HIP;
+        } else if (st == SHK_OK && want == SHK_LAYOUT_SLICED) {
+            st = fail(SHK_ERR_ARG, "shk_csr_create: the sliced layout needs rows with sorted columns near "
+                                   "the diagonal");
+        } else if (st == SHK_OK) {
+            // AUTO: the plain row layout already uploaded serves as is
+            sliced_free(A);
+            A->layout = SHK_LAYOUT_ROWS;
+        }
+    }
     if (st != SHK_OK) {
         shk_csr_destroy(A);
         return st;
@@ -810,6 +834,7 @@ int shk_csr_destroy(shk_csr A)
     hipFree(A->pchunk);
     hipFree(A->chunk);
     hipFree(A->slot_row);
+    sliced_free(A);
     delete A;
     return SHK_OK;
 }

@@ -94,7 +94,7 @@ class Matrix:
     storage: "row" (RowSparse), "sym" (RowSymSparse: upper triangle given),
     "array" (ArrayRowSparse, rows laid end to end)."""
 
-    LAYOUTS = {"auto": 0, "rows": 1, "panels": 2}
+    LAYOUTS = {"auto": 0, "rows": 1, "panels": 2, "sliced": 3}
 
     def __init__(self, csr, storage: str | None = None, dtype=torch.float64, ctx: Context | None = None,
                  layout: str = "auto"):
@@ -147,10 +147,10 @@ class Matrix:
 
     @property
     def layout(self) -> str:
-        """The device layout chosen at creation: "rows" or "panels"."""
+        """The device layout chosen at creation: "rows", "panels" or "sliced"."""
         v = ctypes.c_int()
         L.check(L.load().shk_csr_layout(self.h, ctypes.byref(v)))
-        return {1: "rows", 2: "panels"}[v.value]
+        return {1: "rows", 2: "panels", 3: "sliced"}[v.value]
 
     def GetM(self):
         return self.m
