@@ -93,8 +93,8 @@ SIGNATURES = [
     ("shk_gmres", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_solve_ir", _I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ShkIrParams), _PI]),
     ("shk_sddmm", _I, [_P, _P, _P, _P, _P]),
-    ("shk_cg_batched", _I, [_P, _P, _I, _P, _P, _P, _D, _I, _I, _I, _P, _P]),
-    ("shk_cg_batched_ir", _I, [_P, _P, _P, _I, _P, _P, _P, _D, _I, _D, _I, _I, _P, _P]),
+    ("shk_cg_batched", _I, [_P, _P, _I, _P, _P, _P, _D, _I, _I, _I, _I, _P, _P]),
+    ("shk_cg_batched_ir", _I, [_P, _P, _P, _I, _P, _P, _P, _D, _I, _D, _I, _I, _I, _P, _P]),
     ("shk_time_spmv", _I, [_P, _P, _I, _P, _P, _I, _PD]),
 ]
 

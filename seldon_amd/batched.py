@@ -8,10 +8,11 @@ device work queue, largest systems first), optionally captured in a hipGraph;
 
 The stacked matrices must use the row layout: ``K.Matrix(A, layout="rows")``.
 
-``autotune`` picks the work-group size ("group": threads per system) by timing
-the candidates on the batch itself and caches the choice per batch signature
-(variant, number of systems, rows, stored entries) in ``autotune.json`` next
-to this file.
+``autotune`` picks the work-group size ("group": threads per system) and the
+rows each thread keeps in flight per pass ("rows_per_pass") by timing every
+pair on the batch itself, and caches the choice per batch signature (variant,
+number of systems, rows, stored entries) in ``autotune.json`` next to this
+file.
 """
 from __future__ import annotations
 
@@ -28,7 +29,8 @@ from . import krylov as K
 from . import workloads as wl
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
-CANDIDATES = (256, 512, 1024)
+CANDIDATES = (256, 512, 1024)   # threads per system work-group
+ROWS_PER_PASS = (1, 2, 4, 8)     # rows a thread keeps in flight per pass
 
 
 def stack(systems):
@@ -62,7 +64,7 @@ def aniso_batch(count=512, seed=77, nmin=32, nmax=256, xseed=wl.SEED_XTRUE + 5):
     return A, row_off, b, xt, sides, eps
 
 
-def _run(A64, row_off, x, b, tol, max_iter, group, use_graph, A32=None, inner_tol=1e-4):
+def _run(A64, row_off, x, b, tol, max_iter, group, use_graph, A32=None, inner_tol=1e-4, rows_per_pass=0):
     count = len(row_off) - 1
     its = np.zeros(count, np.int32)
     codes = np.zeros(count, np.int32)
@@ -70,22 +72,24 @@ def _run(A64, row_off, x, b, tol, max_iter, group, use_graph, A32=None, inner_to
     lib = L.load()
     if A32 is None:
         L.check(lib.shk_cg_batched(A64.ctx.h, A64.h, count, ro.ctypes.data_as(ctypes.c_void_p), K._ptr(x), K._ptr(b),
-                                   tol, max_iter, group, int(use_graph), its.ctypes.data_as(ctypes.c_void_p),
-                                   codes.ctypes.data_as(ctypes.c_void_p)))
+                                   tol, max_iter, group, rows_per_pass, int(use_graph),
+                                   its.ctypes.data_as(ctypes.c_void_p), codes.ctypes.data_as(ctypes.c_void_p)))
     else:
         L.check(lib.shk_cg_batched_ir(A64.ctx.h, A64.h, A32.h, count, ro.ctypes.data_as(ctypes.c_void_p), K._ptr(x),
-                                      K._ptr(b), tol, max_iter, inner_tol, group, int(use_graph),
+                                      K._ptr(b), tol, max_iter, inner_tol, group, rows_per_pass, int(use_graph),
                                       its.ctypes.data_as(ctypes.c_void_p), codes.ctypes.data_as(ctypes.c_void_p)))
     return its, codes
 
 
-def cg_batched(A64: "K.Matrix", row_off, x, b, tol=1e-8, max_iter=100000, group=0, use_graph=True):
-    return _run(A64, row_off, x, b, tol, max_iter, group, use_graph)
+def cg_batched(A64: "K.Matrix", row_off, x, b, tol=1e-8, max_iter=100000, group=0, use_graph=True,
+               rows_per_pass=0):
+    return _run(A64, row_off, x, b, tol, max_iter, group, use_graph, rows_per_pass=rows_per_pass)
 
 
 def cg_batched_ir(A64: "K.Matrix", A32: "K.Matrix", row_off, x, b, tol=1e-8, max_iter=100000, inner_tol=1e-4,
-                  group=0, use_graph=True):
-    return _run(A64, row_off, x, b, tol, max_iter, group, use_graph, A32=A32, inner_tol=inner_tol)
+                  group=0, use_graph=True, rows_per_pass=0):
+    return _run(A64, row_off, x, b, tol, max_iter, group, use_graph, A32=A32, inner_tol=inner_tol,
+                rows_per_pass=rows_per_pass)
 
 
 def _signature(variant, A, row_off):
@@ -93,7 +97,10 @@ def _signature(variant, A, row_off):
 
 
 def autotune(A64, row_off, b, variant="fp64", A32=None, tol=1e-8, reps=2, use_cache=True):
-    """Fastest group size for this batch (timed), cached by signature."""
+    """Fastest (group, rows_per_pass) for this batch: every pair of the
+    candidate work-group sizes and rows kept in flight per thread is timed
+    on the batch itself (best of `reps`), and the choice is cached by the
+    batch's signature."""
     key = _signature(variant, A64, row_off)
     cache = {}
     if use_cache and os.path.exists(CACHE):
@@ -102,25 +109,29 @@ def autotune(A64, row_off, b, variant="fp64", A32=None, tol=1e-8, reps=2, use_ca
         except ValueError:
             cache = {}
     if use_cache and key in cache:
-        return cache[key]["group"], cache[key]
+        e = cache[key]
+        return (e["group"], e["rows_per_pass"]), e
     times = {}
     x = torch.zeros_like(b)
     for g in CANDIDATES:
-        best = None
-        for _ in range(reps):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            _run(A64, row_off, x, b, tol, 100000, g, True, A32=A32 if variant == "ir" else None)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        times[g] = best
-    group = min(times, key=times.get)
-    entry = {"group": group, "seconds": {str(k): v for k, v in times.items()}}
+        for u in ROWS_PER_PASS:
+            best = None
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _run(A64, row_off, x, b, tol, 100000, g, True, A32=A32 if variant == "ir" else None,
+                     rows_per_pass=u)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            times[(g, u)] = best
+    choice = min(times, key=times.get)
+    entry = {"group": choice[0], "rows_per_pass": choice[1],
+             "seconds": {f"{g}x{u}": v for (g, u), v in times.items()}}
     if use_cache:
         cache[key] = entry
         try:
             json.dump(cache, open(CACHE, "w"), indent=1)
         except OSError:
             pass
-    return group, entry
+    return choice, entry

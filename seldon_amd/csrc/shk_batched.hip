@@ -74,20 +74,49 @@ __device__ __forceinline__ T row_sum(const RowsView &M, const P *p, int i)
     return temp;
 }
 
-// y_i = (A p)_i for the rows of one system (Mlt: 0 + 1 * temp)
-template <typename T>
-__device__ __forceinline__ T row_apply(const RowsView &M, const T *p, int i)
+// The work-group's rows are visited as in a plain grid-stride loop (thread
+// tid takes rows r0 + tid, r0 + tid + G, ...), U rows of a thread at a time:
+// their loads are all issued before any is used, so a thread keeps U rows'
+// memory accesses in flight instead of one (a large system's phases are
+// latency-bound otherwise).  Every row and every per-thread partial sum is
+// still formed in the same order.
+//
+// Row sums of U rows: step k adds entry k of each row that has one, so each
+// row's additions stay in stored order (Functions_MatVect.cxx:424-430).
+template <typename T, typename P, int U>
+__device__ __forceinline__ void row_sums(const RowsView &M, const P *p, int i0, int stride, int r1, T (&temp)[U])
 {
-    const T temp = row_sum<T, T>(M, p, i);
-    T y = (T)0;
-    y += (T)1 * temp;
-    return y;
+    const T *val = (const T *)M.val;
+    int jb[U], je[U];
+    int L = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int i = i0 + u * stride;
+        const bool in = i < r1;
+        jb[u] = in ? M.ptr[i] : 0;
+        je[u] = in ? M.ptr[i + 1] : 0;
+        temp[u] = (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) L = max(L, je[u] - jb[u]);
+    for (int k = 0; k < L; k++) {
+        T av[U], pv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = jb[u] + k < je[u] ? jb[u] + k : 0;
+            av[u] = val[j];
+            pv[u] = (T)p[M.ind[j]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (jb[u] + k < je[u]) temp[u] += av[u] * pv[u];
+    }
 }
 
 // One Cg (identity) on rows [r0, r1): Seldon's recurrences.  x must hold the
 // initial guess when !zero_x.  Returns the iteration count; *code gets
 // ErrorCode().  facteur is 1/|b| (Init), already checked nonzero.
-template <typename T>
+template <typename T, int U>
 __device__ int group_cg(int r0, int r1, const RowsView &M, const T *b, T *x,
                         T *r, T *p, T *q, double facteur, double tol, int max_iter, T *buf, int &phase,
                         int *code)
@@ -95,11 +124,22 @@ __device__ int group_cg(int r0, int r1, const RowsView &M, const T *b, T *x,
     const int tid = threadIdx.x, G = blockDim.x;
     // r = b; x = 0 (Cg.cxx:63-67)
     T part = (T)0;
-    for (int i = r0 + tid; i < r1; i += G) {
-        const T v = b[i];
-        r[i] = v;
-        x[i] = (T)0;
-        part += v * v;
+    for (int i0 = r0 + tid; i0 < r1; i0 += G * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * G;
+            v[u] = i < r1 ? b[i] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * G;
+            if (i < r1) {
+                r[i] = v[u];
+                x[i] = (T)0;
+                part += v[u] * v[u];
+            }
+        }
     }
     T rr = group_allreduce(part, buf, phase);
     T rho = (T)1, rho_1 = (T)1;
@@ -112,34 +152,78 @@ __device__ int group_cg(int r0, int r1, const RowsView &M, const T *b, T *x,
         if (rho == (T)0) { *code = 1; break; }
         const bool first = nb == 0;
         const T beta = first ? (T)0 : rho / rho_1;
-        for (int i = r0 + tid; i < r1; i += G) {
-            if (first) {
-                p[i] = r[i];
-            } else {
-                T v = p[i] * beta;
-                v += (T)1 * r[i];
-                p[i] = v;
+        for (int i0 = r0 + tid; i0 < r1; i0 += G * U) {
+            T pv[U], rv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * G;
+                const bool in = i < r1;
+                rv[u] = in ? r[i] : (T)0;
+                pv[u] = in && !first ? p[i] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * G;
+                if (i < r1) {
+                    if (first) {
+                        p[i] = rv[u];
+                    } else {
+                        T v = pv[u] * beta;
+                        v += (T)1 * rv[u];
+                        p[i] = v;
+                    }
+                }
             }
         }
         __syncthreads();
         part = (T)0;
-        for (int i = r0 + tid; i < r1; i += G) {
-            const T qi = row_apply<T>(M, p, i);
-            q[i] = qi;
-            part += p[i] * qi;
+        for (int i0 = r0 + tid; i0 < r1; i0 += G * U) {
+            T temp[U], pi[U];
+            row_sums<T, T, U>(M, p, i0, G, r1, temp);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * G;
+                pi[u] = i < r1 ? p[i] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * G;
+                if (i < r1) {
+                    T qi = (T)0;  // Mlt: 0 + 1 * temp
+                    qi += (T)1 * temp[u];
+                    q[i] = qi;
+                    part += pi[u] * qi;
+                }
+            }
         }
         const T delta = group_allreduce(part, buf, phase);
         if (delta == (T)0) { *code = 2; break; }
         const T alpha = rho / delta, malpha = -alpha;
         part = (T)0;
-        for (int i = r0 + tid; i < r1; i += G) {
-            if (alpha != (T)0) x[i] += alpha * p[i];
-            T rv = r[i];
-            if (malpha != (T)0) {
-                rv += malpha * q[i];
-                r[i] = rv;
+        for (int i0 = r0 + tid; i0 < r1; i0 += G * U) {
+            T xv[U], pv[U], rv[U], qv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * G;
+                const bool in = i < r1;
+                xv[u] = in ? x[i] : (T)0;
+                pv[u] = in ? p[i] : (T)0;
+                rv[u] = in ? r[i] : (T)0;
+                qv[u] = in ? q[i] : (T)0;
             }
-            part += rv * rv;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * G;
+                if (i < r1) {
+                    if (alpha != (T)0) x[i] = xv[u] + alpha * pv[u];
+                    T rn = rv[u];
+                    if (malpha != (T)0) {
+                        rn += malpha * qv[u];
+                        r[i] = rn;
+                    }
+                    part += rn * rn;
+                }
+            }
         }
         rr = group_allreduce(part, buf, phase);
         rho_1 = rho;
@@ -169,6 +253,7 @@ struct BatchArgs {
 
 __global__ void k_batch_reset(unsigned *queue) { *queue = 0u; }
 
+template <int U>
 __global__ __launch_bounds__(kMaxGroup) void k_cg_batched(BatchArgs a)
 {
     __shared__ double buf[32];
@@ -189,8 +274,8 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched(BatchArgs a)
         const double bn = sqrt(group_allreduce(part, buf, phase));
         int code = 0, its = 0;
         if (bn != 0.0)
-            its = group_cg<double>(r0, r1, a.A64, a.b, a.x, r, p, q, 1.0 / bn,
-                                   a.tol, a.max_iter, buf, phase, &code);
+            its = group_cg<double, U>(r0, r1, a.A64, a.b, a.x, r, p, q, 1.0 / bn,
+                                      a.tol, a.max_iter, buf, phase, &code);
         if (threadIdx.x == 0) {
             a.its[s] = its;
             a.codes[s] = code;
@@ -199,6 +284,7 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched(BatchArgs a)
 }
 
 // Mixed: FP64 refinement around FP32 inner Cg, per system.
+template <int U>
 __global__ __launch_bounds__(kMaxGroup) void k_cg_batched_ir(BatchArgs a)
 {
     __shared__ double buf[32];
@@ -226,12 +312,24 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched_ir(BatchArgs a)
             for (int k = 0;; k++) {
                 // true residual r = b - A x in FP64
                 part = 0;
-                for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-                    const double temp = row_sum<double, double>(a.A64, a.x, i);
-                    double w = a.b[i];
-                    w += -1.0 * temp;
-                    r64[i] = w;
-                    part += w * w;
+                for (int i0 = r0 + threadIdx.x; i0 < r1; i0 += blockDim.x * U) {
+                    double temp[U], bv[U];
+                    row_sums<double, double, U>(a.A64, a.x, i0, blockDim.x, r1, temp);
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int i = i0 + u * blockDim.x;
+                        bv[u] = i < r1 ? a.b[i] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int i = i0 + u * blockDim.x;
+                        if (i < r1) {
+                            double w = bv[u];
+                            w += -1.0 * temp[u];
+                            r64[i] = w;
+                            part += w * w;
+                        }
+                    }
                 }
                 const double rn = sqrt(group_allreduce(part, buf, phase));
                 if (rn / bn < a.tol) break;
@@ -247,8 +345,8 @@ __global__ __launch_bounds__(kMaxGroup) void k_cg_batched_ir(BatchArgs a)
                 int icode = 0;
                 // right-hand side and residual share r32: Cg copies b into r
                 // first (r = b) and reads b nowhere else
-                total += group_cg<float>(r0, r1, a.A32, r32, d32, r32, p32, q32,
-                                         1.0 / (double)rh, a.inner_tol, a.max_iter, buf32, phase32, &icode);
+                total += group_cg<float, U>(r0, r1, a.A32, r32, d32, r32, p32, q32,
+                                            1.0 / (double)rh, a.inner_tol, a.max_iter, buf32, phase32, &icode);
                 for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) a.x[i] += rn * (double)d32[i];
                 __syncthreads();
                 if (icode > 0) { code = icode; break; }
@@ -266,8 +364,8 @@ struct BatchPlanHost {
 };
 
 static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, const int *row_off_host, void *dx,
-                          const void *db, double tol, int max_iter, double inner_tol, int group, int use_graph,
-                          int *its, int *codes, bool mixed)
+                          const void *db, double tol, int max_iter, double inner_tol, int group, int rpp,
+                          int use_graph, int *its, int *codes, bool mixed)
 {
     SHK_CHECK(ctx && A64 && row_off_host && its && codes, SHK_ERR_ARG, "cg_batched: null argument");
     SHK_CHECK(A64->ctx == ctx, SHK_ERR_ARG, "cg_batched: matrix of another context");
@@ -283,6 +381,8 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
     if (group == 0) group = 512;
     SHK_CHECK(group >= 64 && group <= kMaxGroup && (group & (group - 1)) == 0, SHK_ERR_ARG,
               "cg_batched: group must be a power of two in [64, 1024]");
+    if (rpp == 0) rpp = 4;
+    SHK_CHECK(rpp == 1 || rpp == 2 || rpp == 4 || rpp == 8, SHK_ERR_ARG, "cg_batched: rows_per_pass must be 1, 2, 4 or 8");
     SHK_CHECK(max_iter >= 0, SHK_ERR_ARG, "cg_batched: negative max_iter");
     if (count == 0) return SHK_OK;
     SHK_CHECK(row_off_host[0] == 0 && row_off_host[count] == A64->m, SHK_ERR_DIM,
@@ -337,8 +437,18 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
     const int grid = std::max(1, std::min(count, ctx->num_cus * per_cu));
     auto enqueue = [&]() -> int {
         hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, st, a.queue);
-        if (mixed) hipLaunchKernelGGL(k_cg_batched_ir, dim3(grid), dim3(group), 0, st, a);
-        else hipLaunchKernelGGL(k_cg_batched, dim3(grid), dim3(group), 0, st, a);
+        const dim3 gd(grid), bd(group);
+        if (mixed) {
+            if (rpp == 1) hipLaunchKernelGGL(k_cg_batched_ir<1>, gd, bd, 0, st, a);
+            else if (rpp == 2) hipLaunchKernelGGL(k_cg_batched_ir<2>, gd, bd, 0, st, a);
+            else if (rpp == 4) hipLaunchKernelGGL(k_cg_batched_ir<4>, gd, bd, 0, st, a);
+            else hipLaunchKernelGGL(k_cg_batched_ir<8>, gd, bd, 0, st, a);
+        } else {
+            if (rpp == 1) hipLaunchKernelGGL(k_cg_batched<1>, gd, bd, 0, st, a);
+            else if (rpp == 2) hipLaunchKernelGGL(k_cg_batched<2>, gd, bd, 0, st, a);
+            else if (rpp == 4) hipLaunchKernelGGL(k_cg_batched<4>, gd, bd, 0, st, a);
+            else hipLaunchKernelGGL(k_cg_batched<8>, gd, bd, 0, st, a);
+        }
         SHK_HIP(hipGetLastError());
         return SHK_OK;
     };
@@ -379,18 +489,18 @@ using namespace shk;
 extern "C" {
 
 int shk_cg_batched(shk_ctx ctx, shk_csr A, int count, const int *row_off_host, void *dx, const void *db,
-                   double tol, int max_iter, int group, int use_graph, int *its, int *codes)
+                   double tol, int max_iter, int group, int rows_per_pass, int use_graph, int *its, int *codes)
 {
-    return batched_common(ctx, A, nullptr, count, row_off_host, dx, db, tol, max_iter, 0.0, group, use_graph, its,
-                          codes, false);
+    return batched_common(ctx, A, nullptr, count, row_off_host, dx, db, tol, max_iter, 0.0, group, rows_per_pass,
+                          use_graph, its, codes, false);
 }
 
 int shk_cg_batched_ir(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, const int *row_off_host, void *dx,
-                      const void *db, double tol, int max_iter, double inner_tol, int group, int use_graph,
-                      int *its, int *codes)
+                      const void *db, double tol, int max_iter, double inner_tol, int group, int rows_per_pass,
+                      int use_graph, int *its, int *codes)
 {
-    return batched_common(ctx, A64, A32, count, row_off_host, dx, db, tol, max_iter, inner_tol, group, use_graph,
-                          its, codes, true);
+    return batched_common(ctx, A64, A32, count, row_off_host, dx, db, tol, max_iter, inner_tol, group,
+                          rows_per_pass, use_graph, its, codes, true);
 }
 
 }  // extern "C"

@@ -21,8 +21,8 @@ def _small_batch(count=24):
     return A, row_off, b, systems
 
 
-@pytest.mark.parametrize("group", [64, 256, 1024])
-def test_batched_matches_oracle_per_system(gpu, group):
+@pytest.mark.parametrize("group,rpp", [(64, 1), (256, 4), (512, 2), (1024, 8)])
+def test_batched_matches_oracle_per_system(gpu, group, rpp):
     from oracle import oracle as orc
     from seldon_amd import krylov as K
     from seldon_amd.batched import cg_batched
@@ -30,7 +30,8 @@ def test_batched_matches_oracle_per_system(gpu, group):
     Ag = K.Matrix(A, layout="rows")
     bg = torch.tensor(b, device="cuda")
     x = torch.full_like(bg, 7.0)
-    its, codes = cg_batched(Ag, row_off, x, bg, tol=1e-8, max_iter=5000, group=group, use_graph=False)
+    its, codes = cg_batched(Ag, row_off, x, bg, tol=1e-8, max_iter=5000, group=group, use_graph=False,
+                            rows_per_pass=rpp)
     xh = x.cpu().numpy()
     for s, S in enumerate(systems):
         sl = slice(row_off[s], row_off[s + 1])
@@ -88,9 +89,33 @@ def test_batched_ir_reaches_fp64_tolerance(gpu):
 
 def test_autotune_picks_a_candidate(gpu):
     from seldon_amd import krylov as K
-    from seldon_amd.batched import CANDIDATES, autotune
+    from seldon_amd.batched import CANDIDATES, ROWS_PER_PASS, autotune
     A, row_off, b, systems = _small_batch()
     Ag = K.Matrix(A, layout="rows")
     bg = torch.tensor(b, device="cuda")
-    g, entry = autotune(Ag, row_off, bg, use_cache=False)
-    assert g in CANDIDATES and set(entry["seconds"]) == {str(c) for c in CANDIDATES}
+    (g, u), entry = autotune(Ag, row_off, bg, use_cache=False)
+    assert g in CANDIDATES and u in ROWS_PER_PASS
+    assert set(entry["seconds"]) == {f"{c}x{r}" for c in CANDIDATES for r in ROWS_PER_PASS}
+    assert entry["seconds"][f"{g}x{u}"] == min(entry["seconds"].values())
+
+
+def test_batched_rows_per_pass_results_identical(gpu):
+    """Keeping more rows in flight per thread changes no arithmetic: the
+    same iterations, codes and solutions, bit for bit, for every setting."""
+    from seldon_amd import krylov as K
+    from seldon_amd.batched import cg_batched, cg_batched_ir
+    A, row_off, b, systems = _small_batch()
+    A64 = K.Matrix(A, layout="rows")
+    A32 = K.Matrix(A, dtype=torch.float32, layout="rows")
+    bg = torch.tensor(b, device="cuda")
+    ref = None
+    for u in (1, 2, 4, 8):
+        x = torch.zeros_like(bg)
+        its, codes = cg_batched(A64, row_off, x, bg, group=256, rows_per_pass=u)
+        xi = torch.zeros_like(bg)
+        its_i, codes_i = cg_batched_ir(A64, A32, row_off, xi, bg, tol=1e-10, group=256, rows_per_pass=u)
+        got = (its, codes, x.cpu().numpy(), its_i, codes_i, xi.cpu().numpy())
+        if ref is None:
+            ref = got
+        for a, r in zip(got, ref):
+            assert np.array_equal(a, r), u
