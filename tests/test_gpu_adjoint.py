@@ -148,3 +148,56 @@ def test_adjoint_nonsymmetric_two_solves_one_backward(gpu):
         want = -(lam[rows] * xd[A.ind])
         got = v.grad.cpu().numpy()
         assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)
+
+
+def test_adjoint_diffusion64_full_size(gpu):
+    """The differentiable solve at its stated size: 64^3 cells (262,144
+    rows), theta ~ N(0, 0.5^2), b ~ U[0, 1], x_obs from theta + N(0, 0.1^2).
+    Gradients w.r.t. theta and b against the oracle's FP64 CPU gradient
+    (its Cg forward and adjoint, SDDMM and chain rule) within 1e-6 relative,
+    and against central finite differences (h = 1e-6) of the GPU loss at 32
+    theta and 32 b components.  A central difference here carries the noise
+    of the two FP64 solves it subtracts: the loss (~3e3) is known to about
+    1e-14 of |x - x_obs| |x|, which over 2h = 2e-6 is ~1e-5 of the largest
+    gradient component, so the FD bar is 1e-4 of max |grad| (at 16^3, where
+    that noise is smaller, the FD test holds 1e-6)."""
+    from oracle import oracle as orc
+    from seldon_amd.adjoint import DiffusionOperator
+    n = 64
+    theta, b, dtheta = _problem(n, seed=2025)
+    op = DiffusionOperator(n)
+    th = torch.tensor(theta, device="cuda")
+    bt = torch.tensor(b, device="cuda")
+    with torch.no_grad():
+        x_obs = op.solve(op.values(th + torch.tensor(dtheta, device="cuda")), bt, tol=1e-14)
+    loss, gth, gb, vals = _loss_grad(op, th, bt, x_obs, tol=1e-14)
+    ref = orc.diffusion_grad_reference(n, theta, vals.cpu().numpy(), op.pattern, b, x_obs.cpu().numpy(),
+                                       tol=1e-14)
+    print(f"[adjoint64] loss {loss:.6e} (oracle {ref['loss']:.6e}), oracle its {ref['its']}", flush=True)
+    assert abs(loss - ref["loss"]) <= 1e-9 * abs(ref["loss"])
+    for got, want in ((gth, ref["grad_theta"]), (gb, ref["grad_b"])):
+        rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"[adjoint64] gradient rel. error vs oracle {rel:.2e}", flush=True)
+        assert rel <= 1e-6, rel
+    h = 1e-6
+
+    def L(th_, b_):
+        with torch.no_grad():
+            x = op.solve(op.values(th_), b_, tol=1e-14)
+            return float(torch.sum((x - x_obs) ** 2))
+
+    rng = np.random.default_rng(64)
+    worst = 0.0
+    for grad, which in ((gth, "theta"), (gb, "b")):
+        scale = np.max(np.abs(grad))
+        for j in rng.choice(n ** 3, 32, replace=False):
+            e = torch.zeros_like(th)
+            e[j] = h
+            if which == "theta":
+                fd = (L(th + e, bt) - L(th - e, bt)) / (2 * h)
+            else:
+                fd = (L(th, bt + e) - L(th, bt - e)) / (2 * h)
+            err = abs(fd - grad[j]) / scale
+            worst = max(worst, err)
+            assert abs(fd - grad[j]) <= 1e-4 * scale, (which, j, fd, grad[j])
+    print(f"[adjoint64] worst central-FD deviation {worst:.2e} (of max |grad|)", flush=True)

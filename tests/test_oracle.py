@@ -663,3 +663,33 @@ def test_fp32_cg_and_bicgstab_use_float_arithmetic():
     n = min(h64.size, h32.size)
     rel = np.abs(h32[:n] - h64[:n]) / h64[:n]
     assert rel[1:].max() > 1e-9 and rel[:10].max() < 1e-3
+
+
+def test_diffusion_grad_reference_against_its_own_finite_differences():
+    """The oracle's FP64 gradient of L = |x - x_obs|^2, x = A(theta)^{-1} b
+    (its Cg forward and adjoint, its SDDMM, and the numpy chain rule through
+    the harmonic-mean faces) against central finite differences of the
+    oracle's own solve on the C generator's matrix: every theta component
+    of a 5^3 grid (interior, face, edge and corner cells, so every branch of
+    the chain rule) and every b component."""
+    n = 5
+    N = n ** 3
+    theta = wl.normal(N, 3, 0.0, 0.5)
+    b = wl.uniform(N, 4, 0.0, 1.0)
+    x_obs = orc.solve("cg", wl.diffusion3d(n, theta + wl.normal(N, 5, 0.0, 0.1)), b, precond="jacobi",
+                      tol=1e-15, max_iter=5000).x
+    A = wl.diffusion3d(n, theta)
+    ref = orc.diffusion_grad_reference(n, theta, A.val, A, b, x_obs, tol=1e-15, max_iter=5000)
+
+    def L(th, bb):
+        x = orc.solve("cg", wl.diffusion3d(n, th), bb, precond="jacobi", tol=1e-15, max_iter=5000).x
+        return float(np.sum((x - x_obs) ** 2))
+
+    h = 1e-6
+    gt = np.array([(L(theta + h * e, b) - L(theta - h * e, b)) / (2 * h) for e in np.eye(N)])
+    gb = np.array([(L(theta, b + h * e) - L(theta, b - h * e)) / (2 * h) for e in np.eye(N)])
+    assert np.linalg.norm(gt - ref["grad_theta"]) <= 1e-6 * np.linalg.norm(gt)
+    assert np.linalg.norm(gb - ref["grad_b"]) <= 1e-6 * np.linalg.norm(gb)
+    # componentwise too: a dropped term (a face's neighbour share, a boundary
+    # face) shows in the components it touches, not only in the norm
+    assert np.max(np.abs(gt - ref["grad_theta"])) <= 1e-6 * np.max(np.abs(gt))
