@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--layout", default="auto", choices=["auto", "rows", "panels", "sliced"],
-                    help="device layout of the benchmark matrix (auto: panels where they apply)")
+                    help="device layout of the benchmark matrix (auto: sliced for RowSparse matrices of 2M rows or more)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 128^3 convection-diffusion solves and the batched Cg")
     ap.add_argument("--only", choices=["convdiff128", "batched"],
@@ -78,7 +78,8 @@ def main():
 
     if args.only == "convdiff128":
         torch.cuda.set_device(0)
-        print(json.dumps({"convdiff128": convdiff_solves(K, wl, np, torch, cpu=not args.no_cpu)}), flush=True)
+        print(json.dumps({"convdiff128": convdiff_solves(K, wl, np, torch, cpu=not args.no_cpu, layout=args.layout)}),
+              flush=True)
         return
     if args.only == "batched":
         torch.cuda.set_device(0)
@@ -325,7 +326,7 @@ def cg_config1(K, wl, np, torch, cpu=True):
     return out
 
 
-def convdiff_solves(K, wl, np, torch, cpu=True):
+def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
     """FP64 BiCgStab and Gmres(30) with the Jacobi preconditioner on the
     128^3 7-point upwind convection-diffusion matrix (2,097,152 rows), to a
     1e-10 relative residual: time to solution and iterations/s, plus the
@@ -335,10 +336,10 @@ def convdiff_solves(K, wl, np, torch, cpu=True):
 
     from seldon_amd import _lib as L
     A, b, xt = wl.convdiff(128)
-    Ad = K.Matrix(A, dtype=torch.float64)
+    Ad = K.Matrix(A, dtype=torch.float64, layout=layout)
     bg = torch.tensor(b, device="cuda")
     bn = float(np.linalg.norm(b))
-    out = {"rows": A.m, "nnz": A.nnz}
+    out = {"rows": A.m, "nnz": A.nnz, "layout": Ad.layout}
     y = torch.empty_like(bg)
     avg = ctypes.c_double()
     L.check(L.load().shk_time_spmv(Ad.ctx.h, Ad.h, 0, K._ptr(bg), K._ptr(y), 20, ctypes.byref(avg)))

@@ -268,7 +268,7 @@ int sliced_set_values(shk_csr A);
 void sliced_free(shk_csr A);
 // AUTO picks the sliced layout for RowSparse matrices with at least this
 // many rows (256 panels of 8192 rows)
-constexpr int kSlAutoMinRows = INT32_MAX;
+constexpr int kSlAutoMinRows = 1 << 21;
 int check_device_ptr(const void *p, const char *what);
 
 }  // namespace shk
