@@ -89,17 +89,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t sl_rsrc(const void *p, int nby
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, nbytes, 0x00020000);
 }
 
+// Cache policy of the entry and bit streams: non-temporal (each byte is read
+// once) for FP64, where it measured 6% faster; the default for FP32, where
+// it made no difference.
+template <typename T>
+constexpr int sl_aux() { return sizeof(T) == 8 ? 2 : 0; }
+
 template <typename T>
 __device__ __forceinline__ T sl_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
 template <>
 __device__ __forceinline__ double sl_load<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, sl_aux<double>()));
 }
 template <>
 __device__ __forceinline__ float sl_load<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, sl_aux<float>()));
 }
 
 template <typename T>
@@ -143,14 +149,14 @@ __global__ __launch_bounds__(kSlThreads, 4) void k_sliced(SpmvArgs<T> a, SlicedV
             const uint64_t m = __builtin_amdgcn_ballot_w64(act);
             const int below =
                 (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-            rw[r] = __builtin_amdgcn_raw_buffer_load_b16(rcol, below * 2, ppos * 2, 0);
+            rw[r] = __builtin_amdgcn_raw_buffer_load_b16(rcol, below * 2, ppos * 2, sl_aux<T>());
             rv[r] = sl_load<T>(rval, below * (int)sizeof(T), ppos * (int)sizeof(T));
             ra[r] = act;
             rm[r] = m;
             ppos += __builtin_popcountll(m);
         };
         auto load_bits = [&]() {
-            bload = __builtin_amdgcn_raw_buffer_load_b16(rbits, bofs, 0, 0);
+            bload = __builtin_amdgcn_raw_buffer_load_b16(rbits, bofs, 0, sl_aux<T>());
             bofs += kWave * 2;
         };
         auto next_bits = [&]() {

@@ -88,6 +88,31 @@ def test_sliced_bitwise_vs_oracle(gpu, c):
         assert _same(yg.cpu().numpy(), ref), (name, c, "mlt")
 
 
+def test_sliced_full_size_power_law(gpu):
+    """The 4,194,304-row power-law matrix (113.2M entries): enough 16384-row
+    panels for every CU of the GPU, so the 16-wavefront work-groups run --
+    bitwise against the oracle in FP64 and FP32, Mlt and MltAdd."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    A, _, xt = wl.fem_powerlaw(1 << 22)
+    y0 = wl.uniform(A.m, 31)
+    for c in ("d", "f"):
+        tdt, ndt = DT[c]
+        Ag = K.Matrix(A, dtype=tdt)
+        assert Ag.layout == "sliced"
+        print(f"[sliced] full size {c}", flush=True)
+        x = xt.astype(ndt)
+        xg = _dev(x, ndt)
+        yg = torch.empty(A.m, dtype=tdt, device="cuda")
+        K.Mlt(Ag, xg, yg)
+        assert _same(yg.cpu().numpy(), orc.mlt(A, x, dtype=ndt)), c
+        yg = _dev(y0, ndt)
+        K.MltAdd(0.7, Ag, xg, -0.3, yg)
+        ref = orc.mltadd(ndt(0.7), A, x, ndt(-0.3), y0.astype(ndt), dtype=ndt)
+        assert _same(yg.cpu().numpy(), ref), c
+        del Ag
+
+
 def test_sliced_values_update_and_fused_dot(gpu):
     from oracle import oracle as orc
     from seldon_amd import krylov as K
