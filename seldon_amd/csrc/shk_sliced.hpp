@@ -52,6 +52,7 @@ constexpr int kSlRing = 16;                   // steps per block: 16 activity bi
 constexpr unsigned short kSlNoRow = 0xffffu;
 template <typename T>
 constexpr int sl_win() { return sizeof(T) == 8 ? 8192 : 16384; }  // 64 KiB of x in LDS
+static_assert(sl_win<double>() >= kSlRows && sl_win<float>() >= kSlRows, "a panel's sums fit the window");
 
 // descriptor of one (chunk, wavefront): {blocks, group step counts (bytes
 // 0-15), window start c0, window width}
@@ -171,12 +172,14 @@ __global__ __launch_bounds__(kSlThreads, 4) void k_sliced(SpmvArgs<T> a, SlicedV
         for (int cc = cb; cc < ce; cc++) {
             const SlDesc cd = s.desc[(size_t)cc * kSlWaves + w];
             // stage the window x[c0, c0 + sl_win) straight into LDS: wave w
-            // fills its eighth, 1 KiB per load (reads past x's end return 0;
-            // entries past the window's width are never read)
+            // fills its eighth, 1 KiB per load, as far as the window's width
+            // (reads past x's end return 0; entries past the width are never
+            // read)
             __syncthreads();  // the previous window's reads are done
 #pragma unroll
             for (int j = 0; j < sl_win<T>() / (kSlWaves * kWave * VPL); j++) {
                 const int e0 = (w * (sl_win<T>() / kSlWaves)) + j * kWave * VPL;  // first element of this load
+                if (e0 >= cd.width) break;  // wave-uniform: the rest of this wave's share is past the window
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rx, (__attribute__((address_space(3))) void *)(xs + e0), 16,
                     (cd.c0 + e0 + lane * VPL) * (int)sizeof(T), 0, 0, 0);
@@ -221,17 +224,24 @@ __global__ __launch_bounds__(kSlThreads, 4) void k_sliced(SpmvArgs<T> a, SlicedV
             }
             acc[gi] = cur;
         }
-        // y = beta y + alpha temp (Functions_MatVect.cxx:414, :430)
+        // y = beta y + alpha temp (Functions_MatVect.cxx:414, :430) in row
+        // order: the sums go to LDS at their panel rows (the window's space
+        // holds a panel's 8192 values), then y is written coalesced instead
+        // of at the lanes' length-sorted rows, a partial cache line per lane
         const unsigned short *rmap = s.rowmap + ((size_t)p * kSlWaves + w) * kSlGroups * kWave + lane;
+        __syncthreads();  // the last window's reads are done
 #pragma unroll
         for (int i = 0; i < kSlGroups; i++) {
             const unsigned short lr = rmap[i * kWave];
-            if (lr != kSlNoRow) {
-                const int row = p * kSlRows + lr;
-                T y = panel_y0<T, kCsr>(a, row);
-                y += a.alpha * acc[i];
-                spmv_epilogue<T, kCsr>(a, row, y, c);
-            }
+            if (lr != kSlNoRow) xs[lr] = acc[i];
+        }
+        __syncthreads();
+        const int nr = min(kSlRows, a.m - p * kSlRows);  // every panel row has a slot
+        for (int lr = t; lr < nr; lr += kSlThreads) {
+            const int row = p * kSlRows + lr;
+            T y = panel_y0<T, kCsr>(a, row);
+            y += a.alpha * xs[lr];
+            spmv_epilogue<T, kCsr>(a, row, y, c);
         }
     }
     if (a.dot0 != kDotNone || a.dot1 != kDotNone) block_partial<T, 2>(c, a.partials, gridDim.x, blockIdx.x);
