@@ -647,6 +647,74 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *__restric
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// The same step with 16-byte accesses: thread t takes element pairs 2t, 2t+1
+// of a grid stride (n even, vectors 16-byte aligned; the host checks).  Each
+// element gets the same arithmetic; a thread's partial adds its pairs' two
+// products in element order.
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+    using type = double2;
+};
+template <>
+struct Pair<float> {
+    using type = float2;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_mgs2(int64_t n, const T *__restrict__ vprev, const T *__restrict__ vk,
+                                                    T *__restrict__ w, int hpos, KState<T> *st, const T *prev,
+                                                    int prev_nb, T *partials)
+{
+    using P2 = typename Pair<T>::type;
+    if (st->skip) return;
+    const int64_t n2 = n / 2;
+    const int64_t j0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    const P2 *vp2 = (const P2 *)vprev, *vk2 = (const P2 *)vk;
+    P2 *w2 = (P2 *)w;
+    constexpr int U = kPre / 2;
+    P2 wp[U], vp[U], kp[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int64_t j = j0 + u * stride;
+        if (j < n2) {
+            wp[u] = w2[j];
+            vp[u] = vp2[j];
+            kp[u] = vk2[j];
+        }
+    }
+    const T a = gm_fold_h<T>(prev, prev_nb, hpos, st);
+    const bool aa = a != (T)0;
+    T c[2] = {(T)0, (T)0};
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int64_t j = j0 + u * stride;
+        if (j < n2) {
+            P2 wv = wp[u];
+            if (aa) {
+                wv.x += a * vp[u].x;
+                wv.y += a * vp[u].y;
+                w2[j] = wv;
+            }
+            c[0] += kp[u].x * wv.x;
+            c[0] += kp[u].y * wv.y;
+        }
+    }
+    for (int64_t j = j0 + U * stride; j < n2; j += stride) {
+        P2 wv = w2[j];
+        const P2 v = vp2[j], k = vk2[j];
+        if (aa) {
+            wv.x += a * v.x;
+            wv.y += a * v.y;
+            w2[j] = wv;
+        }
+        c[0] += k.x * wv.x;
+        c[0] += k.y * wv.y;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
 // H(i,i) from the previous partials; last MGS update w += (-H(i,i)) V_i,
 // then the partials of |w|^2 for hi_ip1 (:131-135)
 template <typename T>
@@ -1079,9 +1147,17 @@ int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
                 SHK_TRY(S.spmv(a, FinDefer{}));
                 int prev_nb = a.grid_used;
                 for (int k = 1; k <= i; k++) {
-                    hipLaunchKernelGGL(k_gm_mgs<T>, g, blk, 0, st, n, (const T *)Vk(k - 1), (const T *)Vk(k), w,
-                                       i * (m + 1) + (k - 1), S.dst, (const T *)S.part[(k - 1) & 1], prev_nb,
-                                       S.part[k & 1]);
+                    // 16-byte accesses where the vectors allow them
+                    const bool pairs = n % 2 == 0 &&
+                                       (((uintptr_t)Vk(k - 1) | (uintptr_t)Vk(k) | (uintptr_t)w) & 15) == 0;
+                    if (pairs)
+                        hipLaunchKernelGGL(k_gm_mgs2<T>, g, blk, 0, st, n, (const T *)Vk(k - 1), (const T *)Vk(k), w,
+                                           i * (m + 1) + (k - 1), S.dst, (const T *)S.part[(k - 1) & 1], prev_nb,
+                                           S.part[k & 1]);
+                    else
+                        hipLaunchKernelGGL(k_gm_mgs<T>, g, blk, 0, st, n, (const T *)Vk(k - 1), (const T *)Vk(k), w,
+                                           i * (m + 1) + (k - 1), S.dst, (const T *)S.part[(k - 1) & 1], prev_nb,
+                                           S.part[k & 1]);
                     prev_nb = S.nb;
                 }
                 hipLaunchKernelGGL(k_gm_last<T>, g, blk, 0, st, n, (const T *)Vk(i), w, i * (m + 1) + i, S.dst,
