@@ -173,6 +173,8 @@ def main():
         "upload_plan_seconds": {"fp64": round(up64, 3), "fp32": round(up32, 3)},
     }
 
+    result["transposed"] = transposed_product(K, A64, x64, y64, bytes64, torch)
+
     if not args.no_solve:
         result["solve"] = solve_compare(K, A, A64, A32, b, xt, np, torch, cpu=not args.no_cpu)
 
@@ -192,6 +194,25 @@ def main():
         result["cpu_baseline"] = cpu_baseline(A, xt, bytes64)
         result["host"] = host_info()
     print(json.dumps(result), flush=True)
+
+
+def transposed_product(K, A64, x64, y64, nbytes, torch, reps=10):
+    """Mlt(SeldonTrans, A, X, Y) on the power-law matrix: the first call
+    builds A^T (device sort of the coordinates, layout plan on the host,
+    upload), the later ones are one SpMV with it."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    K.Mlt(K.SeldonTrans, A64, x64, y64)
+    torch.cuda.synchronize()
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        K.Mlt(K.SeldonTrans, A64, x64, y64)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    log(f"transposed product: first call (builds A^T) {first:.3f}s, then {ms:.3f} ms per call")
+    return {"first_call_seconds": round(first, 3), "ms_per_call": round(ms, 4),
+            "GBs_per_call": round(nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def host_info():

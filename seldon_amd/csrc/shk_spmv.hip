@@ -5,6 +5,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "shk_kernels.hpp"
@@ -171,12 +172,24 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
     Q.chunk.clear();
     Q.word.clear();
     Q.src.clear();
-    Q.word.reserve(nnz + nnz / 8 + 64);
-    Q.src.reserve(nnz + nnz / 8 + 64);
     Q.diag_pos.assign(m, -1);
     Q.slot_row.assign((size_t)Q.npanel * kPanelRows, 0);
-    std::vector<int> cols, cur(kPanelRows), slot_of(kPanelRows), order(kPanelRows);
-    for (int p = 0; p < Q.npanel; p++) {
+    // Panels are planned in parallel, each into its own arrays with positions
+    // counted from the panel's start, then laid end to end.  A panel's arrays
+    // start at a multiple of `align` (the previous panel is padded at its
+    // end), so the padding before each chunk is the same as in one
+    // sequential pass and the result is identical.
+    struct Local {
+        std::vector<int> chunk;  // c0 | flag, c1, e0, e1 (e0, e1 from the panel's start)
+        std::vector<uint32_t> word;
+        std::vector<int64_t> src;
+        std::vector<std::pair<int, int64_t>> diag;  // (row, position from the panel's start)
+        double x_bytes = 0;
+        bool ok = true;
+    };
+    std::vector<Local> L(Q.npanel);
+    auto plan_one = [&](int p, Local &Lp) {
+        std::vector<int> cols, cur(kPanelRows), slot_of(kPanelRows), order(kPanelRows);
         const int r0 = p * kPanelRows, r1 = std::min(m, r0 + kPanelRows);
         // Row slots by length, longest first: the panel kernel gives slot s
         // to one thread, so the panel's long rows land in the same wavefronts
@@ -208,13 +221,16 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
                     if ((int64_t)(k2 - i) > budget) break;
                     k = k2;
                 }
-                if (k == i) return false;  // one column with more entries than a chunk holds
-                const int c1 = cols[k - 1] + 1;
-                while ((int64_t)Q.word.size() % align) {
-                    Q.word.push_back(0u);
-                    Q.src.push_back(-1);
+                if (k == i) {  // one column with more entries than a chunk holds
+                    Lp.ok = false;
+                    return;
                 }
-                const int64_t e0 = (int64_t)Q.word.size();
+                const int c1 = cols[k - 1] + 1;
+                while ((int64_t)Lp.word.size() % align) {
+                    Lp.word.push_back(0u);
+                    Lp.src.push_back(-1);
+                }
+                const int64_t e0 = (int64_t)Lp.word.size();
                 for (int r = r0; r < r1; r++) {
                     int j = cur[r - r0];
                     const int jb = j;
@@ -225,26 +241,63 @@ static bool plan_panels(int m, int n, const HostPlan &P, int budget, int win, in
                                      (P.ind[q] & kLowerFlag);
                         if (q == jb) w |= kPanFirst;
                         if (q == j - 1) w |= kPanLast;
-                        if (P.diag_pos[r] == q) Q.diag_pos[r] = (int64_t)Q.word.size();
-                        Q.word.push_back(w);
-                        Q.src.push_back(q);
+                        if (P.diag_pos[r] == q) Lp.diag.emplace_back(r, (int64_t)Lp.word.size());
+                        Lp.word.push_back(w);
+                        Lp.src.push_back(q);
                     }
                     cur[r - r0] = j;
                 }
-                const int64_t e1 = (int64_t)Q.word.size();
-                if (e1 >= (int64_t)INT32_MAX) return false;
-                Q.chunk.push_back((int)((uint32_t)c0 | (pass == 1 && first_chunk ? kChunkDirect : 0u)));
-                Q.chunk.push_back(c1);
-                Q.chunk.push_back((int)e0);
-                Q.chunk.push_back((int)e1);
-                Q.x_bytes += (double)(c1 - c0) * (double)vsize;
+                const int64_t e1 = (int64_t)Lp.word.size();
+                Lp.chunk.push_back((int)((uint32_t)c0 | (pass == 1 && first_chunk ? kChunkDirect : 0u)));
+                Lp.chunk.push_back(c1);
+                Lp.chunk.push_back((int)std::min<int64_t>(e0, INT32_MAX));
+                Lp.chunk.push_back((int)std::min<int64_t>(e1, INT32_MAX));
+                Lp.x_bytes += (double)(c1 - c0) * (double)vsize;
                 first_chunk = false;
                 i = k;
             }
             for (int r = r0; r < r1; r++)
-                if (cur[r - r0] != part_hi(r, pass)) return false;  // not reached with sorted parts
+                if (cur[r - r0] != part_hi(r, pass)) {  // not reached with sorted parts
+                    Lp.ok = false;
+                    return;
+                }
         }
+    };
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+        th.emplace_back([&, t]() {
+            for (int p = t; p < Q.npanel; p += nth) plan_one(p, L[p]);
+        });
+    for (auto &t : th) t.join();
+    size_t total = 0;
+    for (const Local &Lp : L) {
+        if (!Lp.ok) return false;
+        total += Lp.word.size() + (size_t)align;
+    }
+    Q.word.reserve(total);
+    Q.src.reserve(total);
+    for (int p = 0; p < Q.npanel; p++) {
+        Local &Lp = L[p];
+        while ((int64_t)Q.word.size() % align) {
+            Q.word.push_back(0u);
+            Q.src.push_back(-1);
+        }
+        const int64_t base = (int64_t)Q.word.size();
+        if (base + (int64_t)Lp.word.size() >= (int64_t)INT32_MAX) return false;
+        for (size_t c = 0; c < Lp.chunk.size(); c += 4) {
+            Q.chunk.push_back(Lp.chunk[c]);
+            Q.chunk.push_back(Lp.chunk[c + 1]);
+            Q.chunk.push_back((int)(base + Lp.chunk[c + 2]));
+            Q.chunk.push_back((int)(base + Lp.chunk[c + 3]));
+        }
+        for (const auto &d : Lp.diag) Q.diag_pos[d.first] = base + d.second;
+        Q.word.insert(Q.word.end(), Lp.word.begin(), Lp.word.end());
+        Q.src.insert(Q.src.end(), Lp.src.begin(), Lp.src.end());
+        Q.x_bytes += Lp.x_bytes;
         Q.pchunk[p + 1] = (int)(Q.chunk.size() / 4);
+        Local().word.swap(Lp.word);
+        std::vector<int64_t>().swap(Lp.src);
     }
     Q.nslots = (int64_t)Q.word.size();
     (void)n;
@@ -570,7 +623,9 @@ static void coo_to_csr(int nmajor, int nminor, const std::vector<int> &major, co
     }
 }
 
-// The matrix behind the transposed products, built on first use:
+// The matrix behind the transposed products, built on first use (on the
+// device for the row and sliced layouts, below; the panel layout's entries
+// are read back and sorted on the host):
 // RowSparse / ArrayRowSparse: A^T as an ArrayRowSparse matrix (every term
 // "direct": alpha * a * x added onto beta * y) whose row c lists the entries
 // (i, c) of A in increasing i, the order of Seldon's transposed loops
@@ -578,9 +633,71 @@ static void coo_to_csr(int nmajor, int nminor, const std::vector<int> &major, co
 // RowSymSparse: the full symmetric matrix as a RowSparse one with sorted
 // columns, the order of the generic loop behind Mlt(SeldonTrans, ...)
 // (Functions_MatVect.cxx:1307-1313: temp = sum over j of M(j, i) X(j)).
+// coordinates of the row layout's entries (the transposed halves of a
+// RowSymSparse matrix and ArrayRowSparse terms unflagged)
+__global__ void k_rows_coo(int m, const int *ptr, const uint32_t *ind, int *row, int *col)
+{
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock)
+        for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+            row[j] = i;
+            col[j] = (int)(ind[j] & ~kLowerFlag);
+        }
+}
+
+// The same matrix built on the device from the row layout's arrays (row and
+// sliced layouts): the coordinates are sorted by (major, minor) with the
+// library's stable radix sort (shk_coo_to_csr: ties keep their input order,
+// as the host's two stable counting sorts do), so the result is the host
+// path's entry for entry; only the layout plan is made on the host.
+static int build_trans_dev(shk_csr A)
+{
+    shk_ctx ctx = A->ctx;
+    const int64_t nd = A->nnz_dev;
+    const bool sym = A->storage == SHK_ROWSYMSPARSE;
+    const int nmajor = sym ? A->m : A->n, nminor = sym ? A->n : A->m;
+    const size_t vs = dsize(A->dtype);
+    int *row = nullptr, *col = nullptr, *tptr = nullptr, *tind = nullptr;
+    void *tval = nullptr;
+    auto release = [&]() {
+        hipFree(row);
+        hipFree(col);
+        hipFree(tptr);
+        hipFree(tind);
+        hipFree(tval);
+    };
+    const size_t ne = (size_t)std::max<int64_t>(nd, 1);
+    if (hipMalloc(&row, sizeof(int) * ne) != hipSuccess || hipMalloc(&col, sizeof(int) * ne) != hipSuccess ||
+        hipMalloc(&tptr, sizeof(int) * ((size_t)nmajor + 1)) != hipSuccess ||
+        hipMalloc(&tind, sizeof(int) * ne) != hipSuccess || hipMalloc(&tval, vs * ne) != hipSuccess) {
+        release();
+        SHK_CHECK(false, SHK_ERR_NOMEM, "transpose: device memory");
+    }
+    int st = SHK_OK;
+    if (A->m > 0) {
+        hipLaunchKernelGGL(k_rows_coo, dim3(red_blocks(A->m)), dim3(kBlock), 0, ctx->stream, A->m, (const int *)A->ptr,
+                           (const uint32_t *)A->ind, row, col);
+        if (hipGetLastError() != hipSuccess) st = SHK_ERR_HIP;
+    }
+    int mm = 0, nn = 0;
+    if (st == SHK_OK)
+        st = shk_coo_to_csr(ctx, (shk_dtype)A->dtype, nd, sym ? row : col, sym ? col : row, A->val, 0, nmajor,
+                            nminor, nmajor, &mm, &nn, tptr, tind, tval);
+    if (st == SHK_OK && (mm != nmajor || nn != nminor)) st = SHK_ERR_STATE;  // an entry out of range
+    shk_csr T_ = nullptr;
+    if (st == SHK_OK)
+        st = shk_csr_create_layout(ctx, sym ? SHK_ROWSPARSE : SHK_ARRAYROWSPARSE, (shk_dtype)A->dtype, nmajor, nminor,
+                                   nd, tptr, tind, tval, 1, SHK_LAYOUT_AUTO, &T_);
+    release();
+    SHK_CHECK(st != SHK_ERR_STATE, SHK_ERR_STATE, "transpose: entry out of range");
+    SHK_TRY(st);
+    A->trans = T_;
+    return SHK_OK;
+}
+
 template <typename T>
 static int build_trans(shk_csr A)
 {
+    if (A->layout != SHK_LAYOUT_PANELS) return build_trans_dev(A);
     std::vector<int> row, col;
     std::vector<T> val;
     SHK_TRY(coo_download<T>(A, row, col, val));
