@@ -183,20 +183,35 @@ __global__ void k_sell_width(int n, const int *ptr, int *width)
     width[b] = w;
 }
 
-// entries (and, when given, the FP32 copy's values) into the slices
+// entries (and, when given, the FP32 copy's values) into the slices; every
+// column must lie in its row's own system (a column in another system would
+// read that system's p while it is being written): *bad is set otherwise
 template <typename T>
 __global__ void k_sell_fill(int n, const int *ptr, const uint32_t *ind, const T *val, const float *val32,
-                            const int *boff, uint32_t *col, T *sval, float *sval32)
+                            const int *boff, const int *row_off, int count, uint32_t *col, T *sval,
+                            float *sval32, int *bad)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // the system of row i: the last s with row_off[s] <= i
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t c0 = (uint32_t)row_off[lo], c1 = (uint32_t)row_off[lo + 1];
     const int j0 = ptr[i], len = ptr[i + 1] - j0;
     const int q = boff[i >> 6] + (i & 63);
+    bool out = false;
     for (int k = 0; k < len; k++) {
-        col[q + 64 * k] = ind[j0 + k];
+        const uint32_t c = ind[j0 + k];
+        out |= c < c0 || c >= c1;
+        col[q + 64 * k] = c;
         sval[q + 64 * k] = val[j0 + k];
         if (val32) sval32[q + 64 * k] = val32[j0 + k];
     }
+    if (out) atomicOr(bad, 1);
 }
 
 // One Cg (identity): Seldon's recurrences, this work-group's rows [r0, r1)
@@ -632,13 +647,20 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
     double *d_tslot = (double *)w;
     w += al(tslot_bytes);
     unsigned *d_stall = (unsigned *)w;
+    int *d_bad = (int *)ctx->scalars;  // one int
     SHK_HIP(hipMemcpyAsync(d_boff, boff.data(), sizeof(int) * nsl, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_sell_fill<double>, dim3((n + 255) / 256), dim3(256), 0, st, n, (const int *)A64->ptr,
-                       (const uint32_t *)A64->ind, (const double *)A64->val,
-                       mixed ? (const float *)A32->val : (const float *)nullptr, (const int *)d_boff, d_scol,
-                       d_sval, mixed ? d_sval32 : (float *)nullptr);
-    SHK_HIP(hipGetLastError());
     SHK_HIP(hipMemcpyAsync(d_off, row_off_host, off_bytes, hipMemcpyHostToDevice, st));
+    SHK_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+    if (n > 0)
+        hipLaunchKernelGGL(k_sell_fill<double>, dim3((n + 255) / 256), dim3(256), 0, st, n, (const int *)A64->ptr,
+                           (const uint32_t *)A64->ind, (const double *)A64->val,
+                           mixed ? (const float *)A32->val : (const float *)nullptr, (const int *)d_boff,
+                           (const int *)d_off, count, d_scol, d_sval, mixed ? d_sval32 : (float *)nullptr, d_bad);
+    SHK_HIP(hipGetLastError());
+    int bad = 0;
+    SHK_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    SHK_HIP(hipStreamSynchronize(st));
+    SHK_CHECK(bad == 0, SHK_ERR_ARG, "cg_batched: a system has a column outside its own rows");
     SHK_HIP(hipMemcpyAsync(d_slots, slots.data(), slot_bytes, hipMemcpyHostToDevice, st));
     SHK_HIP(hipMemcpyAsync(d_team, team.data(), ord_bytes, hipMemcpyHostToDevice, st));
     int clock_khz = 0;

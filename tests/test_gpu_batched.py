@@ -119,3 +119,22 @@ def test_batched_rows_per_pass_results_identical(gpu):
             ref = got
         for a, r in zip(got, ref):
             assert np.array_equal(a, r), u
+
+
+def test_batched_rejects_a_column_in_another_system(gpu):
+    """A stacked matrix whose system 2 has an entry in system 5's columns is
+    refused (it would read another system's p while that one writes it);
+    nothing is solved and x is left as it was."""
+    from seldon_amd import _lib as L
+    from seldon_amd import krylov as K
+    from seldon_amd.batched import cg_batched
+    A, row_off, b, systems = _small_batch()
+    ind = A.ind.copy()
+    ind[A.ptr[row_off[2]]] = row_off[5]  # first entry of system 2's first row
+    B = wl.CSR(A.m, A.n, A.ptr, ind, A.val)
+    Bg = K.Matrix(B, layout="rows")
+    bg = torch.tensor(b, device="cuda")
+    x = torch.full_like(bg, 3.0)
+    with pytest.raises(L.ShkError, match="outside its own rows"):
+        cg_batched(Bg, row_off, x, bg, group=256)
+    assert torch.all(x == 3.0)
