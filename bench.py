@@ -419,9 +419,12 @@ def batched_cg(K, np, torch, cpu=True):
     """512 anisotropic 2-D Laplacians (n x n, n in [32, 256], anisotropy
     log-uniform in [1, 100]) solved to 1e-8 by the persistent batched Cg,
     FP64 and FP32 inner + FP64 refinement, each under hipGraph with the
-    autotuned group size.  Bytes per Cg iteration of system s (FP64):
+    autotuned work-group size and rows per pass (and, mixed, inner
+    tolerance).  Bytes per Cg iteration of system s (FP64):
     (4+8) nnz_s + 4 (n_s+1) for the matrix and 11 n_s x 8 for the vectors
-    (p = r + beta p: 3; q = A p: 2; x += alpha p, r -= alpha q, |r|^2: 6)."""
+    (p = r + beta p: 3; q = A p: 2; x += alpha p, r -= alpha q, |r|^2: 6);
+    the mixed variant's figure counts its FP32 inner iterations only (its
+    FP64 residuals at each refinement are not in it)."""
     from seldon_amd import batched as B
     A, row_off, b, xt, sides, eps = B.aniso_batch()
     A64 = K.Matrix(A, dtype=torch.float64, layout="rows")
@@ -440,7 +443,7 @@ def batched_cg(K, np, torch, cpu=True):
                                       rows_per_pass=rpp)
         else:
             its, codes = B.cg_batched_ir(A64, A32, row_off, x, bg, tol=1e-8, group=group, use_graph=True,
-                                         rows_per_pass=rpp)
+                                         rows_per_pass=rpp, inner_tol=entry["inner_tol"])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         V = 8.0 if variant == "fp64" else 4.0
@@ -454,6 +457,10 @@ def batched_cg(K, np, torch, cpu=True):
                         "codes_nonzero": int((codes != 0).sum()),
                         ("GBs" if variant == "fp64" else "inner_GBs"): round(gbs, 2),
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "rel_error_vs_xtrue": err}
+        if variant == "ir":
+            out[variant]["inner_tol"] = entry["inner_tol"]
+            out[variant]["tuned_inner_tol_seconds"] = entry["inner_tol_seconds"]
+            out[variant]["speedup_vs_fp64"] = round(out["fp64"]["seconds"] / dt, 3)
         log(f"batched {variant}: {dt:.3f}s group {group}, {int(its.sum())} its, {gbs:.0f} GB/s, "
             f"err {err:.2e}, nonzero codes {int((codes != 0).sum())}")
     if cpu:

@@ -10,9 +10,10 @@ The stacked matrices must use the row layout: ``K.Matrix(A, layout="rows")``.
 
 ``autotune`` picks the work-group size ("group": threads per system) and the
 rows each thread keeps in flight per pass ("rows_per_pass") by timing every
-pair on the batch itself, and caches the choice per batch signature (variant,
-number of systems, rows, stored entries) in ``autotune.json`` next to this
-file.
+pair on the batch itself -- and for the mixed variant then the inner solves'
+tolerance ("inner_tol": a looser one means more refinements of shorter FP32
+solves) -- and caches the choice per batch signature (variant, number of
+systems, rows, stored entries) in ``autotune.json`` next to this file.
 """
 from __future__ import annotations
 
@@ -31,6 +32,7 @@ from . import workloads as wl
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 CANDIDATES = (256, 512, 1024)   # threads per system work-group
 ROWS_PER_PASS = (1, 2, 4, 8)     # rows a thread keeps in flight per pass
+INNER_TOLS = (1e-3, 1e-4, 1e-5)  # mixed variant: each FP32 inner solve's residual reduction
 
 
 def stack(systems):
@@ -99,8 +101,9 @@ def _signature(variant, A, row_off):
 def autotune(A64, row_off, b, variant="fp64", A32=None, tol=1e-8, reps=2, use_cache=True):
     """Fastest (group, rows_per_pass) for this batch: every pair of the
     candidate work-group sizes and rows kept in flight per thread is timed
-    on the batch itself (best of `reps`), and the choice is cached by the
-    batch's signature."""
+    on the batch itself (best of `reps`); for the mixed variant the inner
+    tolerance is then timed with that pair (entry["inner_tol"]).  The choice
+    is cached by the batch's signature."""
     key = _signature(variant, A64, row_off)
     cache = {}
     if use_cache and os.path.exists(CACHE):
@@ -128,6 +131,21 @@ def autotune(A64, row_off, b, variant="fp64", A32=None, tol=1e-8, reps=2, use_ca
     choice = min(times, key=times.get)
     entry = {"group": choice[0], "rows_per_pass": choice[1],
              "seconds": {f"{g}x{u}": v for (g, u), v in times.items()}}
+    if variant == "ir":
+        itimes = {}
+        for itol in INNER_TOLS:
+            best = None
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _run(A64, row_off, x, b, tol, 100000, choice[0], True, A32=A32, inner_tol=itol,
+                     rows_per_pass=choice[1])
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            itimes[itol] = best
+        entry["inner_tol"] = min(itimes, key=itimes.get)
+        entry["inner_tol_seconds"] = {f"{t:g}": v for t, v in itimes.items()}
     if use_cache:
         cache[key] = entry
         try:
