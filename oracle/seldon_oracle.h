@@ -5,9 +5,17 @@
  * RowSparse / RowSymSparse / ArrayRowSparse storages and its transposed
  * forms, the BLAS-1 steps, the Iteration stopping logic, Cg, BiCgStab and
  * Gmres, SOR / SSOR, ConvertMatrix_from_Coordinates), used as the checker
- * for the HIP library.  orc_s_gmres (FP32 vectors, FP64 Hessenberg) has no
- * Seldon counterpart: parity unpinned.  Only the tests, the smoke check and the benchmark's CPU
- * baseline leg may load it; the product never does.
+ * for the HIP library.  The FP32 family (orc_s_*; Gmres with FP32 vectors and
+ * an FP64 Hessenberg) has no Seldon build to compare with; it is pinned by
+ * what the mathematics fixes instead (tests/test_oracle.py): A = I converges
+ * in one step (Gmres through the happy breakdown), a diagonal matrix with k
+ * distinct values in k, the Arnoldi relation M^-1 A V = V Hbar and V's
+ * orthonormality hold to FP32 rounding (with planted slips -- a dropped
+ * term, a transposed Hessenberg, the preconditioner left out -- shown to
+ * fail that check), the FP32 solutions lie within kappa(A) eps_32 of the
+ * FP64 oracle's, and the arithmetic is float, not rounded double.  Only the
+ * tests, the smoke check and the benchmark's CPU baseline leg may load it;
+ * the product never does.
  *
  * Every function follows the reference file:line named in its comment, in
  * the same arithmetic order, and is built with -ffp-contract=off so that
