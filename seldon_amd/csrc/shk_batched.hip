@@ -673,7 +673,9 @@ static int batched_common(shk_ctx ctx, shk_csr A64, shk_csr A32, int count, cons
     a.team_cnt = d_tcnt;
     a.team_slot = d_tslot;
     a.stall = d_stall;
-    a.limit = (long long)std::max(clock_khz, 1) * 1000LL * kTeamWaitSeconds;
+    // wall_clock64 ticks per second (the device reports kHz; 100 MHz when it
+    // reports none)
+    a.limit = (long long)(clock_khz > 0 ? clock_khz : 100000) * 1000LL * kTeamWaitSeconds;
     a.row_off = d_off;
     a.A64 = SellView{(const int *)A64->ptr, d_boff, d_scol, d_sval};
     if (mixed) a.A32 = SellView{(const int *)A64->ptr, d_boff, d_scol, d_sval32};
