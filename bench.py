@@ -66,7 +66,7 @@ def main():
                     help="profiling run: one 1 GiB copy kernel (counter calibration) and 10 FP64 + 10 FP32 SpMVs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only: the sparse Krylov path is "
+        print(json.dumps({"error": "this library runs on one GPU only: the sparse Krylov path is "
                                    "measured on a single MI355X; multi-GPU sharding is design-only"}))
         sys.exit(2)
 

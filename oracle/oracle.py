@@ -7,7 +7,8 @@ DotProd / Norm2 / Add / Mlt / Copy, Iteration, Cg, BiCgStab and Gmres, SOR,
 and ConvertMatrix_from_Coordinates (file:line citations are in
 ``seldon_oracle_body.h``); plus, in Python below, the Harwell-Boeing writer
 and reader.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` legs of ``bench.py`` may use it, and only as the checker.
+``cpu_baseline`` legs of ``bench.py`` may use it, and only as the reference
+the GPU results are compared with.
 
 Parity pins (see ``tests/test_oracle.py``):
   * bitwise equality with drivers compiled from the real Seldon headers

@@ -4,8 +4,8 @@
  * A plain-C CPU restatement of Seldon's sparse Krylov path (SpMV on the
  * RowSparse / RowSymSparse / ArrayRowSparse storages and its transposed
  * forms, the BLAS-1 steps, the Iteration stopping logic, Cg, BiCgStab and
- * Gmres, SOR / SSOR, ConvertMatrix_from_Coordinates), used as the checker
- * for the HIP library.  The FP32 family (orc_s_*; Gmres with FP32 vectors and
+ * Gmres, SOR / SSOR, ConvertMatrix_from_Coordinates): the oracle the HIP
+ * library's results are compared with.  The FP32 family (orc_s_*; Gmres with FP32 vectors and
  * an FP64 Hessenberg) has no Seldon build to compare with; it is pinned by
  * what the mathematics fixes instead (tests/test_oracle.py): A = I converges
  * in one step (Gmres through the happy breakdown), a diagonal matrix with k

@@ -179,7 +179,7 @@ static int run_gmres(Mat& A, Vector<T>& x, const Vector<T>& b, Prec& M, Iteratio
 #ifdef SELDON_WITH_BLAS
   return Gmres(A, x, b, M, it);
 #else
-  cerr << "method not available in this build: gmres" << endl;
+  cerr << "method not available without SELDON_WITH_BLAS: gmres" << endl;
   exit(3);
 #endif
 }
@@ -194,7 +194,7 @@ static int solve_with(const string& method, Mat& A, Vector<T>& x,
     return BiCgStab(A, x, b, M, it);
   if (method == "gmres")
     return run_gmres(A, x, b, M, it, (T*)0, (Titer*)0);
-  cerr << "method not available in this build: " << method << endl;
+  cerr << "method not available: " << method << endl;
   exit(3);
 }
 

@@ -91,8 +91,8 @@ def build_cpp_programs():
 
 
 def build_oracle():
-    """Test-only checkers (oracle/liboracle.so and, where the reference tree is
-    present, oracle/_ref)."""
+    """The test-only oracle (oracle/liboracle.so and, where the reference tree
+    is present, the reference drivers in oracle/_ref)."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "all"])
 
