@@ -64,7 +64,8 @@ def test_batched_graph_equals_stream_and_max_iter(gpu):
     assert np.all(c4[live] == -2) and np.all(i4[live] == 5)
 
 
-def test_batched_ir_reaches_fp64_tolerance(gpu):
+@pytest.mark.parametrize("inner_tol", [1e-3, 1e-4])
+def test_batched_ir_reaches_fp64_tolerance(gpu, inner_tol):
     from seldon_amd import krylov as K
     from seldon_amd.batched import cg_batched, cg_batched_ir
     A, row_off, b, systems = _small_batch()
@@ -72,7 +73,7 @@ def test_batched_ir_reaches_fp64_tolerance(gpu):
     A32 = K.Matrix(A, dtype=torch.float32, layout="rows")
     bg = torch.tensor(b, device="cuda")
     x = torch.zeros_like(bg)
-    its, codes = cg_batched_ir(A64, A32, row_off, x, bg, tol=1e-10, inner_tol=1e-4, group=256)
+    its, codes = cg_batched_ir(A64, A32, row_off, x, bg, tol=1e-10, inner_tol=inner_tol, group=256)
     assert np.all(codes == 0), codes
     xref = torch.zeros_like(bg)
     cg_batched(A64, row_off, xref, bg, tol=1e-12, group=256)
@@ -138,3 +139,36 @@ def test_batched_rejects_a_column_in_another_system(gpu):
     with pytest.raises(L.ShkError, match="outside its own rows"):
         cg_batched(Bg, row_off, x, bg, group=256)
     assert torch.all(x == 3.0)
+
+
+def test_batched_one_large_system_on_a_team_of_sixteen(gpu):
+    """A batch of one 200 x 200 anisotropic Laplacian (40,000 rows): its share
+    of the work is all of it, so 16 work-groups solve it together (members
+    meeting in device memory, the dots added in member order) -- same
+    iterations (within 1) and solution as the oracle's Cg, and the same
+    result bit for bit on a second run (the team's sums have a fixed order)."""
+    from oracle import oracle as orc
+    from seldon_amd import krylov as K
+    from seldon_amd.batched import cg_batched, cg_batched_ir, stack
+    S = wl.aniso2d(200, 200, 30.0)
+    A, row_off = stack([S])
+    xt = wl.uniform(A.m, 17)
+    b = wl.apply(A, xt)
+    Ag = K.Matrix(A, layout="rows")
+    A32 = K.Matrix(A, dtype=torch.float32, layout="rows")
+    bg = torch.tensor(b, device="cuda")
+    r = orc.solve("cg", S, b, tol=1e-8, max_iter=5000)
+    xs = []
+    for _ in range(2):
+        x = torch.zeros_like(bg)
+        its, codes = cg_batched(Ag, row_off, x, bg, tol=1e-8, max_iter=5000, group=256)
+        assert codes[0] == r.code == 0
+        assert abs(int(its[0]) - r.nb_iter) <= 1, (its[0], r.nb_iter)
+        xs.append(x.cpu().numpy())
+    assert np.array_equal(xs[0], xs[1])
+    assert np.linalg.norm(xs[0] - r.x) <= 1e-6 * np.linalg.norm(r.x)
+    x = torch.zeros_like(bg)
+    its, codes = cg_batched_ir(Ag, A32, row_off, x, bg, tol=1e-10, inner_tol=1e-3, group=256)
+    assert codes[0] == 0
+    xh = x.cpu().numpy()
+    assert np.linalg.norm(b - orc.mlt(S, xh)) <= 1e-10 * np.linalg.norm(b)
