@@ -647,19 +647,20 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, const T *__restric
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
-// The same step with 16-byte accesses: thread t takes element pairs 2t, 2t+1
-// of a grid stride (n even, vectors 16-byte aligned; the host checks).  Each
-// element gets the same arithmetic; a thread's partial adds its pairs' two
-// products in element order.
+// The same step with 16-byte accesses: thread t takes the N = 16 / sizeof(T)
+// consecutive elements N t .. N t + N - 1 of a grid stride (n a multiple of
+// N, vectors 16-byte aligned; the host checks).  Each element gets the same
+// arithmetic; a thread's partial adds its group's N products in element
+// order.
 template <typename T>
-struct Pair;
+struct Vec16;
 template <>
-struct Pair<double> {
+struct Vec16<double> {
     using type = double2;
 };
 template <>
-struct Pair<float> {
-    using type = float2;
+struct Vec16<float> {
+    using type = float4;
 };
 
 template <typename T>
@@ -667,50 +668,47 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs2(int64_t n, const T *__restri
                                                     T *__restrict__ w, int hpos, KState<T> *st, const T *prev,
                                                     int prev_nb, T *partials)
 {
-    using P2 = typename Pair<T>::type;
+    using V = typename Vec16<T>::type;
+    constexpr int N = 16 / (int)sizeof(T);
     if (st->skip) return;
-    const int64_t n2 = n / 2;
+    const int64_t nv = n / N;
     const int64_t j0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
-    const P2 *vp2 = (const P2 *)vprev, *vk2 = (const P2 *)vk;
-    P2 *w2 = (P2 *)w;
+    const V *vpv = (const V *)vprev, *vkv = (const V *)vk;
+    V *wv = (V *)w;
     constexpr int U = kPre / 2;
-    P2 wp[U], vp[U], kp[U];
+    V wp[U], vp[U], kp[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int64_t j = j0 + u * stride;
-        if (j < n2) {
-            wp[u] = w2[j];
-            vp[u] = vp2[j];
-            kp[u] = vk2[j];
+        if (j < nv) {
+            wp[u] = wv[j];
+            vp[u] = vpv[j];
+            kp[u] = vkv[j];
         }
     }
     const T a = gm_fold_h<T>(prev, prev_nb, hpos, st);
     const bool aa = a != (T)0;
     T c[2] = {(T)0, (T)0};
+    auto step = [&](V &x, const V &v, const V &k, int64_t j) __attribute__((always_inline)) {
+        T *xe = (T *)&x;
+        const T *ve = (const T *)&v, *ke = (const T *)&k;
+        if (aa) {
+#pragma unroll
+            for (int e = 0; e < N; e++) xe[e] += a * ve[e];
+            wv[j] = x;
+        }
+#pragma unroll
+        for (int e = 0; e < N; e++) c[0] += ke[e] * xe[e];
+    };
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int64_t j = j0 + u * stride;
-        if (j < n2) {
-            P2 wv = wp[u];
-            if (aa) {
-                wv.x += a * vp[u].x;
-                wv.y += a * vp[u].y;
-                w2[j] = wv;
-            }
-            c[0] += kp[u].x * wv.x;
-            c[0] += kp[u].y * wv.y;
-        }
+        if (j < nv) step(wp[u], vp[u], kp[u], j);
     }
-    for (int64_t j = j0 + U * stride; j < n2; j += stride) {
-        P2 wv = w2[j];
-        const P2 v = vp2[j], k = vk2[j];
-        if (aa) {
-            wv.x += a * v.x;
-            wv.y += a * v.y;
-            w2[j] = wv;
-        }
-        c[0] += k.x * wv.x;
-        c[0] += k.y * wv.y;
+    for (int64_t j = j0 + U * stride; j < nv; j += stride) {
+        V x = wv[j];
+        const V v = vpv[j], k = vkv[j];
+        step(x, v, k, j);
     }
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
@@ -1148,7 +1146,7 @@ int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
                 int prev_nb = a.grid_used;
                 for (int k = 1; k <= i; k++) {
                     // 16-byte accesses where the vectors allow them
-                    const bool pairs = n % 2 == 0 &&
+                    const bool pairs = n % (16 / (int64_t)sizeof(T)) == 0 &&
                                        (((uintptr_t)Vk(k - 1) | (uintptr_t)Vk(k) | (uintptr_t)w) & 15) == 0;
                     if (pairs)
                         hipLaunchKernelGGL(k_gm_mgs2<T>, g, blk, 0, st, n, (const T *)Vk(k - 1), (const T *)Vk(k), w,
