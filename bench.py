@@ -348,9 +348,9 @@ def cg_config1(K, wl, np, torch, cpu=True):
 
 
 def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
-    """FP64 BiCgStab and Gmres(30) with the Jacobi preconditioner on the
-    128^3 7-point upwind convection-diffusion matrix (2,097,152 rows), to a
-    1e-10 relative residual: time to solution and iterations/s, plus the
+    """FP64 BiCgStab, Gmres(30), Cgs and BiCg with the Jacobi preconditioner
+    on the 128^3 7-point upwind convection-diffusion matrix (2,097,152 rows),
+    to a 1e-10 relative residual: time to solution and iterations/s, plus the
     SpMV bandwidth on this matrix and the time of one symmetric SOR sweep
     (with the oracle's sequential sweep on one core beside it)."""
     import ctypes
@@ -382,7 +382,16 @@ def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
     ct = time.perf_counter() - t0
     out["ssor_sweep"] = {"seconds": round(st, 5), "cpu_oracle_seconds": round(ct, 4), "cores": 1}
     log(f"convdiff128 SSOR sweep: {st * 1e3:.2f} ms (oracle, 1 core: {ct * 1e3:.1f} ms)")
-    for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30)):
+    # BiCg's A^T (built on the device on first use) is made before its timed
+    # solve; the first call's time is recorded beside it
+    t0 = time.perf_counter()
+    it = K.Iteration(2, 1e-10)
+    it.HideMessages()
+    K.BiCg(Ad, torch.zeros_like(bg), bg, K.JacobiPreconditioner(), it)
+    torch.cuda.synchronize()
+    bicg_first = time.perf_counter() - t0
+    for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30), ("cgs", K.Cgs, None),
+                              ("bicg", K.BiCg, None)):
         x = torch.zeros_like(bg)
         it = K.Iteration(20000, 1e-10)
         if restart:
@@ -396,14 +405,22 @@ def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
         rel = _true_relres(K, Ad, x, bg, bn)
         out[name] = {"seconds": round(dt, 4), "iterations": it.nb_iter, "code": code,
                      "iterations_per_s": round(it.nb_iter / dt, 1), "true_relres": rel}
+        if name == "bicg":
+            out[name]["first_call_seconds_with_transpose_build"] = round(bicg_first, 4)
         log(f"convdiff128 {name}: {dt:.3f}s, {it.nb_iter} its, code {code}, true relres {rel:.2e}")
-        if cpu:
+        if cpu and name == "bicg":
+            # Seldon's BiCg takes A^T p~ through the generic transposed
+            # product (Functions_MatVect.cxx:1279-1316), a dense loop over
+            # every (row, column) pair: 4.4e12 terms per product here
+            out[name]["cpu_oracle"] = {"skipped": "Mlt(SeldonTrans) on RowSparse is the generic dense loop, "
+                                                  "O(rows x columns) per product"}
+        elif cpu:
             # the oracle's solver on one core, a bounded sample of its
             # iterations (a full solve takes minutes); time to solution is
             # projected from its iteration rate and the GPU's count
             cap = 40
             t0 = time.perf_counter()
-            r = orc.solve("bicgstab" if name == "bicgstab" else "gmres", A, b, precond="jacobi", tol=1e-10,
+            r = orc.solve("gmres" if name == "gmres30" else name, A, b, precond="jacobi", tol=1e-10,
                           max_iter=cap, restart=restart or 10)
             ct = time.perf_counter() - t0
             ips = r.nb_iter / ct

@@ -15,15 +15,16 @@
 // are what Seldon's solver templates call unqualified
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
-// more specialised Cg / BiCgStab / Gmres below replace the whole loop by the
+// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
 // JacobiPreconditioner.  Every call goes through the C ABI of include/shk.h.
 //
 // The transposed products (MltAdd / Mlt with SeldonTrans) and Swap are
-// overloaded too, so the transpose-using solvers (BiCg, Qmr, Cgne, Lsqr)
-// also run on the GPU storages through Seldon's own templates.
+// overloaded too, so the transpose-using solvers (Qmr, Cgne, Lsqr, and BiCg
+// with any other preconditioner) also run on the GPU storages through
+// Seldon's own templates.
 //
 // Names: VectHip, RowSparseHip, RowSymSparseHip, ArrayRowSparseHip are the
 // device counterparts of Seldon's VectFull, RowSparse, RowSymSparse and
@@ -647,7 +648,9 @@ namespace Seldon
     shk_ctx ctx = HipContext::Get().Handle();
     if (which == 0) HipCheck(shk_cg(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cg");
     else if (which == 1) HipCheck(shk_bicgstab(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCgStab");
-    else HipCheck(shk_gmres(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gmres");
+    else if (which == 2) HipCheck(shk_gmres(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gmres");
+    else if (which == 3) HipCheck(shk_cgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cgs");
+    else HipCheck(shk_bicg(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCg");
     iter.*Acc::Fac() = Titer(it.facteur_reste);
     const int n = std::min(it.hist_len, it.hist_cap);
     for (int k = 0; k < n; k++)
@@ -659,10 +662,12 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[3][4] = {{"", "Cg breakdown #1", "Cg breakdown #2", ""},
+        static const char* what[5][4] = {{"", "Cg breakdown #1", "Cg breakdown #2", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
                                           "Bicgstab breakdown #3"},
-                                         {"", "", "", ""}};
+                                         {"", "", "", ""},
+                                         {"", "Cgs breakdown #1", "Cgs breakdown #2", ""},
+                                         {"", "Bicg breakdown #1", "Bicg breakdown #2", ""}};
         const int c = it.error_code;
         iter.Fail(c, (c >= 0 && c < 4) ? what[which][c] : "breakdown");
       }
@@ -689,6 +694,25 @@ namespace Seldon
            Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(1, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  // Cgs.cxx:44-46 and BiCg.cxx:45-47: the same specialisation for the two
+  // Lanczos-type solvers the library fuses (BiCg's A^T products on the
+  // library's transpose, built on first use)
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  Cgs(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+      Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(3, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  BiCg(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+       Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(4, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>

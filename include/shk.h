@@ -290,6 +290,18 @@ int shk_bicgstab(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond p
  * solve stay in FP64 on the device for both value types. */
 int shk_gmres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
               shk_iteration *iter, int *code);
+/* Cgs (computation/solver/iterative/Cgs.cxx:46-137): the loop fused on the
+ * device like BiCgStab's (scalars and Finished() in device state, the host
+ * reads the state once per 16 loops); Seldon's vector updates element for
+ * element.  Error codes 1 / 2 are its breakdowns #1 / #2. */
+int shk_cgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+            shk_iteration *iter, int *code);
+/* BiCg (computation/solver/iterative/BiCg.cxx:47-131), fused likewise; its
+ * A^T p~ products run on the transpose the library builds on first use (the
+ * one behind shk_spmv_trans, Seldon's summation order).  With the Jacobi
+ * preconditioner, z~ = r~ / a_ii as z = r / a_ii. */
+int shk_bicg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+             shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

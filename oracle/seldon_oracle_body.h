@@ -519,6 +519,111 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- Cgs: computation/solver/iterative/Cgs.cxx:46-137 -------------- */
+int FN(cgs)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+            orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho_1, rho_2 = 0, alpha, beta, delta;
+    T *p = FN(dup)(N, b), *phat = FN(dup)(N, b), *q = FN(dup)(N, b),
+      *qhat = FN(dup)(N, b), *vhat = FN(dup)(N, b), *u = FN(dup)(N, b),
+      *uhat = FN(dup)(N, b), *r = FN(dup)(N, b), *rtilde = FN(dup)(N, b);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(copy)(N, r, rtilde);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        rho_1 = FN(dot)(N, rtilde, r);
+        if (rho_1 == (T)0) { orc_fail(it, 1); break; }
+        if (it->nb_iter == 0) {
+            FN(copy)(N, r, u);
+            FN(copy)(N, u, p);
+        } else {
+            /* u = r + beta q; p = beta (beta p + q) + u */
+            beta = rho_1 / rho_2;
+            FN(copy)(N, r, u);
+            FN(add)(N, beta, q, u);
+            FN(scal)(N, beta, p);
+            FN(add)(N, (T)1, q, p);
+            FN(scal)(N, beta, p);
+            FN(add)(N, (T)1, u, p);
+        }
+        FN(precond)(precond, N, diag, p, phat);
+        FN(mlt)(A, phat, vhat);
+        it->nb_iter++;
+        delta = FN(dot)(N, rtilde, vhat);
+        if (delta == (T)0) { orc_fail(it, 2); break; }
+        alpha = rho_1 / delta;
+        FN(copy)(N, u, q);
+        FN(add)(N, -alpha, vhat, q);
+        FN(add)(N, (T)1, q, u);
+        FN(precond)(precond, N, diag, u, uhat);
+        FN(add)(N, alpha, uhat, x);
+        FN(mlt)(A, uhat, qhat);
+        FN(add)(N, -alpha, qhat, r);
+        rho_2 = rho_1;
+        it->nb_iter++;
+    }
+done:
+    free(p); free(phat); free(q); free(qhat); free(vhat); free(u); free(uhat);
+    free(r); free(rtilde);
+    return orc_error_code(it);
+}
+
+/* ---- BiCg: computation/solver/iterative/BiCg.cxx:47-131 ------------ *
+ * M.TransSolve of the identity is Copy, of the Jacobi preconditioner the
+ * same division by a_ii (A^T has A's diagonal). */
+int FN(bicg)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+             orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho_1, rho_2 = 0, alpha, beta, delta;
+    T *r = FN(dup)(N, b), *z = FN(dup)(N, b), *p = FN(dup)(N, b), *q = FN(dup)(N, b),
+      *r_tilde = FN(dup)(N, b), *z_tilde = FN(dup)(N, b), *p_tilde = FN(dup)(N, b),
+      *q_tilde = FN(dup)(N, b);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(copy)(N, r, r_tilde);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        FN(precond)(precond, N, diag, r, z);
+        FN(precond)(precond, N, diag, r_tilde, z_tilde);
+        rho_1 = FN(dot)(N, z, r_tilde);
+        if (rho_1 == (T)0) { orc_fail(it, 1); break; }
+        if (it->nb_iter == 0) {
+            FN(copy)(N, z, p);
+            FN(copy)(N, z_tilde, p_tilde);
+        } else {
+            beta = rho_1 / rho_2;
+            FN(scal)(N, beta, p);
+            FN(add)(N, (T)1, z, p);
+            FN(scal)(N, beta, p_tilde);
+            FN(add)(N, (T)1, z_tilde, p_tilde);
+        }
+        FN(mlt)(A, p, q);
+        it->nb_iter++;
+        FN(mlt_trans)(A, p_tilde, q_tilde);
+        delta = FN(dot)(N, p_tilde, q);
+        if (delta == (T)0) { orc_fail(it, 2); break; }
+        alpha = rho_1 / delta;
+        FN(add)(N, alpha, p, x);
+        FN(add)(N, -alpha, q, r);
+        FN(add)(N, -alpha, q_tilde, r_tilde);
+        rho_2 = rho_1;
+        it->nb_iter++;
+    }
+done:
+    free(r); free(z); free(p); free(q); free(r_tilde); free(z_tilde); free(p_tilde);
+    free(q_tilde);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

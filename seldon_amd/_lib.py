@@ -90,6 +90,8 @@ SIGNATURES = [
     ("shk_cg", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_cg_noalloc", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_bicgstab", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_cgs", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_bicg", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_gmres", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_solve_ir", _I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ShkIrParams), _PI]),
     ("shk_sddmm", _I, [_P, _P, _P, _P, _P]),

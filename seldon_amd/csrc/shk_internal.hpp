@@ -265,6 +265,8 @@ int noalloc_scratch_init(int device);
 // free it
 int sliced_build(shk_csr A, const int *ptr, const int *ind, bool *applies);
 int sliced_set_values(shk_csr A);
+// the matrix behind A's transposed products (built on first use)
+int transpose_of(shk_csr A, shk_csr *out);
 void sliced_free(shk_csr A);
 // AUTO picks the sliced layout for RowSparse matrices with at least this
 // many rows (256 panels of 8192 rows)

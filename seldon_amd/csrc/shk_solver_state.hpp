@@ -214,6 +214,51 @@ struct FinBiOmega {  // omega = (t, s) / (t, t) (:129)
     __device__ void operator()(const T (&s)[2]) const { st->omega = s[0] / s[1]; }
 };
 
+// -------------------------------------------------------- Cgs / BiCg ----
+// Loop head shared by Cgs (Cgs.cxx:71-79) and BiCg (BiCg.cxx:72-86):
+// [rho_2 = rho_1; ++iter;] Finished(r); rho_1 = s[1] ((rtilde, r) for Cgs,
+// (z, rtilde) for BiCg); breakdown #1; beta = rho_1 / rho_2 after the
+// first iteration.
+template <typename T>
+struct FinLanczosTop {
+    KState<T> *st;
+    int inc;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (inc) {
+            st->rho_2 = st->rho_1;
+            st->nb_iter++;
+        }
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        st->rho_1 = s[1];
+        if (s[1] == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        if (st->nb_iter != 0) st->beta = s[1] / st->rho_2;
+    }
+};
+
+// After the first product of a loop: ++iter; delta; breakdown #2; alpha =
+// rho_1 / delta (Cgs.cxx:106-117, BiCg.cxx:104-115).
+template <typename T>
+struct FinLanczosDelta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->nb_iter++;
+        st->delta = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 2);
+            return;
+        }
+        st->alpha = st->rho_1 / s[0];
+    }
+};
+
 // ------------------------------------------------------------- Gmres ----
 // GenRot "old BLAS version" (Functions_Vector.cxx:612-649) and ApplyRot
 // (:681-687), in double.

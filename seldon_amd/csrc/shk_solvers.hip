@@ -559,6 +559,210 @@ __global__ __launch_bounds__(kBlock) void k_bi_xr(int64_t n, const T *__restrict
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// --------------------------------------------------------------- Cgs ----
+// [x = 0, r = b]; rtilde = r; sums |r|^2 and (rtilde, r) for the first
+// loop head (Cgs.cxx:63-79).  BiCg starts the same way, its head's sum
+// being (z, rtilde) with z = M^{-1} r, z~ = M^{-T} r~ (BiCg.cxx:64-80; the
+// Jacobi preconditioner divides by a_ii both ways).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lz_init(int64_t n, const T *__restrict__ b, T *__restrict__ x,
+                                                    T *__restrict__ r, T *__restrict__ rt, const T *__restrict__ d,
+                                                    T *__restrict__ z, T *__restrict__ zt, int zero_x, KState<T> *st,
+                                                    T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T v;
+        if (zero_x) {
+            x[i] = (T)0;
+            v = b[i];
+            r[i] = v;
+        } else {
+            v = r[i];
+        }
+        rt[i] = v;
+        c[0] += v * v;
+        if (z) {  // BiCg: (z, rtilde)
+            T zv = v, ztv = v;
+            if (d) {
+                zv = v / d[i];
+                ztv = v / d[i];
+                z[i] = zv;
+                zt[i] = ztv;
+            }
+            c[1] += zv * v;
+        } else {  // Cgs: (rtilde, r)
+            c[1] += v * v;
+        }
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// first ? u = r, p = u : u = r + beta q, p = beta (beta p + q) + u, as
+// Copy / Add / Mlt / Add / Mlt / Add (Cgs.cxx:86-102); phat = M^{-1} p
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgs_pu(int64_t n, const T *__restrict__ r, const T *__restrict__ q,
+                                                   T *__restrict__ p, T *__restrict__ u, const T *__restrict__ d,
+                                                   T *__restrict__ phat, const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T beta = st->beta;
+    const bool ab = beta != (T)0;
+    GRID_STRIDE(i, n) {
+        const T rv = r[i];
+        T uv = rv, pv = rv;
+        if (!first) {
+            const T qv = q[i];
+            if (ab) uv += beta * qv;
+            pv = p[i] * beta;
+            pv += (T)1 * qv;
+            pv *= beta;
+            pv += (T)1 * uv;
+        }
+        u[i] = uv;
+        p[i] = pv;
+        if (d) phat[i] = pv / d[i];
+    }
+}
+
+// ++iter; delta = (rtilde, vhat) from the product's partials, breakdown #2
+// and alpha folded in; q = u - alpha vhat; u += q; uhat = M^{-1} u;
+// x += alpha uhat (Cgs.cxx:106-127)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgs_qux(int64_t n, const T *__restrict__ vhat, T *__restrict__ q,
+                                                    T *__restrict__ u, const T *__restrict__ d, T *__restrict__ uhat,
+                                                    T *__restrict__ x, KState<T> *st, const T *prev, int prev_nb)
+{
+    if (st->skip) return;
+    __shared__ T alpha_sh;
+    __shared__ int fail_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            fail_sh = tot[0] == (T)0;
+            alpha_sh = fail_sh ? (T)0 : st->rho_1 / tot[0];
+            if (blockIdx.x == 0) FinLanczosDelta<T>{st}(tot);
+        }
+        __syncthreads();
+    }
+    if (fail_sh) return;
+    const T alpha = alpha_sh, malpha = -alpha_sh;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    GRID_STRIDE(i, n) {
+        T qv = u[i];
+        if (am) qv += malpha * vhat[i];
+        q[i] = qv;
+        T uv = u[i];
+        uv += (T)1 * qv;
+        u[i] = uv;
+        T uh = uv;
+        if (d) {
+            uh = uv / d[i];
+            uhat[i] = uh;
+        }
+        if (aa) x[i] += alpha * uh;
+    }
+}
+
+// r -= alpha qhat (Cgs.cxx:130); sums |r|^2 and (rtilde, r) for the next
+// loop head
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgs_r(int64_t n, const T *__restrict__ qhat, const T *__restrict__ rt,
+                                                  T *__restrict__ r, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T malpha = -st->alpha;
+    const bool am = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T rv = r[i];
+        if (am) {
+            rv += malpha * qhat[i];
+            r[i] = rv;
+        }
+        c[0] += rv * rv;
+        c[1] += rt[i] * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// -------------------------------------------------------------- BiCg ----
+// first ? p = z, p~ = z~ : p = beta p + z, p~ = beta p~ + z~ as Mlt / Add
+// (BiCg.cxx:88-102)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int64_t n, const T *__restrict__ z, const T *__restrict__ zt,
+                                                   T *__restrict__ p, T *__restrict__ pt, const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T beta = st->beta;
+    GRID_STRIDE(i, n) {
+        if (first) {
+            p[i] = z[i];
+            pt[i] = zt[i];
+        } else {
+            T pv = p[i] * beta;
+            pv += (T)1 * z[i];
+            p[i] = pv;
+            T ptv = pt[i] * beta;
+            ptv += (T)1 * zt[i];
+            pt[i] = ptv;
+        }
+    }
+}
+
+// ++iter; delta = (p~, q) from the first product's partials, breakdown #2
+// and alpha folded in; x += alpha p; r -= alpha q; r~ -= alpha q~
+// (BiCg.cxx:104-125); then z = M^{-1} r, z~ = M^{-T} r~ and the sums |r|^2,
+// (z, r~) for the next loop head
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t n, const T *__restrict__ p, const T *__restrict__ q,
+                                                    const T *__restrict__ qt, T *__restrict__ x, T *__restrict__ r,
+                                                    T *__restrict__ rt, const T *__restrict__ d, T *__restrict__ z,
+                                                    T *__restrict__ zt, KState<T> *st, const T *prev, int prev_nb,
+                                                    T *partials)
+{
+    if (st->skip) return;
+    __shared__ T alpha_sh;
+    __shared__ int fail_sh;
+    {
+        T tot[2];
+        grid_total<T>(prev, prev_nb, tot);
+        if (threadIdx.x == 0) {
+            fail_sh = tot[0] == (T)0;
+            alpha_sh = fail_sh ? (T)0 : st->rho_1 / tot[0];
+            if (blockIdx.x == 0) FinLanczosDelta<T>{st}(tot);
+        }
+        __syncthreads();
+    }
+    if (fail_sh) return;
+    const T alpha = alpha_sh, malpha = -alpha_sh;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (aa) x[i] += alpha * p[i];
+        T rv = r[i], rtv = rt[i];
+        if (am) {
+            rv += malpha * q[i];
+            rtv += malpha * qt[i];
+            r[i] = rv;
+            rt[i] = rtv;
+        }
+        c[0] += rv * rv;
+        T zv = rv;
+        if (d) {
+            zv = rv / d[i];
+            z[i] = zv;
+            zt[i] = rtv / d[i];
+        }
+        c[1] += zv * rtv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -1182,12 +1386,119 @@ int gmres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
     return S.finish(it, code);
 }
 
+// Cgs (Cgs.cxx:46-137): per loop (two of Seldon's iterations) the p / u
+// update, vhat = A phat with (rtilde, vhat), the q / u / x update,
+// qhat = A uhat, the r update with |r|^2 and (rtilde, r), the loop head.
+template <typename T>
+int cgs_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, jac ? 9 : 7, 0, it));
+    const int64_t n = S.n;
+    T *p = S.take(n), *q = S.take(n), *u = S.take(n), *vhat = S.take(n), *qhat = S.take(n), *r = S.take(n),
+      *rt = S.take(n);
+    T *phat = jac ? S.take(n) : p, *uhat = jac ? S.take(n) : u;
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (Cgs.cxx:58)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+    }
+    hipLaunchKernelGGL(k_lz_init<T>, g, blk, 0, st, n, b, x, r, rt, (const T *)nullptr, (T *)nullptr,
+                       (T *)nullptr, it->init_guess_null ? 1 : 0, S.dst, S.partials);
+    S.fin(FinLanczosTop<T>{S.dst, 0});
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            hipLaunchKernelGGL(k_cgs_pu<T>, g, blk, 0, cs, n, r, q, p, u, dg, phat, S.dst);
+            SpmvArgs<T> a{};
+            a.x = phat; a.y_out = vhat; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = rt; a.skip = &S.dst->skip;
+            a.partials = S.part[0];
+            SHK_TRY(S.spmv(a, FinDefer{}));  // delta: finished inside k_cgs_qux
+            hipLaunchKernelGGL(k_cgs_qux<T>, g, blk, 0, cs, n, vhat, q, u, dg, uhat, x, S.dst,
+                               (const T *)S.part[0], a.grid_used);
+            SpmvArgs<T> a2{};
+            a2.x = uhat; a2.y_out = qhat; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a2, FinNone{}));
+            hipLaunchKernelGGL(k_cgs_r<T>, g, blk, 0, cs, n, qhat, rt, r, S.dst, S.partials);
+            S.fin(FinLanczosTop<T>{S.dst, 1});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "cgs: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
+// BiCg (BiCg.cxx:47-131): per loop the p / p~ update, q = A p with (p~, q),
+// q~ = A^T p~ (the library's device-built transpose, Seldon's order), the
+// x / r / r~ update with z, z~, |r|^2 and (z, r~), the loop head.
+template <typename T>
+int bicg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    shk_csr At = nullptr;
+    SHK_TRY(transpose_of(A, &At));
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, jac ? 8 : 6, 0, it));
+    const int64_t n = S.n;
+    T *p = S.take(n), *pt = S.take(n), *q = S.take(n), *qt = S.take(n), *r = S.take(n), *rt = S.take(n);
+    T *z = jac ? S.take(n) : r, *zt = jac ? S.take(n) : rt;
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+    }
+    hipLaunchKernelGGL(k_lz_init<T>, g, blk, 0, st, n, b, x, r, rt, dg, z, zt, it->init_guess_null ? 1 : 0,
+                       S.dst, S.partials);
+    S.fin(FinLanczosTop<T>{S.dst, 0});
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            hipLaunchKernelGGL(k_bicg_p<T>, g, blk, 0, cs, n, z, zt, p, pt, S.dst);
+            SpmvArgs<T> a{};
+            a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = pt; a.skip = &S.dst->skip;
+            a.partials = S.part[0];
+            SHK_TRY(S.spmv(a, FinDefer{}));  // delta: finished inside k_bicg_xr
+            SpmvArgs<T> a2{};
+            a2.x = pt; a2.y_out = qt; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+            SHK_TRY((launch_spmv<T, FinNone>(At, a2, FinNone{})));
+            hipLaunchKernelGGL(k_bicg_xr<T>, g, blk, 0, cs, n, p, q, qt, x, r, rt, dg, z, zt, S.dst,
+                               (const T *)S.part[0], a.grid_used, S.partials);
+            S.fin(FinLanczosTop<T>{S.dst, 1});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "bicg: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
 {
     if (which == 0) return cg_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 1) return bicgstab_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 3) return cgs_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 4) return bicg_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -1348,6 +1659,16 @@ int shk_gmres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec
               int *code)
 {
     return solve_entry(2, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_cgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(3, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_bicg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(4, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

@@ -790,6 +790,15 @@ static int ensure_trans(shk_csr A)
     return A->dtype == SHK_F64 ? build_trans<double>(A) : build_trans<float>(A);
 }
 
+// The matrix behind A's transposed products, built on first use (the fused
+// BiCg runs its A^T p~ products on it).
+int transpose_of(shk_csr A, shk_csr *out)
+{
+    SHK_TRY(ensure_trans(A));
+    *out = A->trans;
+    return SHK_OK;
+}
+
 template <typename T>
 static int csr_write_typed(shk_csr A, const char *path)
 {

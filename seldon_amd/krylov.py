@@ -516,6 +516,17 @@ def Gmres(A, x, b, M, it):
     return _solve(L.load().shk_gmres, A, x, b, M, it)
 
 
+def Cgs(A, x, b, M, it):
+    """Cgs (computation/solver/iterative/Cgs.cxx:46-137) on the GPU, fused."""
+    return _solve(L.load().shk_cgs, A, x, b, M, it)
+
+
+def BiCg(A, x, b, M, it):
+    """BiCg (computation/solver/iterative/BiCg.cxx:47-131) on the GPU, fused;
+    its A^T products run on the library's device-built transpose."""
+    return _solve(L.load().shk_bicg, A, x, b, M, it)
+
+
 def solve_ir(A64: Matrix, A32: Matrix, x, b, M, tol=1e-10, inner_restart=50, inner_tol=1e-6,
              inner_max_iter=2000, max_refinements=50, inner_method="gmres"):
     """FP32 inner Krylov + FP64 iterative refinement (no Seldon counterpart)."""

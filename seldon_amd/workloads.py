@@ -132,6 +132,31 @@ def laplace2d(nx: int, ny: int | None = None) -> CSR:
     return aniso2d(nx, nx if ny is None else ny, 1.0)
 
 
+def upwind2d(g: int, vx: float = 20.0, vy: float = 10.0) -> CSR:
+    """Upwind convection-diffusion on a g x g grid (natural order, sorted
+    columns): -lap u + h (vx d/dx + vy d/dy) u with rows scaled by h^2,
+    h = 1 / (g + 1); nonsymmetric.  The C++ overlay program
+    (tests/cpp/seldon_overlay_check.cpp) builds the same matrix inside Seldon,
+    entry for entry."""
+    n, h = g * g, 1.0 / (g + 1)
+    ptr = np.zeros(n + 1, np.int32)
+    ind, val = [], []
+    for iy in range(g):
+        for ix in range(g):
+            r = iy * g + ix
+            if iy > 0:
+                ind.append(r - g); val.append(-1 - h * vy)
+            if ix > 0:
+                ind.append(r - 1); val.append(-1 - h * vx)
+            ind.append(r); val.append(4 + h * (vx + vy))
+            if ix < g - 1:
+                ind.append(r + 1); val.append(-1.0)
+            if iy < g - 1:
+                ind.append(r + g); val.append(-1.0)
+            ptr[r + 1] = len(ind)
+    return CSR(n, n, ptr, np.array(ind, np.int32), np.array(val, np.float64))
+
+
 def convdiff3d(n: int, v=(1.0, 0.5, 0.25), peclet: float = 10.0) -> CSR:
     lib = _load()
     nu = math.sqrt(v[0] ** 2 + v[1] ** 2 + v[2] ** 2) / peclet
@@ -215,6 +240,14 @@ def cg_laplace(grid: int = 256, seed: int = SEED_XTRUE):
 def convdiff(grid: int = 128, seed: int = SEED_XTRUE + 1):
     A = convdiff3d(grid)
     xt = uniform(A.m, seed)
+    return A, apply(A, xt), xt
+
+
+def upwind(grid: int = 24):
+    """2-D upwind convection-diffusion (vx 20, vy 10); x_true_i = cos(0.21 i)
+    - 0.3 sin(0.7 i); b = A x_true (the overlay program's nonsymmetric case)."""
+    A = upwind2d(grid)
+    xt = np.array([math.cos(0.21 * k) - 0.3 * math.sin(0.7 * k) for k in range(A.m)])
     return A, apply(A, xt), xt
 
 

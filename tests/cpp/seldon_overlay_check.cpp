@@ -14,7 +14,9 @@
 // Symmlq, CoCg, BiCgcr, QmrSym, and Cg with Seldon's SorPreconditioner
 // (SSOR) -- with Seldon's own templates, once on the CPU storages and once on
 // the GPU storages (Mlt(SeldonTrans, ...), SOR, Swap and the rest through the
-// overload set),
+// overload set; BiCg and Cgs with Preconditioner_Base reach the library's
+// fused solvers, and once more, with a user preconditioner type, Seldon's
+// templates: labels bicg_gpu_generic, cgs_gpu_generic),
 // and prints each residual history (Iteration::Finished lines) after a
 // "== <label>" marker, then "ERRCODE e NBITER n".
 #include "Seldon.hxx"
@@ -64,7 +66,7 @@ static void convdiff2d(int g, double vx, double vy, Matrix<double, General, RowS
   A.SetData(n, n, values, ptr, ind);
 }
 
-template <class Mat, class Vec>
+template <class Mat, class Vec, class Prec = Preconditioner_Base>
 static void run_trans_solver(const string& which, const string& label, Mat& A, const Vec& b, int maxit, double tol)
 {
   cout << "== " << which << "_" << label << endl;
@@ -72,7 +74,7 @@ static void run_trans_solver(const string& which, const string& label, Mat& A, c
   x.Fill(0.0);
   Iteration<double> it(maxit, tol);
   it.ShowFullHistory();
-  Preconditioner_Base M;
+  Prec M;
   int code = 0;
   if (which == "bicg") code = BiCg(A, x, b, M, it);
   else if (which == "qmr") code = Qmr(A, x, b, M, it);
@@ -228,6 +230,10 @@ int main(int argc, char** argv)
         const int maxit = (k == 2 || k == 3) ? 4000 : 2000;
         if (cpu) run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
         if (gpu) run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
+        if (gpu && (k == 0 || k == 4))
+          run_trans_solver<Matrix<double, General, RowSparseHip, HipAlloc<double> >,
+                           Vector<double, VectHip, HipAlloc<double> >, UserIdentity>(solvers[k], "gpu_generic",
+                                                                                   Cg_, bcg, maxit, 1e-8);
       }
   }
   {

@@ -693,3 +693,104 @@ def test_diffusion_grad_reference_against_its_own_finite_differences():
     # componentwise too: a dropped term (a face's neighbour share, a boundary
     # face) shows in the components it touches, not only in the norm
     assert np.max(np.abs(gt - ref["grad_theta"])) <= 1e-6 * np.max(np.abs(gt))
+
+
+# ------------------------------------------------------- Cgs and BiCg ----
+
+def _overlay_fixture():
+    return np.load(os.path.join(GOLDEN, "overlay_cpu.npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("method", ["cgs", "bicg"])
+def test_oracle_cgs_bicg_match_seldon_fixture(method):
+    """Seldon's own Cgs / BiCg (Cgs.cxx:46-137, BiCg.cxx:47-131, generic
+    templates, run by the overlay program on its CPU storages) on the 24 x 24
+    upwind convection-diffusion system: residual history, iteration numbers,
+    count and code, bitwise."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve(method, A, b, tol=1e-8, max_iter=2000)
+    assert r.code == int(fx[f"{method}_cpu_code"]) == 0
+    assert r.nb_iter == int(fx[f"{method}_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx[f"{method}_cpu_its"])
+    assert np.array_equal(r.hist, fx[f"{method}_cpu_hist"])
+
+
+def test_oracle_cgs_bicg_fixture_check_has_teeth():
+    """The same comparison fails for a slightly different system (one entry
+    moved by one ulp) and for the other method's history."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    val = A.val.copy()
+    val[100] = np.nextafter(val[100], 0.0)
+    B = wl.CSR(A.m, A.n, A.ptr, A.ind, val)
+    assert not np.array_equal(orc.solve("bicg", B, b, tol=1e-8, max_iter=2000).hist, fx["bicg_cpu_hist"])
+    assert not np.array_equal(orc.solve("cgs", A, b, tol=1e-8, max_iter=2000).hist[:20], fx["bicg_cpu_hist"][:20])
+
+
+@pytest.mark.parametrize("method", ["cgs", "bicg"])
+def test_oracle_cgs_bicg_one_loop_cases(method):
+    """A = 2.5 I with the identity, and a diagonal A with the Jacobi
+    preconditioner (M^-1 A = I): one loop of either method solves the system
+    exactly (two of Seldon's iterations: both methods count the two products
+    of a loop), and the identity on the diagonal A needs at most one loop per
+    distinct value."""
+    n = 120
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    b = wl.uniform(n, 8)
+    r = orc.solve(method, wl.CSR(n, n, *eye, np.full(n, 2.5)), b, tol=1e-12, max_iter=50)
+    assert r.code == 0 and r.nb_iter == 2 and np.allclose(r.x, b / 2.5, rtol=1e-15)
+    d = np.array([1.0, 2.0, 3.5, 7.0])[np.arange(n) % 4]
+    D = wl.CSR(n, n, *eye, d)
+    r = orc.solve(method, D, b, precond="jacobi", tol=1e-12, max_iter=50)
+    assert r.code == 0 and r.nb_iter == 2 and np.allclose(r.x, b / d, rtol=1e-14)
+    r = orc.solve(method, D, b, tol=1e-12, max_iter=50)
+    assert r.code == 0 and r.nb_iter <= 2 * 4 and np.allclose(r.x, b / d, rtol=1e-10)
+
+
+@pytest.mark.parametrize("precond", ["id", "jacobi"])
+def test_oracle_bicg_on_spd_follows_cg(precond):
+    """On a symmetric positive definite A, BiCg with r~0 = r0 builds the same
+    Krylov recurrences as Cg (p~ = p, r~ = r) with two iteration counts per
+    loop: its residual history at iteration 2k equals Cg's at k up to
+    rounding (A^T p~ is summed in the transposed order), with the identity
+    and with Jacobi.  A wrong sign or a swapped operand in the r~ / p~
+    updates breaks the match at once."""
+    A = wl.aniso2d(20, 20, 3.0)
+    b = wl.apply(A, wl.uniform(A.m, 12))
+    cg = orc.solve("cg", A, b, precond=precond, tol=1e-10, max_iter=2000)
+    bi = orc.solve("bicg", A, b, precond=precond, tol=1e-10, max_iter=4000)
+    assert cg.code == bi.code == 0
+    assert abs(bi.nb_iter - 2 * cg.nb_iter) <= 2
+    assert np.array_equal(bi.hist_it[:30], 2 * cg.hist_it[:30])
+    rel = np.abs(bi.hist[:30] - cg.hist[:30]) / cg.hist[:30]
+    assert rel.max() < 1e-9, rel.max()
+
+
+def test_oracle_cgs_squares_the_bicg_residual_polynomial_early():
+    """Cgs's residual is the square of BiCg's residual polynomial applied to
+    r0 (Sonneveld): in exact arithmetic both need the same number of loops on
+    a system whose Krylov space has small dimension -- here a diagonal A with
+    five distinct values and the identity: at most five loops each."""
+    n = 250
+    d = np.array([1.0, 1.5, 3.0, 4.0, 9.0])[np.arange(n) % 5]
+    D = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    b = wl.uniform(n, 21)
+    for m in ("cgs", "bicg"):
+        r = orc.solve(m, D, b, tol=1e-11, max_iter=100)
+        assert r.code == 0 and r.nb_iter <= 10, (m, r.nb_iter)
+        assert np.allclose(r.x, b / d, rtol=1e-9)
+
+
+@pytest.mark.parametrize("method", ["cgs", "bicg"])
+def test_oracle_cgs_bicg_fp32_within_kappa_eps(method):
+    """The FP32 versions land within kappa * eps_32 of the FP64 oracle's
+    solution and are float solves (not a rounded FP64 run)."""
+    A, b, _ = wl.upwind(16)
+    r64 = orc.solve(method, A, b, precond="jacobi", tol=1e-13, max_iter=4000)
+    r32 = orc.solve(method, A, b, precond="jacobi", tol=1e-5, max_iter=4000, dtype=np.float32)
+    assert r64.code == 0 and r32.code == 0
+    kappa = np.linalg.cond(A.to_scipy().toarray())
+    err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert eps32 / 10 <= err <= 10 * kappa * max(eps32, 1e-5), (err, kappa)
