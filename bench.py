@@ -348,7 +348,7 @@ def cg_config1(K, wl, np, torch, cpu=True):
 
 
 def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
-    """FP64 BiCgStab, Gmres(30), Cgs and BiCg with the Jacobi preconditioner
+    """FP64 BiCgStab, Gmres(30), Cgs, BiCg and TfQmr with the Jacobi preconditioner
     on the 128^3 7-point upwind convection-diffusion matrix (2,097,152 rows),
     to a 1e-10 relative residual: time to solution and iterations/s, plus the
     SpMV bandwidth on this matrix and the time of one symmetric SOR sweep
@@ -391,7 +391,7 @@ def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
     torch.cuda.synchronize()
     bicg_first = time.perf_counter() - t0
     for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30), ("cgs", K.Cgs, None),
-                              ("bicg", K.BiCg, None)):
+                              ("bicg", K.BiCg, None), ("tfqmr", K.TfQmr, None)):
         x = torch.zeros_like(bg)
         it = K.Iteration(20000, 1e-10)
         if restart:

@@ -15,7 +15,7 @@
 // are what Seldon's solver templates call unqualified
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
-// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg below replace the whole loop by the
+// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
@@ -650,7 +650,8 @@ namespace Seldon
     else if (which == 1) HipCheck(shk_bicgstab(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCgStab");
     else if (which == 2) HipCheck(shk_gmres(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gmres");
     else if (which == 3) HipCheck(shk_cgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cgs");
-    else HipCheck(shk_bicg(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCg");
+    else if (which == 4) HipCheck(shk_bicg(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCg");
+    else HipCheck(shk_tfqmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "TfQmr");
     iter.*Acc::Fac() = Titer(it.facteur_reste);
     const int n = std::min(it.hist_len, it.hist_cap);
     for (int k = 0; k < n; k++)
@@ -662,14 +663,17 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[5][4] = {{"", "Cg breakdown #1", "Cg breakdown #2", ""},
+        static const char* what[6][6] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
-                                          "Bicgstab breakdown #3"},
-                                         {"", "", "", ""},
-                                         {"", "Cgs breakdown #1", "Cgs breakdown #2", ""},
-                                         {"", "Bicg breakdown #1", "Bicg breakdown #2", ""}};
+                                          "Bicgstab breakdown #3", "", ""},
+                                         {"", "", "", "", "", ""},
+                                         {"", "Cgs breakdown #1", "Cgs breakdown #2", "", "", ""},
+                                         {"", "Bicg breakdown #1", "Bicg breakdown #2", "", "", ""},
+                                         {"", "Tfqmr breakdown: sigma=0", "Tfqmr breakdown: alpha=0",
+                                          "Tfqmr breakdown: tau=0", "Tfqmr breakdown: tau=0",
+                                          "tfqmr breakdown: beta=0"}};
         const int c = it.error_code;
-        iter.Fail(c, (c >= 0 && c < 4) ? what[which][c] : "breakdown");
+        iter.Fail(c, (c >= 0 && c < 6) ? what[which][c] : "breakdown");
       }
     return code;
   }
@@ -696,9 +700,9 @@ namespace Seldon
     return HipRunSolver(1, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
-  // Cgs.cxx:44-46 and BiCg.cxx:45-47: the same specialisation for the two
-  // Lanczos-type solvers the library fuses (BiCg's A^T products on the
-  // library's transpose, built on first use)
+  // Cgs.cxx:44-46, BiCg.cxx:45-47 and TfQmr.cxx:44-46: the same
+  // specialisation for the Lanczos-type solvers the library fuses (BiCg's
+  // A^T products on the library's transpose, built on first use)
   template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
   typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
   Cgs(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
@@ -713,6 +717,14 @@ namespace Seldon
        Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(4, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  TfQmr(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+        Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(5, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>

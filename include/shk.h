@@ -302,6 +302,14 @@ int shk_cgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
  * preconditioner, z~ = r~ / a_ii as z = r / a_ii. */
 int shk_bicg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
              shk_iteration *iter, int *code);
+/* TfQmr (computation/solver/iterative/TfQmr.cxx:45-240), fused likewise
+ * (left preconditioned, as the template: r0 = M^{-1} (b - A x), the
+ * stopping test on kappa = tau sqrt(2 iter + 1 or 2)); its theta, tau and c
+ * are double for both value types, as Titer is in the template.  Error
+ * codes 1-5 are its breakdowns (sigma = 0, alpha = 0, tau = 0 in either
+ * half step, rho0 = 0). */
+int shk_tfqmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+              shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

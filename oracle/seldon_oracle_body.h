@@ -624,6 +624,108 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- TfQmr: computation/solver/iterative/TfQmr.cxx:45-240 ----------- *
+ * Scalars typed as in the template: sigma, alpha, beta, eta, rho, rho0 are
+ * the value type T; c, kappa, tau, theta are Titer = double.  Mlt(s, d)
+ * with a double s scales each T element in double and rounds back
+ * (`d(i) *= s`); eta = c * c * alpha is formed in double and stored as T. */
+static void FN(scal_d)(int n, double s, T *x)
+{
+    for (int i = 0; i < n; i++) x[i] *= s;
+}
+
+int FN(tfqmr)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+              orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T *tmp = FN(dup)(N, b), *r0 = FN(dup)(N, b), *v = FN(dup)(N, b), *h = FN(dup)(N, b),
+      *w = FN(dup)(N, b), *y1 = FN(dup)(N, b), *g = FN(dup)(N, b), *y0 = FN(dup)(N, b),
+      *rtilde = FN(dup)(N, b), *d = FN(dup)(N, b);
+    T sigma, alpha, beta, eta, rho, rho0;
+    double c, kappa, tau, theta;
+    FN(fill)(N, (T)0, tmp);
+    /* 1. r0 = M^{-1} (b - A x) */
+    FN(copy)(N, b, tmp);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, tmp);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(precond)(precond, N, diag, tmp, r0);
+    if (FN(iter_init)(it, N, r0) != 0) goto done;
+    /* 2.-8. */
+    FN(copy)(N, r0, w);
+    FN(copy)(N, r0, y1);
+    FN(copy)(N, y1, v);
+    FN(mlt)(A, v, tmp);
+    FN(precond)(precond, N, diag, tmp, v);
+    it->nb_iter++;
+    FN(copy)(N, v, g);
+    FN(fill)(N, (T)0, d);
+    tau = (double)FN(norm2)(N, r0);
+    theta = 0.0;
+    eta = (T)0;
+    FN(copy)(N, r0, rtilde);
+    rho = FN(dot)(N, rtilde, r0);
+    rho0 = rho;
+    it->nb_iter = 0;
+    for (;;) {
+        /* 9.-11. */
+        sigma = FN(dot)(N, rtilde, v);
+        if (sigma == (T)0) { orc_fail(it, 1); break; }
+        alpha = rho / sigma;
+        FN(copy)(N, y1, y0);
+        FN(add)(N, -alpha, v, y0);
+        /* 12. h = M^{-1} A y0 */
+        FN(copy)(N, y0, h);
+        FN(mlt)(A, h, tmp);
+        FN(precond)(precond, N, diag, tmp, h);
+        /* first half: 13., 18., 14.-17., 19.-21. */
+        FN(add)(N, -alpha, g, w);
+        if (alpha == (T)0) { orc_fail(it, 2); break; }
+        FN(scal_d)(N, theta * theta * (double)eta / (double)alpha, d);
+        FN(add)(N, (T)1, y1, d);
+        if (tau == 0.0) { orc_fail(it, 3); break; }
+        theta = (double)FN(norm2)(N, w) / tau;
+        c = 1.0 / sqrt(1.0 + theta * theta);
+        tau = tau * c * theta;
+        eta = (T)(c * c * (double)alpha);
+        FN(add)(N, eta, d, x);
+        kappa = tau * sqrt(2. * (it->nb_iter + 1));
+        if (orc_finished(it, kappa, it->nb_iter, it->max_iter)) break;
+        it->nb_iter++;
+        FN(copy)(N, h, g);
+        /* second half */
+        FN(add)(N, -alpha, g, w);
+        FN(scal_d)(N, theta * theta * (double)eta / (double)alpha, d);
+        FN(add)(N, (T)1, y0, d);
+        if (tau == 0.0) { orc_fail(it, 4); break; }
+        theta = (double)FN(norm2)(N, w) / tau;
+        c = 1.0 / sqrt(1.0 + theta * theta);
+        tau = tau * c * theta;
+        eta = (T)(c * c * (double)alpha);
+        FN(add)(N, eta, d, x);
+        kappa = tau * sqrt(2. * (it->nb_iter + 1) + 1.);
+        if (orc_finished(it, kappa, it->nb_iter, it->max_iter)) break;
+        /* 22.-26. */
+        rho0 = rho;
+        rho = FN(dot)(N, rtilde, w);
+        if (rho0 == (T)0) { orc_fail(it, 5); break; }
+        beta = rho / rho0;
+        FN(copy)(N, w, y1);
+        FN(add)(N, beta, y0, y1);
+        FN(copy)(N, y1, g);
+        FN(mlt)(A, g, tmp);
+        FN(precond)(precond, N, diag, tmp, g);
+        FN(scal)(N, beta * beta, v);
+        FN(add)(N, beta, h, v);
+        FN(add)(N, (T)1, g, v);
+        it->nb_iter++;
+    }
+done:
+    free(tmp); free(r0); free(v); free(h); free(w); free(y1); free(g); free(y0);
+    free(rtilde); free(d);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

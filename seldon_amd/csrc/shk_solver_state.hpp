@@ -19,12 +19,15 @@ struct KState {
     int max_iter, nb_iter, error_code, fail;
     int done, init_failed, hist_len, hist_cap;
     int skip;          // done, or (GMRES) the current cycle is over
-    int stop_slot;     // BiCgStab: loop slot whose Finished(s) ended the solve
+    int stop_slot;     // BiCgStab / TfQmr: loop slot (TfQmr: half step) that ended the solve
     double *hist;
     int *hist_it;
     // Cg / BiCgStab scalars, typed like Seldon's `Complexe` (= value type)
     T rho, rho_1, rho_2, alpha, beta, delta, omega, sigma;
     T rho_pp[2];       // Cg rho / BiCgStab rho_1 after the loop head of iteration k, in slot k % 2
+    // TfQmr: theta, tau (Titer = double in the template) and eta (value type)
+    double tq_theta, tq_tau;
+    T tq_eta;
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -256,6 +259,90 @@ struct FinLanczosDelta {
             return;
         }
         st->alpha = st->rho_1 / s[0];
+    }
+};
+
+// ------------------------------------------------------------- TfQmr ----
+// After r0 = M^{-1} (b - A x): Init(r0); tau = |r0|; theta = eta = 0;
+// rho = rho0 = (rtilde, r0) with rtilde = r0 (TfQmr.cxx:72-110).
+template <typename T>
+struct FinTqInit {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (!it_init(st, s[0])) return;
+        st->tq_tau = (double)sqrt_t(s[0]);
+        st->tq_theta = 0.0;
+        st->tq_eta = (T)0;
+        st->rho = s[1];
+        st->rho_1 = s[1];
+    }
+};
+
+// sigma = (rtilde, v); breakdown 1; alpha = rho / sigma; breakdown 2 when
+// alpha is 0 (TfQmr.cxx:116-140; the template tests alpha after the w
+// update, which changes nothing then)
+template <typename T>
+struct FinTqSigma {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->sigma = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        const T alpha = st->rho / s[0];
+        st->alpha = alpha;
+        if (alpha == (T)0) it_fail(st, 2);
+    }
+};
+
+// One half step's scalars from |w|^2 (TfQmr.cxx:145-172 and :180-205):
+// breakdown 3 / 4 when tau is 0; theta = |w| / tau; c = 1 / sqrt(1 +
+// theta^2); tau *= c theta; eta = c^2 alpha (formed in double, kept in
+// the value type); kappa = tau sqrt(2 (iter + 1) [+ 1]); Finished(kappa).
+// The x += eta d of this half runs whether or not the solve ends here, so
+// an end records its slot (stop_slot) for the kernel that applies it.  The
+// second half then takes rho0 = rho, rho = (rtilde, w) = s[1],
+// breakdown 5, beta = rho / rho0 and the loop's ++iter (:209-240).
+template <typename T>
+struct FinTqHalf {
+    KState<T> *st;
+    int half, slot;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double tau = st->tq_tau;
+        if (tau == 0.0) {
+            it_fail(st, half ? 4 : 3);
+            return;
+        }
+        const double theta = (double)sqrt_t(s[0]) / tau;
+        const double c = 1.0 / sqrt(1.0 + theta * theta);
+        const double tau2 = tau * c * theta;
+        st->tq_theta = theta;
+        st->tq_tau = tau2;
+        st->tq_eta = (T)(c * c * (double)st->alpha);
+        const double kappa = tau2 * sqrt(half ? 2. * (st->nb_iter + 1) + 1. : 2. * (st->nb_iter + 1));
+        if (it_finished(st, kappa, st->nb_iter, st->max_iter)) {
+            st->stop_slot = 2 * slot + half;
+            it_stop(st);
+            return;
+        }
+        if (!half) {
+            st->nb_iter++;
+            return;
+        }
+        const T rho0 = st->rho;
+        st->rho_1 = rho0;
+        st->rho = s[1];
+        if (rho0 == (T)0) {
+            st->stop_slot = 2 * slot + half;
+            it_fail(st, 5);
+            return;
+        }
+        st->beta = s[1] / rho0;
+        st->nb_iter++;
     }
 };
 

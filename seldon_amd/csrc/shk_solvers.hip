@@ -763,6 +763,164 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t n, const T *__restri
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// ------------------------------------------------------------- TfQmr ----
+// r0 = M^{-1} tmp (tmp = b - A x, or b with x = 0); w = y1 = rtilde = r0;
+// d = 0; sums |r0|^2 and (rtilde, r0) (TfQmr.cxx:58-110)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tq_init(int64_t n, const T *__restrict__ src, const T *__restrict__ dg,
+                                                    T *__restrict__ x, T *__restrict__ w, T *__restrict__ y1,
+                                                    T *__restrict__ rt, T *__restrict__ d, int zero_x, KState<T> *st,
+                                                    T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        T v = src[i];
+        if (dg) v = v / dg[i];
+        w[i] = v;
+        y1[i] = v;
+        rt[i] = v;
+        d[i] = (T)0;
+        c[0] += v * v;
+        c[1] += v * v;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// v = M^{-1} tmp; g = v (the start, :85-93) or, at a loop's end, g = M^{-1}
+// tmp and v = beta^2 v + beta h + g (:227-236); both sum (rtilde, v)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tq_v(int64_t n, const T *__restrict__ tmp, const T *__restrict__ dg,
+                                                 T *__restrict__ g, T *__restrict__ v, const T *__restrict__ h,
+                                                 const T *__restrict__ rt, int start, const KState<T> *st,
+                                                 T *partials)
+{
+    if (st->skip) return;
+    const T beta = st->beta, bb = beta * beta;
+    const bool ab = beta != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T gv = tmp[i];
+        if (dg) gv = gv / dg[i];
+        g[i] = gv;
+        T vv = gv;
+        if (!start) {
+            vv = v[i] * bb;
+            if (ab) vv += beta * h[i];
+            vv += (T)1 * gv;
+        }
+        v[i] = vv;
+        c[0] += rt[i] * vv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// y0 = y1 - alpha v (:123-124)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tq_y0(int64_t n, const T *__restrict__ y1, const T *__restrict__ v,
+                                                  T *__restrict__ y0, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T malpha = -st->alpha;
+    const bool am = malpha != (T)0;
+    GRID_STRIDE(i, n) {
+        T yv = y1[i];
+        if (am) yv += malpha * v[i];
+        y0[i] = yv;
+    }
+}
+
+// first half step: h = M^{-1} tmp (tmp = A y0); w -= alpha g;
+// d = (theta^2 eta / alpha) d + y1, the factor in double (:127-143);
+// sums |w|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tq_h1(int64_t n, const T *__restrict__ tmp, const T *__restrict__ dg,
+                                                  T *__restrict__ h, const T *__restrict__ g, T *__restrict__ w,
+                                                  T *__restrict__ d, const T *__restrict__ y1, const KState<T> *st,
+                                                  T *partials)
+{
+    if (st->skip) return;
+    const T alpha = st->alpha, malpha = -alpha;
+    const bool am = malpha != (T)0;
+    const double s = st->tq_theta * st->tq_theta * (double)st->tq_eta / (double)alpha;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T hv = tmp[i];
+        if (dg) hv = hv / dg[i];
+        h[i] = hv;
+        T wv = w[i];
+        if (am) wv += malpha * g[i];
+        w[i] = wv;
+        T dv = d[i];
+        dv *= s;
+        dv += (T)1 * y1[i];
+        d[i] = dv;
+        c[0] += wv * wv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// x += eta d of the first half (:159, also when the solve ends there); then
+// the second half step: g = h; w -= alpha g; d = (theta^2 eta / alpha) d +
+// y0 (:170-178); sums |w|^2 and (rtilde, w)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tq_h2(int64_t n, const T *__restrict__ h, T *__restrict__ g,
+                                                  T *__restrict__ w, T *__restrict__ d, const T *__restrict__ y0,
+                                                  T *__restrict__ x, const T *__restrict__ rt, const KState<T> *st,
+                                                  int slot, T *partials)
+{
+    const bool here = st->stop_slot == 2 * slot;
+    if (st->skip && !here) return;
+    const T eta = st->tq_eta;
+    const bool ae = eta != (T)0;
+    if (st->skip) {  // the solve ended at this half's Finished
+        if (ae) GRID_STRIDE(i, n) x[i] += eta * d[i];
+        return;
+    }
+    const T alpha = st->alpha, malpha = -alpha;
+    const bool am = malpha != (T)0;
+    const double s = st->tq_theta * st->tq_theta * (double)eta / (double)alpha;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T dv = d[i];
+        if (ae) x[i] += eta * dv;
+        const T hv = h[i];
+        g[i] = hv;
+        T wv = w[i];
+        if (am) wv += malpha * hv;
+        w[i] = wv;
+        dv *= s;
+        dv += (T)1 * y0[i];
+        d[i] = dv;
+        c[0] += wv * wv;
+        c[1] += rt[i] * wv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// x += eta d of the second half (:198, also when the solve ends there);
+// then y1 = w + beta y0 (:216-217)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tq_y1(int64_t n, const T *__restrict__ w, const T *__restrict__ y0,
+                                                  const T *__restrict__ d, T *__restrict__ x, T *__restrict__ y1,
+                                                  const KState<T> *st, int slot)
+{
+    const bool here = st->stop_slot == 2 * slot + 1;
+    if (st->skip && !here) return;
+    const T eta = st->tq_eta, beta = st->beta;
+    const bool ae = eta != (T)0, ab = beta != (T)0;
+    const bool ended = st->skip;
+    GRID_STRIDE(i, n) {
+        if (ae) x[i] += eta * d[i];
+        if (!ended) {
+            T yv = w[i];
+            if (ab) yv += beta * y0[i];
+            y1[i] = yv;
+        }
+    }
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -1491,6 +1649,65 @@ int bicg_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration 
     return S.finish(it, code);
 }
 
+// TfQmr (TfQmr.cxx:45-240): per loop (Seldon's two iterations) sigma and
+// alpha, y0, h = M^{-1} A y0 with the first half step's vector updates,
+// its scalars and Finished, the second half step, its scalars, Finished
+// and beta, y1, g = M^{-1} A y1 and v.
+template <typename T>
+int tfqmr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 9, 0, it));
+    const int64_t n = S.n;
+    T *tmp = S.take(n), *v = S.take(n), *h = S.take(n), *w = S.take(n), *y1 = S.take(n), *g = S.take(n),
+      *y0 = S.take(n), *rt = S.take(n), *d = S.take(n);
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g_(S.nb), blk(kBlock);
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = tmp; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = tmp;
+    }
+    hipLaunchKernelGGL(k_tq_init<T>, g_, blk, 0, st, n, src, dg, x, w, y1, rt, d, it->init_guess_null ? 1 : 0,
+                       S.dst, S.partials);
+    S.fin(FinTqInit<T>{S.dst});
+    {
+        SpmvArgs<T> a{};
+        a.x = y1; a.y_out = tmp; a.alpha = (T)1; a.mlt = 1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+    }
+    hipLaunchKernelGGL(k_tq_v<T>, g_, blk, 0, st, n, tmp, dg, g, v, (const T *)h, rt, 1, S.dst, S.partials);
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int slot = 0; slot < kChunk; slot++) {
+            S.fin(FinTqSigma<T>{S.dst});
+            hipLaunchKernelGGL(k_tq_y0<T>, g_, blk, 0, cs, n, y1, v, y0, S.dst);
+            SpmvArgs<T> a{};
+            a.x = y0; a.y_out = tmp; a.alpha = (T)1; a.mlt = 1; a.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a, FinNone{}));
+            hipLaunchKernelGGL(k_tq_h1<T>, g_, blk, 0, cs, n, tmp, dg, h, g, w, d, y1, S.dst, S.partials);
+            S.fin(FinTqHalf<T>{S.dst, 0, slot});
+            hipLaunchKernelGGL(k_tq_h2<T>, g_, blk, 0, cs, n, h, g, w, d, y0, x, rt, S.dst, slot, S.partials);
+            S.fin(FinTqHalf<T>{S.dst, 1, slot});
+            hipLaunchKernelGGL(k_tq_y1<T>, g_, blk, 0, cs, n, w, y0, d, x, y1, S.dst, slot);
+            SpmvArgs<T> a2{};
+            a2.x = y1; a2.y_out = tmp; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a2, FinNone{}));
+            hipLaunchKernelGGL(k_tq_v<T>, g_, blk, 0, cs, n, tmp, dg, g, v, (const T *)h, rt, 0, S.dst,
+                               S.partials);
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "tfqmr: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -1499,6 +1716,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 1) return bicgstab_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 3) return cgs_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 4) return bicg_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 5) return tfqmr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -1669,6 +1887,11 @@ int shk_cgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, 
 int shk_bicg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(4, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_tfqmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(5, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

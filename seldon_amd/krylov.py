@@ -527,6 +527,11 @@ def BiCg(A, x, b, M, it):
     return _solve(L.load().shk_bicg, A, x, b, M, it)
 
 
+def TfQmr(A, x, b, M, it):
+    """TfQmr (computation/solver/iterative/TfQmr.cxx:45-240) on the GPU, fused."""
+    return _solve(L.load().shk_tfqmr, A, x, b, M, it)
+
+
 def solve_ir(A64: Matrix, A32: Matrix, x, b, M, tol=1e-10, inner_restart=50, inner_tol=1e-6,
              inner_max_iter=2000, max_refinements=50, inner_method="gmres"):
     """FP32 inner Krylov + FP64 iterative refinement (no Seldon counterpart)."""

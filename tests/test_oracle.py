@@ -794,3 +794,68 @@ def test_oracle_cgs_bicg_fp32_within_kappa_eps(method):
     err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
     eps32 = float(np.finfo(np.float32).eps)
     assert eps32 / 10 <= err <= 10 * kappa * max(eps32, 1e-5), (err, kappa)
+
+
+# ------------------------------------------------------------- TfQmr ----
+
+def test_oracle_tfqmr_matches_seldon_fixture():
+    """Seldon's own TfQmr (TfQmr.cxx:45-240, generic templates, the overlay
+    program's CPU leg) on the 24 x 24 upwind system: history (its kappa
+    bounds), iteration numbers, count and code, bitwise."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve("tfqmr", A, b, tol=1e-8, max_iter=2000)
+    assert r.code == int(fx["tfqmr_cpu_code"]) == 0
+    assert r.nb_iter == int(fx["tfqmr_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx["tfqmr_cpu_its"])
+    assert np.array_equal(r.hist, fx["tfqmr_cpu_hist"])
+
+
+def test_oracle_tfqmr_closed_forms():
+    """A = 2.5 I, and a diagonal A under Jacobi (M^-1 A = I): the first half
+    step gives w = 0, so theta = tau = kappa = 0 and x = b / a exactly at
+    iteration 0.  On a general system the quasi-residual bound holds: the
+    true (left-preconditioned) residual at the end is at most the last
+    kappa-based estimate Finished() saw."""
+    n = 90
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    b = wl.uniform(n, 4)
+    r = orc.solve("tfqmr", wl.CSR(n, n, *eye, np.full(n, 2.5)), b, tol=1e-12, max_iter=50)
+    assert r.code == 0 and r.nb_iter == 0 and np.allclose(r.x, b / 2.5, rtol=1e-15)
+    d = np.array([1.0, 2.0, 3.5, 7.0])[np.arange(n) % 4]
+    r = orc.solve("tfqmr", wl.CSR(n, n, *eye, d), b, precond="jacobi", tol=1e-12, max_iter=50)
+    assert r.code == 0 and r.nb_iter == 0 and np.allclose(r.x, b / d, rtol=1e-15)
+    for precond in ("id", "jacobi"):
+        A, b, _ = wl.upwind(20)
+        r = orc.solve("tfqmr", A, b, precond=precond, tol=1e-9, max_iter=3000)
+        assert r.code == 0
+        res = b - orc.mlt(A, r.x)
+        dg = orc.jacobi_diag(A, None, np.float64) if precond == "jacobi" else np.ones(A.m)
+        rel = np.linalg.norm(res / dg) / np.linalg.norm(b / dg)
+        assert rel <= r.hist[-1] * (1 + 1e-6) + 1e-14, (rel, r.hist[-1])
+
+
+def test_oracle_tfqmr_bound_has_teeth():
+    """The same bound fails when the last x update is skipped (a plausible
+    slip: applying x += eta d only when the solve continues)."""
+    A, b, _ = wl.upwind(20)
+    r = orc.solve("tfqmr", A, b, tol=1e-6, max_iter=3000)
+    r_short = orc.solve("tfqmr", A, b, tol=1e-6, max_iter=int(r.hist_it[-1]))
+    assert r_short.code == -2
+    rel = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert rel <= r.hist[-1] * (1 + 1e-6)
+    # the iterate of the previous half step is further from b than the bound
+    prev = orc.solve("tfqmr", A, b, tol=1e-30, max_iter=max(int(r.hist_it[-1]) - 1, 0))
+    relp = np.linalg.norm(b - orc.mlt(A, prev.x)) / np.linalg.norm(b)
+    assert relp > r.hist[-1]
+
+
+def test_oracle_tfqmr_fp32_within_kappa_eps():
+    A, b, _ = wl.upwind(16)
+    r64 = orc.solve("tfqmr", A, b, precond="jacobi", tol=1e-13, max_iter=4000)
+    r32 = orc.solve("tfqmr", A, b, precond="jacobi", tol=1e-5, max_iter=4000, dtype=np.float32)
+    assert r64.code == 0 and r32.code == 0
+    kappa = np.linalg.cond(A.to_scipy().toarray())
+    err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert eps32 / 10 <= err <= 10 * kappa * 1e-5, (err, kappa)
