@@ -15,14 +15,14 @@
 // are what Seldon's solver templates call unqualified
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
-// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr below replace the whole loop by the
+// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
 // JacobiPreconditioner.  Every call goes through the C ABI of include/shk.h.
 //
 // The transposed products (MltAdd / Mlt with SeldonTrans) and Swap are
-// overloaded too, so the transpose-using solvers (Qmr, Cgne, Lsqr, and BiCg
+// overloaded too, so the transpose-using solvers (Cgne, Lsqr, and BiCg / Qmr
 // with any other preconditioner) also run on the GPU storages through
 // Seldon's own templates.
 //
@@ -651,7 +651,8 @@ namespace Seldon
     else if (which == 2) HipCheck(shk_gmres(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gmres");
     else if (which == 3) HipCheck(shk_cgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cgs");
     else if (which == 4) HipCheck(shk_bicg(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCg");
-    else HipCheck(shk_tfqmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "TfQmr");
+    else if (which == 5) HipCheck(shk_tfqmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "TfQmr");
+    else HipCheck(shk_qmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "Qmr");
     iter.*Acc::Fac() = Titer(it.facteur_reste);
     const int n = std::min(it.hist_len, it.hist_cap);
     for (int k = 0; k < n; k++)
@@ -663,17 +664,19 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[6][6] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", ""},
+        static const char* what[7][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
-                                          "Bicgstab breakdown #3", "", ""},
-                                         {"", "", "", "", "", ""},
-                                         {"", "Cgs breakdown #1", "Cgs breakdown #2", "", "", ""},
-                                         {"", "Bicg breakdown #1", "Bicg breakdown #2", "", "", ""},
+                                          "Bicgstab breakdown #3", "", "", ""},
+                                         {"", "", "", "", "", "", ""},
+                                         {"", "Cgs breakdown #1", "Cgs breakdown #2", "", "", "", ""},
+                                         {"", "Bicg breakdown #1", "Bicg breakdown #2", "", "", "", ""},
                                          {"", "Tfqmr breakdown: sigma=0", "Tfqmr breakdown: alpha=0",
                                           "Tfqmr breakdown: tau=0", "Tfqmr breakdown: tau=0",
-                                          "tfqmr breakdown: beta=0"}};
+                                          "tfqmr breakdown: beta=0", ""},
+                                         {"", "Qmr breakdown #1", "Qmr breakdown #2", "Qmr breakdown #3",
+                                          "Qmr breakdown #4", "Qmr breakdown #5", "Qmr breakdown #6"}};
         const int c = it.error_code;
-        iter.Fail(c, (c >= 0 && c < 6) ? what[which][c] : "breakdown");
+        iter.Fail(c, (c >= 0 && c < 7) ? what[which][c] : "breakdown");
       }
     return code;
   }
@@ -700,9 +703,10 @@ namespace Seldon
     return HipRunSolver(1, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
-  // Cgs.cxx:44-46, BiCg.cxx:45-47 and TfQmr.cxx:44-46: the same
-  // specialisation for the Lanczos-type solvers the library fuses (BiCg's
-  // A^T products on the library's transpose, built on first use)
+  // Cgs.cxx:44-46, BiCg.cxx:45-47, TfQmr.cxx:44-46 and Qmr.cxx:41-43: the
+  // same specialisation for the Lanczos-type solvers the library fuses
+  // (BiCg's and Qmr's A^T products on the library's transpose, built on
+  // first use)
   template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
   typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
   Cgs(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
@@ -725,6 +729,14 @@ namespace Seldon
         Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(5, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  Qmr(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+      Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(6, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>

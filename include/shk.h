@@ -310,6 +310,12 @@ int shk_bicg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
  * half step, rho0 = 0). */
 int shk_tfqmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
               shk_iteration *iter, int *code);
+/* Qmr (computation/solver/iterative/Qmr.cxx:42-190), fused likewise; its
+ * A^T q products run on the library's device-built transpose; rho and xi
+ * are double for both value types (Titer in the template), the other
+ * scalars the value type.  Error codes 1-6 are its breakdowns #1-#6. */
+int shk_qmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+            shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

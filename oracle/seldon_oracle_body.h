@@ -726,6 +726,91 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- Qmr: computation/solver/iterative/Qmr.cxx:42-190 --------------- *
+ * Scalars typed as in the template: rho, rho_1, xi are Titer = double;
+ * delta, ep, beta, theta, gamma, eta (and their _1 copies) are the value
+ * type T, each expression evaluated in C++'s types (a double operand
+ * makes the operation double) and rounded where it is stored.  d and s
+ * start as copies of b (Vector1 d(b), s(b)); the branch that would copy p
+ * into d is never taken (First() is tested after ++iter). */
+int FN(qmr)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+            orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T delta, ep = 0, beta, theta_1, gamma_1, theta = 0, gamma = 1, eta = -1;
+    double rho, rho_1, xi;
+    T *r = FN(dup)(N, b), *y = FN(dup)(N, b), *z_tld = FN(dup)(N, b), *v = FN(dup)(N, b),
+      *w = FN(dup)(N, b), *p_tld = FN(dup)(N, b), *p = FN(dup)(N, b), *q = FN(dup)(N, b),
+      *d = FN(dup)(N, b), *s = FN(dup)(N, b);
+    FN(fill)(N, (T)0, r);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(copy)(N, r, v);
+    FN(precond)(precond, N, diag, v, y);
+    rho = (double)FN(norm2)(N, y);
+    FN(copy)(N, r, w);
+    xi = (double)FN(norm2)(N, w);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        if (rho == 0.0) { orc_fail(it, 1); break; }
+        if (xi == 0.0) { orc_fail(it, 2); break; }
+        FN(scal)(N, (T)(1. / rho), v);
+        FN(scal)(N, (T)(1. / rho), y);
+        FN(scal)(N, (T)(1. / xi), w);
+        delta = FN(dot)(N, w, y);
+        if (delta == (T)0) { orc_fail(it, 3); break; }
+        FN(precond)(precond, N, diag, w, z_tld);  /* TransSolve */
+        if (it->nb_iter == 0) {
+            FN(copy)(N, y, p);
+            FN(copy)(N, z_tld, q);
+        } else {
+            FN(scal)(N, (T)(-(xi * (double)delta / (double)ep)), p);
+            FN(add)(N, (T)1, y, p);
+            FN(scal)(N, (T)(-(rho * (double)delta / (double)ep)), q);
+            FN(add)(N, (T)1, z_tld, q);
+        }
+        FN(mlt)(A, p, p_tld);
+        ep = FN(dot)(N, q, p_tld);
+        if (ep == (T)0) { orc_fail(it, 4); break; }
+        beta = ep / delta;
+        if (beta == (T)0) { orc_fail(it, 5); break; }
+        FN(scal)(N, -beta, v);
+        FN(add)(N, (T)1, p_tld, v);
+        FN(precond)(precond, N, diag, v, y);
+        rho_1 = rho;
+        rho = (double)FN(norm2)(N, y);
+        FN(mlt_trans)(A, q, z_tld);
+        FN(scal)(N, -beta, w);
+        FN(add)(N, (T)1, z_tld, w);
+        xi = (double)FN(norm2)(N, w);
+        gamma_1 = gamma;
+        theta_1 = theta;
+        it->nb_iter++;
+        theta = (T)(rho / (double)(gamma_1 * beta));
+        gamma = (T)((double)(T)1 / sqrt(1.0 + (double)(theta * theta)));
+        if (gamma == (T)0) { orc_fail(it, 6); break; }
+        eta = (T)((double)(-eta) * rho_1 * (double)gamma * (double)gamma /
+                  (double)(beta * gamma_1 * gamma_1));
+        {
+            const T tmp = theta_1 * theta_1 * gamma * gamma;
+            FN(scal)(N, tmp, d);
+            FN(add)(N, eta, p, d);
+            FN(scal)(N, tmp, s);
+            FN(add)(N, eta, p_tld, s);
+        }
+        FN(add)(N, (T)1, d, x);
+        FN(add)(N, -(T)1, s, r);
+        it->nb_iter++;
+    }
+done:
+    free(r); free(y); free(z_tld); free(v); free(w); free(p_tld); free(p); free(q);
+    free(d); free(s);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

@@ -93,6 +93,7 @@ SIGNATURES = [
     ("shk_cgs", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_bicg", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_tfqmr", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_qmr", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_gmres", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_solve_ir", _I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ShkIrParams), _PI]),
     ("shk_sddmm", _I, [_P, _P, _P, _P, _P]),

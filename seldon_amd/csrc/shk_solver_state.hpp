@@ -28,6 +28,10 @@ struct KState {
     // TfQmr: theta, tau (Titer = double in the template) and eta (value type)
     double tq_theta, tq_tau;
     T tq_eta;
+    // Qmr: rho, xi (Titer = double in the template); ep, theta, gamma, eta
+    // and the d / s scale theta_1^2 gamma^2 (value type)
+    double qm_rho, qm_xi;
+    T qm_ep, qm_theta, qm_gamma, qm_eta, qm_tmp;
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -343,6 +347,100 @@ struct FinTqHalf {
         }
         st->beta = s[1] / rho0;
         st->nb_iter++;
+    }
+};
+
+// --------------------------------------------------------------- Qmr ----
+// Loop head (Qmr.cxx:79-90): [++iter;] Finished(r); breakdowns #1 (rho =
+// 0) and #2 (xi = 0).  The first head also takes rho = |y| and xi = |w|
+// from s (y = M^{-1} r, w = r) and the scalars' starting values
+// (:49-77): theta = 0, gamma = 1, eta = -1, ep = 0; there r = w, so |r| is
+// sqrt(s[1]).
+template <typename T>
+struct FinQmrTop {
+    KState<T> *st;
+    int first;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        double rnorm;
+        if (first) {
+            st->qm_rho = (double)sqrt_t(s[0]);
+            st->qm_xi = (double)sqrt_t(s[1]);
+            st->qm_theta = (T)0;
+            st->qm_gamma = (T)1;
+            st->qm_eta = (T)-1;
+            st->qm_ep = (T)0;
+            st->nb_iter = 0;
+            rnorm = (double)sqrt_t(s[1]);
+        } else {
+            st->nb_iter++;
+            rnorm = (double)sqrt_t(s[0]);
+        }
+        if (it_finished(st, rnorm, st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        if (st->qm_rho == 0.0) {
+            it_fail(st, 1);
+            return;
+        }
+        if (st->qm_xi == 0.0) it_fail(st, 2);
+    }
+};
+
+// delta = (w, y) after the scaling; breakdown #3 (Qmr.cxx:99-104)
+template <typename T>
+struct FinQmrDelta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->delta = s[0];
+        if (s[0] == (T)0) it_fail(st, 3);
+    }
+};
+
+// ep = (q, p~); breakdown #4; beta = ep / delta; breakdown #5 (:124-137)
+template <typename T>
+struct FinQmrEp {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->qm_ep = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 4);
+            return;
+        }
+        const T beta = s[0] / st->delta;
+        st->beta = beta;
+        if (beta == (T)0) it_fail(st, 5);
+    }
+};
+
+// rho = |y|, xi = |w| (s[0], s[1]); ++iter; theta, gamma, breakdown #6,
+// eta and the d / s scale (Qmr.cxx:143-172), each expression in the
+// template's types
+template <typename T>
+struct FinQmrTheta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double rho_1 = st->qm_rho;
+        const double rho = (double)sqrt_t(s[0]);
+        st->qm_rho = rho;
+        st->qm_xi = (double)sqrt_t(s[1]);
+        const T gamma_1 = st->qm_gamma, theta_1 = st->qm_theta, beta = st->beta;
+        st->nb_iter++;
+        const T theta = (T)(rho / (double)(gamma_1 * beta));
+        const T gamma = (T)((double)(T)1 / sqrt(1.0 + (double)(theta * theta)));
+        st->qm_theta = theta;
+        st->qm_gamma = gamma;
+        if (gamma == (T)0) {
+            it_fail(st, 6);
+            return;
+        }
+        st->qm_eta = (T)((double)(-st->qm_eta) * rho_1 * (double)gamma * (double)gamma /
+                         (double)(beta * gamma_1 * gamma_1));
+        st->qm_tmp = theta_1 * theta_1 * gamma * gamma;
     }
 };
 

@@ -921,6 +921,144 @@ __global__ __launch_bounds__(kBlock) void k_tq_y1(int64_t n, const T *__restrict
     }
 }
 
+// --------------------------------------------------------------- Qmr ----
+// [x = 0]; r = src (b, or b - A x); v = r; y = M^{-1} v; w = r; d = s = b
+// (copies of b, Qmr.cxx:53-76); sums |y|^2 and |w|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qmr_init(int64_t n, const T *__restrict__ src, const T *__restrict__ b,
+                                                     const T *__restrict__ dg, T *__restrict__ x, T *__restrict__ r,
+                                                     T *__restrict__ v, T *__restrict__ y, T *__restrict__ w,
+                                                     T *__restrict__ d, T *__restrict__ s, int zero_x,
+                                                     KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        const T rv = src[i];
+        r[i] = rv;
+        v[i] = rv;
+        T yv = rv;
+        if (dg) yv = rv / dg[i];
+        y[i] = yv;
+        w[i] = rv;
+        const T bv = b[i];
+        d[i] = bv;
+        s[i] = bv;
+        c[0] += yv * yv;
+        c[1] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// v /= rho; y /= rho; w /= xi as Mlt(Complexe(1 / .)) (:92-97);
+// z~ = M^{-T} w (:106); sums (w, y)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qmr_scale(int64_t n, T *__restrict__ v, T *__restrict__ y,
+                                                      T *__restrict__ w, const T *__restrict__ dg, T *__restrict__ z,
+                                                      const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T sv = (T)(1. / st->qm_rho), sw = (T)(1. / st->qm_xi);
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        v[i] = v[i] * sv;
+        const T yv = y[i] * sv;
+        y[i] = yv;
+        const T wv = w[i] * sw;
+        w[i] = wv;
+        if (dg) z[i] = wv / dg[i];
+        c[0] += wv * yv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// first ? p = y, q = z~ : p = y - (xi delta / ep) p, q = z~ - (rho delta /
+// ep) q as Mlt / Add (:108-121)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qmr_pq(int64_t n, const T *__restrict__ y, const T *__restrict__ z,
+                                                   T *__restrict__ p, T *__restrict__ q, const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const double delta = (double)st->delta, ep = (double)st->qm_ep;
+    const T cp = (T)(-(st->qm_xi * delta / ep)), cq = (T)(-(st->qm_rho * delta / ep));
+    GRID_STRIDE(i, n) {
+        if (first) {
+            p[i] = y[i];
+            q[i] = z[i];
+        } else {
+            T pv = p[i] * cp;
+            pv += (T)1 * y[i];
+            p[i] = pv;
+            T qv = q[i] * cq;
+            qv += (T)1 * z[i];
+            q[i] = qv;
+        }
+    }
+}
+
+// v = -beta v + p~; y = M^{-1} v (:139-141)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qmr_vy(int64_t n, const T *__restrict__ pt, const T *__restrict__ dg,
+                                                   T *__restrict__ v, T *__restrict__ y, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T mb = -st->beta;
+    GRID_STRIDE(i, n) {
+        T vv = v[i] * mb;
+        vv += (T)1 * pt[i];
+        v[i] = vv;
+        y[i] = dg ? vv / dg[i] : vv;
+    }
+}
+
+// w = -beta w + z~ (z~ = A^T q, :147-150); sums |y|^2 and |w|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qmr_w(int64_t n, const T *__restrict__ z, const T *__restrict__ y,
+                                                  T *__restrict__ w, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T mb = -st->beta;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T wv = w[i] * mb;
+        wv += (T)1 * z[i];
+        w[i] = wv;
+        const T yv = y[i];
+        c[0] += yv * yv;
+        c[1] += wv * wv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// d = theta_1^2 gamma^2 d + eta p; s likewise with p~; x += d; r -= s
+// (:174-187); sums |r|^2 for the next loop head
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qmr_dsxr(int64_t n, const T *__restrict__ p, const T *__restrict__ pt,
+                                                     T *__restrict__ d, T *__restrict__ s, T *__restrict__ x,
+                                                     T *__restrict__ r, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T tmp = st->qm_tmp, eta = st->qm_eta;
+    const bool ae = eta != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T dv = d[i] * tmp;
+        if (ae) dv += eta * p[i];
+        d[i] = dv;
+        T sv = s[i] * tmp;
+        if (ae) sv += eta * pt[i];
+        s[i] = sv;
+        x[i] += (T)1 * dv;
+        T rv = r[i];
+        rv += -(T)1 * sv;
+        r[i] = rv;
+        c[0] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -1708,6 +1846,68 @@ int tfqmr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration
     return S.finish(it, code);
 }
 
+// Qmr (Qmr.cxx:42-190): per loop (Seldon's two iterations) the scaling of
+// v, y, w with (w, y), the p / q update, p~ = A p with (q, p~), v and y,
+// z~ = A^T q (the device-built transpose) with w, |y| and |w|, the scalar
+// step, and the d / s / x / r update with |r| for the loop head.
+template <typename T>
+int qmr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    shk_csr At = nullptr;
+    SHK_TRY(transpose_of(A, &At));
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 10, 0, it));
+    const int64_t n = S.n;
+    T *r = S.take(n), *y = S.take(n), *z = S.take(n), *v = S.take(n), *w = S.take(n), *pt = S.take(n),
+      *p = S.take(n), *q = S.take(n), *d = S.take(n), *s = S.take(n);
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    const T *zsrc = jac ? z : w;  // M^{-T} w: the identity's is w itself
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (Qmr.cxx:60)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = r;
+    }
+    hipLaunchKernelGGL(k_qmr_init<T>, g, blk, 0, st, n, src, b, dg, x, r, v, y, w, d, s,
+                       it->init_guess_null ? 1 : 0, S.dst, S.partials);
+    S.fin(FinQmrTop<T>{S.dst, 1});
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            hipLaunchKernelGGL(k_qmr_scale<T>, g, blk, 0, cs, n, v, y, w, dg, z, S.dst, S.partials);
+            S.fin(FinQmrDelta<T>{S.dst});
+            hipLaunchKernelGGL(k_qmr_pq<T>, g, blk, 0, cs, n, y, zsrc, p, q, S.dst);
+            SpmvArgs<T> a{};
+            a.x = p; a.y_out = pt; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = q; a.skip = &S.dst->skip;
+            a.partials = S.part[1];
+            SHK_TRY(S.spmv(a, FinDefer{}));
+            grid_finish<T>(cs, (const T *)S.part[1], a.grid_used, (const int *)&S.dst->skip, FinQmrEp<T>{S.dst});
+            hipLaunchKernelGGL(k_qmr_vy<T>, g, blk, 0, cs, n, pt, dg, v, y, S.dst);
+            SpmvArgs<T> a2{};
+            a2.x = q; a2.y_out = z; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+            SHK_TRY((launch_spmv<T, FinNone>(At, a2, FinNone{})));
+            hipLaunchKernelGGL(k_qmr_w<T>, g, blk, 0, cs, n, (const T *)z, (const T *)y, w, S.dst, S.partials);
+            S.fin(FinQmrTheta<T>{S.dst});
+            hipLaunchKernelGGL(k_qmr_dsxr<T>, g, blk, 0, cs, n, p, pt, d, s, x, r, S.dst, S.partials);
+            S.fin(FinQmrTop<T>{S.dst, 0});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "qmr: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -1717,6 +1917,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 3) return cgs_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 4) return bicg_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 5) return tfqmr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 6) return qmr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -1892,6 +2093,11 @@ int shk_bicg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
 int shk_tfqmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(5, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_qmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(6, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

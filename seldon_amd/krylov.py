@@ -532,6 +532,12 @@ def TfQmr(A, x, b, M, it):
     return _solve(L.load().shk_tfqmr, A, x, b, M, it)
 
 
+def Qmr(A, x, b, M, it):
+    """Qmr (computation/solver/iterative/Qmr.cxx:42-190) on the GPU, fused;
+    its A^T products run on the library's device-built transpose."""
+    return _solve(L.load().shk_qmr, A, x, b, M, it)
+
+
 def solve_ir(A64: Matrix, A32: Matrix, x, b, M, tol=1e-10, inner_restart=50, inner_tol=1e-6,
              inner_max_iter=2000, max_refinements=50, inner_method="gmres"):
     """FP32 inner Krylov + FP64 iterative refinement (no Seldon counterpart)."""

@@ -1,11 +1,11 @@
-"""Fused Cgs, BiCg and TfQmr (shk_cgs, shk_bicg, shk_tfqmr) against Seldon's
-own runs and the oracle's restatements of Cgs.cxx:46-137, BiCg.cxx:47-131
-and TfQmr.cxx:45-240.
+"""Fused Cgs, BiCg, TfQmr and Qmr (shk_cgs, shk_bicg, shk_tfqmr, shk_qmr)
+against Seldon's own runs and the oracle's restatements of Cgs.cxx:46-137,
+BiCg.cxx:47-131, TfQmr.cxx:45-240 and Qmr.cxx:42-190.
 
 Bars.  On the overlay program's 24 x 24 upwind system the reference is
 Seldon's own CPU run (tests/golden/overlay_cpu.npz), with the bars of the
-overlay test: BiCg -- same iteration count and numbers, every residual within
-1e-6 relative; Cgs and TfQmr, whose trajectories are rounding-chaotic
+overlay test: BiCg and Qmr -- same iteration count and numbers, every
+residual within 1e-6 relative; Cgs and TfQmr, whose trajectories are rounding-chaotic
 (Seldon's generic, FMA and MKL builds drift apart after 40 and 70
 Finished() calls) -- those first entries within 1e-6 and the count within
 max(2, 1%).  Larger systems against the
@@ -26,7 +26,7 @@ from tests.conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 CHAOTIC = {"cgs": 40, "tfqmr": 70}
-METHODS = ["bicg", "cgs", "tfqmr"]
+METHODS = ["bicg", "cgs", "tfqmr", "qmr"]
 
 
 def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float64, x0=None, storage="row",
@@ -42,7 +42,7 @@ def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float6
     else:
         x = torch.full_like(bg, 3.0)  # overwritten: x.Zero()
     P = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
-    code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr}[method](M, x, bg, P, it)
+    code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr, "qmr": K.Qmr}[method](M, x, bg, P, it)
     n = min(it.hist_len, len(it.history))
     return code, it.nb_iter, it.history[:n].copy(), it.history_it[:n].copy(), x.cpu().numpy()
 
@@ -134,10 +134,11 @@ def test_fused_max_iter_breakdown_and_zero_rhs(gpu, method):
                np.array([v for rw in rows for _, v in rw], np.float64))
     e0 = np.zeros(n)
     e0[0] = 1.0
-    # (TfQmr: its first (rtilde, v) is that product, breakdown 1 at once)
+    # (TfQmr: its first (rtilde, v) is that product, breakdown 1 at once;
+    # Qmr: its first ep = (q, A p), breakdown #4)
     r = orc.solve(method, S, e0, tol=1e-10, max_iter=100)
     code, nb, _, _, x = _gpu(method, S, e0, tol=1e-10, max_iter=100)
-    want = (1, 0) if method == "tfqmr" else (2, 1)
+    want = {"tfqmr": (1, 0), "qmr": (4, 0)}.get(method, (2, 1))
     assert (code, nb) == (r.code, r.nb_iter) == want
     # a zero right-hand side: Init fails; x is left as it was (3.0), except
     # that TfQmr zeroes it before its Init (TfQmr.cxx:61-71)

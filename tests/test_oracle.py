@@ -859,3 +859,53 @@ def test_oracle_tfqmr_fp32_within_kappa_eps():
     err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
     eps32 = float(np.finfo(np.float32).eps)
     assert eps32 / 10 <= err <= 10 * kappa * 1e-5, (err, kappa)
+
+
+# --------------------------------------------------------------- Qmr ----
+
+def test_oracle_qmr_matches_seldon_fixture():
+    """Seldon's own Qmr (Qmr.cxx:42-190, the overlay program's CPU leg) on
+    the 24 x 24 upwind system: history, iteration numbers, count and code,
+    bitwise."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve("qmr", A, b, tol=1e-8, max_iter=2000)
+    assert r.code == int(fx["qmr_cpu_code"]) == 0
+    assert r.nb_iter == int(fx["qmr_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx["qmr_cpu_its"])
+    assert np.array_equal(r.hist, fx["qmr_cpu_hist"])
+
+
+def test_oracle_qmr_closed_forms_and_residual():
+    """A = 2.5 I: one loop gives v = w = 0, eta = |b| / 2.5 and x = b / 2.5
+    exactly, r = 0 at iteration 2.  A diagonal A with four distinct values:
+    at most four loops, x = b / d.  On a general system the residual Qmr
+    carries (r -= s) stays the true one: its last recorded norm matches
+    |b - A x| / |b| to rounding, with the identity and Jacobi -- a slip in
+    the d / s / x updates breaks that at once."""
+    n = 100
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    b = wl.uniform(n, 6)
+    r = orc.solve("qmr", wl.CSR(n, n, *eye, np.full(n, 2.5)), b, tol=1e-12, max_iter=50)
+    assert r.code == 0 and r.nb_iter == 2 and np.allclose(r.x, b / 2.5, rtol=1e-14)
+    d = np.array([1.0, 2.0, 3.5, 7.0])[np.arange(n) % 4]
+    for precond in ("id", "jacobi"):
+        r = orc.solve("qmr", wl.CSR(n, n, *eye, d), b, precond=precond, tol=1e-12, max_iter=50)
+        assert r.code == 0 and r.nb_iter <= 8 and np.allclose(r.x, b / d, rtol=1e-10), (precond, r.nb_iter)
+    A, b, _ = wl.upwind(20)
+    for precond in ("id", "jacobi"):
+        r = orc.solve("qmr", A, b, precond=precond, tol=1e-9, max_iter=3000)
+        assert r.code == 0
+        true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+        assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15, (precond, true, r.hist[-1])
+
+
+def test_oracle_qmr_fp32_within_kappa_eps():
+    A, b, _ = wl.upwind(16)
+    r64 = orc.solve("qmr", A, b, precond="jacobi", tol=1e-13, max_iter=4000)
+    r32 = orc.solve("qmr", A, b, precond="jacobi", tol=1e-5, max_iter=4000, dtype=np.float32)
+    assert r64.code == 0 and r32.code == 0
+    kappa = np.linalg.cond(A.to_scipy().toarray())
+    err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert eps32 / 10 <= err <= 10 * kappa * 1e-5, (err, kappa)
