@@ -348,7 +348,7 @@ def cg_config1(K, wl, np, torch, cpu=True):
 
 
 def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
-    """FP64 BiCgStab, Gmres(30), Cgs, BiCg, TfQmr and Qmr with the Jacobi preconditioner
+    """FP64 BiCgStab, Gmres(30), Cgs, BiCg, TfQmr, Qmr, Cgne and Lsqr with the Jacobi preconditioner
     on the 128^3 7-point upwind convection-diffusion matrix (2,097,152 rows),
     to a 1e-10 relative residual: time to solution and iterations/s, plus the
     SpMV bandwidth on this matrix and the time of one symmetric SOR sweep
@@ -391,7 +391,8 @@ def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
     torch.cuda.synchronize()
     bicg_first = time.perf_counter() - t0
     for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30), ("cgs", K.Cgs, None),
-                              ("bicg", K.BiCg, None), ("tfqmr", K.TfQmr, None), ("qmr", K.Qmr, None)):
+                              ("bicg", K.BiCg, None), ("tfqmr", K.TfQmr, None), ("qmr", K.Qmr, None),
+                              ("cgne", K.Cgne, None), ("lsqr", K.Lsqr, None)):
         x = torch.zeros_like(bg)
         it = K.Iteration(20000, 1e-10)
         if restart:
@@ -408,8 +409,8 @@ def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
         if name == "bicg":
             out[name]["first_call_seconds_with_transpose_build"] = round(bicg_first, 4)
         log(f"convdiff128 {name}: {dt:.3f}s, {it.nb_iter} its, code {code}, true relres {rel:.2e}")
-        if cpu and name in ("bicg", "qmr"):
-            # Seldon's BiCg and Qmr take A^T products through the generic transposed
+        if cpu and name in ("bicg", "qmr", "cgne", "lsqr"):
+            # Seldon's BiCg, Qmr, Cgne and Lsqr take A^T products through the generic transposed
             # product (Functions_MatVect.cxx:1279-1316), a dense loop over
             # every (row, column) pair: 4.4e12 terms per product here
             out[name]["cpu_oracle"] = {"skipped": "Mlt(SeldonTrans) on RowSparse is the generic dense loop, "

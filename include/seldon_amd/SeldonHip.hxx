@@ -15,15 +15,15 @@
 // are what Seldon's solver templates call unqualified
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
-// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr below replace the whole loop by the
+// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr / Cgne / Lsqr below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
 // JacobiPreconditioner.  Every call goes through the C ABI of include/shk.h.
 //
 // The transposed products (MltAdd / Mlt with SeldonTrans) and Swap are
-// overloaded too, so the transpose-using solvers (Cgne, Lsqr, and BiCg / Qmr
-// with any other preconditioner) also run on the GPU storages through
+// overloaded too, so the transpose-using solvers (BiCg, Qmr, Cgne, Lsqr with
+// any other preconditioner type) also run on the GPU storages through
 // Seldon's own templates.
 //
 // Names: VectHip, RowSparseHip, RowSymSparseHip, ArrayRowSparseHip are the
@@ -652,7 +652,9 @@ namespace Seldon
     else if (which == 3) HipCheck(shk_cgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cgs");
     else if (which == 4) HipCheck(shk_bicg(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCg");
     else if (which == 5) HipCheck(shk_tfqmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "TfQmr");
-    else HipCheck(shk_qmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "Qmr");
+    else if (which == 6) HipCheck(shk_qmr(ctx, A, x, b, (shk_precond)prec, &it, &code), "Qmr");
+    else if (which == 7) HipCheck(shk_cgne(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cgne");
+    else HipCheck(shk_lsqr(ctx, A, x, b, (shk_precond)prec, &it, &code), "Lsqr");
     iter.*Acc::Fac() = Titer(it.facteur_reste);
     const int n = std::min(it.hist_len, it.hist_cap);
     for (int k = 0; k < n; k++)
@@ -664,7 +666,7 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[7][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
+        static const char* what[9][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
                                           "Bicgstab breakdown #3", "", "", ""},
                                          {"", "", "", "", "", "", ""},
@@ -674,7 +676,10 @@ namespace Seldon
                                           "Tfqmr breakdown: tau=0", "Tfqmr breakdown: tau=0",
                                           "tfqmr breakdown: beta=0", ""},
                                          {"", "Qmr breakdown #1", "Qmr breakdown #2", "Qmr breakdown #3",
-                                          "Qmr breakdown #4", "Qmr breakdown #5", "Qmr breakdown #6"}};
+                                          "Qmr breakdown #4", "Qmr breakdown #5", "Qmr breakdown #6"},
+                                         {"", "Cgne breakdown #1", "Cgne breakdown #2", "", "", "", ""},
+                                         {"", "Lsqr breakdown #1", "Lsqr breakdown #2", "Lsqr breakdown #3",
+                                          "", "", ""}};
         const int c = it.error_code;
         iter.Fail(c, (c >= 0 && c < 7) ? what[which][c] : "breakdown");
       }
@@ -737,6 +742,24 @@ namespace Seldon
       Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(6, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  // Cgne.cxx:40-42 and Lsqr.cxx:40-42 (Lsqr applies no preconditioner, in
+  // the template as here)
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  Cgne(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+       Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(7, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  Lsqr(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+       Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(8, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>

@@ -316,6 +316,17 @@ int shk_tfqmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec
  * scalars the value type.  Error codes 1-6 are its breakdowns #1-#6. */
 int shk_qmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
             shk_iteration *iter, int *code);
+/* Cgne (computation/solver/iterative/Cgne.cxx:41-124), fused likewise: Cg
+ * on A^T A x = A^T b with M M^T as the preconditioner, the A^T products on
+ * the device-built transpose; two iterations counted per loop. */
+int shk_cgne(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+             shk_iteration *iter, int *code);
+/* Lsqr (computation/solver/iterative/Lsqr.cxx:41-150), fused likewise.  As
+ * the template, it applies no preconditioner (prec is accepted and
+ * ignored) and restarts from x = 0 whatever the initial guess (which only
+ * sets the first residual norm).  Error codes 1-3 are its breakdowns. */
+int shk_lsqr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+             shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

@@ -811,6 +811,128 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- Cgne: computation/solver/iterative/Cgne.cxx:41-124 ------------ *
+ * Cg on A^T A x = A^T b with M M^T as the preconditioner: z = M^{-1}
+ * M^{-T} r; the stopping test on |r| (dp, Titer = double), first on
+ * |A^T b|; two iterations counted per loop. */
+int FN(cgne)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+             orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho = 1, rho_1 = 0, alpha, beta, delta;
+    double dp;
+    T *p = FN(dup)(N, b), *q = FN(dup)(N, b), *r = FN(dup)(N, b), *z = FN(dup)(N, b);
+    FN(mlt_trans)(A, b, q);
+    if (FN(iter_init)(it, N, q) != 0) goto done;
+    if (!it->init_guess_null) {
+        FN(mlt)(A, x, p);
+        FN(mlt_trans)(A, p, r);
+        FN(scal)(N, (T)-1, r);
+        FN(add)(N, (T)1, q, r);
+    } else {
+        FN(copy)(N, q, r);
+        memset(x, 0, sizeof(T) * (size_t)N);
+    }
+    dp = (double)FN(norm2)(N, q);
+    it->nb_iter = 0;
+    while (!orc_finished(it, dp, it->nb_iter, it->max_iter)) {
+        FN(precond)(precond, N, diag, r, q);  /* TransSolve */
+        FN(precond)(precond, N, diag, q, z);
+        rho = FN(dot)(N, r, z);
+        if (rho == (T)0) { orc_fail(it, 1); break; }
+        if (it->nb_iter == 0) {
+            FN(copy)(N, z, p);
+        } else {
+            beta = rho / rho_1;
+            FN(scal)(N, beta, p);
+            FN(add)(N, (T)1, z, p);
+        }
+        FN(mlt)(A, p, q);
+        FN(mlt_trans)(A, q, z);
+        delta = FN(dot)(N, p, z);
+        if (delta == (T)0) { orc_fail(it, 2); break; }
+        alpha = rho / delta;
+        FN(add)(N, alpha, p, x);
+        FN(add)(N, -alpha, z, r);
+        rho_1 = rho;
+        dp = (double)FN(norm2)(N, r);
+        it->nb_iter += 2;
+    }
+done:
+    free(p); free(q); free(r); free(z);
+    return orc_error_code(it);
+}
+
+/* ---- Lsqr: computation/solver/iterative/Lsqr.cxx:41-150 ------------ *
+ * Golub-Kahan bidiagonalisation.  The template never applies M, starts
+ * from x = 0 whatever the guess (only the first residual norm sees x),
+ * and types its scalars as: rho, rho_bar, phi, phi_bar, theta, c, s, tmp
+ * the value type T; beta, alpha, rnorm Titer = double. */
+int FN(lsqr)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+             orc_iter *it)
+{
+    (void)precond; (void)diag;
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho, rho_bar, phi, phi_bar, theta, c, s, tmp;
+    double beta, alpha, rnorm;
+    T *v = FN(dup)(N, b), *v1 = FN(dup)(N, b), *u = FN(dup)(N, b), *u1 = FN(dup)(N, b),
+      *w = FN(dup)(N, b);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, u);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, u);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    rnorm = (double)FN(norm2)(N, u);
+    FN(copy)(N, b, u);
+    beta = (double)FN(norm2)(N, u);
+    tmp = (T)(1.0 / beta);
+    FN(scal)(N, tmp, u);
+    FN(mlt_trans)(A, u, v);
+    alpha = (double)FN(norm2)(N, v);
+    tmp = (T)(1.0 / alpha);
+    FN(scal)(N, tmp, v);
+    FN(copy)(N, v, w);
+    memset(x, 0, sizeof(T) * (size_t)N);
+    phi_bar = (T)beta;
+    rho_bar = (T)alpha;
+    it->nb_iter = 0;
+    while (!orc_finished(it, rnorm, it->nb_iter, it->max_iter)) {
+        FN(mlt)(A, v, u1);
+        FN(add)(N, (T)(-alpha), u, u1);
+        beta = (double)FN(norm2)(N, u1);
+        if (beta == 0.0) { orc_fail(it, 1); break; }
+        tmp = (T)(1.0 / beta);
+        FN(scal)(N, tmp, u1);
+        FN(mlt_trans)(A, u1, v1);
+        FN(add)(N, (T)(-beta), v, v1);
+        alpha = (double)FN(norm2)(N, v1);
+        if (alpha == 0.0) { orc_fail(it, 2); break; }
+        tmp = (T)(1.0 / alpha);
+        FN(scal)(N, tmp, v1);
+        rho = (T)sqrt((double)(rho_bar * rho_bar) + beta * beta);
+        if (rho == (T)0) { orc_fail(it, 3); break; }
+        c = rho_bar / rho;
+        s = (T)(beta / (double)rho);
+        theta = (T)((double)s * alpha);
+        rho_bar = (T)((double)(-c) * alpha);
+        phi = c * phi_bar;
+        phi_bar = s * phi_bar;
+        tmp = phi / rho;
+        FN(add)(N, tmp, w, x);
+        tmp = -theta / rho;
+        FN(scal)(N, tmp, w);
+        FN(add)(N, (T)1, v1, w);
+        rnorm = (double)(phi_bar < 0 ? -phi_bar : phi_bar);
+        { T *t_ = u1; u1 = u; u = t_; }
+        { T *t_ = v1; v1 = v; v = t_; }
+        it->nb_iter += 2;
+    }
+done:
+    free(v); free(v1); free(u); free(u1); free(w);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

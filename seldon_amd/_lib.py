@@ -94,6 +94,8 @@ SIGNATURES = [
     ("shk_bicg", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_tfqmr", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_qmr", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_cgne", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_lsqr", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_gmres", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_solve_ir", _I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ShkIrParams), _PI]),
     ("shk_sddmm", _I, [_P, _P, _P, _P, _P]),

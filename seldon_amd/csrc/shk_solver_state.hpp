@@ -32,6 +32,10 @@ struct KState {
     // and the d / s scale theta_1^2 gamma^2 (value type)
     double qm_rho, qm_xi;
     T qm_ep, qm_theta, qm_gamma, qm_eta, qm_tmp;
+    // Lsqr: beta, alpha, rnorm (Titer = double); rho_bar, phi_bar and the
+    // x / w factors phi / rho, -theta / rho (value type)
+    double ls_beta, ls_alpha, ls_rnorm;
+    T ls_rho_bar, ls_phi_bar, ls_xs, ls_ws;
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -441,6 +445,140 @@ struct FinQmrTheta {
         st->qm_eta = (T)((double)(-st->qm_eta) * rho_1 * (double)gamma * (double)gamma /
                          (double)(beta * gamma_1 * gamma_1));
         st->qm_tmp = theta_1 * theta_1 * gamma * gamma;
+    }
+};
+
+// -------------------------------------------------------------- Cgne ----
+// Init(q) with q = A^T b, and the first Finished(|q|) (Cgne.cxx:55-81)
+template <typename T>
+struct FinCgneInit {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (!it_init(st, s[0])) return;
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+// rho = (r, z); breakdown #1; beta (:88-101)
+template <typename T>
+struct FinCgneRho {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->rho = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        if (st->nb_iter != 0) st->beta = s[0] / st->rho_1;
+    }
+};
+
+// delta = (p, z) with z = A^T A p; breakdown #2; alpha (:104-113)
+template <typename T>
+struct FinCgneDelta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->delta = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 2);
+            return;
+        }
+        st->alpha = st->rho / s[0];
+    }
+};
+
+// rho_1 = rho; dp = |r|; two ++iter; Finished(dp) (:115-123, :82)
+template <typename T>
+struct FinCgneTop {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->rho_1 = st->rho;
+        st->nb_iter += 2;
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+// -------------------------------------------------------------- Lsqr ----
+// rnorm = |b - A x| (s[0]) and beta = |b| (s[1]) (Lsqr.cxx:57-67)
+template <typename T>
+struct FinLsqrNorms {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->ls_rnorm = (double)sqrt_t(s[0]);
+        st->ls_beta = (double)sqrt_t(s[1]);
+    }
+};
+
+// alpha = |A^T u|; phi_bar = beta; rho_bar = alpha; the first
+// Finished(rnorm) (:70-80)
+template <typename T>
+struct FinLsqrStart {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double alpha = (double)sqrt_t(s[0]);
+        st->ls_alpha = alpha;
+        st->ls_phi_bar = (T)st->ls_beta;
+        st->ls_rho_bar = (T)alpha;
+        st->nb_iter = 0;
+        if (it_finished(st, st->ls_rnorm, st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+// beta = |u1|; breakdown #1 (:86-92)
+template <typename T>
+struct FinLsqrBeta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double beta = (double)sqrt_t(s[0]);
+        st->ls_beta = beta;
+        if (beta == 0.0) it_fail(st, 1);
+    }
+};
+
+// alpha = |v1|; breakdown #2; the rotation (rho, breakdown #3, c, s, theta,
+// rho_bar, phi, phi_bar) and the x / w factors, each in the template's
+// types; rnorm = |phi_bar|; two ++iter; Finished(rnorm), whose end records
+// its slot: the x update of this loop still runs (:97-145)
+template <typename T>
+struct FinLsqrAlpha {
+    KState<T> *st;
+    int slot;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double alpha = (double)sqrt_t(s[0]), beta = st->ls_beta;
+        st->ls_alpha = alpha;
+        if (alpha == 0.0) {
+            it_fail(st, 2);
+            return;
+        }
+        const T rho_bar = st->ls_rho_bar;
+        const T rho = (T)sqrt((double)(rho_bar * rho_bar) + beta * beta);
+        if (rho == (T)0) {
+            it_fail(st, 3);
+            return;
+        }
+        const T c = rho_bar / rho;
+        const T sn = (T)(beta / (double)rho);
+        const T theta = (T)((double)sn * alpha);
+        st->ls_rho_bar = (T)((double)(-c) * alpha);
+        const T phi = c * st->ls_phi_bar;
+        const T phi_bar = sn * st->ls_phi_bar;
+        st->ls_phi_bar = phi_bar;
+        st->ls_xs = phi / rho;
+        st->ls_ws = -theta / rho;
+        st->ls_rnorm = (double)(phi_bar < (T)0 ? -phi_bar : phi_bar);
+        st->nb_iter += 2;
+        if (it_finished(st, st->ls_rnorm, st->nb_iter, st->max_iter)) {
+            st->stop_slot = slot;
+            it_stop(st);
+        }
     }
 };
 

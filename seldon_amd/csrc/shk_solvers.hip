@@ -1059,6 +1059,195 @@ __global__ __launch_bounds__(kBlock) void k_qmr_dsxr(int64_t n, const T *__restr
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// -------------------------------------------------------------- Cgne ----
+// |q|^2 of q = A^T b (Iteration::Init and the first stopping test)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, const T *__restrict__ q, const KState<T> *st,
+                                                  T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        const T v = q[i];
+        c[0] += v * v;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// start ? r = q, x = 0 : r = -(A^T A x) + q (Cgne.cxx:64-78; r holds A^T A x)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgne_r0(int64_t n, const T *__restrict__ q, T *__restrict__ r,
+                                                    T *__restrict__ x, int zero_x, const KState<T> *st)
+{
+    if (st->skip) return;
+    GRID_STRIDE(i, n) {
+        if (zero_x) {
+            r[i] = q[i];
+            x[i] = (T)0;
+        } else {
+            T rv = r[i] * (T)-1;
+            rv += (T)1 * q[i];
+            r[i] = rv;
+        }
+    }
+}
+
+// z = M^{-1} M^{-T} r (two divisions by a_ii with Jacobi, :85-86); sums
+// (r, z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgne_z(int64_t n, const T *__restrict__ r, const T *__restrict__ dg,
+                                                   T *__restrict__ z, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        const T rv = r[i];
+        T zv = rv;
+        if (dg) {
+            const T qv = rv / dg[i];
+            zv = qv / dg[i];
+            z[i] = zv;
+        }
+        c[0] += rv * zv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// first ? p = z : p = beta p + z (:94-101)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgne_p(int64_t n, const T *__restrict__ z, T *__restrict__ p,
+                                                   const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T beta = st->beta;
+    GRID_STRIDE(i, n) {
+        if (first) {
+            p[i] = z[i];
+        } else {
+            T pv = p[i] * beta;
+            pv += (T)1 * z[i];
+            p[i] = pv;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha z (z = A^T A p, :114-115); sums |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cgne_xr(int64_t n, const T *__restrict__ p, const T *__restrict__ z,
+                                                    T *__restrict__ x, T *__restrict__ r, const KState<T> *st,
+                                                    T *partials)
+{
+    if (st->skip) return;
+    const T alpha = st->alpha, malpha = -alpha;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (aa) x[i] += alpha * p[i];
+        T rv = r[i];
+        if (am) {
+            rv += malpha * z[i];
+            r[i] = rv;
+        }
+        c[0] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// -------------------------------------------------------------- Lsqr ----
+// sums |u|^2 (u = b - A x, or b) and |b|^2 (Lsqr.cxx:57-65)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lsqr_norms(int64_t n, const T *__restrict__ u, const T *__restrict__ b,
+                                                       const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        const T uv = u[i], bv = b[i];
+        c[0] += uv * uv;
+        c[1] += bv * bv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// u = b * (1 / beta); x = 0 (:66-75: the template zeroes x whatever the
+// initial guess)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lsqr_u0(int64_t n, const T *__restrict__ b, T *__restrict__ u,
+                                                    T *__restrict__ x, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T t = (T)(1.0 / st->ls_beta);
+    GRID_STRIDE(i, n) {
+        u[i] = b[i] * t;
+        x[i] = (T)0;
+    }
+}
+
+// v *= 1 / alpha; w = v (:71-74)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lsqr_v0(int64_t n, T *__restrict__ v, T *__restrict__ w,
+                                                    const KState<T> *st)
+{
+    if (st->skip) return;
+    const T t = (T)(1.0 / st->ls_alpha);
+    GRID_STRIDE(i, n) {
+        const T vv = v[i] * t;
+        v[i] = vv;
+        w[i] = vv;
+    }
+}
+
+// y += (-c) x with c a double scale (Add(-alpha, u, u1), Add(-beta, v, v1));
+// sums |y|^2 (:84-86, :95-97)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lsqr_sub(int64_t n, const T *__restrict__ xv, T *__restrict__ y,
+                                                     int which, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T m = (T)(-(which ? st->ls_beta : st->ls_alpha));
+    const bool am = m != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T yv = y[i];
+        if (am) {
+            yv += m * xv[i];
+            y[i] = yv;
+        }
+        c[0] += yv * yv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// u1 *= 1 / beta (:92-93)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lsqr_scale(int64_t n, T *__restrict__ y, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T t = (T)(1.0 / st->ls_beta);
+    GRID_STRIDE(i, n) y[i] = y[i] * t;
+}
+
+// v1 *= 1 / alpha; x += (phi / rho) w; w = (-theta / rho) w + v1
+// (:103-104, :122-128); also when this loop's Finished ended the solve
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_lsqr_xw(int64_t n, T *__restrict__ v1, T *__restrict__ w,
+                                                    T *__restrict__ x, const KState<T> *st, int slot)
+{
+    if (st->skip && st->stop_slot != slot) return;
+    const T t = (T)(1.0 / st->ls_alpha), xs = st->ls_xs, ws = st->ls_ws;
+    const bool ax = xs != (T)0;
+    GRID_STRIDE(i, n) {
+        const T vv = v1[i] * t;
+        v1[i] = vv;
+        const T wv = w[i];
+        if (ax) x[i] += xs * wv;
+        T wn = wv * ws;
+        wn += (T)1 * vv;
+        w[i] = wn;
+    }
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -1908,6 +2097,131 @@ int qmr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *
     return S.finish(it, code);
 }
 
+// Cgne (Cgne.cxx:41-124): Cg on A^T A x = A^T b; per loop z = M^{-1}
+// M^{-T} r with (r, z), the p update, q = A p, z = A^T q with (p, z)
+// (the device-built transpose), the x / r update with |r|.
+template <typename T>
+int cgne_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    shk_csr At = nullptr;
+    SHK_TRY(transpose_of(A, &At));
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 4, 0, it));
+    const int64_t n = S.n;
+    T *p = S.take(n), *q = S.take(n), *r = S.take(n), *z = S.take(n);
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    const T *zsrc = jac ? z : r;  // the identity's M^{-1} M^{-T} r is r itself
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    SpmvArgs<T> a0{};
+    a0.x = b; a0.y_out = q; a0.alpha = (T)1; a0.mlt = 1; a0.skip = &S.dst->skip;
+    SHK_TRY((launch_spmv<T, FinNone>(At, a0, FinNone{})));
+    hipLaunchKernelGGL(k_sumsq<T>, g, blk, 0, st, n, (const T *)q, S.dst, S.partials);
+    S.fin(FinCgneInit<T>{S.dst});
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_out = p; a.alpha = (T)1; a.mlt = 1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        SpmvArgs<T> a2{};
+        a2.x = p; a2.y_out = r; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+        SHK_TRY((launch_spmv<T, FinNone>(At, a2, FinNone{})));
+    }
+    hipLaunchKernelGGL(k_cgne_r0<T>, g, blk, 0, st, n, (const T *)q, r, x, it->init_guess_null ? 1 : 0, S.dst);
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            hipLaunchKernelGGL(k_cgne_z<T>, g, blk, 0, cs, n, (const T *)r, dg, z, S.dst, S.partials);
+            S.fin(FinCgneRho<T>{S.dst});
+            hipLaunchKernelGGL(k_cgne_p<T>, g, blk, 0, cs, n, zsrc, p, S.dst);
+            SpmvArgs<T> a{};
+            a.x = p; a.y_out = q; a.alpha = (T)1; a.mlt = 1; a.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a, FinNone{}));
+            SpmvArgs<T> a2{};
+            a2.x = q; a2.y_out = z; a2.alpha = (T)1; a2.mlt = 1;
+            a2.dot0 = kDotVec; a2.u0 = p; a2.skip = &S.dst->skip;
+            a2.partials = S.part[1];
+            SHK_TRY((launch_spmv<T, FinDefer>(At, a2, FinDefer{})));
+            grid_finish<T>(cs, (const T *)S.part[1], a2.grid_used, (const int *)&S.dst->skip,
+                           FinCgneDelta<T>{S.dst});
+            hipLaunchKernelGGL(k_cgne_xr<T>, g, blk, 0, cs, n, (const T *)p, (const T *)z, x, r, S.dst,
+                               S.partials);
+            S.fin(FinCgneTop<T>{S.dst});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "cgne: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
+// Lsqr (Lsqr.cxx:41-150): per loop u1 = A v, u1 -= alpha u with |u1|, the
+// scaling, v1 = A^T u1 (the device-built transpose), v1 -= beta v with
+// |v1|, the rotation's scalars, and the x / w update; u / u1 and v / v1
+// swap roles from loop to loop (Swap).  The template applies no
+// preconditioner, so none is applied here either.
+template <typename T>
+int lsqr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, shk_iteration *it, int *code)
+{
+    shk_csr At = nullptr;
+    SHK_TRY(transpose_of(A, &At));
+    Solve<T> S;
+    SHK_TRY(S.setup(ctx, A, 5, 0, it));
+    const int64_t n = S.n;
+    T *u = S.take(n), *u1 = S.take(n), *v = S.take(n), *v1 = S.take(n), *w = S.take(n);
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (Lsqr.cxx:55)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = u; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = u;
+    }
+    hipLaunchKernelGGL(k_lsqr_norms<T>, g, blk, 0, st, n, src, b, S.dst, S.partials);
+    S.fin(FinLsqrNorms<T>{S.dst});
+    hipLaunchKernelGGL(k_lsqr_u0<T>, g, blk, 0, st, n, b, u, x, S.dst);
+    {
+        SpmvArgs<T> a{};
+        a.x = u; a.y_out = v; a.alpha = (T)1; a.mlt = 1; a.dot0 = kDotSelf; a.skip = &S.dst->skip;
+        a.partials = S.part[1];
+        SHK_TRY((launch_spmv<T, FinDefer>(At, a, FinDefer{})));
+        grid_finish<T>(st, (const T *)S.part[1], a.grid_used, (const int *)&S.dst->skip, FinLsqrStart<T>{S.dst});
+    }
+    hipLaunchKernelGGL(k_lsqr_v0<T>, g, blk, 0, st, n, v, w, S.dst);
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            // Swap(u1, u); Swap(v1, v) after every loop: even slots use the
+            // first assignment, odd ones the other (a chunk has an even count)
+            T *uc = (k & 1) ? u1 : u, *un = (k & 1) ? u : u1;
+            T *vc = (k & 1) ? v1 : v, *vn = (k & 1) ? v : v1;
+            SpmvArgs<T> a{};
+            a.x = vc; a.y_out = un; a.alpha = (T)1; a.mlt = 1; a.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a, FinNone{}));
+            hipLaunchKernelGGL(k_lsqr_sub<T>, g, blk, 0, cs, n, (const T *)uc, un, 0, S.dst, S.partials);
+            S.fin(FinLsqrBeta<T>{S.dst});
+            hipLaunchKernelGGL(k_lsqr_scale<T>, g, blk, 0, cs, n, un, S.dst);
+            SpmvArgs<T> a2{};
+            a2.x = un; a2.y_out = vn; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+            SHK_TRY((launch_spmv<T, FinNone>(At, a2, FinNone{})));
+            hipLaunchKernelGGL(k_lsqr_sub<T>, g, blk, 0, cs, n, (const T *)vc, vn, 1, S.dst, S.partials);
+            S.fin(FinLsqrAlpha<T>{S.dst, k});
+            hipLaunchKernelGGL(k_lsqr_xw<T>, g, blk, 0, cs, n, vn, w, x, S.dst, k);
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "lsqr: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -1918,6 +2232,8 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 4) return bicg_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 5) return tfqmr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 6) return qmr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 7) return cgne_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 8) return lsqr_impl<T>(ctx, A, (T *)dx, (const T *)db, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -2098,6 +2414,16 @@ int shk_tfqmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec
 int shk_qmr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(6, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_cgne(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(7, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_lsqr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(8, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

@@ -538,6 +538,17 @@ def Qmr(A, x, b, M, it):
     return _solve(L.load().shk_qmr, A, x, b, M, it)
 
 
+def Cgne(A, x, b, M, it):
+    """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
+    return _solve(L.load().shk_cgne, A, x, b, M, it)
+
+
+def Lsqr(A, x, b, M, it):
+    """Lsqr (computation/solver/iterative/Lsqr.cxx:41-150) on the GPU, fused;
+    like the template it applies no preconditioner."""
+    return _solve(L.load().shk_lsqr, A, x, b, M, it)
+
+
 def solve_ir(A64: Matrix, A32: Matrix, x, b, M, tol=1e-10, inner_restart=50, inner_tol=1e-6,
              inner_max_iter=2000, max_refinements=50, inner_method="gmres"):
     """FP32 inner Krylov + FP64 iterative refinement (no Seldon counterpart)."""

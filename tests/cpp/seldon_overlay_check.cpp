@@ -14,10 +14,10 @@
 // Symmlq, CoCg, BiCgcr, QmrSym, and Cg with Seldon's SorPreconditioner
 // (SSOR) -- with Seldon's own templates, once on the CPU storages and once on
 // the GPU storages (Mlt(SeldonTrans, ...), SOR, Swap and the rest through the
-// overload set; BiCg, Qmr, Cgs and TfQmr with Preconditioner_Base reach
-// the library's fused solvers, and once more, with a user preconditioner
-// type, Seldon's templates: labels bicg_gpu_generic, qmr_gpu_generic,
-// cgs_gpu_generic, tfqmr_gpu_generic),
+// overload set; BiCg, Qmr, Cgne, Lsqr, Cgs and TfQmr with
+// Preconditioner_Base reach the library's fused solvers, and once more,
+// with a user preconditioner type, Seldon's templates: labels
+// <solver>_gpu_generic),
 // and prints each residual history (Iteration::Finished lines) after a
 // "== <label>" marker, then "ERRCODE e NBITER n".
 #include "Seldon.hxx"
@@ -231,7 +231,7 @@ int main(int argc, char** argv)
         const int maxit = (k == 2 || k == 3) ? 4000 : 2000;
         if (cpu) run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
         if (gpu) run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
-        if (gpu && (k == 0 || k == 1 || k == 4 || k == 6))
+        if (gpu && k != 5 && k < 7)
           run_trans_solver<Matrix<double, General, RowSparseHip, HipAlloc<double> >,
                            Vector<double, VectHip, HipAlloc<double> >, UserIdentity>(solvers[k], "gpu_generic",
                                                                                    Cg_, bcg, maxit, 1e-8);

@@ -909,3 +909,81 @@ def test_oracle_qmr_fp32_within_kappa_eps():
     err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
     eps32 = float(np.finfo(np.float32).eps)
     assert eps32 / 10 <= err <= 10 * kappa * 1e-5, (err, kappa)
+
+
+# ------------------------------------------------------- Cgne and Lsqr ----
+
+@pytest.mark.parametrize("method", ["cgne", "lsqr"])
+def test_oracle_cgne_lsqr_match_seldon_fixture(method):
+    """Seldon's own Cgne (Cgne.cxx:41-124) and Lsqr (Lsqr.cxx:41-150), the
+    overlay program's CPU legs on the 24 x 24 upwind system (4000
+    iterations allowed): history, iteration numbers, count and code,
+    bitwise."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve(method, A, b, tol=1e-8, max_iter=4000)
+    assert r.code == int(fx[f"{method}_cpu_code"]) == 0
+    assert r.nb_iter == int(fx[f"{method}_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx[f"{method}_cpu_its"])
+    assert np.array_equal(r.hist, fx[f"{method}_cpu_hist"])
+
+
+def test_oracle_cgne_is_cg_on_the_normal_equations():
+    """Cgne's loop is Cg on A^T A x = A^T b (two iterations per loop): its
+    history at 2k matches the oracle's Cg on the explicit A^T A with A^T b
+    (as a CSR matrix) up to rounding, with the identity; its solution is
+    A^-1 b.  A = 2.5 I takes one loop."""
+    A, b, xt = wl.upwind(10)
+    D = A.to_scipy().toarray()
+    N = D.T @ D
+    rows, cols = np.nonzero(N)
+    ptr = np.searchsorted(rows, np.arange(A.m + 1)).astype(np.int32)
+    AtA = wl.CSR(A.m, A.m, ptr, cols.astype(np.int32), N[rows, cols])
+    atb = orc.mlt_trans(A, b) if hasattr(orc, "mlt_trans") else D.T @ b
+    cg = orc.solve("cg", AtA, atb, tol=1e-10, max_iter=2000)
+    ne = orc.solve("cgne", A, b, tol=1e-10, max_iter=4000)
+    assert cg.code == ne.code == 0
+    k = min(30, len(cg.hist), len(ne.hist))
+    assert np.array_equal(ne.hist_it[:k], 2 * cg.hist_it[:k])
+    assert np.max(np.abs(ne.hist[:k] - cg.hist[:k]) / cg.hist[:k]) < 1e-8
+    assert np.allclose(ne.x, xt, rtol=0, atol=1e-7)
+    n = 50
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    bb = wl.uniform(n, 2)
+    r = orc.solve("cgne", wl.CSR(n, n, *eye, np.full(n, 2.5)), bb, tol=1e-12, max_iter=20)
+    assert r.code == 0 and r.nb_iter == 2 and np.allclose(r.x, bb / 2.5, rtol=1e-14)
+
+
+def test_oracle_lsqr_residual_estimate_and_breakdown():
+    """Lsqr's rnorm = |phi_bar| is the true residual norm: at the end it
+    matches |b - A x| / |b| to rounding and the solution is A^-1 b.  A = 2.5 I
+    takes one loop.  With A = 2 I and b = e_0 every step is exact, so the
+    first u1 = A v - alpha u is exactly 0: breakdown #1 with x still zero, as
+    the template has it."""
+    A, b, xt = wl.upwind(12)
+    r = orc.solve("lsqr", A, b, tol=1e-10, max_iter=6000)
+    assert r.code == 0
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15, (true, r.hist[-1])
+    assert np.allclose(r.x, xt, rtol=0, atol=1e-6)
+    n = 40
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    bb = wl.uniform(n, 3)
+    r = orc.solve("lsqr", wl.CSR(n, n, *eye, np.full(n, 2.5)), bb, tol=1e-12, max_iter=20)
+    assert r.code == 0 and r.nb_iter == 2 and np.allclose(r.x, bb / 2.5, rtol=1e-14)
+    e0 = np.zeros(n)
+    e0[0] = 1.0
+    r = orc.solve("lsqr", wl.CSR(n, n, *eye, np.full(n, 2.0)), e0, tol=1e-12, max_iter=20)
+    assert r.code == 1 and r.nb_iter == 0 and np.all(r.x == 0)
+
+
+@pytest.mark.parametrize("method", ["cgne", "lsqr"])
+def test_oracle_cgne_lsqr_fp32_within_kappa_eps(method):
+    A, b, _ = wl.upwind(8)
+    r64 = orc.solve(method, A, b, tol=1e-13, max_iter=8000)
+    r32 = orc.solve(method, A, b, tol=1e-5, max_iter=8000, dtype=np.float32)
+    assert r64.code == 0 and r32.code == 0
+    kappa = np.linalg.cond(A.to_scipy().toarray())
+    err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
+    eps32 = float(np.finfo(np.float32).eps)
+    assert eps32 / 10 <= err <= 10 * kappa ** 2 * 1e-5, (err, kappa)
