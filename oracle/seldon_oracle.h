@@ -4,13 +4,14 @@
  * A plain-C CPU restatement of Seldon's sparse Krylov path (SpMV on the
  * RowSparse / RowSymSparse / ArrayRowSparse storages and its transposed
  * forms, the BLAS-1 steps, the Iteration stopping logic, Cg, BiCgStab,
- * Gmres, Cgs, BiCg, TfQmr and Qmr, SOR / SSOR,
+ * Gmres, Cgs, BiCg, TfQmr, Qmr, Cgne and Lsqr, SOR / SSOR,
  * ConvertMatrix_from_Coordinates): the oracle the HIP library's results are
- * compared with.  Cgs, BiCg, TfQmr and Qmr are pinned bitwise to Seldon's
- * own runs of them (the overlay program's CPU legs), and by closed one-loop
- * cases, BiCg following Cg on an SPD matrix, TfQmr's quasi-residual bound
- * and Qmr's carried residual; their FP32 versions by kappa(A) eps_32.  The
- * FP32 family (orc_s_*; Gmres with FP32 vectors and
+ * compared with.  Cgs, BiCg, TfQmr, Qmr, Cgne and Lsqr are pinned bitwise to
+ * Seldon's own runs of them (the overlay program's CPU legs), and by closed
+ * one-loop cases, BiCg following Cg on an SPD matrix, TfQmr's
+ * quasi-residual bound, Qmr's carried residual, Cgne as Cg on the explicit
+ * A^T A and Lsqr's residual estimate; their FP32 versions by kappa(A)
+ * eps_32.  The FP32 family (orc_s_*; Gmres with FP32 vectors and
  * an FP64 Hessenberg) has no Seldon build to compare with; it is pinned by
  * what the mathematics fixes instead (tests/test_oracle.py): A = I converges
  * in one step (Gmres through the happy breakdown), a diagonal matrix with k
