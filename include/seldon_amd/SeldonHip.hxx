@@ -15,7 +15,7 @@
 // are what Seldon's solver templates call unqualified
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
-// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr / Cgne / Lsqr below replace the whole loop by the
+// more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr / Cgne / Lsqr / CoCg below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
@@ -646,7 +646,7 @@ namespace Seldon
       }
     int code = 0;
     shk_ctx ctx = HipContext::Get().Handle();
-    if (which == 0) HipCheck(shk_cg(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cg");
+    if (which == 0 || which == 9) HipCheck(shk_cg(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cg");
     else if (which == 1) HipCheck(shk_bicgstab(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCgStab");
     else if (which == 2) HipCheck(shk_gmres(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gmres");
     else if (which == 3) HipCheck(shk_cgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "Cgs");
@@ -666,7 +666,7 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[9][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
+        static const char* what[10][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
                                           "Bicgstab breakdown #3", "", "", ""},
                                          {"", "", "", "", "", "", ""},
@@ -679,7 +679,8 @@ namespace Seldon
                                           "Qmr breakdown #4", "Qmr breakdown #5", "Qmr breakdown #6"},
                                          {"", "Cgne breakdown #1", "Cgne breakdown #2", "", "", "", ""},
                                          {"", "Lsqr breakdown #1", "Lsqr breakdown #2", "Lsqr breakdown #3",
-                                          "", "", ""}};
+                                          "", "", ""},
+                                         {"", "Cocg breakdown #1", "Cocg breakdown #2", "", "", "", ""}};
         const int c = it.error_code;
         iter.Fail(c, (c >= 0 && c < 7) ? what[which][c] : "breakdown");
       }
@@ -742,6 +743,16 @@ namespace Seldon
       Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(6, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  // CoCg.cxx:43-45: for real values the loop is Cg's, operation for
+  // operation (DotProd and DotProdConj agree on reals): the fused Cg runs it
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  CoCg(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+       Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(9, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   // Cgne.cxx:40-42 and Lsqr.cxx:40-42 (Lsqr applies no preconditioner, in

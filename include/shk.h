@@ -268,7 +268,10 @@ typedef struct {
  * (Iterative.cxx:344-354): 0, -2 at max_iter, or the breakdown code.
  * A negative SHK_ERR_* (<= -1, != -2 ...) cannot be confused: solver status
  * is returned through *code, the function result is the call status. */
-/* Cg (computation/solver/iterative/Cg.cxx:45-116). */
+/* Cg (computation/solver/iterative/Cg.cxx:45-116).  For real values it is
+ * also Seldon's CoCg (CoCg.cxx:44-115): that loop is Cg's operation for
+ * operation (DotProd and DotProdConj agree on reals), so the C++ overlay
+ * routes CoCg here. */
 int shk_cg(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
            shk_iteration *iter, int *code);
 /* shk_cg without any device allocation: no counterpart in the reference.

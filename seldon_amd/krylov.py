@@ -538,6 +538,12 @@ def Qmr(A, x, b, M, it):
     return _solve(L.load().shk_qmr, A, x, b, M, it)
 
 
+def CoCg(A, x, b, M, it):
+    """CoCg (computation/solver/iterative/CoCg.cxx:44-115): for real values
+    its loop is Cg's operation for operation, so it runs the fused Cg."""
+    return _solve(L.load().shk_cg, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

@@ -178,3 +178,27 @@ def test_fused_fp32(gpu, method):
     err = np.linalg.norm(x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
     # the normal equations square the condition number
     assert err <= 10 * (kappa ** 2 if method in NORMAL else kappa) * 1e-5, (err, kappa)
+
+
+def test_cocg_runs_the_fused_cg(gpu):
+    """CoCg on reals is Cg's loop (CoCg.cxx:44-115): K.CoCg runs the fused Cg
+    and matches Seldon's own CoCg run on the overlay program's 64 x 64
+    Laplacian -- same count and iteration numbers, every entry within 1e-6
+    (the reductions' order)."""
+    import math
+    from seldon_amd import krylov as K
+    fx = np.load(f"{GOLDEN}/overlay_cpu.npz", allow_pickle=False)
+    A = wl.laplace2d(64)
+    xt = np.array([math.sin(0.37 * i) + 0.1 * math.cos(1.3 * i) for i in range(A.m)])
+    b = orc.mlt(A, xt)
+    M = K.Matrix(A)
+    bg = torch.tensor(b, device="cuda")
+    x = torch.zeros_like(bg)
+    it = K.Iteration(2000, 1e-8)
+    it.HideMessages()
+    code = K.CoCg(M, x, bg, K.Preconditioner_Base(), it)
+    hist = it.history
+    ref = fx["cocg_cpu_hist"]
+    assert code == 0 and it.nb_iter == int(fx["cocg_cpu_nb"])
+    assert np.array_equal(it.history_it, fx["cocg_cpu_its"])
+    assert np.max(np.abs(hist - ref) / ref) < 1e-6

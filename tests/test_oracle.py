@@ -987,3 +987,19 @@ def test_oracle_cgne_lsqr_fp32_within_kappa_eps(method):
     err = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
     eps32 = float(np.finfo(np.float32).eps)
     assert eps32 / 10 <= err <= 10 * kappa ** 2 * 1e-5, (err, kappa)
+
+
+def test_oracle_cg_is_seldons_cocg_on_reals():
+    """Seldon's CoCg (CoCg.cxx:44-115, the overlay program's CPU leg on the
+    64 x 64 Laplacian) has Cg's loop for real values: the oracle's Cg
+    reproduces its history, iteration numbers and count bitwise, which is
+    why the library runs CoCg as its fused Cg."""
+    import math
+    fx = _overlay_fixture()
+    A = wl.laplace2d(64)
+    xt = np.array([math.sin(0.37 * i) + 0.1 * math.cos(1.3 * i) for i in range(A.m)])
+    b = orc.mlt(A, xt)
+    assert np.array_equal(np.frombuffer(fx["b_cpu"].tobytes()[4:], np.float64), b)  # Vector::Write: int n, then b
+    r = orc.solve("cg", A, b, tol=1e-8, max_iter=2000)
+    assert r.nb_iter == int(fx["cocg_cpu_nb"]) and r.code == int(fx["cocg_cpu_code"]) == 0
+    assert np.array_equal(r.hist_it, fx["cocg_cpu_its"]) and np.array_equal(r.hist, fx["cocg_cpu_hist"])
