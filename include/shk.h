@@ -330,6 +330,12 @@ int shk_cgne(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
  * sets the first residual norm).  Error codes 1-3 are its breakdowns. */
 int shk_lsqr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
              shk_iteration *iter, int *code);
+/* BiCgcr (computation/solver/iterative/BiCgcr.cxx:45-117), fused likewise:
+ * one product per iteration, (w, v), (w, a) and (w, z) from fused
+ * reductions; the start's product counts as the first iteration, as in
+ * the template.  Error codes 1 / 2 are its breakdowns. */
+int shk_bicgcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+               shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

@@ -933,6 +933,51 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- BiCgcr: computation/solver/iterative/BiCgcr.cxx:45-117 --------- */
+int FN(bicgcr)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+               orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho, mu, alpha, beta, tau;
+    T *v = FN(dup)(N, b), *w = FN(dup)(N, b), *s = FN(dup)(N, b), *z = FN(dup)(N, b),
+      *p = FN(dup)(N, b), *a = FN(dup)(N, b);
+    FN(fill)(N, (T)0, v); FN(fill)(N, (T)0, w); FN(fill)(N, (T)0, s);
+    FN(fill)(N, (T)0, z); FN(fill)(N, (T)0, p); FN(fill)(N, (T)0, a);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, v);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, v);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    it->nb_iter = 0;
+    FN(precond)(precond, N, diag, v, s);
+    FN(copy)(N, s, p);
+    FN(mlt)(A, p, a);
+    FN(precond)(precond, N, diag, a, w);
+    it->nb_iter++;
+    while (!orc_finished(it, (double)FN(norm2)(N, v), it->nb_iter, it->max_iter)) {
+        rho = FN(dot)(N, w, v);
+        mu = FN(dot)(N, w, a);
+        if (mu == (T)0) { orc_fail(it, 1); break; }
+        alpha = rho / mu;
+        FN(add)(N, alpha, p, x);
+        FN(add)(N, -alpha, a, v);
+        FN(add)(N, -alpha, w, s);
+        FN(mlt)(A, s, z);
+        tau = FN(dot)(N, w, z);
+        if (tau == (T)0) { orc_fail(it, 2); break; }
+        beta = tau / mu;
+        FN(scal)(N, -beta, p);
+        FN(add)(N, (T)1, s, p);
+        FN(scal)(N, -beta, a);
+        FN(add)(N, (T)1, z, a);
+        FN(precond)(precond, N, diag, a, w);
+        it->nb_iter++;
+    }
+done:
+    free(v); free(w); free(s); free(z); free(p); free(a);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

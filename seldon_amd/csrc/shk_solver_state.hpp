@@ -582,6 +582,52 @@ struct FinLsqrAlpha {
     }
 };
 
+// ------------------------------------------------------------ BiCgcr ----
+// Loop head: [++iter;] Finished(v) (BiCgcr.cxx:73-80); the first head
+// counts the start's product (nb_iter = 1)
+template <typename T>
+struct FinBcrTop {
+    KState<T> *st;
+    int first;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (first) st->nb_iter = 1;
+        else st->nb_iter++;
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+// rho = (w, v), mu = (w, a); breakdown #1; alpha = rho / mu (:82-90)
+template <typename T>
+struct FinBcrMu {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->rho = s[0];
+        st->delta = s[1];  // mu
+        if (s[1] == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        st->alpha = s[0] / s[1];
+    }
+};
+
+// tau = (w, z); breakdown #2; beta = tau / mu (:96-104)
+template <typename T>
+struct FinBcrTau {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->sigma = s[0];  // tau
+        if (s[0] == (T)0) {
+            it_fail(st, 2);
+            return;
+        }
+        st->beta = s[0] / st->delta;
+    }
+};
+
 // ------------------------------------------------------------- Gmres ----
 // GenRot "old BLAS version" (Functions_Vector.cxx:612-649) and ApplyRot
 // (:681-687), in double.

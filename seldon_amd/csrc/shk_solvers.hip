@@ -1248,6 +1248,85 @@ __global__ __launch_bounds__(kBlock) void k_lsqr_xw(int64_t n, T *__restrict__ v
     }
 }
 
+// ------------------------------------------------------------ BiCgcr ----
+// v = src (b, or b - A x); [x = 0]; s = M^{-1} v; p = s (BiCgcr.cxx:62-73);
+// sums |v|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bcr_init(int64_t n, const T *__restrict__ src, const T *__restrict__ dg,
+                                                     T *__restrict__ x, T *__restrict__ v, T *__restrict__ s,
+                                                     T *__restrict__ p, int zero_x, const KState<T> *st,
+                                                     T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        const T vv = src[i];
+        v[i] = vv;
+        T sv = vv;
+        if (dg) sv = vv / dg[i];
+        s[i] = sv;
+        p[i] = sv;
+        c[0] += vv * vv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// start ? (w = M^{-1} a) : (p = -beta p + s; a = -beta a + z; w = M^{-1} a)
+// (:75, :106-111); sums (w, v) and (w, a) for the next loop
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bcr_paw(int64_t n, const T *__restrict__ sv, const T *__restrict__ z,
+                                                    const T *__restrict__ dg, const T *__restrict__ v,
+                                                    T *__restrict__ p, T *__restrict__ a, T *__restrict__ w,
+                                                    int start, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T mb = -st->beta;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T av = a[i];
+        if (!start) {
+            T pv = p[i] * mb;
+            pv += (T)1 * sv[i];
+            p[i] = pv;
+            av = av * mb;
+            av += (T)1 * z[i];
+            a[i] = av;
+        }
+        T wv = av;
+        if (dg) {
+            wv = av / dg[i];
+            w[i] = wv;
+        }
+        c[0] += wv * v[i];
+        c[1] += wv * av;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// x += alpha p; v -= alpha a; s -= alpha w (:91-95); sums |v|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bcr_xvs(int64_t n, const T *__restrict__ p, const T *__restrict__ a,
+                                                    const T *__restrict__ w, T *__restrict__ x, T *__restrict__ v,
+                                                    T *__restrict__ s, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T alpha = st->alpha, malpha = -alpha;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (aa) x[i] += alpha * p[i];
+        T vv = v[i];
+        if (am) {
+            vv += malpha * a[i];
+            v[i] = vv;
+            s[i] += malpha * w[i];
+        }
+        c[0] += vv * vv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -2222,6 +2301,66 @@ int lsqr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, shk_iteration *it, int *
     return S.finish(it, code);
 }
 
+// BiCgcr (BiCgcr.cxx:45-117): per iteration rho = (w, v) and mu = (w, a),
+// the x / v / s update with |v|, z = A s with (w, z), and the p / a / w
+// update with the next (w, v), (w, a); one product per iteration.
+template <typename T>
+int bicgcr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, jac ? 6 : 5, 0, it));
+    const int64_t n = S.n;
+    T *v = S.take(n), *s = S.take(n), *z = S.take(n), *p = S.take(n), *a = S.take(n);
+    T *w = jac ? S.take(n) : a;  // the identity's M^{-1} a is a itself
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (BiCgcr.cxx:58)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a0{};
+        a0.x = x; a0.y_in = b; a0.y_out = v; a0.alpha = (T)-1; a0.beta = (T)1; a0.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a0, FinNone{}));
+        src = v;
+    }
+    hipLaunchKernelGGL(k_bcr_init<T>, g, blk, 0, st, n, src, dg, x, v, s, p, it->init_guess_null ? 1 : 0, S.dst,
+                       S.part[0]);
+    {
+        SpmvArgs<T> a1{};
+        a1.x = p; a1.y_out = a; a1.alpha = (T)1; a1.mlt = 1; a1.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a1, FinNone{}));
+    }
+    hipLaunchKernelGGL(k_bcr_paw<T>, g, blk, 0, st, n, (const T *)s, (const T *)z, dg, (const T *)v, p, a, w, 1,
+                       S.dst, S.part[1]);
+    grid_finish<T>(st, (const T *)S.part[0], S.nb, (const int *)&S.dst->skip, FinBcrTop<T>{S.dst, 1});
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            grid_finish<T>(cs, (const T *)S.part[1], S.nb, (const int *)&S.dst->skip, FinBcrMu<T>{S.dst});
+            hipLaunchKernelGGL(k_bcr_xvs<T>, g, blk, 0, cs, n, (const T *)p, (const T *)a, (const T *)w, x, v, s,
+                               S.dst, S.part[0]);
+            SpmvArgs<T> a2{};
+            a2.x = s; a2.y_out = z; a2.alpha = (T)1; a2.mlt = 1;
+            a2.dot0 = kDotVec; a2.u0 = w; a2.skip = &S.dst->skip;
+            a2.partials = S.part[1];
+            SHK_TRY(S.spmv(a2, FinDefer{}));
+            grid_finish<T>(cs, (const T *)S.part[1], a2.grid_used, (const int *)&S.dst->skip, FinBcrTau<T>{S.dst});
+            hipLaunchKernelGGL(k_bcr_paw<T>, g, blk, 0, cs, n, (const T *)s, (const T *)z, dg, (const T *)v, p, a, w,
+                               0, S.dst, S.part[1]);
+            grid_finish<T>(cs, (const T *)S.part[0], S.nb, (const int *)&S.dst->skip, FinBcrTop<T>{S.dst, 0});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "bicgcr: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -2234,6 +2373,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 6) return qmr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 7) return cgne_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 8) return lsqr_impl<T>(ctx, A, (T *)dx, (const T *)db, it, code);
+    if (which == 10) return bicgcr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -2424,6 +2564,11 @@ int shk_cgne(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
 int shk_lsqr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(8, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_bicgcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(10, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

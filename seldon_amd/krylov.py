@@ -544,6 +544,11 @@ def CoCg(A, x, b, M, it):
     return _solve(L.load().shk_cg, A, x, b, M, it)
 
 
+def BiCgcr(A, x, b, M, it):
+    """BiCgcr (computation/solver/iterative/BiCgcr.cxx:45-117) on the GPU, fused."""
+    return _solve(L.load().shk_bicgcr, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

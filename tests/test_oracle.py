@@ -1003,3 +1003,28 @@ def test_oracle_cg_is_seldons_cocg_on_reals():
     r = orc.solve("cg", A, b, tol=1e-8, max_iter=2000)
     assert r.nb_iter == int(fx["cocg_cpu_nb"]) and r.code == int(fx["cocg_cpu_code"]) == 0
     assert np.array_equal(r.hist_it, fx["cocg_cpu_its"]) and np.array_equal(r.hist, fx["cocg_cpu_hist"])
+
+
+def test_oracle_bicgcr_matches_seldon_fixture_and_closed_forms():
+    """Seldon's own BiCgcr (BiCgcr.cxx:45-117, the overlay program's CPU leg
+    on the 64 x 64 Laplacian): history, iteration numbers, count, bitwise.
+    A = 2.5 I: the first loop solves it (x = b / 2.5); where v rounds to
+    exactly 0, tau = (w, A s) is 0 and the template reports its breakdown #2
+    at iteration 1, else Finished() ends it at iteration 2.  The carried
+    residual v stays the true one."""
+    import math
+    fx = _overlay_fixture()
+    A = wl.laplace2d(64)
+    xt = np.array([math.sin(0.37 * i) + 0.1 * math.cos(1.3 * i) for i in range(A.m)])
+    b = orc.mlt(A, xt)
+    r = orc.solve("bicgcr", A, b, tol=1e-8, max_iter=2000)
+    assert r.nb_iter == int(fx["bicgcr_cpu_nb"]) and r.code == int(fx["bicgcr_cpu_code"]) == 0
+    assert np.array_equal(r.hist_it, fx["bicgcr_cpu_its"]) and np.array_equal(r.hist, fx["bicgcr_cpu_hist"])
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-4 * r.hist[-1] + 1e-15
+    n = 60
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    bb = wl.uniform(n, 9)
+    for precond in ("id", "jacobi"):
+        r = orc.solve("bicgcr", wl.CSR(n, n, *eye, np.full(n, 2.5)), bb, precond=precond, tol=1e-12, max_iter=20)
+        assert (r.code, r.nb_iter) in ((2, 1), (0, 2)) and np.allclose(r.x, bb / 2.5, rtol=1e-14)
