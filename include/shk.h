@@ -336,6 +336,11 @@ int shk_lsqr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
  * the template.  Error codes 1 / 2 are its breakdowns. */
 int shk_bicgcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
                shk_iteration *iter, int *code);
+/* QmrSym (computation/solver/iterative/QmrSym.cxx:42-166), fused likewise:
+ * Qmr for symmetric A and M, no transposed products; rho double, the other
+ * scalars the value type.  Error codes 1, 3-6 are its breakdowns. */
+int shk_qmrsym(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+               shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

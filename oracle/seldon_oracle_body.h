@@ -978,6 +978,77 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- QmrSym: computation/solver/iterative/QmrSym.cxx:42-166 ---------- *
+ * Qmr for a symmetric A and M (no transposed products); scalar types as
+ * in Qmr (rho, rho_1 double; the others the value type).  Here First() is
+ * tested before ++iter, so the first loop starts d and s from p, p~. */
+int FN(qmrsym)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+               orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T delta, ep = 0, beta, theta_1, gamma_1, theta = 0, gamma = 1, eta = -1;
+    double rho, rho_1;
+    T *r = FN(dup)(N, b), *y = FN(dup)(N, b), *v = FN(dup)(N, b), *p_tld = FN(dup)(N, b),
+      *p = FN(dup)(N, b), *d = FN(dup)(N, b), *s = FN(dup)(N, b);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(copy)(N, r, v);
+    FN(precond)(precond, N, diag, v, y);
+    rho = (double)FN(norm2)(N, y);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        if (rho == 0.0) { orc_fail(it, 1); break; }
+        FN(scal)(N, (T)(1. / rho), v);
+        FN(scal)(N, (T)(1. / rho), y);
+        delta = FN(dot)(N, v, y);
+        if (delta == (T)0) { orc_fail(it, 3); break; }
+        if (it->nb_iter == 0) {
+            FN(copy)(N, y, p);
+        } else {
+            FN(scal)(N, (T)(-(rho * (double)delta / (double)ep)), p);
+            FN(add)(N, (T)1, y, p);
+        }
+        FN(mlt)(A, p, p_tld);
+        ep = FN(dot)(N, p, p_tld);
+        if (ep == (T)0) { orc_fail(it, 4); break; }
+        beta = ep / delta;
+        if (beta == (T)0) { orc_fail(it, 5); break; }
+        FN(scal)(N, -beta, v);
+        FN(add)(N, (T)1, p_tld, v);
+        FN(precond)(precond, N, diag, v, y);
+        rho_1 = rho;
+        rho = (double)FN(norm2)(N, y);
+        gamma_1 = gamma;
+        theta_1 = theta;
+        theta = (T)(rho / (double)(gamma_1 * beta));
+        gamma = (T)((double)(T)1 / sqrt(1.0 + (double)(theta * theta)));
+        if (gamma == (T)0) { orc_fail(it, 6); break; }
+        eta = (T)((double)(-eta) * rho_1 * (double)gamma * (double)gamma /
+                  (double)(beta * gamma_1 * gamma_1));
+        if (it->nb_iter == 0) {
+            FN(copy)(N, p, d);
+            FN(scal)(N, eta, d);
+            FN(copy)(N, p_tld, s);
+            FN(scal)(N, eta, s);
+        } else {
+            const T tmp = theta_1 * theta_1 * gamma * gamma;
+            FN(scal)(N, tmp, d);
+            FN(add)(N, eta, p, d);
+            FN(scal)(N, tmp, s);
+            FN(add)(N, eta, p_tld, s);
+        }
+        FN(add)(N, (T)1, d, x);
+        FN(add)(N, -(T)1, s, r);
+        it->nb_iter++;
+    }
+done:
+    free(r); free(y); free(v); free(p_tld); free(p); free(d); free(s);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

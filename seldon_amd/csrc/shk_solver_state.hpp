@@ -628,6 +628,91 @@ struct FinBcrTau {
     }
 };
 
+// ------------------------------------------------------------ QmrSym ----
+// Loop head (QmrSym.cxx:75-85): [++iter;] Finished(r); breakdown #1
+// (rho = 0).  The first head also takes rho = |y| (s[0]) and the scalars'
+// starting values; |r| is sqrt(s[1]) there and sqrt(s[0]) later.
+template <typename T>
+struct FinQmsTop {
+    KState<T> *st;
+    int first;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        double rnorm;
+        if (first) {
+            st->qm_rho = (double)sqrt_t(s[0]);
+            st->qm_theta = (T)0;
+            st->qm_gamma = (T)1;
+            st->qm_eta = (T)-1;
+            st->qm_ep = (T)0;
+            st->nb_iter = 0;
+            rnorm = (double)sqrt_t(s[1]);
+        } else {
+            st->nb_iter++;
+            rnorm = (double)sqrt_t(s[0]);
+        }
+        if (it_finished(st, rnorm, st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        if (st->qm_rho == 0.0) it_fail(st, 1);
+    }
+};
+
+// delta = (v, y); its breakdown is error code 3 (:92-96)
+template <typename T>
+struct FinQmsDelta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->delta = s[0];
+        if (s[0] == (T)0) it_fail(st, 3);
+    }
+};
+
+// ep = (p, p~), code 4; beta = ep / delta, code 5 (:107-119)
+template <typename T>
+struct FinQmsEp {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->qm_ep = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 4);
+            return;
+        }
+        const T beta = s[0] / st->delta;
+        st->beta = beta;
+        if (beta == (T)0) it_fail(st, 5);
+    }
+};
+
+// rho = |y|; theta, gamma (code 6 when 0), eta and the d / s scale, in the
+// template's types (:124-140); the first loop's d / s start is chosen by
+// the kernel from nb_iter (still 0 there)
+template <typename T>
+struct FinQmsTheta {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double rho_1 = st->qm_rho;
+        const double rho = (double)sqrt_t(s[0]);
+        st->qm_rho = rho;
+        const T gamma_1 = st->qm_gamma, theta_1 = st->qm_theta, beta = st->beta;
+        const T theta = (T)(rho / (double)(gamma_1 * beta));
+        const T gamma = (T)((double)(T)1 / sqrt(1.0 + (double)(theta * theta)));
+        st->qm_theta = theta;
+        st->qm_gamma = gamma;
+        if (gamma == (T)0) {
+            it_fail(st, 6);
+            return;
+        }
+        st->qm_eta = (T)((double)(-st->qm_eta) * rho_1 * (double)gamma * (double)gamma /
+                         (double)(beta * gamma_1 * gamma_1));
+        st->qm_tmp = theta_1 * theta_1 * gamma * gamma;
+    }
+};
+
 // ------------------------------------------------------------- Gmres ----
 // GenRot "old BLAS version" (Functions_Vector.cxx:612-649) and ApplyRot
 // (:681-687), in double.

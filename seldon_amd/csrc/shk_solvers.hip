@@ -1327,6 +1327,120 @@ __global__ __launch_bounds__(kBlock) void k_bcr_xvs(int64_t n, const T *__restri
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// ------------------------------------------------------------ QmrSym ----
+// [x = 0]; r = src; v = r; y = M^{-1} v (QmrSym.cxx:62-72); sums |y|^2, |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qms_init(int64_t n, const T *__restrict__ src, const T *__restrict__ dg,
+                                                     T *__restrict__ x, T *__restrict__ r, T *__restrict__ v,
+                                                     T *__restrict__ y, int zero_x, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        const T rv = src[i];
+        r[i] = rv;
+        v[i] = rv;
+        T yv = rv;
+        if (dg) yv = rv / dg[i];
+        y[i] = yv;
+        c[0] += yv * yv;
+        c[1] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// v /= rho; y /= rho (:88-89); sums (v, y)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qms_scale(int64_t n, T *__restrict__ v, T *__restrict__ y,
+                                                      const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T sv = (T)(1. / st->qm_rho);
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        const T vv = v[i] * sv;
+        v[i] = vv;
+        const T yv = y[i] * sv;
+        y[i] = yv;
+        c[0] += vv * yv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// first ? p = y : p = y - (rho delta / ep) p (:98-104)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qms_p(int64_t n, const T *__restrict__ y, T *__restrict__ p,
+                                                  const KState<T> *st)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T cp = (T)(-(st->qm_rho * (double)st->delta / (double)st->qm_ep));
+    GRID_STRIDE(i, n) {
+        if (first) {
+            p[i] = y[i];
+        } else {
+            T pv = p[i] * cp;
+            pv += (T)1 * y[i];
+            p[i] = pv;
+        }
+    }
+}
+
+// v = -beta v + p~; y = M^{-1} v (:121-122); sums |y|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qms_vy(int64_t n, const T *__restrict__ pt, const T *__restrict__ dg,
+                                                   T *__restrict__ v, T *__restrict__ y, const KState<T> *st,
+                                                   T *partials)
+{
+    if (st->skip) return;
+    const T mb = -st->beta;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T vv = v[i] * mb;
+        vv += (T)1 * pt[i];
+        v[i] = vv;
+        const T yv = dg ? vv / dg[i] : vv;
+        y[i] = yv;
+        c[0] += yv * yv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// first ? d = eta p, s = eta p~ : d = tmp d + eta p, s = tmp s + eta p~;
+// x += d; r -= s (:142-157); sums |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qms_dsxr(int64_t n, const T *__restrict__ p, const T *__restrict__ pt,
+                                                     T *__restrict__ d, T *__restrict__ s, T *__restrict__ x,
+                                                     T *__restrict__ r, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const bool first = st->nb_iter == 0;
+    const T tmp = st->qm_tmp, eta = st->qm_eta;
+    const bool ae = eta != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T dv, sv;
+        if (first) {
+            dv = p[i] * eta;
+            sv = pt[i] * eta;
+        } else {
+            dv = d[i] * tmp;
+            if (ae) dv += eta * p[i];
+            sv = s[i] * tmp;
+            if (ae) sv += eta * pt[i];
+        }
+        d[i] = dv;
+        s[i] = sv;
+        x[i] += (T)1 * dv;
+        T rv = r[i];
+        rv += -(T)1 * sv;
+        r[i] = rv;
+        c[0] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -2361,6 +2475,61 @@ int bicgcr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteratio
     return S.finish(it, code);
 }
 
+// QmrSym (QmrSym.cxx:42-166): Qmr without the transposed side; per loop
+// the v / y scaling with (v, y), the p update, p~ = A p with (p, p~), v and
+// y with |y|, the scalar step, the d / s / x / r update with |r|.
+template <typename T>
+int qmrsym_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 7, 0, it));
+    const int64_t n = S.n;
+    T *r = S.take(n), *y = S.take(n), *v = S.take(n), *pt = S.take(n), *p = S.take(n), *d = S.take(n),
+      *s = S.take(n);
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (QmrSym.cxx:59)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = r;
+    }
+    hipLaunchKernelGGL(k_qms_init<T>, g, blk, 0, st, n, src, dg, x, r, v, y, it->init_guess_null ? 1 : 0, S.dst,
+                       S.partials);
+    S.fin(FinQmsTop<T>{S.dst, 1});
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            hipLaunchKernelGGL(k_qms_scale<T>, g, blk, 0, cs, n, v, y, S.dst, S.partials);
+            S.fin(FinQmsDelta<T>{S.dst});
+            hipLaunchKernelGGL(k_qms_p<T>, g, blk, 0, cs, n, (const T *)y, p, S.dst);
+            SpmvArgs<T> a{};
+            a.x = p; a.y_out = pt; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = p; a.skip = &S.dst->skip;
+            a.partials = S.part[1];
+            SHK_TRY(S.spmv(a, FinDefer{}));
+            grid_finish<T>(cs, (const T *)S.part[1], a.grid_used, (const int *)&S.dst->skip, FinQmsEp<T>{S.dst});
+            hipLaunchKernelGGL(k_qms_vy<T>, g, blk, 0, cs, n, (const T *)pt, dg, v, y, S.dst, S.partials);
+            S.fin(FinQmsTheta<T>{S.dst});
+            hipLaunchKernelGGL(k_qms_dsxr<T>, g, blk, 0, cs, n, (const T *)p, (const T *)pt, d, s, x, r, S.dst,
+                               S.partials);
+            S.fin(FinQmsTop<T>{S.dst, 0});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "qmrsym: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -2374,6 +2543,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 7) return cgne_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 8) return lsqr_impl<T>(ctx, A, (T *)dx, (const T *)db, it, code);
     if (which == 10) return bicgcr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 11) return qmrsym_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -2569,6 +2739,11 @@ int shk_lsqr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
 int shk_bicgcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(10, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_qmrsym(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(11, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

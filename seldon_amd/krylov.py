@@ -549,6 +549,11 @@ def BiCgcr(A, x, b, M, it):
     return _solve(L.load().shk_bicgcr, A, x, b, M, it)
 
 
+def QmrSym(A, x, b, M, it):
+    """QmrSym (computation/solver/iterative/QmrSym.cxx:42-166) on the GPU, fused."""
+    return _solve(L.load().shk_qmrsym, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

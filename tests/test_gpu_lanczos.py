@@ -47,7 +47,7 @@ def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float6
         x = torch.full_like(bg, 3.0)  # overwritten: x.Zero()
     P = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
     code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr, "qmr": K.Qmr, "cgne": K.Cgne,
-            "lsqr": K.Lsqr, "bicgcr": K.BiCgcr}[method](M, x, bg, P, it)
+            "lsqr": K.Lsqr, "bicgcr": K.BiCgcr, "qmrsym": K.QmrSym}[method](M, x, bg, P, it)
     n = min(it.hist_len, len(it.history))
     return code, it.nb_iter, it.history[:n].copy(), it.history_it[:n].copy(), x.cpu().numpy()
 
@@ -211,21 +211,27 @@ def _laplace64_overlay():
     return A, orc.mlt(A, xt)
 
 
-def test_bicgcr_matches_seldon_run(gpu):
-    """The fused BiCgcr against Seldon's own BiCgcr run (the overlay
+SYM = ["bicgcr", "qmrsym"]
+
+
+@pytest.mark.parametrize("method", SYM)
+def test_sym_matches_seldon_run(gpu, method):
+    """The fused BiCgcr and QmrSym against Seldon's own runs (the overlay
     program's 64 x 64 Laplacian): same count and iteration numbers, every
     entry within 1e-6 (Seldon's generic, FMA and MKL builds agree bitwise
-    on it)."""
+    on them)."""
     fx = np.load(f"{GOLDEN}/overlay_cpu.npz", allow_pickle=False)
     A, b = _laplace64_overlay()
-    got = _gpu("bicgcr", A, b, tol=1e-8, max_iter=2000)
-    ref = (fx["bicgcr_cpu_hist"], fx["bicgcr_cpu_its"], int(fx["bicgcr_cpu_code"]), int(fx["bicgcr_cpu_nb"]))
-    rel = _check("bicgcr", got, *ref, len(ref[0]), 0)
+    got = _gpu(method, A, b, tol=1e-8, max_iter=2000)
+    ref = (fx[f"{method}_cpu_hist"], fx[f"{method}_cpu_its"], int(fx[f"{method}_cpu_code"]),
+           int(fx[f"{method}_cpu_nb"]))
+    rel = _check(method, got, *ref, len(ref[0]), 0)
     assert len(got[2]) == len(ref[0]) and rel.max() < 1e-6
 
 
+@pytest.mark.parametrize("method", SYM)
 @pytest.mark.parametrize("case", ["laplace256_id", "aniso128_jacobi", "aniso128_sliced", "x0"])
-def test_bicgcr_matches_oracle(gpu, case):
+def test_sym_matches_oracle(gpu, method, case):
     if case.startswith("laplace"):
         A = wl.laplace2d(256)
     else:
@@ -234,24 +240,25 @@ def test_bicgcr_matches_oracle(gpu, case):
     precond = "id" if case.endswith("_id") else "jacobi"
     layout = "sliced" if case.endswith("sliced") else "auto"
     x0 = wl.uniform(A.m, 45) if case == "x0" else None
-    r = orc.solve("bicgcr", A, b, precond=precond, tol=1e-9, max_iter=8000, x0=x0)
-    got = _gpu("bicgcr", A, b, precond=precond, tol=1e-9, max_iter=8000, layout=layout, x0=x0)
-    _check("bicgcr", got, r.hist, r.hist_it, r.code, r.nb_iter, 20, max(4, r.nb_iter // 50))
+    r = orc.solve(method, A, b, precond=precond, tol=1e-9, max_iter=8000, x0=x0)
+    got = _gpu(method, A, b, precond=precond, tol=1e-9, max_iter=8000, layout=layout, x0=x0)
+    _check(method, got, r.hist, r.hist_it, r.code, r.nb_iter, 20, max(4, r.nb_iter // 50))
     assert np.linalg.norm(b - orc.mlt(A, got[4])) <= 1e-8 * np.linalg.norm(b)
 
 
-def test_bicgcr_storages_max_iter_zero_rhs(gpu):
+@pytest.mark.parametrize("method", SYM)
+def test_sym_storages_max_iter_zero_rhs(gpu, method):
     A = wl.aniso2d(40, 40, 4.0)
     b = wl.apply(A, wl.uniform(A.m, 3))
     for st, M in (("sym", wl.upper(A)), ("array", A)):
-        r = orc.solve("bicgcr", M, b, precond="jacobi", tol=1e-9, max_iter=4000, storage=st)
-        got = _gpu("bicgcr", M, b, precond="jacobi", tol=1e-9, max_iter=4000, storage=st)
-        _check("bicgcr", got, r.hist, r.hist_it, r.code, r.nb_iter, 20, max(4, r.nb_iter // 50))
-    r = orc.solve("bicgcr", A, b, tol=1e-14, max_iter=7)
-    code, nb, hist, its, _ = _gpu("bicgcr", A, b, tol=1e-14, max_iter=7)
+        r = orc.solve(method, M, b, precond="jacobi", tol=1e-9, max_iter=4000, storage=st)
+        got = _gpu(method, M, b, precond="jacobi", tol=1e-9, max_iter=4000, storage=st)
+        _check(method, got, r.hist, r.hist_it, r.code, r.nb_iter, 20, max(4, r.nb_iter // 50))
+    r = orc.solve(method, A, b, tol=1e-14, max_iter=7)
+    code, nb, hist, its, _ = _gpu(method, A, b, tol=1e-14, max_iter=7)
     assert code == r.code == -2 and nb == r.nb_iter and np.array_equal(its, r.hist_it)
-    code, nb, _, _, x = _gpu("bicgcr", A, np.zeros(A.m), tol=1e-8)
-    assert code == orc.solve("bicgcr", A, np.zeros(A.m), tol=1e-8).code and np.all(x == 3.0)
-    a1 = _gpu("bicgcr", A, b, precond="jacobi", tol=1e-9, max_iter=4000)
-    a2 = _gpu("bicgcr", A, b, precond="jacobi", tol=1e-9, max_iter=4000)
+    code, nb, _, _, x = _gpu(method, A, np.zeros(A.m), tol=1e-8)
+    assert code == orc.solve(method, A, np.zeros(A.m), tol=1e-8).code and np.all(x == 3.0)
+    a1 = _gpu(method, A, b, precond="jacobi", tol=1e-9, max_iter=4000)
+    a2 = _gpu(method, A, b, precond="jacobi", tol=1e-9, max_iter=4000)
     assert a1[1] == a2[1] and np.array_equal(a1[2], a2[2]) and np.array_equal(a1[4], a2[4])

@@ -1028,3 +1028,25 @@ def test_oracle_bicgcr_matches_seldon_fixture_and_closed_forms():
     for precond in ("id", "jacobi"):
         r = orc.solve("bicgcr", wl.CSR(n, n, *eye, np.full(n, 2.5)), bb, precond=precond, tol=1e-12, max_iter=20)
         assert (r.code, r.nb_iter) in ((2, 1), (0, 2)) and np.allclose(r.x, bb / 2.5, rtol=1e-14)
+
+
+def test_oracle_qmrsym_matches_seldon_fixture_and_residual():
+    """Seldon's own QmrSym (QmrSym.cxx:42-166, the overlay program's CPU leg
+    on the 64 x 64 Laplacian): history, iteration numbers, count, bitwise;
+    its carried residual (r -= s) stays the true one; A = 2.5 I is solved
+    in one loop."""
+    import math
+    fx = _overlay_fixture()
+    A = wl.laplace2d(64)
+    xt = np.array([math.sin(0.37 * i) + 0.1 * math.cos(1.3 * i) for i in range(A.m)])
+    b = orc.mlt(A, xt)
+    r = orc.solve("qmrsym", A, b, tol=1e-8, max_iter=2000)
+    assert r.nb_iter == int(fx["qmrsym_cpu_nb"]) and r.code == int(fx["qmrsym_cpu_code"]) == 0
+    assert np.array_equal(r.hist_it, fx["qmrsym_cpu_its"]) and np.array_equal(r.hist, fx["qmrsym_cpu_hist"])
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15
+    n = 60
+    eye = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    bb = wl.uniform(n, 19)
+    r = orc.solve("qmrsym", wl.CSR(n, n, *eye, np.full(n, 2.5)), bb, tol=1e-12, max_iter=20)
+    assert r.code == 0 and r.nb_iter == 1 and np.allclose(r.x, bb / 2.5, rtol=1e-14)
