@@ -341,6 +341,12 @@ int shk_bicgcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
  * scalars the value type.  Error codes 1, 3-6 are its breakdowns. */
 int shk_qmrsym(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
                shk_iteration *iter, int *code);
+/* MinRes (computation/solver/iterative/MinRes.cxx:47-151), fused likewise:
+ * the stopping test on the residual estimate np (|b| times the product of
+ * the rotations' |s|, double as Titer in the template); the other scalars
+ * the value type.  Error codes 1 / 2 are its breakdowns. */
+int shk_minres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+               shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

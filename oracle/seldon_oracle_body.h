@@ -1049,6 +1049,84 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- MinRes: computation/solver/iterative/MinRes.cxx:47-151 ---------- *
+ * All scalars the value type T except the residual estimate np (Titer =
+ * double); the stopping test is on np, which starts at |b| and is
+ * multiplied by |s| each iteration. */
+int FN(minres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+               orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T *u_old = FN(dup)(N, b), *u = FN(dup)(N, b), *r = FN(dup)(N, b), *v_old = FN(dup)(N, b),
+      *v = FN(dup)(N, b), *w_old = FN(dup)(N, b), *w = FN(dup)(N, b), *z = FN(dup)(N, b),
+      *w_oold = FN(dup)(N, b);
+    T dp, beta, ibeta, beta_old, alpha, eta, ceta;
+    T cold, coold, c, soold, sold, s, rho0, rho1, rho2, rho3;
+    double np;
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(fill)(N, (T)0, u_old); FN(fill)(N, (T)0, v_old); FN(fill)(N, (T)0, w_old);
+    FN(fill)(N, (T)0, w); FN(fill)(N, (T)0, w_oold);
+    FN(precond)(precond, N, diag, r, z);
+    dp = FN(dot)(N, r, z);
+    dp = (T)sqrt(dp);
+    beta = dp;
+    eta = beta;
+    FN(copy)(N, r, v);
+    FN(copy)(N, z, u);
+    ibeta = (T)(1.0 / (double)beta);
+    FN(scal)(N, ibeta, v);
+    FN(scal)(N, ibeta, u);
+    c = 1; s = 0; cold = 1; sold = 0;
+    np = (double)FN(norm2)(N, b);
+    it->nb_iter = 0;
+    while (!orc_finished(it, np, it->nb_iter, it->max_iter)) {
+        FN(mlt)(A, u, r);
+        alpha = FN(dot)(N, r, u);
+        FN(precond)(precond, N, diag, r, z);
+        FN(add)(N, -alpha, v, r);
+        FN(add)(N, -alpha, u, z);
+        FN(add)(N, -beta, v_old, r);
+        FN(add)(N, -beta, u_old, z);
+        beta_old = beta;
+        dp = FN(dot)(N, r, z);
+        beta = (T)sqrt(dp);
+        coold = cold; cold = c; soold = sold; sold = s;
+        rho0 = cold * alpha - coold * sold * beta_old;
+        rho1 = (T)sqrt(rho0 * rho0 + beta * beta);
+        rho2 = sold * alpha + coold * cold * beta_old;
+        rho3 = soold * beta_old;
+        if (rho1 == (T)0) { orc_fail(it, 1); break; }
+        c = rho0 / rho1;
+        s = beta / rho1;
+        FN(copy)(N, w_old, w_oold);
+        FN(copy)(N, w, w_old);
+        FN(copy)(N, u, w);
+        FN(add)(N, -rho2, w_old, w);
+        FN(add)(N, -rho3, w_oold, w);
+        FN(scal)(N, (T)(1. / (double)rho1), w);
+        ceta = c * eta;
+        FN(add)(N, ceta, w, x);
+        eta = -s * eta;
+        FN(copy)(N, v, v_old);
+        FN(copy)(N, u, u_old);
+        FN(copy)(N, r, v);
+        FN(copy)(N, z, u);
+        if (beta == (T)0) { orc_fail(it, 2); break; }
+        ibeta = (T)(1.0 / (double)beta);
+        FN(scal)(N, ibeta, v);
+        FN(scal)(N, ibeta, u);
+        np *= (double)(s < 0 ? -s : s);
+        it->nb_iter++;
+    }
+done:
+    free(u_old); free(u); free(r); free(v_old); free(v); free(w_old); free(w); free(z); free(w_oold);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

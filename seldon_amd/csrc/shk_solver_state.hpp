@@ -36,6 +36,10 @@ struct KState {
     // x / w factors phi / rho, -theta / rho (value type)
     double ls_beta, ls_alpha, ls_rnorm;
     T ls_rho_bar, ls_phi_bar, ls_xs, ls_ws;
+    // MinRes: the residual estimate np (Titer = double) and the value-type
+    // scalars of its recurrence (indices kMr*)
+    double mr_np;
+    T mr[14];
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -710,6 +714,86 @@ struct FinQmsTheta {
         st->qm_eta = (T)((double)(-st->qm_eta) * rho_1 * (double)gamma * (double)gamma /
                          (double)(beta * gamma_1 * gamma_1));
         st->qm_tmp = theta_1 * theta_1 * gamma * gamma;
+    }
+};
+
+// ------------------------------------------------------------ MinRes ----
+enum { kMrBeta, kMrEta, kMrIbeta, kMrAlpha, kMrC, kMrS, kMrCold, kMrSold, kMrRho2, kMrRho3, kMrIrho1, kMrCeta };
+
+// dp = sqrt((r, z)); beta = eta = dp; ibeta = 1 / beta; the rotation's
+// start; np = |b|; the first Finished(np) (MinRes.cxx:74-89)
+template <typename T>
+struct FinMrInit {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        T *m = st->mr;
+        const T dp = sqrt_t(s[0]);
+        m[kMrBeta] = dp;
+        m[kMrEta] = dp;
+        m[kMrIbeta] = (T)(1.0 / (double)dp);
+        m[kMrC] = (T)1;
+        m[kMrS] = (T)0;
+        m[kMrCold] = (T)1;
+        m[kMrSold] = (T)0;
+        st->mr_np = (double)sqrt_t(s[1]);
+        st->nb_iter = 0;
+        if (it_finished(st, st->mr_np, st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+template <typename T>
+struct FinMrAlpha {  // alpha = (A u, u) (:94)
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const { st->mr[kMrAlpha] = s[0]; }
+};
+
+// beta = sqrt((r, z)); the QR step (rho0..rho3, breakdown #1, c, s); the
+// factors of the w / x update; breakdown #2 (beta = 0, after that update);
+// ibeta; np *= |s|; ++iter; Finished(np).  An end after the w / x update
+// records its slot: that update still runs (:106-148)
+template <typename T>
+struct FinMrStep {
+    KState<T> *st;
+    int slot;
+    __device__ void operator()(const T (&sm)[2]) const
+    {
+        T *m = st->mr;
+        const T beta_old = m[kMrBeta], alpha = m[kMrAlpha];
+        const T beta = sqrt_t(sm[0]);
+        m[kMrBeta] = beta;
+        const T coold = m[kMrCold], cold = m[kMrC], soold = m[kMrSold], sold = m[kMrS];
+        m[kMrCold] = cold;
+        m[kMrSold] = sold;
+        const T rho0 = cold * alpha - coold * sold * beta_old;
+        const T rho1 = sqrt_t(rho0 * rho0 + beta * beta);
+        const T rho2 = sold * alpha + coold * cold * beta_old;
+        const T rho3 = soold * beta_old;
+        if (rho1 == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        const T c = rho0 / rho1, s = beta / rho1;
+        m[kMrC] = c;
+        m[kMrS] = s;
+        m[kMrRho2] = rho2;
+        m[kMrRho3] = rho3;
+        m[kMrIrho1] = (T)(1. / (double)rho1);
+        const T eta = m[kMrEta];
+        m[kMrCeta] = c * eta;
+        m[kMrEta] = -s * eta;
+        if (beta == (T)0) {
+            st->stop_slot = slot;
+            it_fail(st, 2);
+            return;
+        }
+        m[kMrIbeta] = (T)(1.0 / (double)beta);
+        st->mr_np *= (double)(s < (T)0 ? -s : s);
+        st->nb_iter++;
+        if (it_finished(st, st->mr_np, st->nb_iter, st->max_iter)) {
+            st->stop_slot = slot;
+            it_stop(st);
+        }
     }
 };
 

@@ -1441,6 +1441,109 @@ __global__ __launch_bounds__(kBlock) void k_qms_dsxr(int64_t n, const T *__restr
     block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
 }
 
+// ------------------------------------------------------------ MinRes ----
+// [x = 0]; r = src; z = M^{-1} r (MinRes.cxx:66-79); sums (r, z), |b|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mr_init(int64_t n, const T *__restrict__ src, const T *__restrict__ b,
+                                                    const T *__restrict__ dg, T *__restrict__ x, T *__restrict__ r,
+                                                    T *__restrict__ z, int zero_x, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        const T rv = src[i];
+        r[i] = rv;
+        T zv = rv;
+        if (dg) zv = rv / dg[i];
+        z[i] = zv;
+        c[0] += rv * zv;
+        const T bv = b[i];
+        c[1] += bv * bv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// v = r ibeta; u = z ibeta; u_old, v_old, w_old, w, w_oold = 0 (:73-84)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mr_start(int64_t n, const T *__restrict__ r, const T *__restrict__ z,
+                                                     T *__restrict__ v, T *__restrict__ u, T *__restrict__ v_old,
+                                                     T *__restrict__ u_old, T *__restrict__ w0, T *__restrict__ w1,
+                                                     T *__restrict__ w2, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T ib = st->mr[kMrIbeta];
+    GRID_STRIDE(i, n) {
+        v[i] = r[i] * ib;
+        u[i] = z[i] * ib;
+        v_old[i] = (T)0;
+        u_old[i] = (T)0;
+        w0[i] = (T)0;
+        w1[i] = (T)0;
+        w2[i] = (T)0;
+    }
+}
+
+// z = M^{-1} r (r = A u); r -= alpha v; z -= alpha u; r -= beta v_old;
+// z -= beta u_old (:96-104); sums (r, z)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mr_rz(int64_t n, const T *__restrict__ dg, const T *__restrict__ v,
+                                                  const T *__restrict__ u, const T *__restrict__ v_old,
+                                                  const T *__restrict__ u_old, T *__restrict__ r, T *__restrict__ z,
+                                                  const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T ma = -st->mr[kMrAlpha], mb = -st->mr[kMrBeta];
+    const bool aa = ma != (T)0, ab = mb != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T rv = r[i];
+        T zv = rv;
+        if (dg) zv = rv / dg[i];
+        if (aa) {
+            rv += ma * v[i];
+            zv += ma * u[i];
+        }
+        if (ab) {
+            rv += mb * v_old[i];
+            zv += mb * u_old[i];
+        }
+        r[i] = rv;
+        z[i] = zv;
+        c[0] += rv * zv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// w_new = (u - rho2 w - rho3 w_old) / rho1 into w_oold's buffer; x +=
+// c eta w_new (:120-131, also when the solve ends here); then v_new =
+// r ibeta, u_new = z ibeta into the old v_old / u_old buffers (:133-143)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mr_wx(int64_t n, const T *__restrict__ u, const T *__restrict__ w,
+                                                  const T *__restrict__ w_old, T *__restrict__ w_new,
+                                                  T *__restrict__ x, const T *__restrict__ r,
+                                                  const T *__restrict__ z, T *__restrict__ v_new,
+                                                  T *__restrict__ u_new, const KState<T> *st, int slot)
+{
+    if (st->skip && st->stop_slot != slot) return;
+    const T *m = st->mr;
+    const T m2 = -m[kMrRho2], m3 = -m[kMrRho3], ir = m[kMrIrho1], ce = m[kMrCeta], ib = m[kMrIbeta];
+    const bool a2 = m2 != (T)0, a3 = m3 != (T)0, ac = ce != (T)0;
+    const bool ended = st->skip;
+    GRID_STRIDE(i, n) {
+        T wv = u[i];
+        if (a2) wv += m2 * w[i];
+        if (a3) wv += m3 * w_old[i];
+        wv *= ir;
+        w_new[i] = wv;
+        if (ac) x[i] += ce * wv;
+        if (!ended) {
+            v_new[i] = r[i] * ib;
+            u_new[i] = z[i] * ib;
+        }
+    }
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -2530,6 +2633,65 @@ int qmrsym_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteratio
     return S.finish(it, code);
 }
 
+// MinRes (MinRes.cxx:47-151): per iteration r = A u with (r, u), z and the
+// r / z recurrences with (r, z), the QR step, and the w / x / v / u update;
+// the w triple and the v, u pairs rotate through their buffers instead of
+// being copied (Copy(w_old, w_oold); Copy(w, w_old); ...).
+template <typename T>
+int minres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 9, 0, it));
+    const int64_t n = S.n;
+    T *r = S.take(n), *z = S.take(n);
+    T *wb[3] = {S.take(n), S.take(n), S.take(n)};
+    T *vb[2] = {S.take(n), S.take(n)}, *ub[2] = {S.take(n), S.take(n)};
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (MinRes.cxx:63)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = r;
+    }
+    hipLaunchKernelGGL(k_mr_init<T>, g, blk, 0, st, n, src, b, dg, x, r, z, it->init_guess_null ? 1 : 0, S.dst,
+                       S.partials);
+    S.fin(FinMrInit<T>{S.dst});
+    hipLaunchKernelGGL(k_mr_start<T>, g, blk, 0, st, n, (const T *)r, (const T *)z, vb[0], ub[0], vb[1], ub[1],
+                       wb[0], wb[1], wb[2], S.dst);
+    SHK_HIP(hipGetLastError());
+    int64_t t = 0;  // iterations enqueued: the buffers' rotation
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++, t++) {
+            T *u = ub[t % 2], *u_old = ub[(t + 1) % 2], *v = vb[t % 2], *v_old = vb[(t + 1) % 2];
+            T *w = wb[t % 3], *w_old = wb[(t + 2) % 3], *w_new = wb[(t + 1) % 3];
+            SpmvArgs<T> a{};
+            a.x = u; a.y_out = r; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = u; a.skip = &S.dst->skip;
+            a.partials = S.part[1];
+            SHK_TRY(S.spmv(a, FinDefer{}));
+            grid_finish<T>(cs, (const T *)S.part[1], a.grid_used, (const int *)&S.dst->skip, FinMrAlpha<T>{S.dst});
+            hipLaunchKernelGGL(k_mr_rz<T>, g, blk, 0, cs, n, dg, (const T *)v, (const T *)u, (const T *)v_old,
+                               (const T *)u_old, r, z, S.dst, S.partials);
+            S.fin(FinMrStep<T>{S.dst, k});
+            hipLaunchKernelGGL(k_mr_wx<T>, g, blk, 0, cs, n, (const T *)u, (const T *)w, (const T *)w_old, w_new, x,
+                               (const T *)r, (const T *)z, v_old, u_old, S.dst, k);
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "minres: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -2544,6 +2706,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 8) return lsqr_impl<T>(ctx, A, (T *)dx, (const T *)db, it, code);
     if (which == 10) return bicgcr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 11) return qmrsym_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 12) return minres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -2744,6 +2907,11 @@ int shk_bicgcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
 int shk_qmrsym(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(11, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_minres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(12, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

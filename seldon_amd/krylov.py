@@ -554,6 +554,11 @@ def QmrSym(A, x, b, M, it):
     return _solve(L.load().shk_qmrsym, A, x, b, M, it)
 
 
+def MinRes(A, x, b, M, it):
+    """MinRes (computation/solver/iterative/MinRes.cxx:47-151) on the GPU, fused."""
+    return _solve(L.load().shk_minres, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

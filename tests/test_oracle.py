@@ -1050,3 +1050,22 @@ def test_oracle_qmrsym_matches_seldon_fixture_and_residual():
     bb = wl.uniform(n, 19)
     r = orc.solve("qmrsym", wl.CSR(n, n, *eye, np.full(n, 2.5)), bb, tol=1e-12, max_iter=20)
     assert r.code == 0 and r.nb_iter == 1 and np.allclose(r.x, bb / 2.5, rtol=1e-14)
+
+
+def test_oracle_minres_matches_seldon_fixture_and_estimate():
+    """Seldon's own MinRes (MinRes.cxx:47-151, the overlay program's CPU leg
+    on the 64 x 64 Laplacian): history, iteration numbers, count, bitwise.
+    Its stopping quantity np = |b| prod |s| is the true residual norm of
+    the iterate in exact arithmetic: at the end it matches |b - A x| to
+    rounding (identity), and the iterate solves A x = b."""
+    import math
+    fx = _overlay_fixture()
+    A = wl.laplace2d(64)
+    xt = np.array([math.sin(0.37 * i) + 0.1 * math.cos(1.3 * i) for i in range(A.m)])
+    b = orc.mlt(A, xt)
+    r = orc.solve("minres", A, b, tol=1e-8, max_iter=2000)
+    assert r.nb_iter == int(fx["minres_cpu_nb"]) and r.code == int(fx["minres_cpu_code"]) == 0
+    assert np.array_equal(r.hist_it, fx["minres_cpu_its"]) and np.array_equal(r.hist, fx["minres_cpu_hist"])
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15, (true, r.hist[-1])
+    assert np.linalg.norm(r.x - xt) <= 1e-8 * 2e3 * np.linalg.norm(xt)  # kappa(A) ~ 1.7e3 times the tolerance
