@@ -347,6 +347,12 @@ int shk_qmrsym(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
  * the value type.  Error codes 1 / 2 are its breakdowns. */
 int shk_minres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
                shk_iteration *iter, int *code);
+/* Symmlq (computation/solver/iterative/Symmlq.cxx:45-161), fused likewise:
+ * the stopping test on np = s_prod / |c| (double, as Titer), the last
+ * x += ceta_bar w_bar after the loop, as the template; it reports no
+ * breakdown (error code 0 or -2). */
+int shk_symmlq(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+               shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

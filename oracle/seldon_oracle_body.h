@@ -1127,6 +1127,90 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- Symmlq: computation/solver/iterative/Symmlq.cxx:45-161 ---------- *
+ * All scalars the value type T except np and s_prod (Titer = double); the
+ * last step x += ceta_bar w_bar follows the loop, whatever ended it. */
+int FN(symmlq)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+               orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T alpha, beta, ibeta, beta_old, beta1, ceta = 0, ceta_oold, ceta_old, ceta_bar;
+    T c, cold, s, sold, coold, soold, rho0, rho1, rho2, rho3, dp;
+    double np, s_prod;
+    T *r = FN(dup)(N, b), *z = FN(dup)(N, b), *u = FN(dup)(N, b), *v = FN(dup)(N, b),
+      *w = FN(dup)(N, b), *u_old = FN(dup)(N, b), *v_old = FN(dup)(N, b), *w_bar = FN(dup)(N, b);
+    FN(fill)(N, (T)0, u_old); FN(fill)(N, (T)0, v_old); FN(fill)(N, (T)0, w); FN(fill)(N, (T)0, w_bar);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    ceta_oold = 0; ceta_old = 0;
+    c = 1; cold = 1; s = 0; sold = 0;
+    FN(precond)(precond, N, diag, r, z);
+    dp = FN(dot)(N, r, z);
+    dp = (T)sqrt(dp);
+    beta = dp;
+    beta1 = beta;
+    s_prod = (double)(beta1 < 0 ? -beta1 : beta1);
+    FN(copy)(N, r, v);
+    FN(copy)(N, z, u);
+    ibeta = (T)(1.0 / (double)beta);
+    FN(scal)(N, ibeta, v);
+    FN(scal)(N, ibeta, u);
+    FN(copy)(N, u, w_bar);
+    np = (double)FN(norm2)(N, b);
+    it->nb_iter = 0;
+    while (!orc_finished(it, np, it->nb_iter, it->max_iter)) {
+        if (it->nb_iter != 0) {
+            FN(copy)(N, v, v_old);
+            FN(copy)(N, u, u_old);
+            ibeta = (T)(1.0 / (double)beta);
+            FN(copy)(N, r, v);
+            FN(scal)(N, ibeta, v);
+            FN(copy)(N, z, u);
+            FN(scal)(N, ibeta, u);
+            FN(copy)(N, w_bar, w);
+            FN(scal)(N, c, w);
+            FN(add)(N, s, u, w);
+            FN(scal)(N, -s, w_bar);
+            FN(add)(N, c, u, w_bar);
+            FN(add)(N, ceta, w, x);
+            ceta_oold = ceta_old;
+            ceta_old = ceta;
+        }
+        FN(mlt)(A, u, r);
+        alpha = FN(dot)(N, u, r);
+        FN(precond)(precond, N, diag, r, z);
+        FN(add)(N, -alpha, v, r);
+        FN(add)(N, -alpha, u, z);
+        FN(add)(N, -beta, v_old, r);
+        FN(add)(N, -beta, u_old, z);
+        beta_old = beta;
+        dp = FN(dot)(N, r, z);
+        beta = (T)sqrt(dp);
+        coold = cold; cold = c; soold = sold; sold = s;
+        rho0 = cold * alpha - coold * sold * beta_old;
+        rho1 = (T)sqrt(rho0 * rho0 + beta * beta);
+        rho2 = sold * alpha + coold * cold * beta_old;
+        rho3 = soold * beta_old;
+        c = rho0 / rho1;
+        s = beta / rho1;
+        if (it->nb_iter == 0) ceta = beta1 / rho1;
+        else ceta = -(rho2 * ceta_old + rho3 * ceta_oold) / rho1;
+        s_prod *= (double)(s < 0 ? -s : s);
+        if (c == (T)0) np = s_prod * 1e16;
+        else np = s_prod / (double)(c < 0 ? -c : c);
+        it->nb_iter++;
+    }
+    if (c == (T)0) ceta_bar = (T)((double)ceta * 1e15);
+    else ceta_bar = ceta / c;
+    FN(add)(N, ceta_bar, w_bar, x);
+done:
+    free(r); free(z); free(u); free(v); free(w); free(u_old); free(v_old); free(w_bar);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

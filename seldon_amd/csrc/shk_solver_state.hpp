@@ -38,7 +38,7 @@ struct KState {
     T ls_rho_bar, ls_phi_bar, ls_xs, ls_ws;
     // MinRes: the residual estimate np (Titer = double) and the value-type
     // scalars of its recurrence (indices kMr*)
-    double mr_np;
+    double mr_np, sy_sprod;  // Symmlq keeps its np in mr_np too
     T mr[14];
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
@@ -794,6 +794,72 @@ struct FinMrStep {
             st->stop_slot = slot;
             it_stop(st);
         }
+    }
+};
+
+// ------------------------------------------------------------ Symmlq ----
+// Symmlq's value-type scalars share MinRes's array: beta, alpha, c, s,
+// cold, sold as there, and these
+enum { kSyBeta1 = kMrRho2, kSyCeta = kMrRho3, kSyCetaOld = kMrIrho1 };
+
+// dp = sqrt((r, z)); beta = beta1 = dp; s_prod = |beta1|; the rotation's
+// start; ceta = ceta_old = 0; np = |b|; the first Finished(np)
+// (Symmlq.cxx:74-92)
+template <typename T>
+struct FinSyInit {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        T *m = st->mr;
+        const T dp = sqrt_t(s[0]);
+        m[kMrBeta] = dp;
+        m[kSyBeta1] = dp;
+        st->sy_sprod = (double)(dp < (T)0 ? -dp : dp);
+        m[kMrIbeta] = (T)(1.0 / (double)dp);
+        m[kMrC] = (T)1;
+        m[kMrS] = (T)0;
+        m[kMrCold] = (T)1;
+        m[kMrSold] = (T)0;
+        m[kSyCeta] = (T)0;
+        m[kSyCetaOld] = (T)0;
+        st->mr_np = (double)sqrt_t(s[1]);
+        st->nb_iter = 0;
+        if (it_finished(st, st->mr_np, st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+// beta = sqrt((r, z)); the QR step and Givens rotation; ceta (the first
+// iteration from beta1); s_prod *= |s|; np = s_prod / |c| (or s_prod 1e16
+// when c = 0); ++iter; Finished(np) (:122-150).  kMrIbeta keeps 1 / beta
+// for the next iteration's v, u (:99)
+template <typename T>
+struct FinSyStep {
+    KState<T> *st;
+    __device__ void operator()(const T (&sm)[2]) const
+    {
+        T *m = st->mr;
+        const T beta_old = m[kMrBeta], alpha = m[kMrAlpha];
+        const T beta = sqrt_t(sm[0]);
+        m[kMrBeta] = beta;
+        const T coold = m[kMrCold], cold = m[kMrC], soold = m[kMrSold], sold = m[kMrS];
+        m[kMrCold] = cold;
+        m[kMrSold] = sold;
+        const T rho0 = cold * alpha - coold * sold * beta_old;
+        const T rho1 = sqrt_t(rho0 * rho0 + beta * beta);
+        const T rho2 = sold * alpha + coold * cold * beta_old;
+        const T rho3 = soold * beta_old;
+        const T c = rho0 / rho1, s = beta / rho1;
+        m[kMrC] = c;
+        m[kMrS] = s;
+        const T ceta_old = m[kSyCeta], ceta_oold = m[kSyCetaOld];
+        const T ceta = st->nb_iter == 0 ? m[kSyBeta1] / rho1 : -(rho2 * ceta_old + rho3 * ceta_oold) / rho1;
+        m[kSyCetaOld] = ceta_old;
+        m[kSyCeta] = ceta;
+        m[kMrIbeta] = (T)(1.0 / (double)beta);
+        st->sy_sprod *= (double)(s < (T)0 ? -s : s);
+        st->mr_np = c == (T)0 ? st->sy_sprod * 1e16 : st->sy_sprod / (double)(c < (T)0 ? -c : c);
+        st->nb_iter++;
+        if (it_finished(st, st->mr_np, st->nb_iter, st->max_iter)) it_stop(st);
     }
 };
 

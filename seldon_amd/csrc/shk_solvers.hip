@@ -1544,6 +1544,65 @@ __global__ __launch_bounds__(kBlock) void k_mr_wx(int64_t n, const T *__restrict
     }
 }
 
+// ------------------------------------------------------------ Symmlq ----
+// v = r ibeta; u = z ibeta; w_bar = u; v_old = u_old = 0 (Symmlq.cxx:84-89)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sy_start(int64_t n, const T *__restrict__ r, const T *__restrict__ z,
+                                                     T *__restrict__ v, T *__restrict__ u, T *__restrict__ v_old,
+                                                     T *__restrict__ u_old, T *__restrict__ w_bar,
+                                                     const KState<T> *st)
+{
+    if (st->skip) return;
+    const T ib = st->mr[kMrIbeta];
+    GRID_STRIDE(i, n) {
+        v[i] = r[i] * ib;
+        const T uv = z[i] * ib;
+        u[i] = uv;
+        w_bar[i] = uv;
+        v_old[i] = (T)0;
+        u_old[i] = (T)0;
+    }
+}
+
+// the update of every iteration after the first (:96-113): v = r ibeta,
+// u = z ibeta (into the old v_old / u_old buffers); w = c w_bar + s u;
+// w_bar = -s w_bar + c u; x += ceta w (w is not kept)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sy_upd(int64_t n, const T *__restrict__ r, const T *__restrict__ z,
+                                                   T *__restrict__ v_new, T *__restrict__ u_new,
+                                                   T *__restrict__ w_bar, T *__restrict__ x, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T *m = st->mr;
+    const T ib = m[kMrIbeta], c = m[kMrC], s = m[kMrS], ms = -m[kMrS], ceta = m[kSyCeta];
+    const bool as = s != (T)0, ac = c != (T)0, ae = ceta != (T)0;
+    GRID_STRIDE(i, n) {
+        v_new[i] = r[i] * ib;
+        const T uv = z[i] * ib;
+        u_new[i] = uv;
+        const T wb = w_bar[i];
+        T wv = wb * c;
+        if (as) wv += s * uv;
+        T wbn = wb * ms;
+        if (ac) wbn += c * uv;
+        w_bar[i] = wbn;
+        if (ae) x[i] += ceta * wv;
+    }
+}
+
+// after the loop: x += ceta_bar w_bar, ceta_bar = ceta / c (ceta 1e15 when
+// c = 0) (:153-158)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sy_final(int64_t n, const T *__restrict__ w_bar, T *__restrict__ x,
+                                                     const KState<T> *st)
+{
+    if (st->init_failed) return;
+    const T c = st->mr[kMrC], ceta = st->mr[kSyCeta];
+    const T cb = c == (T)0 ? (T)((double)ceta * 1e15) : ceta / c;
+    if (cb == (T)0) return;
+    GRID_STRIDE(i, n) x[i] += cb * w_bar[i];
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -2692,6 +2751,66 @@ int minres_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteratio
     return S.finish(it, code);
 }
 
+// Symmlq (Symmlq.cxx:45-161): per iteration (after the first) the v / u /
+// w_bar / x update, then r = A u with (u, r), z and the r / z recurrences
+// with (r, z) (MinRes's kernel), the QR step; after the loop the last
+// x += ceta_bar w_bar.  v and u rotate through two buffers each.
+template <typename T>
+int symmlq_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 7, 0, it));
+    const int64_t n = S.n;
+    T *r = S.take(n), *z = S.take(n), *w_bar = S.take(n);
+    T *vb[2] = {S.take(n), S.take(n)}, *ub[2] = {S.take(n), S.take(n)};
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (Symmlq.cxx:65)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = r;
+    }
+    hipLaunchKernelGGL(k_mr_init<T>, g, blk, 0, st, n, src, b, dg, x, r, z, it->init_guess_null ? 1 : 0, S.dst,
+                       S.partials);
+    S.fin(FinSyInit<T>{S.dst});
+    hipLaunchKernelGGL(k_sy_start<T>, g, blk, 0, st, n, (const T *)r, (const T *)z, vb[0], ub[0], vb[1], ub[1],
+                       w_bar, S.dst);
+    SHK_HIP(hipGetLastError());
+    int64_t t = 0;  // iterations enqueued: the v / u buffers' rotation
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) + 2;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++, t++) {
+            T *u = ub[t % 2], *u_old = ub[(t + 1) % 2], *v = vb[t % 2], *v_old = vb[(t + 1) % 2];
+            if (t > 0)
+                hipLaunchKernelGGL(k_sy_upd<T>, g, blk, 0, cs, n, (const T *)r, (const T *)z, v, u, w_bar, x,
+                                   S.dst);
+            SpmvArgs<T> a{};
+            a.x = u; a.y_out = r; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = u; a.skip = &S.dst->skip;
+            a.partials = S.part[1];
+            SHK_TRY(S.spmv(a, FinDefer{}));
+            grid_finish<T>(cs, (const T *)S.part[1], a.grid_used, (const int *)&S.dst->skip, FinMrAlpha<T>{S.dst});
+            hipLaunchKernelGGL(k_mr_rz<T>, g, blk, 0, cs, n, dg, (const T *)v, (const T *)u, (const T *)v_old,
+                               (const T *)u_old, r, z, S.dst, S.partials);
+            S.fin(FinSyStep<T>{S.dst});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "symmlq: solver did not reach a stopping state");
+    hipLaunchKernelGGL(k_sy_final<T>, g, blk, 0, st, n, (const T *)w_bar, x, S.dst);
+    SHK_HIP(hipGetLastError());
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -2707,6 +2826,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 10) return bicgcr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 11) return qmrsym_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 12) return minres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 13) return symmlq_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -2912,6 +3032,11 @@ int shk_qmrsym(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
 int shk_minres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(12, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_symmlq(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(13, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

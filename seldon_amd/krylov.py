@@ -559,6 +559,11 @@ def MinRes(A, x, b, M, it):
     return _solve(L.load().shk_minres, A, x, b, M, it)
 
 
+def Symmlq(A, x, b, M, it):
+    """Symmlq (computation/solver/iterative/Symmlq.cxx:45-161) on the GPU, fused."""
+    return _solve(L.load().shk_symmlq, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

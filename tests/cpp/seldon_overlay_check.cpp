@@ -14,8 +14,8 @@
 // Symmlq, CoCg, BiCgcr, QmrSym, and Cg with Seldon's SorPreconditioner
 // (SSOR) -- with Seldon's own templates, once on the CPU storages and once on
 // the GPU storages (Mlt(SeldonTrans, ...), SOR, Swap and the rest through the
-// overload set; BiCg, Qmr, Cgne, Lsqr, Cgs, TfQmr, CoCg, BiCgcr, QmrSym and
-// MinRes with
+// overload set; BiCg, Qmr, Cgne, Lsqr, Cgs, TfQmr, CoCg, BiCgcr, QmrSym,
+// MinRes and Symmlq with
 // Preconditioner_Base reach the library's fused solvers, and once more,
 // with a user preconditioner type, Seldon's templates: labels
 // <solver>_gpu_generic),
@@ -245,7 +245,7 @@ int main(int argc, char** argv)
       {
         if (cpu) run_trans_solver(solvers[k], "cpu", A, bh, 2000, 1e-8);
         if (gpu) run_trans_solver(solvers[k], "gpu", Ag, bg, 2000, 1e-8);
-        if (gpu && k != 1)
+        if (gpu)
           run_trans_solver<Matrix<double, General, RowSparseHip, HipAlloc<double> >,
                            Vector<double, VectHip, HipAlloc<double> >, UserIdentity>(solvers[k], "gpu_generic",
                                                                                    Ag, bg, 2000, 1e-8);

@@ -47,7 +47,8 @@ def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float6
         x = torch.full_like(bg, 3.0)  # overwritten: x.Zero()
     P = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
     code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr, "qmr": K.Qmr, "cgne": K.Cgne,
-            "lsqr": K.Lsqr, "bicgcr": K.BiCgcr, "qmrsym": K.QmrSym, "minres": K.MinRes}[method](M, x, bg, P, it)
+            "lsqr": K.Lsqr, "bicgcr": K.BiCgcr, "qmrsym": K.QmrSym, "minres": K.MinRes,
+            "symmlq": K.Symmlq}[method](M, x, bg, P, it)
     n = min(it.hist_len, len(it.history))
     return code, it.nb_iter, it.history[:n].copy(), it.history_it[:n].copy(), x.cpu().numpy()
 
@@ -211,12 +212,12 @@ def _laplace64_overlay():
     return A, orc.mlt(A, xt)
 
 
-SYM = ["bicgcr", "qmrsym", "minres"]
+SYM = ["bicgcr", "qmrsym", "minres", "symmlq"]
 
 
 @pytest.mark.parametrize("method", SYM)
 def test_sym_matches_seldon_run(gpu, method):
-    """The fused BiCgcr, QmrSym and MinRes against Seldon's own runs (the overlay
+    """The fused BiCgcr, QmrSym, MinRes and Symmlq against Seldon's own runs (the overlay
     program's 64 x 64 Laplacian): same count and iteration numbers, every
     entry within 1e-6 (Seldon's generic, FMA and MKL builds agree bitwise
     on them)."""

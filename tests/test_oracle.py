@@ -1069,3 +1069,21 @@ def test_oracle_minres_matches_seldon_fixture_and_estimate():
     true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
     assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15, (true, r.hist[-1])
     assert np.linalg.norm(r.x - xt) <= 1e-8 * 2e3 * np.linalg.norm(xt)  # kappa(A) ~ 1.7e3 times the tolerance
+
+
+def test_oracle_symmlq_matches_seldon_fixture():
+    """Seldon's own Symmlq (Symmlq.cxx:45-161, the overlay program's CPU leg
+    on the 64 x 64 Laplacian): history, iteration numbers, count, bitwise;
+    after its last step x solves A x = b to kappa times the tolerance; and
+    without that step (max_iter reached) x is further from it."""
+    import math
+    fx = _overlay_fixture()
+    A = wl.laplace2d(64)
+    xt = np.array([math.sin(0.37 * i) + 0.1 * math.cos(1.3 * i) for i in range(A.m)])
+    b = orc.mlt(A, xt)
+    r = orc.solve("symmlq", A, b, tol=1e-8, max_iter=2000)
+    assert r.nb_iter == int(fx["symmlq_cpu_nb"]) and r.code == int(fx["symmlq_cpu_code"]) == 0
+    assert np.array_equal(r.hist_it, fx["symmlq_cpu_its"]) and np.array_equal(r.hist, fx["symmlq_cpu_hist"])
+    assert np.linalg.norm(r.x - xt) <= 1e-8 * 2e3 * np.linalg.norm(xt)
+    r2 = orc.solve("symmlq", A, b, tol=1e-8, max_iter=60)
+    assert r2.code == -2 and np.linalg.norm(r2.x - xt) > np.linalg.norm(r.x - xt)
