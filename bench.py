@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--only", choices=["convdiff128", "batched"],
                     help="run one extra section alone and print its JSON (for profiling its kernels)")
     ap.add_argument("--probe", action="store_true",
-                    help="profiling run: one 1 GiB copy kernel (counter calibration) and 10 FP64 + 10 FP32 SpMVs")
+                    help="profiling run: one 1 GiB copy kernel (counter calibration) and 40 FP64 + 40 FP32 SpMVs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU only: the sparse Krylov path is "
@@ -114,9 +114,9 @@ def main():
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)
         dst.copy_(src)
-        for _ in range(10):
+        for _ in range(40):
             K.Mlt(A64, x64, y64)
-        for _ in range(10):
+        for _ in range(40):
             K.Mlt(A32, x32, y32)
         torch.cuda.synchronize()
         log("probe done")
