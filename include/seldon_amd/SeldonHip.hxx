@@ -16,7 +16,7 @@
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
 // more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr / Cgne / Lsqr / CoCg / BiCgcr /
-// QmrSym / MinRes / Symmlq below replace the whole loop by the
+// QmrSym / MinRes / Symmlq / QCgs below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
@@ -659,7 +659,8 @@ namespace Seldon
     else if (which == 10) HipCheck(shk_bicgcr(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCgcr");
     else if (which == 11) HipCheck(shk_qmrsym(ctx, A, x, b, (shk_precond)prec, &it, &code), "QmrSym");
     else if (which == 12) HipCheck(shk_minres(ctx, A, x, b, (shk_precond)prec, &it, &code), "MinRes");
-    else HipCheck(shk_symmlq(ctx, A, x, b, (shk_precond)prec, &it, &code), "Symmlq");
+    else if (which == 13) HipCheck(shk_symmlq(ctx, A, x, b, (shk_precond)prec, &it, &code), "Symmlq");
+    else HipCheck(shk_qcgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "QCgs");
     iter.*Acc::Fac() = Titer(it.facteur_reste);
     const int n = std::min(it.hist_len, it.hist_cap);
     for (int k = 0; k < n; k++)
@@ -671,7 +672,7 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[14][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
+        static const char* what[15][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
                                           "Bicgstab breakdown #3", "", "", ""},
                                          {"", "", "", "", "", "", ""},
@@ -690,7 +691,9 @@ namespace Seldon
                                          {"", "Qmr breakdown #1", "", "Qmr breakdown #2", "Qmr breakdown #3",
                                           "Qmr breakdown #4", "Qmr breakdown #5"},
                                          {"", "Minres breakdown #1", "MinRes breakdown #2", "", "", "", ""},
-                                         {"", "", "", "", "", "", ""}};
+                                         {"", "", "", "", "", "", ""},
+                                         {"", "QCgs breakdown #1", "Qcgs breakdown #2", "Qcgs breakdown #3", "", "",
+                                          ""}};
         const int c = it.error_code;
         iter.Fail(c, (c >= 0 && c < 7) ? what[which][c] : "breakdown");
       }
@@ -763,6 +766,15 @@ namespace Seldon
        Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(9, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  // QCgs.cxx:45-47
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  QCgs(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+       Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(14, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   // Symmlq.cxx:44-46

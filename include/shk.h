@@ -353,6 +353,13 @@ int shk_minres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
  * breakdown (error code 0 or -2). */
 int shk_symmlq(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
                shk_iteration *iter, int *code);
+/* QCgs (computation/solver/iterative/QCgs.cxx:47-164), fused likewise: Cgs
+ * with the quasi-minimal-residual smoothing step (its five dot products in
+ * one reduction), the stopping test on the smoothed residual, and x =
+ * M^{-1} x_qcgs at the end, as the template.  Error codes 1-3 are its
+ * breakdowns. */
+int shk_qcgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+             shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

@@ -1211,6 +1211,83 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- QCgs: computation/solver/iterative/QCgs.cxx:47-164 ------------- *
+ * Cgs with a quasi-minimal-residual smoothing step: the 2 x 2 system of
+ * (u, u), (v, v), (u, v) with u = r_qcgs - r; the stopping test is on
+ * r_qcgs; after the loop x = M^{-1} x_qcgs, as the template has it. */
+int FN(qcgs)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+             orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    T rho_1, rho_2, mu, nu, alpha, beta, sigma, delta;
+    T b00, b11, b10, m00, m11, m10, rn0, rn1;
+    T *p = FN(dup)(N, b), *q = FN(dup)(N, b), *r = FN(dup)(N, b), *rtilde = FN(dup)(N, b),
+      *u = FN(dup)(N, b), *phat = FN(dup)(N, b), *r_qcgs = FN(dup)(N, b), *x_qcgs = FN(dup)(N, b),
+      *v = FN(dup)(N, b);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, r);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(copy)(N, r, rtilde);
+    rho_1 = (T)1;
+    FN(fill)(N, (T)0, q);
+    FN(fill)(N, (T)0, p);
+    FN(copy)(N, r, r_qcgs);
+    FN(copy)(N, x, x_qcgs);
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r_qcgs), it->nb_iter, it->max_iter)) {
+        rho_2 = FN(dot)(N, rtilde, r);
+        if (rho_1 == (T)0) { orc_fail(it, 1); break; }
+        beta = rho_2 / rho_1;
+        FN(copy)(N, r, u);
+        FN(add)(N, beta, q, u);
+        FN(scal)(N, beta, p);
+        FN(add)(N, (T)1, q, p);
+        FN(scal)(N, beta, p);
+        FN(add)(N, (T)1, u, p);
+        FN(precond)(precond, N, diag, p, phat);
+        FN(mlt)(A, phat, v);
+        it->nb_iter++;
+        sigma = FN(dot)(N, rtilde, v);
+        if (sigma == (T)0) { orc_fail(it, 2); break; }
+        alpha = rho_2 / sigma;
+        FN(copy)(N, u, q);
+        FN(add)(N, -alpha, v, q);
+        FN(add)(N, (T)1, q, u);
+        FN(add)(N, alpha, u, x);
+        FN(precond)(precond, N, diag, u, phat);
+        FN(mlt)(A, phat, u);
+        FN(add)(N, -alpha, u, r);
+        FN(copy)(N, r_qcgs, u);
+        FN(add)(N, -(T)1, r, u);
+        b00 = FN(dot)(N, u, u);
+        b11 = FN(dot)(N, v, v);
+        b10 = FN(dot)(N, u, v);
+        delta = b00 * b11 - b10 * b10;
+        if (delta == (T)0) { orc_fail(it, 3); break; }
+        m00 = b11 / delta;
+        m11 = b00 / delta;
+        m10 = -b10 / delta;
+        rn0 = -FN(dot)(N, u, r);
+        rn1 = -FN(dot)(N, v, r);
+        mu = m00 * rn0 + m10 * rn1;
+        nu = m10 * rn0 + m11 * rn1;
+        FN(copy)(N, r, r_qcgs);
+        FN(add)(N, mu, u, r_qcgs);
+        FN(add)(N, nu, v, r_qcgs);
+        FN(scal)(N, mu, x_qcgs);
+        FN(add)(N, (T)1 - mu, x, x_qcgs);
+        FN(add)(N, -nu, p, x_qcgs);
+        rho_1 = rho_2;
+        it->nb_iter++;
+    }
+    FN(precond)(precond, N, diag, x_qcgs, x);
+done:
+    free(p); free(q); free(r); free(rtilde); free(u); free(phat); free(r_qcgs); free(x_qcgs); free(v);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

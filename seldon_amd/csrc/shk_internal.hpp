@@ -232,6 +232,22 @@ __global__ __launch_bounds__(kBlock) void k_grid_finish(const T *partials, int n
     if (threadIdx.x == 0) fin(out);
 }
 
+// The same for six values (three pairs at partials + 2 k nblocks).
+template <typename T, typename Fin>
+__global__ __launch_bounds__(kBlock) void k_grid_finish6(const T *partials, int nblocks, const int *skip, Fin fin)
+{
+    if (skip && *skip) return;
+    T out[6];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        T t[2];
+        grid_total<T>(partials + (size_t)2 * k * nblocks, nblocks, t);
+        out[2 * k] = t[0];
+        out[2 * k + 1] = t[1];
+    }
+    if (threadIdx.x == 0) fin(out);
+}
+
 // The second launch of a split grid reduction (see block_partial).
 template <typename T, typename Fin>
 inline void grid_finish(hipStream_t st, const T *partials, int nblocks, const int *skip, Fin fin)

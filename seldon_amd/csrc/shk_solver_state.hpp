@@ -40,6 +40,7 @@ struct KState {
     // scalars of its recurrence (indices kMr*)
     double mr_np, sy_sprod;  // Symmlq keeps its np in mr_np too
     T mr[14];
+    T qc_mu, qc_nu;  // QCgs's smoothing coefficients
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -860,6 +861,69 @@ struct FinSyStep {
         st->mr_np = c == (T)0 ? st->sy_sprod * 1e16 : st->sy_sprod / (double)(c < (T)0 ? -c : c);
         st->nb_iter++;
         if (it_finished(st, st->mr_np, st->nb_iter, st->max_iter)) it_stop(st);
+    }
+};
+
+// -------------------------------------------------------------- QCgs ----
+// Loop head (QCgs.cxx:80-90): [++iter;] Finished(r_qcgs); rho_2 =
+// (rtilde, r); breakdown #1 (rho_1 = 0); beta = rho_2 / rho_1.  The first
+// head starts rho_1 at 1 (:70).
+template <typename T>
+struct FinQcTop {
+    KState<T> *st;
+    int first;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (first) st->rho_1 = (T)1;
+        else st->nb_iter++;
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        st->rho_2 = s[1];
+        if (st->rho_1 == (T)0) {
+            it_fail(st, 1);
+            return;
+        }
+        st->beta = s[1] / st->rho_1;
+    }
+};
+
+// ++iter; sigma = (rtilde, v); breakdown #2; alpha = rho_2 / sigma (:103-112)
+template <typename T>
+struct FinQcSigma {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->nb_iter++;
+        st->sigma = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 2);
+            return;
+        }
+        st->alpha = st->rho_2 / s[0];
+    }
+};
+
+// The smoothing step's 2 x 2 system from (u, u), (v, v), (u, v), (u, r),
+// (v, r): breakdown #3 when its determinant is 0; mu, nu; rho_1 = rho_2
+// (:130-152, bt_b symmetric: bt_b(0,1) = bt_b(1,0))
+template <typename T>
+struct FinQcSmooth {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[6]) const
+    {
+        const T b00 = s[0], b11 = s[1], b10 = s[2];
+        const T delta = b00 * b11 - b10 * b10;
+        if (delta == (T)0) {
+            it_fail(st, 3);
+            return;
+        }
+        const T m00 = b11 / delta, m11 = b00 / delta, m10 = -b10 / delta;
+        const T rn0 = -s[3], rn1 = -s[4];
+        st->qc_mu = m00 * rn0 + m10 * rn1;
+        st->qc_nu = m10 * rn0 + m11 * rn1;
+        st->rho_1 = st->rho_2;
     }
 };
 

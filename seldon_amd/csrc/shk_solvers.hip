@@ -1603,6 +1603,145 @@ __global__ __launch_bounds__(kBlock) void k_sy_final(int64_t n, const T *__restr
     GRID_STRIDE(i, n) x[i] += cb * w_bar[i];
 }
 
+// -------------------------------------------------------------- QCgs ----
+// [x = 0]; r = src; rtilde = r_qcgs = r; x_qcgs = x; q = p = 0
+// (QCgs.cxx:64-76); sums |r_qcgs|^2 and (rtilde, r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qc_init(int64_t n, const T *__restrict__ src, T *__restrict__ x,
+                                                    T *__restrict__ r, T *__restrict__ rt, T *__restrict__ rq,
+                                                    T *__restrict__ xq, T *__restrict__ q, T *__restrict__ p,
+                                                    int zero_x, const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        const T rv = src[i];
+        r[i] = rv;
+        rt[i] = rv;
+        rq[i] = rv;
+        xq[i] = x[i];
+        q[i] = (T)0;
+        p[i] = (T)0;
+        c[0] += rv * rv;
+        c[1] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// u = r + beta q; p = beta (beta p + q) + u; phat = M^{-1} p (:92-103)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qc_up(int64_t n, const T *__restrict__ r, const T *__restrict__ q,
+                                                  T *__restrict__ p, T *__restrict__ u, const T *__restrict__ dg,
+                                                  T *__restrict__ phat, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T beta = st->beta;
+    const bool ab = beta != (T)0;
+    GRID_STRIDE(i, n) {
+        const T qv = q[i];
+        T uv = r[i];
+        if (ab) uv += beta * qv;
+        u[i] = uv;
+        T pv = p[i] * beta;
+        pv += (T)1 * qv;
+        pv *= beta;
+        pv += (T)1 * uv;
+        p[i] = pv;
+        if (dg) phat[i] = pv / dg[i];
+    }
+}
+
+// q = u - alpha v; u += q; x += alpha u; uhat = M^{-1} u (:113-122)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qc_qu(int64_t n, const T *__restrict__ v, T *__restrict__ q,
+                                                  T *__restrict__ u, T *__restrict__ x, const T *__restrict__ dg,
+                                                  T *__restrict__ uhat, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T alpha = st->alpha, malpha = -alpha;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    GRID_STRIDE(i, n) {
+        const T u0 = u[i];
+        T qv = u0;
+        if (am) qv += malpha * v[i];
+        q[i] = qv;
+        T uv = u0;
+        uv += (T)1 * qv;
+        u[i] = uv;
+        if (aa) x[i] += alpha * uv;
+        if (dg) uhat[i] = uv / dg[i];
+    }
+}
+
+// r -= alpha A uhat; u = r_qcgs - r (:125-129); sums (u, u), (v, v),
+// (u, v), (u, r), (v, r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qc_r(int64_t n, const T *__restrict__ w, const T *__restrict__ rq,
+                                                 const T *__restrict__ v, T *__restrict__ r, T *__restrict__ u,
+                                                 const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T malpha = -st->alpha;
+    const bool am = malpha != (T)0;
+    T c[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T rv = r[i];
+        if (am) {
+            rv += malpha * w[i];
+            r[i] = rv;
+        }
+        T uv = rq[i];
+        uv += -(T)1 * rv;
+        u[i] = uv;
+        const T vv = v[i];
+        c[0] += uv * uv;
+        c[1] += vv * vv;
+        c[2] += uv * vv;
+        c[3] += uv * rv;
+        c[4] += vv * rv;
+    }
+    block_partial<T, 6>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// r_qcgs = r + mu u + nu v; x_qcgs = mu x_qcgs + (1 - mu) x - nu p
+// (:154-160); sums |r_qcgs|^2 and (rtilde, r) for the next loop head
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qc_smooth(int64_t n, const T *__restrict__ r, const T *__restrict__ u,
+                                                      const T *__restrict__ v, const T *__restrict__ x,
+                                                      const T *__restrict__ p, const T *__restrict__ rt,
+                                                      T *__restrict__ rq, T *__restrict__ xq, const KState<T> *st,
+                                                      T *partials)
+{
+    if (st->skip) return;
+    const T mu = st->qc_mu, nu = st->qc_nu, omu = (T)1 - st->qc_mu, mnu = -st->qc_nu;
+    const bool amu = mu != (T)0, anu = nu != (T)0, aomu = omu != (T)0, amnu = mnu != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        const T rv = r[i];
+        T rqv = rv;
+        if (amu) rqv += mu * u[i];
+        if (anu) rqv += nu * v[i];
+        rq[i] = rqv;
+        T xqv = xq[i] * mu;
+        if (aomu) xqv += omu * x[i];
+        if (amnu) xqv += mnu * p[i];
+        xq[i] = xqv;
+        c[0] += rqv * rqv;
+        c[1] += rt[i] * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// after the loop: x = M^{-1} x_qcgs (:162)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qc_final(int64_t n, const T *__restrict__ xq, const T *__restrict__ dg,
+                                                     T *__restrict__ x, const KState<T> *st)
+{
+    if (st->init_failed) return;
+    GRID_STRIDE(i, n) x[i] = dg ? xq[i] / dg[i] : xq[i];
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -1860,7 +1999,8 @@ struct Solve {
     double *dhist = nullptr;
     int *dhist_it = nullptr;
     T *partials;
-    T *part[2];             // alternating partial buffers (Gmres's folded finishes)
+    T *part[3];             // partial buffers: two alternating pairs (Gmres's folded finishes) and one of
+                            // six values (QCgs's smoothing step)
     char *ws;
     size_t off = 0;
     // caller-provided work memory (shk_cg_noalloc): vectors, history and
@@ -1889,7 +2029,7 @@ struct Solve {
         size_t hist_cap = it->hist && it->hist_cap > 0 ? (size_t)it->hist_cap : 0;
         size_t need = vec_count * vb + extra_bytes + 256 + hist_cap * (sizeof(double) + sizeof(int)) + 512;
         const size_t maxb = (size_t)std::max(nb, a->nblk_launch);
-        const size_t red = sizeof(T) * 4 * maxb;  // two buffers of [2][maxb]
+        const size_t red = sizeof(T) * 10 * maxb;  // two buffers of [2][maxb], one of [6][maxb]
         if (ext) {
             need = (need + 255) & ~(size_t)255;
             SHK_CHECK(need + red <= ext_cap, SHK_ERR_ARG,
@@ -1907,6 +2047,7 @@ struct Solve {
         off = 0;
         part[0] = partials;
         part[1] = partials + 2 * maxb;
+        part[2] = partials + 4 * maxb;
         dst = (KState<T> *)ctx->state;
         hst = (KState<T> *)ctx->state_host;
         if (hist_cap) {
@@ -2811,6 +2952,71 @@ int symmlq_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteratio
     return S.finish(it, code);
 }
 
+// QCgs (QCgs.cxx:47-164): Cgs's loop plus the smoothing step, whose five
+// dots come from one six-value reduction; after the loop x = M^{-1}
+// x_qcgs, as the template does.
+template <typename T>
+int qcgs_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, jac ? 11 : 10, 0, it));
+    const int64_t n = S.n;
+    T *p = S.take(n), *q = S.take(n), *r = S.take(n), *rt = S.take(n), *u = S.take(n), *rq = S.take(n),
+      *xq = S.take(n), *v = S.take(n), *w = S.take(n), *phat = S.take(n);
+    T *uhat = jac ? S.take(n) : u;
+    const T *pin = jac ? phat : p;  // the identity's M^{-1} p is p itself
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // Init(b) (QCgs.cxx:59)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = r; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = r;
+    }
+    hipLaunchKernelGGL(k_qc_init<T>, g, blk, 0, st, n, src, x, r, rt, rq, xq, q, p, it->init_guess_null ? 1 : 0,
+                       S.dst, S.part[0]);
+    SHK_HIP(hipGetLastError());
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / 2 + 2;
+    bool first = true;
+    SHK_TRY(S.run_chunks(limit, kChunk, [&](hipStream_t cs) -> int {
+        for (int k = 0; k < kChunk; k++) {
+            grid_finish<T>(cs, (const T *)S.part[0], S.nb, (const int *)&S.dst->skip,
+                           FinQcTop<T>{S.dst, first ? 1 : 0});
+            first = false;
+            hipLaunchKernelGGL(k_qc_up<T>, g, blk, 0, cs, n, (const T *)r, (const T *)q, p, u, dg, phat, S.dst);
+            SpmvArgs<T> a{};
+            a.x = pin; a.y_out = v; a.alpha = (T)1; a.mlt = 1;
+            a.dot0 = kDotVec; a.u0 = rt; a.skip = &S.dst->skip;
+            a.partials = S.part[1];
+            SHK_TRY(S.spmv(a, FinDefer{}));
+            grid_finish<T>(cs, (const T *)S.part[1], a.grid_used, (const int *)&S.dst->skip, FinQcSigma<T>{S.dst});
+            hipLaunchKernelGGL(k_qc_qu<T>, g, blk, 0, cs, n, (const T *)v, q, u, x, dg, uhat, S.dst);
+            SpmvArgs<T> a2{};
+            a2.x = uhat; a2.y_out = w; a2.alpha = (T)1; a2.mlt = 1; a2.skip = &S.dst->skip;
+            SHK_TRY(S.spmv(a2, FinNone{}));
+            hipLaunchKernelGGL(k_qc_r<T>, g, blk, 0, cs, n, (const T *)w, (const T *)rq, (const T *)v, r, u, S.dst,
+                               S.part[2]);
+            hipLaunchKernelGGL((k_grid_finish6<T, FinQcSmooth<T>>), dim3(1), dim3(kBlock), 0, cs,
+                               (const T *)S.part[2], S.nb, (const int *)&S.dst->skip, FinQcSmooth<T>{S.dst});
+            hipLaunchKernelGGL(k_qc_smooth<T>, g, blk, 0, cs, n, (const T *)r, (const T *)u, (const T *)v,
+                               (const T *)x, (const T *)p, (const T *)rt, rq, xq, S.dst, S.part[0]);
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "qcgs: solver did not reach a stopping state");
+    hipLaunchKernelGGL(k_qc_final<T>, g, blk, 0, st, n, (const T *)xq, dg, x, S.dst);
+    SHK_HIP(hipGetLastError());
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -2827,6 +3033,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 11) return qmrsym_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 12) return minres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 13) return symmlq_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 14) return qcgs_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -3037,6 +3244,11 @@ int shk_minres(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
 int shk_symmlq(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(13, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_qcgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(14, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

@@ -564,6 +564,11 @@ def Symmlq(A, x, b, M, it):
     return _solve(L.load().shk_symmlq, A, x, b, M, it)
 
 
+def QCgs(A, x, b, M, it):
+    """QCgs (computation/solver/iterative/QCgs.cxx:47-164) on the GPU, fused."""
+    return _solve(L.load().shk_qcgs, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

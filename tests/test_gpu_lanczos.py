@@ -28,8 +28,8 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-CHAOTIC = {"cgs": 40, "tfqmr": 70, "cgne": 100, "lsqr": 100}
-METHODS = ["bicg", "cgs", "tfqmr", "qmr", "cgne", "lsqr"]
+CHAOTIC = {"cgs": 40, "qcgs": 40, "tfqmr": 70, "cgne": 100, "lsqr": 100}
+METHODS = ["bicg", "cgs", "qcgs", "tfqmr", "qmr", "cgne", "lsqr"]
 NORMAL = ("cgne", "lsqr")  # compared on smaller systems
 
 
@@ -48,7 +48,7 @@ def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float6
     P = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
     code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr, "qmr": K.Qmr, "cgne": K.Cgne,
             "lsqr": K.Lsqr, "bicgcr": K.BiCgcr, "qmrsym": K.QmrSym, "minres": K.MinRes,
-            "symmlq": K.Symmlq}[method](M, x, bg, P, it)
+            "symmlq": K.Symmlq, "qcgs": K.QCgs}[method](M, x, bg, P, it)
     n = min(it.hist_len, len(it.history))
     return code, it.nb_iter, it.history[:n].copy(), it.history_it[:n].copy(), x.cpu().numpy()
 
@@ -147,7 +147,7 @@ def test_fused_max_iter_breakdown_and_zero_rhs(gpu, method):
     # Qmr: its first ep = (q, A p), breakdown #4)
     r = orc.solve(method, S, e0, tol=1e-10, max_iter=100)
     code, nb, _, _, x = _gpu(method, S, e0, tol=1e-10, max_iter=100)
-    want = {"tfqmr": (1, 0), "qmr": (4, 0), "cgs": (2, 1), "bicg": (2, 1)}.get(method)
+    want = {"tfqmr": (1, 0), "qmr": (4, 0), "cgs": (2, 1), "qcgs": (2, 1), "bicg": (2, 1)}.get(method)
     assert (code, nb) == (r.code, r.nb_iter)
     assert want is None or (code, nb) == want
     # a zero right-hand side: Init fails; x is left as it was (3.0), except

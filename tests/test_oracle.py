@@ -1087,3 +1087,23 @@ def test_oracle_symmlq_matches_seldon_fixture():
     assert np.linalg.norm(r.x - xt) <= 1e-8 * 2e3 * np.linalg.norm(xt)
     r2 = orc.solve("symmlq", A, b, tol=1e-8, max_iter=60)
     assert r2.code == -2 and np.linalg.norm(r2.x - xt) > np.linalg.norm(r.x - xt)
+
+
+def test_oracle_qcgs_matches_seldon_fixture():
+    """Seldon's own QCgs (QCgs.cxx:47-164, the overlay program's CPU leg on
+    the 24 x 24 upwind system): history, iteration numbers, count, bitwise;
+    the smoothed residual r_qcgs it stops on is the true residual of
+    x = x_qcgs at the end (identity), and a diagonal A is solved."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve("qcgs", A, b, tol=1e-8, max_iter=2000)
+    assert r.code == int(fx["qcgs_cpu_code"]) == 0 and r.nb_iter == int(fx["qcgs_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx["qcgs_cpu_its"]) and np.array_equal(r.hist, fx["qcgs_cpu_hist"])
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15, (true, r.hist[-1])
+    n = 80
+    d = np.array([1.0, 2.0, 3.5, 7.0])[np.arange(n) % 4]
+    D = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    bb = wl.uniform(n, 13)
+    r = orc.solve("qcgs", D, bb, tol=1e-11, max_iter=100)
+    assert r.code == 0 and np.allclose(r.x, bb / d, rtol=1e-9)
