@@ -360,6 +360,13 @@ int shk_symmlq(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
  * breakdowns. */
 int shk_qcgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
              shk_iteration *iter, int *code);
+/* Gcr (computation/solver/iterative/Gcr.cxx:45-139), fused likewise,
+ * restarted every m = iter->parameter_restart iterations; both its outer and
+ * inner stopping tests enter the history, as their Finished() calls do in
+ * Seldon.  Error code 1 is its breakdown (w_j = 0), which here ends the
+ * solve (Seldon's outer loop would go on). */
+int shk_gcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+            shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

@@ -1288,6 +1288,67 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- Gcr: computation/solver/iterative/Gcr.cxx:45-139 --------------- *
+ * Restarted every m = parameter_restart iterations; the inner Iteration is
+ * the outer one with nb_iter's limit set to (its start + m), and both
+ * loops' Finished() calls enter the history.  Seldon's outer loop keeps
+ * going after breakdown #1 (Fail does not end it); here the solve ends
+ * there. */
+int FN(gcr)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+            orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    const int m = it->parameter_restart;
+    T **p = (T **)malloc(sizeof(T *) * (size_t)(m + 1)), **w = (T **)malloc(sizeof(T *) * (size_t)(m + 1));
+    T *beta = (T *)calloc((size_t)(m + 1), sizeof(T));
+    for (int i = 0; i <= m; i++) {
+        p[i] = FN(dup)(N, b); FN(fill)(N, (T)0, p[i]);
+        w[i] = FN(dup)(N, b); FN(fill)(N, (T)0, w[i]);
+    }
+    T *r = FN(dup)(N, b), *q = FN(dup)(N, b), *u = FN(dup)(N, b);
+    T alpha, delta;
+    double normr;
+    int failed = 0;
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, b, u);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, u);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    FN(precond)(precond, N, diag, u, r);
+    normr = (double)FN(norm2)(N, r);
+    it->nb_iter = 0;
+    while (!failed && !orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, it->max_iter)) {
+        const int inner_max = it->nb_iter + m;
+        FN(copy)(N, r, p[0]);
+        for (int i = 0; i < N; i++) p[0][i] *= 1.0 / normr;  /* Mlt(Titer(1) / normr, p[0]) */
+        int j = 0;
+        while (!orc_finished(it, (double)FN(norm2)(N, r), it->nb_iter, inner_max)) {
+            FN(mlt)(A, p[j], u);
+            FN(precond)(precond, N, diag, u, w[j]);
+            beta[j] = FN(dot)(N, w[j], w[j]);
+            if (beta[j] == (T)0) { orc_fail(it, 1); failed = 1; break; }
+            alpha = FN(dot)(N, w[j], r) / beta[j];
+            FN(add)(N, alpha, p[j], x);
+            FN(add)(N, -alpha, w[j], r);
+            it->nb_iter++;
+            FN(mlt)(A, r, u);
+            FN(precond)(precond, N, diag, u, q);
+            FN(copy)(N, r, p[j + 1]);
+            for (int i = 0; i <= j; i++) {
+                delta = -FN(dot)(N, w[i], q) / beta[i];
+                FN(add)(N, delta, p[i], p[j + 1]);
+            }
+            it->nb_iter++;
+            j++;
+        }
+        normr = (double)FN(norm2)(N, r);
+    }
+done:
+    for (int i = 0; i <= m; i++) { free(p[i]); free(w[i]); }
+    free(p); free(w); free(beta); free(r); free(q); free(u);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

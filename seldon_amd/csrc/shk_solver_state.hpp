@@ -41,6 +41,10 @@ struct KState {
     double mr_np, sy_sprod;  // Symmlq keeps its np in mr_np too
     T mr[14];
     T qc_mu, qc_nu;  // QCgs's smoothing coefficients
+    // Gcr: the cycle's inner limit, its norm |r| and gc_skip (1 when the
+    // inner loop of the cycle is over or the solve has ended)
+    double gc_normr;
+    int gc_inner_max, gc_skip;
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -924,6 +928,72 @@ struct FinQcSmooth {
         st->qc_mu = m00 * rn0 + m10 * rn1;
         st->qc_nu = m10 * rn0 + m11 * rn1;
         st->rho_1 = st->rho_2;
+    }
+};
+
+// --------------------------------------------------------------- Gcr ----
+// Outer loop head (Gcr.cxx:82-89): Finished(r) on the outer Iteration; a new
+// cycle's inner Iteration gets the limit nb_iter + m; normr = |r|
+template <typename T>
+struct FinGcOuter {
+    KState<T> *st;
+    int m;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const double rn = (double)sqrt_t(s[0]);
+        st->gc_normr = rn;
+        if (it_finished(st, rn, st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            st->gc_skip = 1;
+            return;
+        }
+        st->gc_inner_max = st->nb_iter + m;
+        st->gc_skip = 0;
+    }
+};
+
+// Inner loop head (:94): Finished(r) against the cycle's limit; true ends
+// the cycle's inner loop (its kernels then return)
+template <typename T>
+struct FinGcInner {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->gc_inner_max)) st->gc_skip = 1;
+    }
+};
+
+// beta_j = (w_j, w_j); breakdown #1; alpha = (w_j, r) / beta_j; the two
+// ++iter of the inner iteration (:103-133; nothing reads the count between
+// them).  The template's outer loop would go on after the breakdown; here
+// the solve ends.
+template <typename T>
+struct FinGcAlpha {
+    KState<T> *st;
+    T *betas;
+    int j;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        betas[j] = s[0];
+        if (s[0] == (T)0) {
+            it_fail(st, 1);
+            st->gc_skip = 1;
+            return;
+        }
+        st->alpha = s[1] / s[0];
+        st->nb_iter += 2;
+    }
+};
+
+// delta_i = -(w_i, q) / beta_i for the group i = g0 .. g0 + cnt - 1 (:124-128)
+template <typename T>
+struct FinGcDelta {
+    const T *betas;
+    T *deltas;
+    int g0, cnt;
+    __device__ void operator()(const T (&s)[6]) const
+    {
+        for (int k = 0; k < cnt; k++) deltas[g0 + k] = -s[k] / betas[g0 + k];
     }
 };
 

@@ -1742,6 +1742,122 @@ __global__ __launch_bounds__(kBlock) void k_qc_final(int64_t n, const T *__restr
     GRID_STRIDE(i, n) x[i] = dg ? xq[i] / dg[i] : xq[i];
 }
 
+// --------------------------------------------------------------- Gcr ----
+// [x = 0]; r = M^{-1} src (src = b - A x, or b) (Gcr.cxx:71-80); sums |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gc_init(int64_t n, const T *__restrict__ src, const T *__restrict__ dg,
+                                                    T *__restrict__ x, T *__restrict__ r, int zero_x,
+                                                    const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        T rv = src[i];
+        if (dg) rv = rv / dg[i];
+        r[i] = rv;
+        c[0] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// p_0 = r * (1 / normr), the factor a double (:90-91)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gc_p0(int64_t n, const T *__restrict__ r, T *__restrict__ p0,
+                                                  const KState<T> *st)
+{
+    if (st->gc_skip) return;
+    const double s = 1.0 / st->gc_normr;
+    GRID_STRIDE(i, n) {
+        T pv = r[i];
+        pv *= s;
+        p0[i] = pv;
+    }
+}
+
+// w_j = M^{-1} (A p_j) (:98-99); sums (w_j, w_j) and (w_j, r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gc_w(int64_t n, const T *__restrict__ u, const T *__restrict__ dg,
+                                                 const T *__restrict__ r, T *__restrict__ w, const KState<T> *st,
+                                                 T *partials)
+{
+    if (st->gc_skip) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T wv = u[i];
+        if (dg) wv = wv / dg[i];
+        w[i] = wv;
+        c[0] += wv * wv;
+        c[1] += wv * r[i];
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// x += alpha p_j; r -= alpha w_j (:109-111); sums |r|^2
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gc_xr(int64_t n, const T *__restrict__ p, const T *__restrict__ w,
+                                                  T *__restrict__ x, T *__restrict__ r, const KState<T> *st,
+                                                  T *partials)
+{
+    if (st->gc_skip) return;
+    const T alpha = st->alpha, malpha = -alpha;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (aa) x[i] += alpha * p[i];
+        T rv = r[i];
+        if (am) {
+            rv += malpha * w[i];
+            r[i] = rv;
+        }
+        c[0] += rv * rv;
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// [q = M^{-1} (A r)] (:116-117); sums (w_i, q) for up to six i from g0
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gc_q(int64_t n, const T *__restrict__ u, const T *__restrict__ dg,
+                                                 T *__restrict__ q, int write_q, const T *__restrict__ wbase,
+                                                 int64_t stride, int g0, int cnt, const KState<T> *st,
+                                                 T *partials)
+{
+    if (st->gc_skip) return;
+    T c[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        T qv;
+        if (write_q) {
+            qv = u[i];
+            if (dg) qv = qv / dg[i];
+            q[i] = qv;
+        } else {
+            qv = q[i];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+            if (k < cnt) c[k] += wbase[(size_t)(g0 + k) * stride + i] * qv;
+    }
+    block_partial<T, 6>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// p_{j+1} = r + sum over i <= j of delta_i p_i, added in i order (:119-129)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gc_p(int64_t n, const T *__restrict__ r, T *__restrict__ pbase,
+                                                 int64_t stride, int j, const T *__restrict__ deltas,
+                                                 const KState<T> *st)
+{
+    if (st->gc_skip) return;
+    T *pn = pbase + (size_t)(j + 1) * stride;
+    GRID_STRIDE(i, n) {
+        T pv = r[i];
+        for (int k = 0; k <= j; k++) {
+            const T d = deltas[k];
+            if (d != (T)0) pv += d * pbase[(size_t)k * stride + i];
+        }
+        pn[i] = pv;
+    }
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -3017,6 +3133,92 @@ int qcgs_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration 
     return S.finish(it, code);
 }
 
+// Gcr (Gcr.cxx:45-139), restarted every m = parameter_restart iterations:
+// per cycle the outer head and p_0; per inner iteration (two of Seldon's
+// iterations, at most ceil(m / 2) per cycle) its head, w_j = M^{-1} A p_j
+// with (w_j, w_j) and (w_j, r), the x / r update, q = M^{-1} A r with the
+// (w_i, q) in groups of six, and p_{j+1}.
+template <typename T>
+int gcr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    const int m = it->parameter_restart;
+    SHK_CHECK(m >= 1 && m <= 1000, SHK_ERR_ARG, "gcr: the restart parameter must be in 1 .. 1000");
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    SHK_TRY(S.setup(ctx, A, 3 + 2 * (size_t)(m + 1), 2 * (((size_t)(m + 1) * sizeof(T) + 255) & ~(size_t)255), it));
+    const int64_t n = S.n;
+    T *r = S.take(n), *q = S.take(n), *u = S.take(n);
+    T *pbase = S.take(n);
+    for (int i = 1; i <= m; i++) S.take(n);
+    T *wbase = S.take(n);
+    for (int i = 1; i <= m; i++) S.take(n);
+    const int64_t stride = wbase - pbase == 0 ? 0 : (int64_t)((wbase - pbase) / (m + 1));
+    SHK_CHECK(stride * (m + 1) == (int64_t)(wbase - pbase), SHK_ERR_STATE, "gcr: work vectors not evenly spaced");
+    T *betas = S.take(m + 1), *deltas = S.take(m + 1);
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    const int *gskip = (const int *)&S.dst->gc_skip;
+    // no cycle is open until the first outer head opens one
+    S.hst->gc_skip = 1;
+    SHK_HIP(hipMemcpyAsync(&S.dst->gc_skip, &S.hst->gc_skip, sizeof(int), hipMemcpyHostToDevice, st));
+    // Init(b) (Gcr.cxx:55)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = u; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = u;
+    }
+    hipLaunchKernelGGL(k_gc_init<T>, g, blk, 0, st, n, src, dg, x, r, it->init_guess_null ? 1 : 0, S.dst,
+                       S.part[0]);
+    SHK_HIP(hipGetLastError());
+    const int ni = (m + 1) / 2;  // inner iterations of a full cycle
+    const int cycles = std::max(1, kChunk / ni);
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / m + 2;
+    SHK_TRY(S.run_chunks(limit, cycles, [&](hipStream_t cs) -> int {
+        for (int c = 0; c < cycles; c++) {
+            grid_finish<T>(cs, (const T *)S.part[0], S.nb, (const int *)&S.dst->skip, FinGcOuter<T>{S.dst, m});
+            hipLaunchKernelGGL(k_gc_p0<T>, g, blk, 0, cs, n, (const T *)r, pbase, S.dst);
+            for (int j = 0; j < ni; j++) {
+                T *pj = pbase + (size_t)j * stride, *wj = wbase + (size_t)j * stride;
+                grid_finish<T>(cs, (const T *)S.part[0], S.nb, gskip, FinGcInner<T>{S.dst});
+                SpmvArgs<T> a{};
+                a.x = pj; a.y_out = u; a.alpha = (T)1; a.mlt = 1; a.skip = gskip;
+                SHK_TRY(S.spmv(a, FinNone{}));
+                hipLaunchKernelGGL(k_gc_w<T>, g, blk, 0, cs, n, (const T *)u, dg, (const T *)r, wj, S.dst,
+                                   S.part[1]);
+                grid_finish<T>(cs, (const T *)S.part[1], S.nb, gskip, FinGcAlpha<T>{S.dst, betas, j});
+                hipLaunchKernelGGL(k_gc_xr<T>, g, blk, 0, cs, n, (const T *)pj, (const T *)wj, x, r, S.dst,
+                                   S.part[0]);
+                SpmvArgs<T> a2{};
+                a2.x = r; a2.y_out = u; a2.alpha = (T)1; a2.mlt = 1; a2.skip = gskip;
+                SHK_TRY(S.spmv(a2, FinNone{}));
+                for (int g0 = 0; g0 <= j; g0 += 6) {
+                    const int cnt = std::min(6, j + 1 - g0);
+                    hipLaunchKernelGGL(k_gc_q<T>, g, blk, 0, cs, n, (const T *)u, dg, q, g0 == 0 ? 1 : 0,
+                                       (const T *)wbase, stride, g0, cnt, S.dst, S.part[2]);
+                    hipLaunchKernelGGL((k_grid_finish6<T, FinGcDelta<T>>), dim3(1), dim3(kBlock), 0, cs,
+                                       (const T *)S.part[2], S.nb, gskip,
+                                       FinGcDelta<T>{(const T *)betas, deltas, g0, cnt});
+                }
+                hipLaunchKernelGGL(k_gc_p<T>, g, blk, 0, cs, n, (const T *)r, pbase, stride, j,
+                                   (const T *)deltas, S.dst);
+            }
+            // the inner head after the cycle's last iteration: it records its
+            // Finished() and ends the inner loop (nb_iter reached the limit)
+            grid_finish<T>(cs, (const T *)S.part[0], S.nb, gskip, FinGcInner<T>{S.dst});
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "gcr: solver did not reach a stopping state");
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -3034,6 +3236,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 12) return minres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 13) return symmlq_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 14) return qcgs_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 15) return gcr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -3249,6 +3452,11 @@ int shk_symmlq(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond pre
 int shk_qcgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(14, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_gcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
+{
+    return solve_entry(15, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

@@ -569,6 +569,12 @@ def QCgs(A, x, b, M, it):
     return _solve(L.load().shk_qcgs, A, x, b, M, it)
 
 
+def Gcr(A, x, b, M, it):
+    """Gcr (computation/solver/iterative/Gcr.cxx:45-139) on the GPU, fused;
+    restarted every it.parameter_restart iterations."""
+    return _solve(L.load().shk_gcr, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

@@ -5,8 +5,8 @@ Qmr.cxx:42-190, Cgne.cxx:41-124 and Lsqr.cxx:41-150.
 
 Bars.  On the overlay program's 24 x 24 upwind system the reference is
 Seldon's own CPU run (tests/golden/overlay_cpu.npz), with the bars of the
-overlay test: BiCg and Qmr -- same iteration count and numbers, every
-residual within 1e-6 relative; Cgs, TfQmr, Cgne and Lsqr, whose
+overlay test: BiCg, Qmr and Gcr (restarted every 10) -- same iteration count
+and numbers, every residual within 1e-6 relative; Cgs, TfQmr, Cgne and Lsqr, whose
 trajectories are rounding-chaotic (Seldon's generic, FMA and MKL builds
 drift apart after 40, 70, 100 and 100 Finished() calls) -- those first
 entries within 1e-6 and the count within max(2, 1%).  The oracle's
@@ -29,7 +29,7 @@ from tests.conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 CHAOTIC = {"cgs": 40, "qcgs": 40, "tfqmr": 70, "cgne": 100, "lsqr": 100}
-METHODS = ["bicg", "cgs", "qcgs", "tfqmr", "qmr", "cgne", "lsqr"]
+METHODS = ["bicg", "cgs", "qcgs", "tfqmr", "qmr", "cgne", "lsqr", "gcr"]
 NORMAL = ("cgne", "lsqr")  # compared on smaller systems
 
 
@@ -48,7 +48,7 @@ def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float6
     P = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
     code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr, "qmr": K.Qmr, "cgne": K.Cgne,
             "lsqr": K.Lsqr, "bicgcr": K.BiCgcr, "qmrsym": K.QmrSym, "minres": K.MinRes,
-            "symmlq": K.Symmlq, "qcgs": K.QCgs}[method](M, x, bg, P, it)
+            "symmlq": K.Symmlq, "qcgs": K.QCgs, "gcr": K.Gcr}[method](M, x, bg, P, it)
     n = min(it.hist_len, len(it.history))
     return code, it.nb_iter, it.history[:n].copy(), it.history_it[:n].copy(), x.cpu().numpy()
 

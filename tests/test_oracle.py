@@ -1107,3 +1107,18 @@ def test_oracle_qcgs_matches_seldon_fixture():
     bb = wl.uniform(n, 13)
     r = orc.solve("qcgs", D, bb, tol=1e-11, max_iter=100)
     assert r.code == 0 and np.allclose(r.x, bb / d, rtol=1e-9)
+
+
+def test_oracle_gcr_matches_seldon_fixture():
+    """Seldon's own Gcr (Gcr.cxx:45-139, restart 10, the overlay program's
+    CPU leg on the 24 x 24 upwind system): history -- the outer and inner
+    Finished() calls, three entries at each restart -- iteration numbers,
+    count, bitwise; its residual r stays the true one."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve("gcr", A, b, tol=1e-8, max_iter=2000, restart=10)
+    assert r.code == int(fx["gcr_cpu_code"]) == 0 and r.nb_iter == int(fx["gcr_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx["gcr_cpu_its"]) and np.array_equal(r.hist, fx["gcr_cpu_hist"])
+    assert list(r.hist_it[:9]) == [0, 0, 2, 4, 6, 8, 10, 10, 10]
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15
