@@ -348,7 +348,7 @@ def cg_config1(K, wl, np, torch, cpu=True):
 
 
 def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
-    """FP64 BiCgStab, Gmres(30), Cgs, BiCg, TfQmr, Qmr, Cgne and Lsqr with the Jacobi preconditioner
+    """FP64 BiCgStab, Gmres(30), Cgs, BiCg, TfQmr, Qmr, Cgne, Lsqr and BiCgStab(10) with the Jacobi preconditioner
     on the 128^3 7-point upwind convection-diffusion matrix (2,097,152 rows),
     to a 1e-10 relative residual: time to solution and iterations/s, plus the
     SpMV bandwidth on this matrix and the time of one symmetric SOR sweep
@@ -392,7 +392,7 @@ def convdiff_solves(K, wl, np, torch, cpu=True, layout="auto"):
     bicg_first = time.perf_counter() - t0
     for name, fn, restart in (("bicgstab", K.BiCgStab, None), ("gmres30", K.Gmres, 30), ("cgs", K.Cgs, None),
                               ("bicg", K.BiCg, None), ("tfqmr", K.TfQmr, None), ("qmr", K.Qmr, None),
-                              ("cgne", K.Cgne, None), ("lsqr", K.Lsqr, None)):
+                              ("cgne", K.Cgne, None), ("lsqr", K.Lsqr, None), ("bicgstabl", K.BiCgStabl, None)):
         x = torch.zeros_like(bg)
         it = K.Iteration(20000, 1e-10)
         if restart:

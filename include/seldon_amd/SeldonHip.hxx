@@ -16,7 +16,7 @@
 // (computation/solver/iterative/Cg.cxx:45-116, doc/guide/iterative.dox:62,223),
 // so Seldon's own templates run primitive by primitive on the GPU; and the
 // more specialised Cg / BiCgStab / Gmres / Cgs / BiCg / TfQmr / Qmr / Cgne / Lsqr / CoCg / BiCgcr /
-// QmrSym / MinRes / Symmlq / QCgs / Gcr below replace the whole loop by the
+// QmrSym / MinRes / Symmlq / QCgs / Gcr / BiCgStabl below replace the whole loop by the
 // library's fused, device-driven solvers (same Iteration semantics, residual
 // history replayed through Iteration::Finished so messages print as Seldon's)
 // when the preconditioner is Preconditioner_Base (identity) or
@@ -661,7 +661,8 @@ namespace Seldon
     else if (which == 12) HipCheck(shk_minres(ctx, A, x, b, (shk_precond)prec, &it, &code), "MinRes");
     else if (which == 13) HipCheck(shk_symmlq(ctx, A, x, b, (shk_precond)prec, &it, &code), "Symmlq");
     else if (which == 14) HipCheck(shk_qcgs(ctx, A, x, b, (shk_precond)prec, &it, &code), "QCgs");
-    else HipCheck(shk_gcr(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gcr");
+    else if (which == 15) HipCheck(shk_gcr(ctx, A, x, b, (shk_precond)prec, &it, &code), "Gcr");
+    else HipCheck(shk_bicgstabl(ctx, A, x, b, (shk_precond)prec, &it, &code), "BiCgStabl");
     iter.*Acc::Fac() = Titer(it.facteur_reste);
     const int n = std::min(it.hist_len, it.hist_cap);
     for (int k = 0; k < n; k++)
@@ -673,7 +674,7 @@ namespace Seldon
     iter.SetNumberIteration(it.nb_iter);
     if (it.fail_convergence)
       {
-        static const char* what[16][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
+        static const char* what[17][7] = {{"", "Cg breakdown #1", "Cg breakdown #2", "", "", "", ""},
                                          {"", "Bicgstab breakdown #1", "Bicgstab breakdown #2",
                                           "Bicgstab breakdown #3", "", "", ""},
                                          {"", "", "", "", "", "", ""},
@@ -695,7 +696,9 @@ namespace Seldon
                                          {"", "", "", "", "", "", ""},
                                          {"", "QCgs breakdown #1", "Qcgs breakdown #2", "Qcgs breakdown #3", "", "",
                                           ""},
-                                         {"", "Gcr breakdown #1", "", "", "", "", ""}};
+                                         {"", "Gcr breakdown #1", "", "", "", "", ""},
+                                         {"", "Bicgstabl breakdown #1", "Bicgstabl Breakdown #2", "", "", "",
+                                          ""}};
         const int c = it.error_code;
         iter.Fail(c, (c >= 0 && c < 7) ? what[which][c] : "breakdown");
       }
@@ -777,6 +780,15 @@ namespace Seldon
       Prec& M, Iteration<Titer>& iter)
   {
     return HipRunSolver(15, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
+  }
+
+  // BiCgStabl.cxx:42-44
+  template <class Titer, class T, class Prop, class Storage, class Alloc, class AllocV, class Prec>
+  typename std::enable_if<IsHipSparse<Storage>::value && (HipPrecKind<Prec>::value >= 0), int>::type
+  BiCgStabl(Matrix<T, Prop, Storage, Alloc>& A, Vector<T, VectHip, AllocV>& x, const Vector<T, VectHip, AllocV>& b,
+            Prec& M, Iteration<Titer>& iter)
+  {
+    return HipRunSolver(16, A.GetHandle(), x.GetData(), b.GetData(), HipPrecKind<Prec>::value, iter);
   }
 
   // QCgs.cxx:45-47

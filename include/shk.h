@@ -367,6 +367,13 @@ int shk_qcgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
  * solve (Seldon's outer loop would go on). */
 int shk_gcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
             shk_iteration *iter, int *code);
+/* BiCgStabl (computation/solver/iterative/BiCgStabl.cxx:43-180), fused
+ * likewise, with l = iter->parameter_restart (1 .. 100) BiCG steps per cycle
+ * and right preconditioning (x is mapped by M^{-1} at the end, as there).
+ * Error codes 1 and 2 are its breakdowns (rho_0 = 0, sigma = 0), which here
+ * end the solve (Seldon's loop would go on after the cycle's MR part). */
+int shk_bicgstabl(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
+                  shk_iteration *iter, int *code);
 
 /* ---- FP32 inner solve + FP64 iterative refinement (no counterpart in
  *      Seldon; its Gmres is the inner solver) ------------------------------ */

@@ -105,7 +105,8 @@ def lib():
             getattr(L, f"orc_{sfx}_dot").restype = T
             getattr(L, f"orc_{sfx}_norm2").argtypes = [ctypes.c_int, vp]
             getattr(L, f"orc_{sfx}_norm2").restype = T
-            for solver in ("cg", "bicgstab", "gmres", "cgs", "bicg", "tfqmr", "qmr", "cgne", "lsqr", "bicgcr", "qmrsym", "minres", "symmlq", "qcgs", "gcr"):
+            for solver in ("cg", "bicgstab", "gmres", "cgs", "bicg", "tfqmr", "qmr", "cgne", "lsqr", "bicgcr", "qmrsym", "minres", "symmlq", "qcgs", "gcr",
+                           "bicgstabl"):
                 f = getattr(L, f"orc_{sfx}_{solver}")
                 f.argtypes = [ctypes.POINTER(OrcMat), vp, vp, ctypes.c_int, vp,
                               ctypes.POINTER(OrcIter)]

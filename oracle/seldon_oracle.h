@@ -134,6 +134,8 @@ void orc_coo_dims(int nnz, const int *row, const int *col, int index, int m_min,
                          const T *diag, orc_iter *it);                         \
     int orc_##SFX##_gcr(const orc_mat *A, T *x, const T *b, int precond,       \
                         const T *diag, orc_iter *it);                          \
+    int orc_##SFX##_bicgstabl(const orc_mat *A, T *x, const T *b, int precond, \
+                              const T *diag, orc_iter *it);                    \
     int orc_##SFX##_gmres(const orc_mat *A, T *x, const T *b, int precond,     \
                           const T *diag, orc_iter *it);                        \
     void orc_##SFX##_sddmm(int m, const int *ptr, const int *ind,              \

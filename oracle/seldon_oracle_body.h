@@ -1349,6 +1349,98 @@ done:
     return orc_error_code(it);
 }
 
+/* ---- BiCgStabl: computation/solver/iterative/BiCgStabl.cxx:43-180 -- *
+ * l = parameter_restart BiCG steps per cycle, then the MR part's modified
+ * Gram-Schmidt on r_1 .. r_l, the solve for gamma and the update; right
+ * preconditioning (x is mapped by M^{-1} at the end).  tau and gamma' keep
+ * their values from earlier cycles where a zero gamma(i) skips them, as
+ * there.  Seldon's loop goes on after a breakdown (Fail only ends the BiCG
+ * part); here the solve ends there, before the MR part. */
+int FN(bicgstabl)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
+                  orc_iter *it)
+{
+    const int N = A->m;
+    if (N <= 0) return 0;
+    const int l = it->parameter_restart, ld = l + 1;
+    T **r = (T **)malloc(sizeof(T *) * (size_t)ld), **u = (T **)malloc(sizeof(T *) * (size_t)ld);
+    for (int i = 0; i <= l; i++) {
+        r[i] = FN(dup)(N, b); FN(fill)(N, (T)0, r[i]);
+        u[i] = FN(dup)(N, b); FN(fill)(N, (T)0, u[i]);
+    }
+    T *q = FN(dup)(N, b), *r0 = FN(dup)(N, b);
+    T *gamma = (T *)calloc((size_t)ld, sizeof(T)), *gp = (T *)calloc((size_t)ld, sizeof(T));
+    T *gt = (T *)calloc((size_t)ld, sizeof(T)), *tau = (T *)calloc((size_t)ld * ld, sizeof(T));
+    T rho_0, rho_1 = 0, alpha, beta, omega, sigma;
+    int failed = 0;
+    FN(copy)(N, b, r[0]);
+    if (!it->init_guess_null) FN(mltadd)((T)-1, A, x, (T)1, r[0]);
+    else memset(x, 0, sizeof(T) * (size_t)N);
+    if (FN(iter_init)(it, N, b) != 0) goto done;
+    FN(copy)(N, r[0], r0);
+    rho_0 = 1; alpha = 0; omega = 1;
+    it->nb_iter = 0;
+    while (!orc_finished(it, (double)FN(norm2)(N, r[0]), it->nb_iter, it->max_iter)) {
+        rho_0 *= -omega;
+        for (int j = 0; j < l; j++) {
+            rho_1 = FN(dot)(N, r[j], r0);
+            if (rho_0 == (T)0) { orc_fail(it, 1); failed = 1; break; }
+            beta = alpha * (rho_1 / rho_0);
+            rho_0 = rho_1;
+            for (int i = 0; i <= j; i++) {
+                FN(scal)(N, -beta, u[i]);
+                FN(add)(N, (T)1, r[i], u[i]);
+            }
+            FN(precond)(precond, N, diag, u[j], q);
+            FN(mlt)(A, q, u[j + 1]);
+            it->nb_iter++;
+            sigma = FN(dot)(N, u[j + 1], r0);
+            if (sigma == (T)0) { orc_fail(it, 2); failed = 1; break; }
+            alpha = rho_1 / sigma;
+            FN(add)(N, alpha, u[0], x);
+            for (int i = 0; i <= j; i++) FN(add)(N, -alpha, u[i + 1], r[i]);
+            FN(precond)(precond, N, diag, r[j], q);
+            FN(mlt)(A, q, r[j + 1]);
+            it->nb_iter++;
+        }
+        if (failed) break;
+        for (int j = 1; j <= l; j++) {
+            for (int i = 1; i < j; i++)
+                if (gamma[i] != (T)0) {
+                    tau[i * ld + j] = FN(dot)(N, r[j], r[i]) / gamma[i];
+                    FN(add)(N, -tau[i * ld + j], r[i], r[j]);
+                }
+            gamma[j] = FN(dot)(N, r[j], r[j]);
+            if (gamma[j] != (T)0) gp[j] = FN(dot)(N, r[0], r[j]) / gamma[j];
+        }
+        gamma[l] = gp[l];
+        omega = gamma[l];
+        for (int j = l - 1; j >= 1; j--) {
+            sigma = 0;
+            for (int i = j + 1; i <= l; i++) sigma += tau[j * ld + i] * gamma[i];
+            gamma[j] = gp[j] - sigma;
+        }
+        for (int j = 1; j <= l - 1; j++) {
+            sigma = 0;
+            for (int i = j + 1; i <= l - 1; i++) sigma += tau[j * ld + i] * gamma[i + 1];
+            gt[j] = gamma[j + 1] + sigma;
+        }
+        FN(add)(N, gamma[1], r[0], x);
+        FN(add)(N, -gp[l], r[l], r[0]);
+        FN(add)(N, -gamma[l], u[l], u[0]);
+        for (int j = 1; j <= l - 1; j++) {
+            FN(add)(N, -gamma[j], u[j], u[0]);
+            FN(add)(N, gt[j], r[j], x);
+            FN(add)(N, -gp[j], r[j], r[0]);
+        }
+    }
+    FN(copy)(N, x, q);
+    FN(precond)(precond, N, diag, q, x);
+done:
+    for (int i = 0; i <= l; i++) { free(r[i]); free(u[i]); }
+    free(r); free(u); free(q); free(r0); free(gamma); free(gp); free(gt); free(tau);
+    return orc_error_code(it);
+}
+
 /* ---- Gmres(m): computation/solver/iterative/Gmres.cxx:45-182 ------- */
 int FN(gmres)(const orc_mat *A, T *x, const T *b, int precond, const T *diag,
               orc_iter *outer)

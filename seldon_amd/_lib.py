@@ -102,6 +102,7 @@ SIGNATURES = [
     ("shk_symmlq", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_qcgs", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_gcr", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
+    ("shk_bicgstabl", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_gmres", _I, [_P, _P, _P, _P, _I, ctypes.POINTER(ShkIteration), _PI]),
     ("shk_solve_ir", _I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(ShkIrParams), _PI]),
     ("shk_sddmm", _I, [_P, _P, _P, _P, _P]),

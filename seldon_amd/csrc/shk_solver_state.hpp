@@ -45,6 +45,10 @@ struct KState {
     // inner loop of the cycle is over or the solve has ended)
     double gc_normr;
     int gc_inner_max, gc_skip;
+    // BiCgStabl: the BiCG part's rho_0, rho_1, alpha, beta, the MR part's
+    // omega, and bl_apply (1 when the MGS step's tau(i, j) was formed)
+    T bl_rho0, bl_rho1, bl_alpha, bl_beta, bl_omega;
+    int bl_apply;
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -994,6 +998,94 @@ struct FinGcDelta {
     __device__ void operator()(const T (&s)[6]) const
     {
         for (int k = 0; k < cnt; k++) deltas[g0 + k] = -s[k] / betas[g0 + k];
+    }
+};
+
+// --------------------------------------------------------- BiCgStabl ----
+// Loop head (BiCgStabl.cxx:87-101): Finished(r_0); rho_0 *= -omega; then the
+// BiCG part's j = 0 head: rho_1 = (r_0, r0~), breakdown #1, beta, rho_0.
+// s = {|r_0|^2, (r_0, r0~)}.
+template <typename T>
+__device__ inline void bl_rho_head(KState<T> *st, T rho1)
+{
+    st->bl_rho1 = rho1;
+    if (st->bl_rho0 == (T)0) {
+        it_fail(st, 1);
+        return;
+    }
+    st->bl_beta = st->bl_alpha * (rho1 / st->bl_rho0);
+    st->bl_rho0 = rho1;
+}
+
+template <typename T>
+struct FinBlTop {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        if (it_finished(st, (double)sqrt_t(s[0]), st->nb_iter, st->max_iter)) {
+            it_stop(st);
+            return;
+        }
+        st->bl_rho0 *= -st->bl_omega;
+        bl_rho_head(st, s[1]);
+    }
+};
+
+// ++iter; sigma = (u_{j+1}, r0~); breakdown #2; alpha = rho_1 / sigma
+// (:110-118)
+template <typename T>
+struct FinBlSigma {
+    KState<T> *st;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->nb_iter++;
+        if (s[0] == (T)0) {
+            it_fail(st, 2);
+            return;
+        }
+        st->bl_alpha = st->bl_rho1 / s[0];
+    }
+};
+
+// ++iter after r_{j+1} = A M^{-1} r_j (:125); for j + 1 < l the next BiCG
+// head on s[0] = (r_{j+1}, r0~)
+template <typename T>
+struct FinBlRho {
+    KState<T> *st;
+    int head;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        st->nb_iter++;
+        if (head) bl_rho_head(st, s[0]);
+    }
+};
+
+// MGS step of the MR part (:131-138): tau(i, j) = (r_j, r_i) / gamma(i),
+// formed only where gamma(i) != 0 (tau keeps its old value otherwise)
+template <typename T>
+struct FinBlTau {
+    KState<T> *st;
+    const T *gamma;
+    T *tau;  // tau(i, j) at [i * (l + 1) + j]
+    int idx, i;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        const T gi = gamma[i];
+        st->bl_apply = gi != (T)0;
+        if (gi != (T)0) tau[idx] = s[0] / gi;
+    }
+};
+
+// gamma(j) = (r_j, r_j); gamma'(j) = (r_0, r_j) / gamma(j) where gamma(j)
+// != 0 (:139-141)
+template <typename T>
+struct FinBlGamma {
+    T *gamma, *gp;
+    int j;
+    __device__ void operator()(const T (&s)[2]) const
+    {
+        gamma[j] = s[0];
+        if (s[0] != (T)0) gp[j] = s[1] / s[0];
     }
 };
 

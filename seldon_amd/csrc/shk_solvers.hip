@@ -1858,6 +1858,169 @@ __global__ __launch_bounds__(kBlock) void k_gc_p(int64_t n, const T *__restrict_
     }
 }
 
+// --------------------------------------------------------- BiCgStabl ----
+// [x = 0]; r_0 = src (b - A x, or b); r0~ = r_0 (BiCgStabl.cxx:66-80); sums
+// |r_0|^2 for the loop head's norm and for (r_0, r0~).  After a failed
+// Init(b) only x.Zero(), which precedes it there, is done.  src may be r_0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bl_init(int64_t n, const T *src, T *__restrict__ x, T *r0,
+                                                    T *__restrict__ rt, int zero_x, const KState<T> *st,
+                                                    T *partials)
+{
+    const bool dead = st->skip;
+    if (dead && !zero_x) return;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(i, n) {
+        if (zero_x) x[i] = (T)0;
+        if (dead) continue;
+        const T rv = src[i];
+        r0[i] = rv;
+        rt[i] = rv;
+        c[0] += rv * rv;
+    }
+    if (dead) return;
+    c[1] = c[0];
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// u_i = r_i - beta u_i for i <= j (:103-106: Mlt(-beta, u_i), Add(1, r_i,
+// u_i)); [q = M^{-1} u_j]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bl_u(int64_t n, const T *__restrict__ rb, T *__restrict__ ub,
+                                                 int64_t stride, int j, const T *__restrict__ dg,
+                                                 T *__restrict__ q, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T mb = -st->bl_beta;
+    GRID_STRIDE(i, n) {
+        T uv = (T)0;
+        for (int k = 0; k <= j; k++) {
+            uv = ub[(size_t)k * stride + i] * mb;
+            uv += rb[(size_t)k * stride + i];
+            ub[(size_t)k * stride + i] = uv;
+        }
+        if (dg) q[i] = uv / dg[i];
+    }
+}
+
+// x += alpha u_0; r_i -= alpha u_{i+1} for i <= j (:117-120); [q = M^{-1} r_j]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bl_xr(int64_t n, T *__restrict__ rb, const T *__restrict__ ub,
+                                                  int64_t stride, int j, const T *__restrict__ dg,
+                                                  T *__restrict__ q, T *__restrict__ x, const KState<T> *st)
+{
+    if (st->skip) return;
+    const T alpha = st->bl_alpha, malpha = -alpha;
+    const bool aa = alpha != (T)0, am = malpha != (T)0;
+    GRID_STRIDE(i, n) {
+        if (aa) x[i] += alpha * ub[i];
+        T rv = (T)0;
+        for (int k = 0; k <= j; k++) {
+            rv = rb[(size_t)k * stride + i];
+            if (am) {
+                rv += malpha * ub[(size_t)(k + 1) * stride + i];
+                rb[(size_t)k * stride + i] = rv;
+            }
+        }
+        if (dg) q[i] = rv / dg[i];
+    }
+}
+
+// MR part, modified Gram-Schmidt step (j, i) (:131-141): for i > 1, r_j -=
+// tau(i-1, j) r_{i-1} where that tau was formed; then sums (r_j, r_i) for
+// i < j, or |r_j|^2 and (r_0, r_j) for i == j
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bl_mr(int64_t n, T *__restrict__ rb, int64_t stride, int j, int i,
+                                                  const T *__restrict__ tau, int ld, const KState<T> *st,
+                                                  T *partials)
+{
+    if (st->skip) return;
+    const T t = (i > 1 && st->bl_apply) ? -tau[(size_t)(i - 1) * ld + j] : (T)0;
+    const bool sub = t != (T)0;
+    T *rj = rb + (size_t)j * stride;
+    const T *rp = rb + (size_t)(i - 1) * stride, *ri = rb + (size_t)i * stride;
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(k, n) {
+        T v = rj[k];
+        if (sub) {
+            v += t * rp[k];
+            rj[k] = v;
+        }
+        if (i < j) {
+            c[0] += v * ri[k];
+        } else {
+            c[0] += v * v;
+            c[1] += rb[k] * v;
+        }
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// gamma = tau^{-1} gamma', omega = gamma(l), gamma'' = T S gamma
+// (:144-162), by one thread
+template <typename T>
+__global__ __launch_bounds__(64) void k_bl_solve(KState<T> *st, const T *__restrict__ tau, T *__restrict__ gamma,
+                                                 const T *__restrict__ gp, T *__restrict__ gt, int l)
+{
+    if (st->skip || threadIdx.x != 0) return;
+    const int ld = l + 1;
+    gamma[l] = gp[l];
+    st->bl_omega = gamma[l];
+    for (int j = l - 1; j >= 1; j--) {
+        T sigma = (T)0;
+        for (int i = j + 1; i <= l; i++) sigma += tau[j * ld + i] * gamma[i];
+        gamma[j] = gp[j] - sigma;
+    }
+    for (int j = 1; j <= l - 1; j++) {
+        T sigma = (T)0;
+        for (int i = j + 1; i <= l - 1; i++) sigma += tau[j * ld + i] * gamma[i + 1];
+        gt[j] = gamma[j + 1] + sigma;
+    }
+}
+
+// x += gamma_1 r_0; r_0 -= gamma'_l r_l; u_0 -= gamma_l u_l; then for j = 1
+// .. l-1: u_0 -= gamma_j u_j, x += gamma''_j r_j, r_0 -= gamma'_j r_j
+// (:165-174); sums |r_0|^2 and (r_0, r0~) for the next loop head
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bl_up(int64_t n, T *__restrict__ rb, T *__restrict__ ub,
+                                                  int64_t stride, int l, const T *__restrict__ gamma,
+                                                  const T *__restrict__ gp, const T *__restrict__ gt,
+                                                  const T *__restrict__ rt, T *__restrict__ x,
+                                                  const KState<T> *st, T *partials)
+{
+    if (st->skip) return;
+    const T g1 = gamma[1], gpl = -gp[l], gl = -gamma[l];
+    T c[2] = {(T)0, (T)0};
+    GRID_STRIDE(k, n) {
+        T r0 = rb[k], u0 = ub[k], xv = x[k];
+        if (g1 != (T)0) xv += g1 * r0;
+        if (gpl != (T)0) r0 += gpl * rb[(size_t)l * stride + k];
+        if (gl != (T)0) u0 += gl * ub[(size_t)l * stride + k];
+        for (int j = 1; j <= l - 1; j++) {
+            const T a = -gamma[j], bt = gt[j], cp = -gp[j];
+            if (a != (T)0) u0 += a * ub[(size_t)j * stride + k];
+            if (bt != (T)0) xv += bt * rb[(size_t)j * stride + k];
+            if (cp != (T)0) r0 += cp * rb[(size_t)j * stride + k];
+        }
+        rb[k] = r0;
+        ub[k] = u0;
+        x[k] = xv;
+        c[0] += r0 * r0;
+        c[1] += r0 * rt[k];
+    }
+    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+}
+
+// the change of coordinates of the right preconditioning (:177): x =
+// M^{-1} x; not after a failed Init(b), which returns before it
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bl_final(int64_t n, const T *__restrict__ dg, T *__restrict__ x,
+                                                     const KState<T> *st)
+{
+    if (st->init_failed) return;
+    GRID_STRIDE(i, n) x[i] = x[i] / dg[i];
+}
+
 // ------------------------------------------------------------- Gmres ----
 // x = 0 (Gmres.cxx:86); r = M^{-1} b (:89); sum r^2
 template <typename T>
@@ -3219,6 +3382,109 @@ int gcr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *
     return S.finish(it, code);
 }
 
+// BiCgStabl (BiCgStabl.cxx:43-180), l = parameter_restart: per cycle the
+// loop head, the BiCG part (per j: the u update, u_{j+1} = A M^{-1} u_j with
+// (u_{j+1}, r0~), the x / r update, r_{j+1} = A M^{-1} r_j with (r_{j+1},
+// r0~)), the MR part's modified Gram-Schmidt (one kernel per (j, i), the
+// subtraction of step i-1 fused with the dot of step i), the small solve
+// for gamma, and the x / r_0 / u_0 update with the next head's two sums.
+// The template's loop goes on after a breakdown (Fail only ends the BiCG
+// part); here the solve ends there.
+template <typename T>
+int bicgstabl_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *it, int *code)
+{
+    const int l = it->parameter_restart;
+    SHK_CHECK(l >= 1 && l <= 100, SHK_ERR_ARG, "bicgstabl: the restart parameter (l) must be in 1 .. 100");
+    Solve<T> S;
+    const bool jac = prec == SHK_PREC_JACOBI;
+    const size_t ld = (size_t)l + 1;
+    const auto r256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    SHK_TRY(S.setup(ctx, A, 2 * ld + 2, r256(ld * ld * sizeof(T)) + 3 * r256(ld * sizeof(T)), it));
+    const int64_t n = S.n;
+    T *rbase = S.take(n);
+    for (int i = 1; i <= l; i++) S.take(n);
+    T *ubase = S.take(n);
+    for (int i = 1; i <= l; i++) S.take(n);
+    T *rt = S.take(n), *q = S.take(n);
+    const int64_t stride = (int64_t)((ubase - rbase) / (int64_t)ld);
+    SHK_CHECK(stride * (int64_t)ld == (int64_t)(ubase - rbase) && rt - ubase == stride * (int64_t)ld,
+              SHK_ERR_STATE, "bicgstabl: work vectors not evenly spaced");
+    T *tau = S.take((int64_t)(ld * ld)), *gamma = S.take((int64_t)ld), *gp = S.take((int64_t)ld),
+      *gt = S.take((int64_t)ld);
+    const T *dg = jac ? (const T *)A->diag : nullptr;
+    hipStream_t st = ctx->stream;
+    dim3 g(S.nb), blk(kBlock);
+    // u_0, tau, gamma, gamma', gamma'' start at zero (:58-65); rho_0 = 1,
+    // alpha = 0, omega = 1 (:85)
+    SHK_HIP(hipMemsetAsync(ubase, 0, sizeof(T) * (size_t)n, st));
+    SHK_HIP(hipMemsetAsync(tau, 0, sizeof(T) * ld * ld, st));
+    SHK_HIP(hipMemsetAsync(gamma, 0, sizeof(T) * ld, st));
+    SHK_HIP(hipMemsetAsync(gp, 0, sizeof(T) * ld, st));
+    SHK_HIP(hipMemsetAsync(gt, 0, sizeof(T) * ld, st));
+    S.hst->bl_rho0 = (T)1;
+    S.hst->bl_alpha = (T)0;
+    S.hst->bl_omega = (T)1;
+    SHK_HIP(hipMemcpyAsync(S.dst, S.hst, sizeof(KState<T>), hipMemcpyHostToDevice, st));
+    // Init(b) (:76)
+    hipLaunchKernelGGL(k_axpy_dot<T>, g, blk, 0, st, n, (T)0, (const T *)nullptr, (T *)b, b,
+                       (const T *)nullptr, S.partials, (const int *)&S.dst->skip);
+    S.fin(FinInitNorm<T>{S.dst});
+    const T *src = b;
+    if (!it->init_guess_null) {
+        SpmvArgs<T> a{};
+        a.x = x; a.y_in = b; a.y_out = rbase; a.alpha = (T)-1; a.beta = (T)1; a.skip = &S.dst->skip;
+        SHK_TRY(S.spmv(a, FinNone{}));
+        src = rbase;
+    }
+    hipLaunchKernelGGL(k_bl_init<T>, g, blk, 0, st, n, src, x, rbase, rt, it->init_guess_null ? 1 : 0, S.dst,
+                       S.part[0]);
+    SHK_HIP(hipGetLastError());
+    const int cycles = std::max(1, kChunk / l);
+    const int64_t limit = (int64_t)std::max(it->max_iter, 0) / (2 * l) + 2;
+    SHK_TRY(S.run_chunks(limit, cycles, [&](hipStream_t cs) -> int {
+        const int *skip = (const int *)&S.dst->skip;
+        for (int c = 0; c < cycles; c++) {
+            grid_finish<T>(cs, (const T *)S.part[0], S.nb, skip, FinBlTop<T>{S.dst});
+            for (int j = 0; j < l; j++) {
+                T *uj = ubase + (size_t)j * stride, *rj = rbase + (size_t)j * stride;
+                hipLaunchKernelGGL(k_bl_u<T>, g, blk, 0, cs, n, (const T *)rbase, ubase, stride, j, dg, q, S.dst);
+                SpmvArgs<T> a{};
+                a.x = jac ? q : uj; a.y_out = uj + stride; a.alpha = (T)1; a.mlt = 1;
+                a.dot0 = kDotVec; a.u0 = rt; a.skip = skip; a.partials = S.part[1];
+                SHK_TRY(S.spmv(a, FinBlSigma<T>{S.dst}));
+                hipLaunchKernelGGL(k_bl_xr<T>, g, blk, 0, cs, n, rbase, (const T *)ubase, stride, j, dg, q, x,
+                                   S.dst);
+                SpmvArgs<T> a2{};
+                a2.x = jac ? q : rj; a2.y_out = rj + stride; a2.alpha = (T)1; a2.mlt = 1;
+                a2.dot0 = kDotVec; a2.u0 = rt; a2.skip = skip; a2.partials = S.part[1];
+                SHK_TRY(S.spmv(a2, FinBlRho<T>{S.dst, j + 1 < l ? 1 : 0}));
+            }
+            for (int j = 1; j <= l; j++)
+                for (int i = 1; i <= j; i++) {
+                    hipLaunchKernelGGL(k_bl_mr<T>, g, blk, 0, cs, n, rbase, stride, j, i, (const T *)tau, (int)ld,
+                                       S.dst, S.part[0]);
+                    if (i < j)
+                        grid_finish<T>(cs, (const T *)S.part[0], S.nb, skip,
+                                       FinBlTau<T>{S.dst, (const T *)gamma, tau, (int)(i * ld + j), i});
+                    else
+                        grid_finish<T>(cs, (const T *)S.part[0], S.nb, skip, FinBlGamma<T>{gamma, gp, j});
+                }
+            hipLaunchKernelGGL(k_bl_solve<T>, dim3(1), dim3(64), 0, cs, S.dst, (const T *)tau, gamma,
+                               (const T *)gp, gt, l);
+            hipLaunchKernelGGL(k_bl_up<T>, g, blk, 0, cs, n, rbase, ubase, stride, l, (const T *)gamma,
+                               (const T *)gp, (const T *)gt, (const T *)rt, x, S.dst, S.part[0]);
+        }
+        SHK_HIP(hipGetLastError());
+        return SHK_OK;
+    }));
+    SHK_CHECK(S.hst->done, SHK_ERR_STATE, "bicgstabl: solver did not reach a stopping state");
+    if (jac) {
+        hipLaunchKernelGGL(k_bl_final<T>, g, blk, 0, st, n, dg, x, S.dst);
+        SHK_HIP(hipGetLastError());
+    }
+    return S.finish(it, code);
+}
+
 template <typename T>
 int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, int prec,
                    shk_iteration *it, int *code)
@@ -3237,6 +3503,7 @@ int solve_dispatch(int which, shk_ctx ctx, shk_csr A, void *dx, const void *db, 
     if (which == 13) return symmlq_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 14) return qcgs_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     if (which == 15) return gcr_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
+    if (which == 16) return bicgstabl_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
     return gmres_impl<T>(ctx, A, (T *)dx, (const T *)db, prec, it, code);
 }
 
@@ -3457,6 +3724,12 @@ int shk_qcgs(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec,
 int shk_gcr(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it, int *code)
 {
     return solve_entry(15, ctx, A, dx, db, prec, it, code);
+}
+
+int shk_bicgstabl(shk_ctx ctx, shk_csr A, void *dx, const void *db, shk_precond prec, shk_iteration *it,
+                  int *code)
+{
+    return solve_entry(16, ctx, A, dx, db, prec, it, code);
 }
 
 }  // extern "C"

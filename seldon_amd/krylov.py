@@ -575,6 +575,12 @@ def Gcr(A, x, b, M, it):
     return _solve(L.load().shk_gcr, A, x, b, M, it)
 
 
+def BiCgStabl(A, x, b, M, it):
+    """BiCgStabl (computation/solver/iterative/BiCgStabl.cxx:43-180) on the
+    GPU, fused; l = it.parameter_restart BiCG steps per cycle."""
+    return _solve(L.load().shk_bicgstabl, A, x, b, M, it)
+
+
 def Cgne(A, x, b, M, it):
     """Cgne (computation/solver/iterative/Cgne.cxx:41-124) on the GPU, fused."""
     return _solve(L.load().shk_cgne, A, x, b, M, it)

@@ -14,8 +14,8 @@
 // Symmlq, CoCg, BiCgcr, QmrSym, and Cg with Seldon's SorPreconditioner
 // (SSOR) -- with Seldon's own templates, once on the CPU storages and once on
 // the GPU storages (Mlt(SeldonTrans, ...), SOR, Swap and the rest through the
-// overload set; BiCg, Qmr, Cgne, Lsqr, Cgs, QCgs, TfQmr, Gcr, CoCg, BiCgcr,
-// QmrSym, MinRes and Symmlq with
+// overload set; BiCg, Qmr, Cgne, Lsqr, Cgs, QCgs, TfQmr, Gcr, BiCgStabl,
+// CoCg, BiCgcr, QmrSym, MinRes and Symmlq with
 // Preconditioner_Base reach the library's fused solvers, and once more,
 // with a user preconditioner type, Seldon's templates: labels
 // <solver>_gpu_generic),
@@ -232,7 +232,7 @@ int main(int argc, char** argv)
         const int maxit = (k == 2 || k == 3) ? 4000 : 2000;
         if (cpu) run_trans_solver(solvers[k], "cpu", C, bc, maxit, 1e-8);
         if (gpu) run_trans_solver(solvers[k], "gpu", Cg_, bcg, maxit, 1e-8);
-        if (gpu && k < 8)
+        if (gpu)
           run_trans_solver<Matrix<double, General, RowSparseHip, HipAlloc<double> >,
                            Vector<double, VectHip, HipAlloc<double> >, UserIdentity>(solvers[k], "gpu_generic",
                                                                                    Cg_, bcg, maxit, 1e-8);

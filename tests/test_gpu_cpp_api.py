@@ -148,7 +148,7 @@ def test_overlay_in_real_seldon(gpu):
     # must match within 1e-6 and the counts within 1% (GPU sums round
     # differently from the first dot product on).
     agree = {"cgne": 100, "lsqr": 100, "cgs": 40, "qcgs": 40, "tfqmr": 70, "bicgstabl": 2}
-    # BiCg, Qmr, Cgne, Lsqr, Cgs, QCgs, TfQmr, Gcr, CoCg (= Cg on reals), BiCgcr, QmrSym, MinRes and Symmlq with
+    # BiCg, Qmr, Cgne, Lsqr, Cgs, QCgs, TfQmr, Gcr, BiCgStabl, CoCg (= Cg on reals), BiCgcr, QmrSym, MinRes and Symmlq with
     # Preconditioner_Base run the
     # library's fused solvers
     # (label _gpu), and Seldon's templates with a user preconditioner type
@@ -156,8 +156,8 @@ def test_overlay_in_real_seldon(gpu):
     legs = [(solver, "gpu") for solver in ("bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr",
                                            "bicgstabl", "minres", "symmlq", "cocg", "bicgcr", "qmrsym",
                                            "ssor_cg", "ssor_cg_sym")]
-    legs += [(m, "gpu_generic") for m in ("bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr", "cocg",
-                                          "bicgcr", "qmrsym", "minres", "symmlq")]
+    legs += [(m, "gpu_generic") for m in ("bicg", "qmr", "cgne", "lsqr", "cgs", "qcgs", "tfqmr", "gcr", "bicgstabl",
+                                          "cocg", "bicgcr", "qmrsym", "minres", "symmlq")]
     for solver, leg in legs:
         h_c, it_c, c_c, n_c = cpu(f"{solver}_cpu")
         h_g, it_g, c_g, n_g = s[f"{solver}_{leg}"]

@@ -1,7 +1,11 @@
-"""Fused Cgs, BiCg, TfQmr, Qmr, Cgne and Lsqr (shk_cgs, shk_bicg, shk_tfqmr,
-shk_qmr, shk_cgne, shk_lsqr) against Seldon's own runs and the oracle's
+"""Fused Cgs, BiCg, TfQmr, Qmr, Cgne, Lsqr and BiCgStabl (shk_cgs, shk_bicg,
+shk_tfqmr, shk_qmr, shk_cgne, shk_lsqr, shk_bicgstabl; also QCgs, Gcr and
+the symmetric ones) against Seldon's own runs and the oracle's
 restatements of Cgs.cxx:46-137, BiCg.cxx:47-131, TfQmr.cxx:45-240,
-Qmr.cxx:42-190, Cgne.cxx:41-124 and Lsqr.cxx:41-150.
+Qmr.cxx:42-190, Cgne.cxx:41-124, Lsqr.cxx:41-150 and BiCgStabl.cxx:43-180.
+BiCgStabl records one Finished() per cycle of 2 l = 20 iterations and its
+builds drift after the second, so its bars are the first two entries and the
+count within one cycle.
 
 Bars.  On the overlay program's 24 x 24 upwind system the reference is
 Seldon's own CPU run (tests/golden/overlay_cpu.npz), with the bars of the
@@ -28,8 +32,12 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-CHAOTIC = {"cgs": 40, "qcgs": 40, "tfqmr": 70, "cgne": 100, "lsqr": 100}
-METHODS = ["bicg", "cgs", "qcgs", "tfqmr", "qmr", "cgne", "lsqr", "gcr"]
+CHAOTIC = {"cgs": 40, "qcgs": 40, "tfqmr": 70, "cgne": 100, "lsqr": 100, "bicgstabl": 2}
+METHODS = ["bicg", "cgs", "qcgs", "tfqmr", "qmr", "cgne", "lsqr", "gcr", "bicgstabl"]
+# at most this many leading history entries are compared, and the counts may
+# differ by this many iterations (BiCgStabl(10): one cycle)
+HEAD_MAX = {"bicgstabl": 2}
+COUNT_SLACK = {"bicgstabl": 20}
 NORMAL = ("cgne", "lsqr")  # compared on smaller systems
 
 
@@ -48,7 +56,7 @@ def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float6
     P = K.JacobiPreconditioner() if precond == "jacobi" else K.Preconditioner_Base()
     code = {"cgs": K.Cgs, "bicg": K.BiCg, "tfqmr": K.TfQmr, "qmr": K.Qmr, "cgne": K.Cgne,
             "lsqr": K.Lsqr, "bicgcr": K.BiCgcr, "qmrsym": K.QmrSym, "minres": K.MinRes,
-            "symmlq": K.Symmlq, "qcgs": K.QCgs, "gcr": K.Gcr}[method](M, x, bg, P, it)
+            "symmlq": K.Symmlq, "qcgs": K.QCgs, "gcr": K.Gcr, "bicgstabl": K.BiCgStabl}[method](M, x, bg, P, it)
     n = min(it.hist_len, len(it.history))
     return code, it.nb_iter, it.history[:n].copy(), it.history_it[:n].copy(), x.cpu().numpy()
 
@@ -57,7 +65,8 @@ def _check(method, got, ref_hist, ref_its, ref_code, ref_nb, head, count_tol):
     code, nb, hist, its, _ = got
     k = min(len(hist), len(ref_hist))
     rel = np.abs(hist[:k] - ref_hist[:k]) / np.abs(ref_hist[:k])
-    h = min(head, k)
+    h = min(head, k, HEAD_MAX.get(method, head))
+    count_tol = max(count_tol, COUNT_SLACK.get(method, 0))
     print(f"[{method}] code {code}/{ref_code} its {nb}/{ref_nb} max rel dev {rel.max():.2e}, "
           f"first {h}: {rel[:h].max():.2e}", flush=True)
     assert code == ref_code
@@ -151,10 +160,11 @@ def test_fused_max_iter_breakdown_and_zero_rhs(gpu, method):
     assert (code, nb) == (r.code, r.nb_iter)
     assert want is None or (code, nb) == want
     # a zero right-hand side: Init fails; x is left as it was (3.0), except
-    # that TfQmr zeroes it before its Init (TfQmr.cxx:61-71)
+    # that TfQmr and BiCgStabl zero it before their Init (TfQmr.cxx:61-71,
+    # BiCgStabl.cxx:70-76)
     code, nb, _, _, x = _gpu(method, A, np.zeros(A.m), tol=1e-8)
     r = orc.solve(method, A, np.zeros(A.m), tol=1e-8)
-    assert code == r.code and np.all(x == (0.0 if method == "tfqmr" else 3.0))
+    assert code == r.code and np.all(x == (0.0 if method in ("tfqmr", "bicgstabl") else 3.0))
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -173,12 +183,22 @@ def test_fused_fp32(gpu, method):
     assert code == r32.code == 0
     if method == "bicg":
         assert abs(nb - r32.nb_iter) <= max(4, r32.nb_iter // 10), (nb, r32.nb_iter)
-    k = min(10, len(hist), len(r32.hist))
+    # FP32 BiCgStab(10) is rounding-chaotic within its first cycle: a one-ulp
+    # change of b moves the oracle's own second entry from 2.7e-2 to 1.4e-3
+    # (FP64: 3.1e-4), so only the first entry is compared
+    k = min(10, len(hist), len(r32.hist), 1 if method == "bicgstabl" else 10)
     assert np.max(np.abs(hist[:k] - r32.hist[:k]) / r32.hist[:k]) < 1e-3
     kappa = np.linalg.cond(A.to_scipy().toarray())
     err = np.linalg.norm(x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
     # the normal equations square the condition number
-    assert err <= 10 * (kappa ** 2 if method in NORMAL else kappa) * 1e-5, (err, kappa)
+    bound = 10 * (kappa ** 2 if method in NORMAL else kappa) * 1e-5
+    if method == "bicgstabl":
+        # its FP32 recurrence residual parts from the true one: the oracle's
+        # own float run stops at 1.7e-6 with a true residual of 1.2e-2 and
+        # its x 3.6e-2 from the FP64 solution; the bar is that run's error
+        err32 = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
+        bound = max(bound, 2 * err32)
+    assert err <= bound, (err, kappa)
 
 
 def test_cocg_runs_the_fused_cg(gpu):

@@ -1122,3 +1122,36 @@ def test_oracle_gcr_matches_seldon_fixture():
     assert list(r.hist_it[:9]) == [0, 0, 2, 4, 6, 8, 10, 10, 10]
     true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
     assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-15
+
+
+def test_oracle_bicgstabl_matches_seldon_fixture_and_closed_forms():
+    """Seldon's own BiCgStabl (BiCgStabl.cxx:43-180, l = 10, the overlay
+    program's CPU leg on the 24 x 24 upwind system): history -- one
+    Finished() per cycle of 2 l iterations -- count and code, bitwise.  The
+    recurrence residual r_0 stays the true one.  Closed forms: with Jacobi on
+    a diagonal matrix A M^{-1} = I, so BiCgStab(1)'s first cycle solves the
+    system and only the final change of coordinates x = M^{-1} y gives
+    x = b / d (a dropped final map or a sign slip in the x update fails it);
+    without a preconditioner the BiCG polynomial gains degree l per cycle, so
+    four distinct eigenvalues stop BiCgStab(l) after ceil(4 / l) cycles of
+    2 l iterations, with x = b / d."""
+    fx = _overlay_fixture()
+    A, b, _ = wl.upwind(24)
+    r = orc.solve("bicgstabl", A, b, tol=1e-8, max_iter=2000, restart=10)
+    assert r.code == int(fx["bicgstabl_cpu_code"]) == 0 and r.nb_iter == int(fx["bicgstabl_cpu_nb"])
+    assert np.array_equal(r.hist_it, fx["bicgstabl_cpu_its"]) and np.array_equal(r.hist, fx["bicgstabl_cpu_hist"])
+    assert list(r.hist_it) == [0, 20, 40, 60, 80, 100]
+    # (the last entry, 1.3e-11, is near the rounding floor of the true
+    # residual, so the gap is bounded absolutely there)
+    true = np.linalg.norm(b - orc.mlt(A, r.x)) / np.linalg.norm(b)
+    assert abs(true - r.hist[-1]) <= 1e-3 * r.hist[-1] + 1e-11
+    n = 80
+    d = np.array([1.0, 2.0, 3.5, 7.0])[np.arange(n) % 4]
+    D = wl.CSR(n, n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d)
+    bb = wl.uniform(n, 13)
+    r = orc.solve("bicgstabl", D, bb, precond="jacobi", tol=1e-12, max_iter=100, restart=1)
+    assert r.code == 0 and r.nb_iter == 2 and np.allclose(r.x, bb / d, rtol=1e-12)
+    for l, cycles in ((1, 4), (2, 2), (3, 2), (4, 1)):
+        r = orc.solve("bicgstabl", D, bb, tol=1e-11, max_iter=100, restart=l)
+        assert r.code == 0 and r.nb_iter == 2 * l * cycles, (l, r.nb_iter)
+        assert np.allclose(r.x, bb / d, rtol=1e-9)
