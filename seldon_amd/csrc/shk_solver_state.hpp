@@ -46,9 +46,8 @@ struct KState {
     double gc_normr;
     int gc_inner_max, gc_skip;
     // BiCgStabl: the BiCG part's rho_0, rho_1, alpha, beta, the MR part's
-    // omega, and bl_apply (1 when the MGS step's tau(i, j) was formed)
+    // omega
     T bl_rho0, bl_rho1, bl_alpha, bl_beta, bl_omega;
-    int bl_apply;
     // Gmres(m)
     int gm_m, gm_i, gm_cycle_done, gm_inner_nb, gm_inner_max, gm_active;
     int cyc_skip;      // 1 when no cycle is open (cycle-end kernels return)
@@ -1060,32 +1059,25 @@ struct FinBlRho {
     }
 };
 
-// MGS step of the MR part (:131-138): tau(i, j) = (r_j, r_i) / gamma(i),
-// formed only where gamma(i) != 0 (tau keeps its old value otherwise)
+// Step i of the MR part's Gram-Schmidt for the columns j0 .. j0+cnt-1
+// (:131-141): where the group starts at column i, gamma(i) = |r_i|^2 and
+// gamma'(i) = (r_0, r_i) / gamma(i) (where gamma(i) != 0) from s[0], s[1];
+// then tau(i, j) = (r_j, r_i) / gamma(i) for the group's j > i, formed only
+// where gamma(i) != 0 (tau keeps its old value otherwise).
 template <typename T>
-struct FinBlTau {
-    KState<T> *st;
-    const T *gamma;
-    T *tau;  // tau(i, j) at [i * (l + 1) + j]
-    int idx, i;
-    __device__ void operator()(const T (&s)[2]) const
+struct FinBlMgs {
+    T *gamma, *gp, *tau;
+    int i, j0, cnt, ld;
+    __device__ void operator()(const T (&s)[6]) const
     {
+        const bool head = j0 == i;
+        if (head) {
+            gamma[i] = s[0];
+            if (s[0] != (T)0) gp[i] = s[1] / s[0];
+        }
         const T gi = gamma[i];
-        st->bl_apply = gi != (T)0;
-        if (gi != (T)0) tau[idx] = s[0] / gi;
-    }
-};
-
-// gamma(j) = (r_j, r_j); gamma'(j) = (r_0, r_j) / gamma(j) where gamma(j)
-// != 0 (:139-141)
-template <typename T>
-struct FinBlGamma {
-    T *gamma, *gp;
-    int j;
-    __device__ void operator()(const T (&s)[2]) const
-    {
-        gamma[j] = s[0];
-        if (s[0] != (T)0) gp[j] = s[1] / s[0];
+        if (gi == (T)0) return;
+        for (int k = head ? 1 : 0; k < cnt; k++) tau[(size_t)i * ld + j0 + k] = s[head ? k + 1 : k] / gi;
     }
 };
 

@@ -1926,34 +1926,51 @@ __global__ __launch_bounds__(kBlock) void k_bl_xr(int64_t n, T *__restrict__ rb,
     }
 }
 
-// MR part, modified Gram-Schmidt step (j, i) (:131-141): for i > 1, r_j -=
-// tau(i-1, j) r_{i-1} where that tau was formed; then sums (r_j, r_i) for
-// i < j, or |r_j|^2 and (r_0, r_j) for i == j
+// MR part (:131-141), step i of its modified Gram-Schmidt taken column by
+// column of tau ("right-looking"): for each j of the group j0 .. j0+cnt-1
+// (j >= i), r_j -= tau(i-1, j) r_{i-1} where that tau was formed (gamma(i-1)
+// != 0, i > 1); then sums (r_j, r_i) for j > i, and, when the group starts
+// at j0 = i, |r_i|^2 and (r_0, r_i) first.  Each r_j still takes its
+// subtractions in i order, each tau(i, j) from r_j as it stands after
+// i - 1 of them, as in the template's loop over j then i.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bl_mr(int64_t n, T *__restrict__ rb, int64_t stride, int j, int i,
-                                                  const T *__restrict__ tau, int ld, const KState<T> *st,
-                                                  T *partials)
+__global__ __launch_bounds__(kBlock) void k_bl_mgs(int64_t n, T *__restrict__ rb, int64_t stride, int i, int j0,
+                                                   int cnt, const T *__restrict__ tau, const T *__restrict__ gamma,
+                                                   int ld, const KState<T> *st, T *partials)
 {
     if (st->skip) return;
-    const T t = (i > 1 && st->bl_apply) ? -tau[(size_t)(i - 1) * ld + j] : (T)0;
-    const bool sub = t != (T)0;
-    T *rj = rb + (size_t)j * stride;
-    const T *rp = rb + (size_t)(i - 1) * stride, *ri = rb + (size_t)i * stride;
-    T c[2] = {(T)0, (T)0};
-    GRID_STRIDE(k, n) {
-        T v = rj[k];
-        if (sub) {
-            v += t * rp[k];
-            rj[k] = v;
+    const bool formed = i > 1 && gamma[i - 1] != (T)0;
+    T t[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) t[k] = (formed && k < cnt) ? -tau[(size_t)(i - 1) * ld + j0 + k] : (T)0;
+    const bool head = j0 == i;
+    T c[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+    const T *rp = rb + (size_t)(i > 1 ? i - 1 : 0) * stride;
+    T *ri_p = rb + (size_t)i * stride;
+    GRID_STRIDE(e, n) {
+        const T pv = formed ? rp[e] : (T)0;
+        T ri = ri_p[e];
+        if (head) {
+            if (t[0] != (T)0) {
+                ri += t[0] * pv;
+                ri_p[e] = ri;
+            }
+            c[0] += ri * ri;
+            c[1] += rb[e] * ri;
         }
-        if (i < j) {
-            c[0] += v * ri[k];
-        } else {
-            c[0] += v * v;
-            c[1] += rb[k] * v;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            if (k >= cnt || (head && k == 0)) continue;
+            T *rj = rb + (size_t)(j0 + k) * stride;
+            T v = rj[e];
+            if (t[k] != (T)0) {
+                v += t[k] * pv;
+                rj[e] = v;
+            }
+            c[head ? k + 1 : k] += v * ri;
         }
     }
-    block_partial<T, 2>(c, partials, gridDim.x, blockIdx.x);
+    block_partial<T, 6>(c, partials, gridDim.x, blockIdx.x);
 }
 
 // gamma = tau^{-1} gamma', omega = gamma(l), gamma'' = T S gamma
@@ -3385,8 +3402,9 @@ int gcr_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_iteration *
 // BiCgStabl (BiCgStabl.cxx:43-180), l = parameter_restart: per cycle the
 // loop head, the BiCG part (per j: the u update, u_{j+1} = A M^{-1} u_j with
 // (u_{j+1}, r0~), the x / r update, r_{j+1} = A M^{-1} r_j with (r_{j+1},
-// r0~)), the MR part's modified Gram-Schmidt (one kernel per (j, i), the
-// subtraction of step i-1 fused with the dot of step i), the small solve
+// r0~)), the MR part's modified Gram-Schmidt (per step i one kernel for
+// each group of up to six columns j >= i, the subtraction of step i-1 fused
+// with the dots of step i), the small solve
 // for gamma, and the x / r_0 / u_0 update with the next head's two sums.
 // The template's loop goes on after a breakdown (Fail only ends the BiCG
 // part); here the solve ends there.
@@ -3459,15 +3477,18 @@ int bicgstabl_impl(shk_ctx ctx, shk_csr A, T *x, const T *b, int prec, shk_itera
                 a2.dot0 = kDotVec; a2.u0 = rt; a2.skip = skip; a2.partials = S.part[1];
                 SHK_TRY(S.spmv(a2, FinBlRho<T>{S.dst, j + 1 < l ? 1 : 0}));
             }
-            for (int j = 1; j <= l; j++)
-                for (int i = 1; i <= j; i++) {
-                    hipLaunchKernelGGL(k_bl_mr<T>, g, blk, 0, cs, n, rbase, stride, j, i, (const T *)tau, (int)ld,
-                                       S.dst, S.part[0]);
-                    if (i < j)
-                        grid_finish<T>(cs, (const T *)S.part[0], S.nb, skip,
-                                       FinBlTau<T>{S.dst, (const T *)gamma, tau, (int)(i * ld + j), i});
-                    else
-                        grid_finish<T>(cs, (const T *)S.part[0], S.nb, skip, FinBlGamma<T>{gamma, gp, j});
+            // the MR part, step i over the columns j >= i: the first group
+            // holds column i (its last subtraction, gamma(i), gamma'(i))
+            // and up to four more, the next groups six each
+            for (int i = 1; i <= l; i++)
+                for (int j0 = i; j0 <= l;) {
+                    const int cnt = std::min(j0 == i ? 5 : 6, l + 1 - j0);
+                    hipLaunchKernelGGL(k_bl_mgs<T>, g, blk, 0, cs, n, rbase, stride, i, j0, cnt, (const T *)tau,
+                                       (const T *)gamma, (int)ld, S.dst, S.part[2]);
+                    hipLaunchKernelGGL((k_grid_finish6<T, FinBlMgs<T>>), dim3(1), dim3(kBlock), 0, cs,
+                                       (const T *)S.part[2], S.nb, skip,
+                                       FinBlMgs<T>{gamma, gp, tau, i, j0, cnt, (int)ld});
+                    j0 += cnt;
                 }
             hipLaunchKernelGGL(k_bl_solve<T>, dim3(1), dim3(64), 0, cs, S.dst, (const T *)tau, gamma,
                                (const T *)gp, gt, l);
