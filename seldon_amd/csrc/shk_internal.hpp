@@ -232,19 +232,21 @@ __global__ __launch_bounds__(kBlock) void k_grid_finish(const T *partials, int n
     if (threadIdx.x == 0) fin(out);
 }
 
-// The same for six values (three pairs at partials + 2 k nblocks).
+// The same for six values (at partials + k nblocks), in one pass: each
+// value's per-thread sums and block sum in the same order as grid_total's.
 template <typename T, typename Fin>
 __global__ __launch_bounds__(kBlock) void k_grid_finish6(const T *partials, int nblocks, const int *skip, Fin fin)
 {
     if (skip && *skip) return;
+    __shared__ T lds[6 * 16];
     T out[6];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        T t[2];
-        grid_total<T>(partials + (size_t)2 * k * nblocks, nblocks, t);
-        out[2 * k] = t[0];
-        out[2 * k + 1] = t[1];
+    for (int k = 0; k < 6; k++) out[k] = (T)0;
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) out[k] += partials[(size_t)k * nblocks + i];
     }
+    block_sum<T, 6>(out, lds);
     if (threadIdx.x == 0) fin(out);
 }
 
