@@ -42,11 +42,13 @@ NORMAL = ("cgne", "lsqr")  # compared on smaller systems
 
 
 def _gpu(method, A, b, precond="id", tol=1e-8, max_iter=2000, dtype=torch.float64, x0=None, storage="row",
-         layout="auto"):
+         layout="auto", restart=None):
     from seldon_amd import krylov as K
     M = K.Matrix(A, storage=storage, dtype=dtype, layout=layout)
     bg = torch.tensor(b, device="cuda", dtype=dtype)
     it = K.Iteration(max_iter, tol)
+    if restart is not None:
+        it.SetRestart(restart)
     it.HideMessages()
     if x0 is not None:
         x = torch.tensor(x0, device="cuda", dtype=dtype)
@@ -199,6 +201,24 @@ def test_fused_fp32(gpu, method):
         err32 = np.linalg.norm(r32.x.astype(np.float64) - r64.x) / np.linalg.norm(r64.x)
         bound = max(bound, 2 * err32)
     assert err <= bound, (err, kappa)
+
+
+@pytest.mark.parametrize("l", [1, 2, 4, 5, 6, 7, 12])
+def test_bicgstabl_other_l(gpu, l):
+    """BiCgStab(l) for other l than the default 10: the Gram-Schmidt steps
+    group up to six columns per kernel (the group holding column i up to
+    five), so these l cover a single column, a lone head group, a head group
+    of exactly five, one and two further groups.  Against the oracle: the
+    first two entries, the count within one cycle, the true residual."""
+    A, b, _ = wl.upwind(32)
+    for precond in ("id", "jacobi"):
+        r = orc.solve("bicgstabl", A, b, precond=precond, tol=1e-9, max_iter=20000, restart=l)
+        got = _gpu("bicgstabl", A, b, precond=precond, tol=1e-9, max_iter=20000, restart=l)
+        _check("bicgstabl", got, r.hist, r.hist_it, r.code, r.nb_iter, 2, 2 * l)
+        assert got[1] % (2 * l) == 0 and np.all(got[3] % (2 * l) == 0)
+        # (the recurrence residual drifts from the true one as l grows: the
+        # oracle's own true residual is 6.7e-9 at l = 12 for a 1e-9 stop)
+        assert np.linalg.norm(b - orc.mlt(A, got[4])) <= 1e-7 * np.linalg.norm(b)
 
 
 def test_cocg_runs_the_fused_cg(gpu):
