@@ -4,7 +4,7 @@ ctypes front-end over ``oracle/liboracle.so`` (built from ``seldon_oracle.c``
 by ``oracle/Makefile``), a plain-C restatement of Seldon's generic templates:
 SpMV on RowSparse / RowSymSparse / ArrayRowSparse and its transposed forms,
 DotProd / Norm2 / Add / Mlt / Copy, Iteration, Cg, BiCgStab, Gmres, Cgs, BiCg, TfQmr, Qmr, Cgne,
-Lsqr, BiCgcr, QmrSym, MinRes, Symmlq, QCgs and Gcr, SOR,
+Lsqr, BiCgcr, QmrSym, MinRes, Symmlq, QCgs, Gcr and BiCgStabl, SOR,
 and ConvertMatrix_from_Coordinates (file:line citations are in
 ``seldon_oracle_body.h``); plus, in Python below, the Harwell-Boeing writer
 and reader.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
@@ -16,8 +16,8 @@ Parity pins (see ``tests/test_oracle.py``):
     (``oracle/_ref``) on SpMV (plain and transposed), Cg, BiCgStab, SOR on
     the three storages, the coordinate conversion and the Harwell-Boeing
     files, and on committed fixtures (``tests/golden``) made by them;
-  * Cgs, QCgs, BiCg, TfQmr, Qmr, Cgne, Lsqr, Gcr, BiCgcr, QmrSym, MinRes and Symmlq (and Cg as
-    Seldon's CoCg):
+  * Cgs, QCgs, BiCg, TfQmr, Qmr, Cgne, Lsqr, Gcr, BiCgStabl, BiCgcr, QmrSym, MinRes and Symmlq
+    (and Cg as Seldon's CoCg):
     bitwise against Seldon's own
     runs of them (the overlay program's CPU legs,
     ``tests/golden/overlay_cpu.npz``), closed cases (A = 2.5 I; diagonal A
@@ -25,7 +25,10 @@ Parity pins (see ``tests/test_oracle.py``):
     following Cg on an SPD matrix (history at 2k = Cg's at k within 1e-9),
     Cgne following Cg on the explicit A^T A, TfQmr's quasi-residual bound on
     the true residual, Qmr's carried residual and Lsqr's |phi_bar| equal to
-    the true one, FP32 within kappa eps_32;
+    the true one, FP32 within kappa eps_32; BiCgStab(l) on a diagonal A
+    with four distinct values stopping after ceil(4 / l) cycles with
+    x = A^-1 b, and BiCgStab(1) under Jacobi solving it in one cycle through
+    the final change of coordinates (both also in float);
   * Gmres: iteration counts and residual histories against the MKL-linked
     real Seldon build (tolerance, MKL's reductions differ in order);
   * closed forms (5-point Laplacian eigenpairs, SOR as forward substitution

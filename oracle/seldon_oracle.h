@@ -4,11 +4,11 @@
  * A plain-C CPU restatement of Seldon's sparse Krylov path (SpMV on the
  * RowSparse / RowSymSparse / ArrayRowSparse storages and its transposed
  * forms, the BLAS-1 steps, the Iteration stopping logic, Cg, BiCgStab,
- * Gmres, Cgs, QCgs, BiCg, TfQmr, Qmr, Cgne, Lsqr, Gcr, BiCgcr, QmrSym, MinRes
- * and Symmlq, SOR / SSOR,
+ * Gmres, Cgs, QCgs, BiCg, TfQmr, Qmr, Cgne, Lsqr, Gcr, BiCgStabl, BiCgcr,
+ * QmrSym, MinRes and Symmlq, SOR / SSOR,
  * ConvertMatrix_from_Coordinates): the oracle the HIP library's results are
- * compared with.  Cgs, QCgs, BiCg, TfQmr, Qmr, Cgne, Lsqr, Gcr, BiCgcr, QmrSym, MinRes and Symmlq
- * are pinned bitwise to
+ * compared with.  Cgs, QCgs, BiCg, TfQmr, Qmr, Cgne, Lsqr, Gcr, BiCgStabl, BiCgcr, QmrSym, MinRes
+ * and Symmlq are pinned bitwise to
  * Seldon's own runs of them (the overlay program's CPU legs), and by closed
  * one-loop cases, BiCg following Cg on an SPD matrix, TfQmr's
  * quasi-residual bound, Qmr's carried residual, Cgne as Cg on the explicit
